@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Walk-engine benchmark: walk-steps/s on one MI355X.
+
+Workload (one bench step): single-source PPR (visit counts) and RWR
+(endpoints) on an R-MAT scale-22 graph (a/b/c/d = .57/.19/.19/.05, edge
+factor 16, symmetrized, deduplicated, self-loops and isolated ids removed,
+u32 CSR), from two sources -- the highest-degree vertex and one uniform
+random vertex (seed 1) -- with 1e8 walkers per source and app, termination
+probability 0.2, Philox seed 1.  Graph and outputs stay in HBM; the timed
+region covers the four walks.
+
+Printed: one JSON line with walk-steps/s, the roofline of the dominant
+kernel (PPR visit counting: 20 algorithmic bytes per step), the measured
+random-gather rate at the CSR's footprint, and the reference process timed
+on the host cores (xorshift* CPU path) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--walkers", type=float, default=1e8)
+    ap.add_argument("--tp", type=float, default=0.2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
+                    help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(walkers shard by id range with one counter all-reduce; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ptwalker_amd as ptw
+
+    W = int(args.walkers)
+    ctx = ptw.Context(0)
+    t0 = time.time()
+    gi = ctx.rmat(scale=args.scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1)
+    build_s = time.time() - t0
+    rng = np.random.default_rng(1)
+    hub = int(gi.max_degree_vertex)
+    rnd = int(rng.integers(0, gi.num_vertices))
+    sources = [hub, rnd]
+    V = gi.num_vertices
+    counts = torch.zeros(V, dtype=torch.int64, device="cuda")
+    ends = torch.zeros(W, dtype=torch.int32, device="cuda")
+
+    def one_step(record):
+        steps = 0
+        for s in sources:
+            d = ptw.make_desc(ptw.PPR(args.tp, s), W, seed=1, flags=ptw.FLAG_OUT_DEVICE)
+            o = ctx.walk(d, counts=counts.data_ptr(), device_ptrs=True)
+            steps += o.steps
+            if record is not None:
+                record["ppr_ms"].append(o.kernel_ms)
+                record["ppr_steps"].append(o.steps)
+            d = ptw.make_desc(ptw.RWR(args.tp, s), W, seed=1, flags=ptw.FLAG_OUT_DEVICE)
+            o = ctx.walk(d, ends=ends.data_ptr(), device_ptrs=True)
+            steps += o.steps
+            if record is not None:
+                record["rwr_ms"].append(o.kernel_ms)
+                record["rwr_steps"].append(o.steps)
+        return steps
+
+    for _ in range(args.warmup):
+        one_step(None)
+    rec = {"ppr_ms": [], "ppr_steps": [], "rwr_ms": [], "rwr_steps": []}
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    total_steps = 0
+    for _ in range(args.steps):
+        total_steps += one_step(rec)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    value = total_steps / elapsed
+
+    # roofline of the dominant kernel: PPR visit counting, 20 B per step
+    ppr_ms = float(np.mean(rec["ppr_ms"]))
+    ppr_steps = float(np.mean(rec["ppr_steps"]))
+    ppr_bytes = 20.0 * ppr_steps
+    achieved = ppr_bytes / (ppr_ms * 1e-3) / 1e9
+    rwr_ms = float(np.mean(rec["rwr_ms"]))
+    rwr_steps = float(np.mean(rec["rwr_steps"]))
+    rwr_gbs = (12.0 * rwr_steps + 4.0 * W) / (rwr_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    # measured random-gather roofline at the CSR's col[] footprint
+    col_bytes = 4 * gi.num_edges
+    gps, _ = ctx.bench_gather(col_bytes, 1 << 30, seed=1)
+    ppr_sps = ppr_steps / (ppr_ms * 1e-3)
+    gather_frac = 2.0 * ppr_sps / gps
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx, hub, args)
+
+    line = {
+        "metric": "walk-steps/s",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT graph generated in-process (seeded Philox); no dataset",
+        "config": {
+            "workload": f"single-source PPR + RWR, R-MAT scale {args.scale} ef16 symmetrized, "
+                        f"sources = max-degree + 1 uniform, {W:.0e} walkers per source per app, tp {args.tp}",
+            "vertices": V,
+            "edges": gi.num_edges,
+            "walkers_per_source": W,
+            "sources": sources,
+            "tp": args.tp,
+            "walk_steps_per_bench_step": total_steps // args.steps,
+            "graph_build_s": round(build_s, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "walk_kernel<count> (PPR)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_step": 20,
+            "kernel_ms": ppr_ms,
+            "steps_per_launch": ppr_steps,
+        },
+        "random_gather": {
+            "gathers_per_s": gps,
+            "table_bytes": col_bytes,
+            "frac": gather_frac,
+            "note": "(PPR steps/s x 2 random gathers per step) / measured random u32 gathers/s",
+        },
+        "rwr": {"kernel_ms": rwr_ms, "achieved_GBs": rwr_gbs, "bytes_per_step": 12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(ctx, hub, args):
+    """The reference process with its own RNG (oracle: xorshift*, quantized geometric,
+    OpenMP over walkers), PPR from the same source, on a bounded walker sample."""
+    import oracle
+
+    rp, col = ctx.export_csr()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    probe = 200_000
+    t = time.perf_counter()
+    r = oracle.ref_walk(rp, col, oracle.APP_PPR, probe, tp=args.tp, source=hub, start_uniform=False, seed=1,
+                        threads=cores)
+    dt = time.perf_counter() - t
+    rate = r["steps"] / max(dt, 1e-9)
+    n = int(min(5e8, max(probe, rate * args.cpu_seconds * args.tp)))
+    t = time.perf_counter()
+    r = oracle.ref_walk(rp, col, oracle.APP_PPR, n, tp=args.tp, source=hub, start_uniform=False, seed=1,
+                        threads=cores)
+    dt = time.perf_counter() - t
+    return {
+        "value": r["steps"] / dt,
+        "unit": "steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"PPR from the max-degree vertex, {n} walkers ({r['steps']} steps, {dt:.1f} s), "
+                  f"xorshift* reference-semantics CPU path, {cores} threads",
+    }
+
+
+if __name__ == "__main__":
+    main()

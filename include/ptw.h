@@ -1,0 +1,191 @@
+/*
+ * ptw.h -- C ABI of the MI355X probabilistic-termination random-walk engine.
+ *
+ * One context owns one HIP device, one stream and one resident graph.  Every
+ * function returns 0 on success or a negative PTW_ERR_* code; the message of
+ * the last failure is available from ptw_last_error().  No C++ exception
+ * crosses this boundary.  A context is not thread-safe: use one per host
+ * thread.
+ *
+ * Vertex ids at this boundary are "graph ids": for a loaded edge list they
+ * are the first-appearance compact ids that the reference assigns while
+ * loading (src/core/graph.hpp:293-319); for a CSR handed in they are the
+ * CSR's own indices.  The device holds a degree-descending relabel of the
+ * graph, which is invisible here: every output is mapped back.
+ *
+ * Walk process (one Philox4x32-10 block per step, keyed by seed and counted
+ * by (walker id, step, stream)) -- see ptw_walk().
+ */
+#ifndef PTW_H
+#define PTW_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTW_VERSION_MAJOR 0
+#define PTW_VERSION_MINOR 1
+
+/* status codes */
+#define PTW_OK 0
+#define PTW_ERR_INVALID_ARG (-1)
+#define PTW_ERR_IO (-2)
+#define PTW_ERR_HIP (-3)
+#define PTW_ERR_NO_GRAPH (-4)
+#define PTW_ERR_OOM (-5)
+#define PTW_ERR_DANGLING (-6)
+#define PTW_ERR_FORMAT (-7)
+#define PTW_ERR_UNSUPPORTED (-8)
+#define PTW_ERR_ADMISSION (-9)
+#define PTW_ERR_CANCELLED (-10)
+#define PTW_ERR_TIMEOUT (-11)
+
+/* graph formats: include/type.hpp:12-15 (BinaryGraphFormat, TextGraphFormat) */
+#define PTW_FORMAT_BINARY 0
+#define PTW_FORMAT_TEXT 1
+
+/* apps: src/core/{pagerank,ppr,RWR,deepwalk}.cpp */
+#define PTW_APP_PAGERANK 0
+#define PTW_APP_PPR 1
+#define PTW_APP_RWR 2
+#define PTW_APP_DEEPWALK 3
+
+/* step-budget semantics for the probabilistic-termination apps */
+#define PTW_SEM_REFERENCE 0 /* L = 1 + Geometric(tp), src/core/pagerank.cpp:44-67 */
+#define PTW_SEM_TEXTBOOK 1  /* L = Geometric(tp) (may be 0) */
+
+/* start vertex of a walker */
+#define PTW_START_UNIFORM 0    /* uniform over vertices, src/core/pagerank.cpp:70-80 */
+#define PTW_START_SOURCE 1     /* every walker at desc.source, src/core/ppr.cpp:91-108 */
+#define PTW_START_ROUNDROBIN 2 /* walker w at vertex w mod V ("walks per vertex") */
+
+/* what a walker does at a vertex without out-edges while budget is left */
+#define PTW_DANGLING_ERROR 0 /* report PTW_ERR_DANGLING (the reference divides by 0) */
+#define PTW_DANGLING_STOP 1  /* the walker ends there */
+
+/* ptw_walk_desc.flags */
+#define PTW_FLAG_OUT_DEVICE 1u /* output pointers are device pointers */
+#define PTW_FLAG_ACCUMULATE 2u /* counts += instead of counts = */
+
+/* ptw_graph_rmat flags */
+#define PTW_RMAT_KEEP_ISOLATED 1u /* keep degree-0 ids (default: compact them away) */
+#define PTW_GRAPH_NO_RELABEL 2u   /* keep graph ids on the device (no degree relabel) */
+
+#define PTW_INVALID_VERTEX 0xFFFFFFFFu
+
+/* ptw_create device argument for a context without a device: graph ingest,
+ * export and writing work; walks return PTW_ERR_UNSUPPORTED. */
+#define PTW_DEVICE_HOST_ONLY (-1)
+
+typedef struct ptw_ctx ptw_ctx;
+
+typedef struct ptw_graph_info {
+    uint32_t num_vertices;
+    uint32_t max_degree;
+    uint64_t num_edges;
+    uint32_t max_degree_vertex; /* graph id; lowest id among ties */
+    uint32_t num_dangling;      /* vertices with out-degree 0 */
+    uint32_t hub_vertices;      /* device ids below this count in LDS hub counters */
+    uint32_t relabelled;        /* 1 if the device holds a degree relabel */
+    uint64_t device_bytes;      /* HBM held by the resident graph */
+} ptw_graph_info;
+
+/* Plain-data replacement for the reference's Alg template concept
+ * (src/core/solver.hpp:1049-1050; type/walking_step/walker_init/update,
+ * src/core/pagerank.cpp:10-90). */
+typedef struct ptw_walk_desc {
+    int32_t app;             /* PTW_APP_* */
+    int32_t semantics;       /* PTW_SEM_* */
+    int32_t start_mode;      /* PTW_START_* */
+    int32_t dangling_policy; /* PTW_DANGLING_* */
+    uint32_t source;         /* graph id, PTW_START_SOURCE */
+    uint32_t walk_len;       /* DEEPWALK: vertices per path (reference -l) */
+    double tp;               /* termination probability (reference --sp), (0, 1] */
+    uint64_t seed;           /* Philox key */
+    uint64_t wid_base;       /* first walker id (for split or resumed runs) */
+    uint64_t walkers;        /* walker count (reference -w) */
+    uint32_t stream;         /* 4th Philox counter word (query id) */
+    uint32_t flags;          /* PTW_FLAG_* */
+} ptw_walk_desc;
+
+typedef struct ptw_walk_out {
+    uint64_t *counts; /* PAGERANK/PPR: u64[V] visit counts, or NULL */
+    uint32_t *ends;   /* RWR: u32[walkers] end vertex per walker, or NULL */
+    uint32_t *paths;  /* DEEPWALK: u32[walkers * walk_len], or NULL */
+    /* filled on return */
+    uint64_t steps;    /* sum of transitions over walkers */
+    uint64_t visits;   /* count increments (count apps) */
+    uint64_t dangling; /* walkers stopped at a degree-0 vertex */
+    double kernel_ms;  /* device time of the walk kernel (HIP events) */
+    double total_ms;   /* wall time of the call */
+} ptw_walk_out;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces FMobSolver(Graph*, MultiThreadConfig), src/core/solver.hpp:120,
+ * and init_concurrency, include/numa_helper.hpp:37-127. */
+int ptw_create(ptw_ctx **out, int device); /* device >= 0, or PTW_DEVICE_HOST_ONLY */
+void ptw_destroy(ptw_ctx *ctx);
+const char *ptw_last_error(const ptw_ctx *ctx);
+/* Library version as major * 1000 + minor. */
+int ptw_version(void);
+
+/* ---- graph ------------------------------------------------------------ */
+/* Graph::load(path, GraphFormat, as_undirected, bodary),
+ * src/core/graph.hpp:277-366, with read_text_graph / read_binary_graph,
+ * include/io.hpp:39-53,76-90.  Ids are compacted by first appearance and
+ * adjacency keeps file order.  flags: PTW_GRAPH_NO_RELABEL. */
+int ptw_graph_load(ptw_ctx *ctx, const char *path, int format, int as_undirected, uint32_t flags);
+
+/* A CSR in graph ids (row_ptr[V + 1], col[E]); replaces the graph build of
+ * src/core/graph.hpp:331-366 for callers that already hold a CSR. */
+int ptw_graph_from_csr(ptw_ctx *ctx, const uint32_t *row_ptr, const uint32_t *col,
+                       uint32_t num_vertices, uint64_t num_edges, uint32_t flags);
+
+/* Seeded R-MAT generator: 2^scale vertices, edge_factor * 2^scale draws with
+ * quadrant probabilities a, b, c (d = 1 - a - b - c); symmetrized,
+ * deduplicated, self-loops dropped, isolated ids compacted away unless
+ * PTW_RMAT_KEEP_ISOLATED.  Replaces gen_undirected_graph of
+ * src/test/test.hpp:62-88 as the synthetic-input source. */
+int ptw_graph_rmat(ptw_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a, double b,
+                   double c, uint64_t seed, uint32_t flags);
+
+int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info);
+
+/* Copy the resident graph's CSR in graph ids to host memory. */
+int ptw_graph_export_csr(ptw_ctx *ctx, uint32_t *row_ptr, uint32_t *col);
+
+/* Write the resident graph as an edge list (write_text_graph /
+ * write_binary_graph, include/io.hpp:55-105). */
+int ptw_graph_write(ptw_ctx *ctx, const char *path, int format);
+
+/* ---- walks ------------------------------------------------------------ */
+/* walk(FMobSolver*, walker_num, walk_len, mem_quota, Alg&, walker_size),
+ * src/core/solver.hpp:1049-1074, with the walker loop of
+ * src/core/solver.hpp:194-498.  Process for walker w (wid = wid_base + w):
+ *   start:  SOURCE: desc.source; ROUNDROBIN: wid mod V; UNIFORM:
+ *           mulhi64(Philox(wid, 0xFFFFFFFF, stream).xy, V)
+ *   step i: r = Philox(ctr = (wid_lo, wid_hi, i, stream), key = seed);
+ *           PT apps stop when r.z >= floor((1 - tp) 2^32) (from i >= 1 under
+ *           PTW_SEM_REFERENCE); DEEPWALK stops at i == walk_len - 1;
+ *           v = col[row_ptr[v] + mulhi64(r.xy, deg v)]
+ * PAGERANK/PPR count the start and every step (type 0); RWR records the
+ * last vertex (type 1); DEEPWALK writes the path (PTW_INVALID_VERTEX after a
+ * dangling stop).  Synchronous: returns when outputs are complete. */
+int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Random-gather microbenchmark: `gathers` dependent-free random u32 loads
+ * from a table of `table_bytes`, Philox indices.  Reports gathers/s and the
+ * kernel time.  No counterpart in the reference: it is the roofline the walk
+ * kernel is measured against. */
+int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed,
+                     double *gathers_per_s, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTW_H */

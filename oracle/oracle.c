@@ -1,0 +1,375 @@
+/*
+ * oracle.c -- CPU oracle for the walk engine.  TEST INFRASTRUCTURE ONLY:
+ * see ptw_oracle.h for what is here and who may call it.
+ *
+ * Reference files followed (PTWalker):
+ *   walker loop, visit counting:      src/core/solver.hpp:242-262
+ *   walker loop, endpoint callback:   src/core/solver.hpp:284-305, src/core/RWR.cpp:99-105
+ *   start vertex per walker:          src/core/solver.hpp:767-776, src/core/pagerank.cpp:70-80
+ *   step budget 1 + Geometric(tp):    src/core/pagerank.cpp:22-36,41-68
+ *   DeepWalk fixed budget:            src/core/deepwalk.cpp:25-28
+ *   uniform neighbour choice:         src/core/sampler.hpp:214-217
+ *   xorshift* generator:              include/random.hpp:144-180
+ */
+#include "ptw_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon et al., SC'11, section 4.2): S-box round on   */
+/* (x0,x1,x2,x3) with multipliers M0, M1 and a Weyl key schedule.       */
+/* ------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * x0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+uint32_t orc_mulhi64(uint32_t lo, uint32_t hi, uint32_t n)
+{
+    /* floor((hi * 2^32 + lo) * n / 2^64), exact in 64-bit pieces */
+    uint64_t a = (uint64_t)hi * n;
+    uint64_t b = ((uint64_t)lo * n) >> 32;
+    return (uint32_t)((a + b) >> 32);
+}
+
+uint32_t orc_continue_threshold(double tp)
+{
+    double c = floor((1.0 - tp) * 4294967296.0);
+    if (c < 0.0) c = 0.0;
+    if (c > 4294967295.0) c = 4294967295.0;
+    return (uint32_t)c;
+}
+
+static inline void draw(uint64_t seed, uint32_t stream, uint64_t wid, uint32_t step, uint32_t r[4])
+{
+    uint32_t ctr[4] = {(uint32_t)wid, (uint32_t)(wid >> 32), step, stream};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    orc_philox4x32_10(ctr, key, r);
+}
+
+uint32_t orc_walk_length(uint64_t seed, uint32_t stream, uint64_t wid, double tp, int semantics)
+{
+    uint32_t thr = orc_continue_threshold(tp);
+    uint32_t r[4];
+    for (uint32_t i = 0;; ++i) {
+        draw(seed, stream, wid, i, r);
+        int coin = (semantics == ORC_SEM_TEXTBOOK) || i >= 1;
+        if (coin && r[2] >= thr) return i;
+    }
+}
+
+static int is_pt_app(int app) { return app != ORC_APP_DEEPWALK; }
+
+int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                    const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
+                    uint32_t *paths, orc_walk_stats *st, int threads)
+{
+    if (V == 0 || d->walkers == 0) {
+        if (st) memset(st, 0, sizeof(*st));
+        return 0;
+    }
+    if (d->start_mode == ORC_START_SOURCE && d->source >= V) return -1;
+    if (d->app == ORC_APP_DEEPWALK && d->walk_len == 0) return -1;
+    const uint32_t thr = orc_continue_threshold(d->tp);
+    const int pt = is_pt_app(d->app);
+    const int count_app = (d->app == ORC_APP_PAGERANK || d->app == ORC_APP_PPR);
+    uint64_t tot_steps = 0, tot_visits = 0, tot_dangling = 0;
+    const int64_t W = (int64_t)d->walkers;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt) reduction(+ : tot_steps, tot_visits, tot_dangling)
+#else
+    (void)threads;
+#endif
+    for (int64_t w = 0; w < W; ++w) {
+        uint64_t wid = d->wid_base + (uint64_t)w;
+        uint32_t r[4];
+        uint32_t v;
+        if (d->start_mode == ORC_START_SOURCE) {
+            v = d->source;
+        } else if (d->start_mode == ORC_START_ROUNDROBIN) {
+            v = (uint32_t)(wid % V);
+        } else {
+            draw(d->seed, d->stream, wid, 0xFFFFFFFFu, r);
+            v = orc_mulhi64(r[0], r[1], V);
+        }
+        if (count_app && counts) {
+            __atomic_fetch_add(&counts[v], 1, __ATOMIC_RELAXED);
+        }
+        tot_visits += count_app;
+        uint32_t *path = paths ? paths + (uint64_t)w * d->walk_len : NULL;
+        if (path) path[0] = v;
+        uint32_t i = 0;
+        for (;; ++i) {
+            if (!pt && i + 1 >= d->walk_len) break;
+            draw(d->seed, d->stream, wid, i, r);
+            if (pt) {
+                int coin = (d->semantics == ORC_SEM_TEXTBOOK) || i >= 1;
+                if (coin && r[2] >= thr) break;
+            }
+            uint32_t beg = row_ptr[v], deg = row_ptr[v + 1] - beg;
+            if (deg == 0) { tot_dangling++; break; }
+            v = col[beg + orc_mulhi64(r[0], r[1], deg)];
+            if (count_app && counts) __atomic_fetch_add(&counts[v], 1, __ATOMIC_RELAXED);
+            tot_visits += count_app;
+            if (path) path[i + 1] = v;
+        }
+        tot_steps += i;
+        if (path) {
+            for (uint32_t k = i + 1; k < d->walk_len; ++k) path[k] = ORC_INVALID;
+        }
+        if (ends) ends[w] = v;
+    }
+    if (st) {
+        st->steps = tot_steps;
+        st->visits = tot_visits;
+        st->dangling = tot_dangling;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reference process with the reference's own RNG                       */
+/* ------------------------------------------------------------------ */
+typedef struct { uint64_t s; char pad[56]; } xorshift_t;
+
+/* include/random.hpp:161-168: output from the pre-update state */
+static inline uint32_t xs_gen(xorshift_t *g, uint32_t bound)
+{
+    uint32_t ret = (uint32_t)(g->s * UINT64_C(0x2545F4914F6CDD1D));
+    g->s ^= g->s >> 12;
+    g->s ^= g->s << 25;
+    g->s ^= g->s >> 27;
+    return ret % bound;
+}
+
+/* include/random.hpp:169-176: low 16 bits / 65535 (can be exactly 1.0) */
+static inline float xs_gen_float(xorshift_t *g, float upper)
+{
+    uint32_t ret = (uint32_t)((g->s * UINT64_C(0x2545F4914F6CDD1D)) & 0xFFFF);
+    g->s ^= g->s >> 12;
+    g->s ^= g->s << 25;
+    g->s ^= g->s >> 27;
+    return (float)ret / (float)65535 * upper;
+}
+
+static uint64_t splitmix64(uint64_t *x)
+{
+    uint64_t z = (*x += UINT64_C(0x9E3779B97F4A7C15));
+    z = (z ^ (z >> 30)) * UINT64_C(0xBF58476D1CE4E5B9);
+    z = (z ^ (z >> 27)) * UINT64_C(0x94D049BB133111EB);
+    return z ^ (z >> 31);
+}
+
+static void xs_seed(xorshift_t *g, uint64_t seed, uint64_t index)
+{
+    uint64_t x = seed ^ (index * UINT64_C(0xD1B54A32D192ED03));
+    g->s = splitmix64(&x);
+    if (g->s == 0) g->s = 1; /* 0 is a fixed point of xorshift */
+}
+
+typedef struct { double cdf[128]; double tot; } geo_table;
+
+/* src/core/pagerank.cpp:22-36 */
+static void geo_init(geo_table *t, double tp)
+{
+    for (int si = 0; si < 128; ++si) t->cdf[si] = pow(1.0 - tp, si) * tp;
+    for (int si = 0; si < 127; ++si) t->cdf[si + 1] += t->cdf[si];
+    t->tot = t->cdf[127];
+}
+
+/* src/core/pagerank.cpp:41-68 */
+static inline uint32_t geo_draw(const geo_table *t, xorshift_t *g)
+{
+    uint32_t stepid = 1;
+    for (;;) {
+        double u = xs_gen_float(g, 1.0f);
+        if (u > t->tot) {
+            stepid += 128;
+            continue;
+        }
+        for (int si = 0; si < 128; ++si) {
+            if (t->cdf[si] >= u) {
+                stepid += (uint32_t)si;
+                break;
+            }
+        }
+        return stepid;
+    }
+}
+
+void orc_ref_step_histogram(double tp, uint64_t seed, uint64_t n, uint64_t *hist, uint32_t hist_len)
+{
+    geo_table t;
+    geo_init(&t, tp);
+    xorshift_t g;
+    xs_seed(&g, seed, 0);
+    memset(hist, 0, sizeof(uint64_t) * hist_len);
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t L = geo_draw(&t, &g);
+        if (L >= hist_len) L = hist_len - 1;
+        hist[L]++;
+    }
+}
+
+void orc_xorshift_gen(uint64_t state, uint32_t bound, uint64_t n, uint32_t *out)
+{
+    xorshift_t g;
+    g.s = state;
+    for (uint64_t i = 0; i < n; ++i) out[i] = xs_gen(&g, bound);
+}
+
+int orc_ref_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                 int app, uint32_t source, int start_uniform, double tp, uint32_t walk_len,
+                 uint64_t walkers, uint64_t seed, int threads,
+                 uint64_t *counts, uint32_t *ends, orc_walk_stats *st)
+{
+    if (V == 0) return -1;
+    if (!start_uniform && source >= V) return -1;
+    geo_table t;
+    geo_init(&t, tp);
+    const int count_app = (app == ORC_APP_PAGERANK || app == ORC_APP_PPR);
+    uint64_t tot_steps = 0, tot_visits = 0, tot_dangling = 0;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    int nt = 1;
+    (void)threads;
+#endif
+    xorshift_t *gens = (xorshift_t *)aligned_alloc(64, sizeof(xorshift_t) * (size_t)nt);
+    for (int k = 0; k < nt; ++k) xs_seed(&gens[k], seed, (uint64_t)k);
+    const int64_t W = (int64_t)walkers;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(+ : tot_steps, tot_visits, tot_dangling)
+#endif
+    {
+#ifdef _OPENMP
+        xorshift_t *g = &gens[omp_get_thread_num()];
+#pragma omp for schedule(static)
+#else
+        xorshift_t *g = &gens[0];
+#endif
+        for (int64_t w = 0; w < W; ++w) {
+            /* walker_init then walking_step, src/core/solver.hpp:767-781 */
+            uint32_t v = start_uniform ? xs_gen(g, V) : source;
+            uint32_t L = (app == ORC_APP_DEEPWALK) ? walk_len - 1 : geo_draw(&t, g);
+            if (count_app) {
+                __atomic_fetch_add(&counts[v], 1, __ATOMIC_RELAXED);
+                tot_visits++;
+            }
+            uint32_t i = 0;
+            for (; i < L; ++i) {
+                uint32_t beg = row_ptr[v], deg = row_ptr[v + 1] - beg;
+                if (deg == 0) { tot_dangling++; break; }
+                v = col[beg + xs_gen(g, deg)];
+                if (count_app) {
+                    __atomic_fetch_add(&counts[v], 1, __ATOMIC_RELAXED);
+                    tot_visits++;
+                }
+            }
+            tot_steps += i;
+            if (ends) ends[w] = v;
+        }
+    }
+    free(gens);
+    if (st) {
+        st->steps = tot_steps;
+        st->visits = tot_visits;
+        st->dangling = tot_dangling;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 power iteration                                                 */
+/* ------------------------------------------------------------------ */
+/* y <- y P (row-stochastic uniform transitions; dangling rows lose mass) */
+static void push(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                 const double *y, double *out)
+{
+    memset(out, 0, sizeof(double) * V);
+    for (uint32_t u = 0; u < V; ++u) {
+        uint32_t beg = row_ptr[u], deg = row_ptr[u + 1] - beg;
+        if (deg == 0 || y[u] == 0.0) continue;
+        double m = y[u] / (double)deg;
+        for (uint32_t k = 0; k < deg; ++k) out[col[beg + k]] += m;
+    }
+}
+
+int orc_power_iteration(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                        uint32_t start, double tp, int semantics, int mode,
+                        double tol, int max_iter, double *out)
+{
+    double *y = (double *)calloc(V, sizeof(double));
+    double *z = (double *)calloc(V, sizeof(double));
+    if (!y || !z) { free(y); free(z); return -1; }
+    if (start == ORC_INVALID) {
+        for (uint32_t v = 0; v < V; ++v) y[v] = 1.0 / (double)V;
+    } else {
+        y[start] = 1.0;
+    }
+    memset(out, 0, sizeof(double) * V);
+    int it = 0;
+    if (mode == 0) {
+        /* E[count] per walker = e_s + sum_{i>=1} P(L >= i) e_s P^i, with
+         * P(L >= i) = (1-tp)^(i-1) (reference) or (1-tp)^i (textbook). */
+        for (uint32_t v = 0; v < V; ++v) out[v] = y[v];
+        double w = (semantics == ORC_SEM_TEXTBOOK) ? (1.0 - tp) : 1.0;
+        for (it = 1; it <= max_iter; ++it) {
+            push(row_ptr, col, V, y, z);
+            double mass = 0.0;
+            for (uint32_t v = 0; v < V; ++v) {
+                out[v] += w * z[v];
+                mass += z[v];
+            }
+            double *tmp = y; y = z; z = tmp;
+            if (w * mass < tol) break;
+            w *= (1.0 - tp);
+        }
+    } else {
+        /* E[end] = sum_{L} P(L) e_s P^L; reference: P(L = k+1) = tp (1-tp)^k */
+        double w = (semantics == ORC_SEM_TEXTBOOK) ? tp : 0.0;
+        if (semantics == ORC_SEM_TEXTBOOK) {
+            for (uint32_t v = 0; v < V; ++v) out[v] = w * y[v];
+        }
+        double pl = (semantics == ORC_SEM_TEXTBOOK) ? tp * (1.0 - tp) : tp;
+        for (it = 1; it <= max_iter; ++it) {
+            push(row_ptr, col, V, y, z);
+            double mass = 0.0;
+            for (uint32_t v = 0; v < V; ++v) {
+                out[v] += pl * z[v];
+                mass += z[v];
+            }
+            double *tmp = y; y = z; z = tmp;
+            /* remaining weight sum_{j>it} P(L=j) = P(L > it) */
+            if (mass * (pl / tp) * (1.0 - tp) < tol) break;
+            pl *= (1.0 - tp);
+        }
+    }
+    free(y);
+    free(z);
+    return it;
+}

@@ -1,0 +1,128 @@
+/*
+ * ptw_oracle.h -- CPU oracle for the probabilistic-termination walk engine.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libptw, the CLI
+ * binaries, the Python package) links, loads or calls this code.  It is
+ * used by tests/, by __graft_entry__.smoke() as the checker, and by
+ * bench.py's cpu_baseline leg as the timed CPU path.
+ *
+ * Three things live here:
+ *
+ *  1. Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
+ *     as easy as 1, 2, 3", SC'11), written from the paper's definition.
+ *
+ *  2. A scalar replay of the engine's walk process on a CSR graph.  The
+ *     process is the reference walker loop (src/core/solver.hpp:242-262 for
+ *     visit counting, :284-305 for the endpoint callback; step budget
+ *     L = 1 + K, K ~ Geometric(tp), src/core/pagerank.cpp:41-68; DeepWalk's
+ *     fixed budget walk_len - 1, src/core/deepwalk.cpp:25-28) with every
+ *     random decision taken from Philox keyed by (seed, walker id, step):
+ *
+ *       key  = (seed_lo, seed_hi)
+ *       init = Philox(ctr = (wid_lo, wid_hi, 0xFFFFFFFF, stream))
+ *              start = mulhi64(init.x | init.y << 32, V)     (uniform start)
+ *       step i (i = 0, 1, ...):  r = Philox(ctr = (wid_lo, wid_hi, i, stream))
+ *              stop if the budget is spent:
+ *                 PT apps, reference semantics: i >= 1 and r.z >= thr
+ *                 PT apps, textbook semantics:  r.z >= thr
+ *                 fixed-length apps:            i == walk_len - 1
+ *              stop if deg(v) == 0 (dangling; counted in *dangling)
+ *              v = col[row_ptr[v] + mulhi64(r.x | r.y << 32, deg(v))]
+ *       thr  = min(floor((1 - tp) * 2^32), 2^32 - 1)   (continue iff r.z < thr)
+ *
+ *     so that one Philox block decides one step, and the result does not
+ *     depend on which lane, thread or order advanced the walker.
+ *
+ *  3. The reference's own CPU process with its own RNG: xorshift* with the
+ *     output taken before the state update and a 16-bit float
+ *     (include/random.hpp:144-180), the 128-entry geometric CDF table
+ *     (src/core/pagerank.cpp:22-36,41-68), uniform neighbour choice by
+ *     modulo (src/core/sampler.hpp:214-217).  OpenMP over walkers.
+ *
+ *  4. An FP64 power-iteration oracle for the expected visit vector, the
+ *     expected endpoint vector, and the per-step position distributions.
+ */
+#ifndef PTW_ORACLE_H
+#define PTW_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* app kinds and semantics: identical numbering to include/ptw.h */
+enum { ORC_APP_PAGERANK = 0, ORC_APP_PPR = 1, ORC_APP_RWR = 2, ORC_APP_DEEPWALK = 3 };
+enum { ORC_SEM_REFERENCE = 0, ORC_SEM_TEXTBOOK = 1 };
+enum { ORC_START_UNIFORM = 0, ORC_START_SOURCE = 1, ORC_START_ROUNDROBIN = 2 };
+
+#define ORC_INVALID 0xFFFFFFFFu
+
+/* Philox4x32-10 single block. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* floor(r64 * n / 2^64) for r64 = hi:lo */
+uint32_t orc_mulhi64(uint32_t lo, uint32_t hi, uint32_t n);
+
+/* continuation threshold for termination probability tp */
+uint32_t orc_continue_threshold(double tp);
+
+typedef struct {
+    int app;            /* ORC_APP_* */
+    int semantics;      /* ORC_SEM_* (PT apps only) */
+    int start_mode;     /* ORC_START_* */
+    uint32_t source;    /* start vertex when start_mode == SOURCE */
+    double tp;          /* termination probability (PT apps) */
+    uint32_t walk_len;  /* vertices per path (DEEPWALK) */
+    uint64_t seed;
+    uint32_t stream;    /* query / stream id (4th counter word) */
+    uint64_t wid_base;  /* first walker id */
+    uint64_t walkers;   /* number of walkers */
+} orc_walk_desc;
+
+typedef struct {
+    uint64_t steps;     /* sum over walkers of transitions taken */
+    uint64_t visits;    /* sum of count increments (count apps) */
+    uint64_t dangling;  /* walkers stopped at a degree-0 vertex with budget left */
+} orc_walk_stats;
+
+/* Philox replay.  counts: u64[V] accumulated (+=), may be NULL;
+ * ends: u32[walkers] (RWR), may be NULL; paths: u32[walkers * walk_len]
+ * (DEEPWALK), may be NULL.  threads <= 0 means all OpenMP threads. */
+int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                    const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
+                    uint32_t *paths, orc_walk_stats *st, int threads);
+
+/* Walk length of one walker under the Philox process (for closed-form checks). */
+uint32_t orc_walk_length(uint64_t seed, uint32_t stream, uint64_t wid, double tp, int semantics);
+
+/* Reference-semantics CPU path with xorshift* (own RNG).  Per-thread
+ * generators are seeded from `seed` and the thread index.  counts u64[V]
+ * (+=, atomic), ends u32[walkers] or NULL. */
+int orc_ref_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                 int app, uint32_t source, int start_uniform, double tp, uint32_t walk_len,
+                 uint64_t walkers, uint64_t seed, int threads,
+                 uint64_t *counts, uint32_t *ends, orc_walk_stats *st);
+
+/* The reference's quantized geometric draw: 1 + K with the 128-entry CDF and
+ * xorshift* gen_float.  Runs `n` draws from one generator seeded with `seed`
+ * and writes a histogram of min(L, hist_len - 1) into hist (u64[hist_len]). */
+void orc_ref_step_histogram(double tp, uint64_t seed, uint64_t n, uint64_t *hist, uint32_t hist_len);
+
+/* xorshift* generator (include/random.hpp:161-176) exposed for tests:
+ * writes n outputs of gen(bound) starting from `state`. */
+void orc_xorshift_gen(uint64_t state, uint32_t bound, uint64_t n, uint32_t *out);
+
+/* FP64 oracle.  mode 0: expected visits per walker (count apps), mode 1:
+ * expected endpoint distribution (RWR).  start: source vertex, or
+ * ORC_INVALID for the uniform start.  out: double[V].  Iterates until the
+ * remaining mass is below tol or max_iter is reached; returns iterations. */
+int orc_power_iteration(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                        uint32_t start, double tp, int semantics, int mode,
+                        double tol, int max_iter, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,336 @@
+"""ptwalker_amd -- MI355X probabilistic-termination random-walk engine.
+
+Python mirror of the reference's in-process interface, over the C ABI of
+``libptw.so`` (include/ptw.h):
+
+* ``Graph`` -- ``Graph(MultiThreadConfig)`` + ``Graph::load(path, format,
+  as_undirected, bodary)`` (src/core/graph.hpp:151,277); also a CSR handed
+  in and the seeded R-MAT generator.
+* ``walk(graph, walker_num, app, ...)`` -- ``walk(FMobSolver*, walker_num,
+  walk_len, mem_quota, Alg&)`` (src/core/solver.hpp:1049-1074).
+* apps -- the reference's Alg plug-ins as plain data: ``PageRank``
+  (src/core/pagerank.cpp:10-90), ``PPR`` (src/core/ppr.cpp:10-119),
+  ``RWR`` (src/core/RWR.cpp:15-107), ``DeepWalk`` (src/core/deepwalk.cpp:10-42).
+
+There is no CPU fallback: importing works without a GPU, but every call
+that walks needs the HIP library and a device, and fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libptw.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+# status codes, apps, flags: include/ptw.h
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_IO = -2
+ERR_HIP = -3
+ERR_NO_GRAPH = -4
+ERR_OOM = -5
+ERR_DANGLING = -6
+ERR_FORMAT = -7
+ERR_UNSUPPORTED = -8
+ERR_ADMISSION = -9
+ERR_CANCELLED = -10
+ERR_TIMEOUT = -11
+
+FORMAT_BINARY = 0
+FORMAT_TEXT = 1
+
+APP_PAGERANK = 0
+APP_PPR = 1
+APP_RWR = 2
+APP_DEEPWALK = 3
+
+SEM_REFERENCE = 0
+SEM_TEXTBOOK = 1
+
+START_UNIFORM = 0
+START_SOURCE = 1
+START_ROUNDROBIN = 2
+
+DANGLING_ERROR = 0
+DANGLING_STOP = 1
+
+FLAG_OUT_DEVICE = 1
+FLAG_ACCUMULATE = 2
+
+RMAT_KEEP_ISOLATED = 1
+GRAPH_NO_RELABEL = 2
+
+INVALID_VERTEX = 0xFFFFFFFF
+
+
+class PtwError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"ptw error {code}: {message}")
+        self.code = code
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_vertices", ctypes.c_uint32),
+        ("max_degree", ctypes.c_uint32),
+        ("num_edges", ctypes.c_uint64),
+        ("max_degree_vertex", ctypes.c_uint32),
+        ("num_dangling", ctypes.c_uint32),
+        ("hub_vertices", ctypes.c_uint32),
+        ("relabelled", ctypes.c_uint32),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class WalkDesc(ctypes.Structure):
+    _fields_ = [
+        ("app", ctypes.c_int32),
+        ("semantics", ctypes.c_int32),
+        ("start_mode", ctypes.c_int32),
+        ("dangling_policy", ctypes.c_int32),
+        ("source", ctypes.c_uint32),
+        ("walk_len", ctypes.c_uint32),
+        ("tp", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("wid_base", ctypes.c_uint64),
+        ("walkers", ctypes.c_uint64),
+        ("stream", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class WalkOut(ctypes.Structure):
+    _fields_ = [
+        ("counts", ctypes.c_void_p),
+        ("ends", ctypes.c_void_p),
+        ("paths", ctypes.c_void_p),
+        ("steps", ctypes.c_uint64),
+        ("visits", ctypes.c_uint64),
+        ("dangling", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+
+# every symbol include/ptw.h declares, with its ctypes signature
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "ptw_version": (ctypes.c_int, []),
+    "ptw_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "ptw_destroy": (None, [_P]),
+    "ptw_last_error": (ctypes.c_char_p, [_P]),
+    "ptw_graph_load": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
+    "ptw_graph_from_csr": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32]),
+    "ptw_graph_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32]),
+    "ptw_graph_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
+    "ptw_graph_export_csr": (ctypes.c_int, [_P, _P, _P]),
+    "ptw_graph_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "ptw_walk": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
+    "ptw_bench_gather": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libptw.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data
+
+
+class Context:
+    """One HIP device, one stream, one resident graph (ptw_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.ptw_create(ctypes.byref(h), device)
+        if rc != OK:
+            raise PtwError(rc, f"cannot create a context on HIP device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ptw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, rc: int):
+        if rc != OK:
+            raise PtwError(rc, self.lib.ptw_last_error(self.h).decode(errors="replace"))
+        return rc
+
+    # ---- graph -----------------------------------------------------------
+    def load(self, path: str, fmt: int = FORMAT_TEXT, as_undirected: bool = False, flags: int = 0):
+        self.check(self.lib.ptw_graph_load(self.h, path.encode(), fmt, int(as_undirected), flags))
+        return self.info()
+
+    def from_csr(self, row_ptr: np.ndarray, col: np.ndarray, flags: int = 0):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.check(self.lib.ptw_graph_from_csr(self.h, _ptr(row_ptr), _ptr(col), len(row_ptr) - 1, len(col), flags))
+        return self.info()
+
+    def rmat(self, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, flags: int = 0):
+        self.check(self.lib.ptw_graph_rmat(self.h, scale, edge_factor, a, b, c, seed, flags))
+        return self.info()
+
+    def info(self) -> GraphInfo:
+        gi = GraphInfo()
+        self.check(self.lib.ptw_graph_info_get(self.h, ctypes.byref(gi)))
+        return gi
+
+    def export_csr(self):
+        gi = self.info()
+        row_ptr = np.empty(gi.num_vertices + 1, dtype=np.uint32)
+        col = np.empty(max(gi.num_edges, 1), dtype=np.uint32)
+        self.check(self.lib.ptw_graph_export_csr(self.h, _ptr(row_ptr), _ptr(col)))
+        return row_ptr, col[: gi.num_edges]
+
+    def write(self, path: str, fmt: int = FORMAT_TEXT):
+        self.check(self.lib.ptw_graph_write(self.h, path.encode(), fmt))
+
+    # ---- walks -----------------------------------------------------------
+    def walk(self, desc: WalkDesc, counts=None, ends=None, paths=None, device_ptrs: bool = False) -> WalkOut:
+        out = WalkOut()
+        if device_ptrs:
+            out.counts, out.ends, out.paths = counts, ends, paths
+        else:
+            out.counts, out.ends, out.paths = _ptr(counts), _ptr(ends), _ptr(paths)
+        self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
+        return out
+
+    def bench_gather(self, table_bytes: int, gathers: int, seed: int = 1):
+        gps = ctypes.c_double()
+        ms = ctypes.c_double()
+        self.check(self.lib.ptw_bench_gather(self.h, table_bytes, gathers, seed, ctypes.byref(gps), ctypes.byref(ms)))
+        return gps.value, ms.value
+
+
+# ---- the reference's apps as plain data --------------------------------------
+@dataclass
+class PageRank:
+    """src/core/pagerank.cpp:10-90: uniform starts, 1 + Geometric(tp) steps, visit counts."""
+    tp: float
+    semantics: int = SEM_REFERENCE
+
+
+@dataclass
+class PPR:
+    """src/core/ppr.cpp:10-119: every walker starts at `source`, visit counts."""
+    tp: float
+    source: int
+    semantics: int = SEM_REFERENCE
+
+
+@dataclass
+class RWR:
+    """src/core/RWR.cpp:15-107: end vertex per walker (source, or uniform if None)."""
+    tp: float
+    source: Optional[int] = None
+    semantics: int = SEM_REFERENCE
+
+
+@dataclass
+class DeepWalk:
+    """src/core/deepwalk.cpp:10-42: fixed walk_len vertices per path (walk_len - 1 steps)."""
+    walk_len: int
+    start: int = START_UNIFORM
+
+
+def make_desc(app, walkers: int, seed: int = 1, wid_base: int = 0, stream: int = 0,
+              dangling: int = DANGLING_ERROR, flags: int = 0) -> WalkDesc:
+    d = WalkDesc()
+    d.walkers = walkers
+    d.seed = seed
+    d.wid_base = wid_base
+    d.stream = stream
+    d.dangling_policy = dangling
+    d.flags = flags
+    if isinstance(app, PageRank):
+        d.app, d.tp, d.semantics, d.start_mode = APP_PAGERANK, app.tp, app.semantics, START_UNIFORM
+    elif isinstance(app, PPR):
+        d.app, d.tp, d.semantics, d.start_mode, d.source = APP_PPR, app.tp, app.semantics, START_SOURCE, app.source
+    elif isinstance(app, RWR):
+        d.app, d.tp, d.semantics = APP_RWR, app.tp, app.semantics
+        if app.source is None:
+            d.start_mode = START_UNIFORM
+        else:
+            d.start_mode, d.source = START_SOURCE, app.source
+    elif isinstance(app, DeepWalk):
+        d.app, d.walk_len, d.start_mode = APP_DEEPWALK, app.walk_len, app.start
+    else:
+        raise TypeError(f"unknown app {app!r}")
+    return d
+
+
+@dataclass
+class WalkResult:
+    counts: Optional[np.ndarray]
+    ends: Optional[np.ndarray]
+    paths: Optional[np.ndarray]
+    steps: int
+    visits: int
+    dangling: int
+    kernel_ms: float
+    total_ms: float
+
+
+def walk(ctx: Context, walker_num: int, app, seed: int = 1, wid_base: int = 0, stream: int = 0,
+         dangling: int = DANGLING_ERROR) -> WalkResult:
+    """Run `walker_num` walkers of `app` on the context's graph (host outputs)."""
+    gi = ctx.info()
+    d = make_desc(app, walker_num, seed=seed, wid_base=wid_base, stream=stream, dangling=dangling)
+    counts = ends = paths = None
+    if d.app in (APP_PAGERANK, APP_PPR):
+        counts = np.zeros(gi.num_vertices, dtype=np.uint64)
+    elif d.app == APP_RWR:
+        ends = np.zeros(max(walker_num, 1), dtype=np.uint32)
+    else:
+        paths = np.zeros(max(walker_num * d.walk_len, 1), dtype=np.uint32)
+    out = ctx.walk(d, counts=counts, ends=ends, paths=paths)
+    if ends is not None:
+        ends = ends[:walker_num]
+    if paths is not None:
+        paths = paths[: walker_num * d.walk_len].reshape(walker_num, d.walk_len)
+    return WalkResult(counts, ends, paths, out.steps, out.visits, out.dangling, out.kernel_ms, out.total_ms)

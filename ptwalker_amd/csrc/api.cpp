@@ -1,0 +1,449 @@
+// C ABI of the walk engine (include/ptw.h).  Host orchestration only: the
+// walk itself runs in the HIP kernels of walk.hip; there is no CPU path.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/ptw.h"
+#include "engine.hpp"
+
+struct ptw_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    bool has_graph = false;
+    ptw::HostCsr host;
+    ptw::DeviceGraph dev;
+    uint32_t max_degree = 0, max_degree_vertex = 0, num_dangling = 0;
+
+    unsigned long long *counts_dev = nullptr;  // V device-id counters
+    unsigned long long *scratch = nullptr;     // 8 words
+    ptw::LaunchShape shape;
+};
+
+namespace {
+
+using ptw::HostCsr;
+
+int fail(ptw_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? PTW_ERR_OOM : PTW_ERR_HIP,             \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                       \
+    } while (0)
+
+void free_graph(ptw_ctx *ctx)
+{
+    auto &d = ctx->dev;
+    if (d.row_ptr) (void)hipFree(d.row_ptr);
+    if (d.col) (void)hipFree(d.col);
+    if (d.perm) (void)hipFree(d.perm);
+    if (d.inv) (void)hipFree(d.inv);
+    if (ctx->counts_dev) (void)hipFree(ctx->counts_dev);
+    d = ptw::DeviceGraph{};
+    ctx->counts_dev = nullptr;
+    ctx->has_graph = false;
+    ctx->host = HostCsr{};
+}
+
+template <typename T>
+int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
+{
+    HIP_TRY(ctx, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
+    if (n) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return PTW_OK;
+}
+
+// Takes ownership of `g` (graph ids), builds the device copy.
+int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
+{
+    free_graph(ctx);
+    ctx->host = std::move(g);
+    HostCsr &h = ctx->host;
+    const uint32_t V = h.V;
+    // degree statistics
+    uint32_t maxd = 0, maxv = 0, dang = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        uint32_t d = h.row_ptr[v + 1] - h.row_ptr[v];
+        if (d > maxd) {
+            maxd = d;
+            maxv = v;
+        }
+        dang += d == 0;
+    }
+    ctx->max_degree = maxd;
+    ctx->max_degree_vertex = maxv;
+    ctx->num_dangling = dang;
+
+    auto &d = ctx->dev;
+    d.V = V;
+    d.E = h.E();
+    if (ctx->device < 0) {  // host-only context: no device copy
+        ctx->has_graph = true;
+        return PTW_OK;
+    }
+    int rc;
+    if (flags & PTW_GRAPH_NO_RELABEL) {
+        if ((rc = upload(ctx, &d.row_ptr, h.row_ptr.data(), h.row_ptr.size()))) return rc;
+        if ((rc = upload(ctx, &d.col, h.col.data(), h.col.size()))) return rc;
+        d.relabelled = false;
+        d.hubs = 0;
+    } else {
+        std::vector<uint32_t> perm, inv, rp, col;
+        ptw::relabel_by_degree(h, perm, inv, rp, col);
+        if ((rc = upload(ctx, &d.row_ptr, rp.data(), rp.size()))) return rc;
+        if ((rc = upload(ctx, &d.col, col.data(), col.size()))) return rc;
+        if ((rc = upload(ctx, &d.perm, perm.data(), perm.size()))) return rc;
+        if ((rc = upload(ctx, &d.inv, inv.data(), inv.size()))) return rc;
+        d.relabelled = true;
+        d.hubs = V;
+    }
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->counts_dev, (size_t)(V ? V : 1) * sizeof(unsigned long long)));
+    d.bytes = ((uint64_t)V + 1) * 4 + d.E * 4 + (d.relabelled ? 8ull * V : 0) + 8ull * V;
+    ctx->has_graph = true;
+    return PTW_OK;
+}
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptw_version(void) { return PTW_VERSION_MAJOR * 1000 + PTW_VERSION_MINOR; }
+
+const char *ptw_last_error(const ptw_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int ptw_create(ptw_ctx **out, int device)
+{
+    if (!out) return PTW_ERR_INVALID_ARG;
+    *out = nullptr;
+    ptw_ctx *ctx = new (std::nothrow) ptw_ctx;
+    if (!ctx) return PTW_ERR_OOM;
+    if (device == PTW_DEVICE_HOST_ONLY) {
+        ctx->device = -1;  // graph ingest only; walks report PTW_ERR_UNSUPPORTED
+        *out = ctx;
+        return PTW_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        delete ctx;
+        return PTW_ERR_HIP;  // no HIP device: the engine has no CPU path
+    }
+    if (device < 0 || device >= n) {
+        delete ctx;
+        return PTW_ERR_INVALID_ARG;
+    }
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipMalloc((void **)&ctx->scratch, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        ptw_destroy(ctx);
+        return PTW_ERR_HIP;
+    }
+    *out = ctx;
+    return PTW_OK;
+}
+
+void ptw_destroy(ptw_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->device < 0) {
+        ctx->host = HostCsr{};
+        delete ctx;
+        return;
+    }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_graph(ctx);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int ptw_graph_load(ptw_ctx *ctx, const char *path, int format, int as_undirected, uint32_t flags)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    try {
+        if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+        HostCsr g;
+        int rc = ptw::load_edge_list(path, format, as_undirected != 0, g, ctx->err);
+        if (rc) return rc;
+        if (g.V == 0) return fail(ctx, PTW_ERR_FORMAT, std::string("graph file '") + path + "' holds no edges");
+        return install_graph(ctx, std::move(g), flags);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed while loading the graph");
+    } catch (const std::exception &ex) {
+        return fail(ctx, PTW_ERR_IO, ex.what());
+    }
+}
+
+int ptw_graph_from_csr(ptw_ctx *ctx, const uint32_t *row_ptr, const uint32_t *col, uint32_t num_vertices,
+                       uint64_t num_edges, uint32_t flags)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    try {
+        if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+        int rc = ptw::validate_csr(row_ptr, col, num_vertices, num_edges, ctx->err);
+        if (rc) return rc;
+        HostCsr g;
+        g.V = num_vertices;
+        g.row_ptr.assign(row_ptr, row_ptr + num_vertices + 1);
+        g.col.assign(col, col + num_edges);
+        return install_graph(ctx, std::move(g), flags);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed while copying the CSR");
+    }
+}
+
+int ptw_graph_rmat(ptw_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                   uint64_t seed, uint32_t flags)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    try {
+        if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+        HostCsr g;
+        int rc = ptw::build_rmat(scale, edge_factor, a, b, c, seed, (flags & PTW_RMAT_KEEP_ISOLATED) != 0, g,
+                                 ctx->err);
+        if (rc) return rc;
+        if (g.V == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "R-MAT produced no edges");
+        return install_graph(ctx, std::move(g), flags);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed while generating R-MAT");
+    }
+}
+
+int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info)
+{
+    if (!ctx || !info) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    info->num_vertices = ctx->dev.V;
+    info->num_edges = ctx->dev.E;
+    info->max_degree = ctx->max_degree;
+    info->max_degree_vertex = ctx->max_degree_vertex;
+    info->num_dangling = ctx->num_dangling;
+    info->hub_vertices = ctx->dev.hubs < (uint32_t)ctx->shape.lds_hubs ? ctx->dev.hubs : (uint32_t)ctx->shape.lds_hubs;
+    info->relabelled = ctx->dev.relabelled ? 1 : 0;
+    info->device_bytes = ctx->dev.bytes;
+    return PTW_OK;
+}
+
+int ptw_graph_export_csr(ptw_ctx *ctx, uint32_t *row_ptr, uint32_t *col)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (!row_ptr || (ctx->host.E() && !col)) return fail(ctx, PTW_ERR_INVALID_ARG, "null output buffer");
+    memcpy(row_ptr, ctx->host.row_ptr.data(), ctx->host.row_ptr.size() * 4);
+    if (ctx->host.E()) memcpy(col, ctx->host.col.data(), ctx->host.col.size() * 4);
+    return PTW_OK;
+}
+
+int ptw_graph_write(ptw_ctx *ctx, const char *path, int format)
+{
+    if (!ctx || !path) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (format != PTW_FORMAT_TEXT && format != PTW_FORMAT_BINARY)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown graph format");
+    return ptw::write_edge_list(ctx->host, path, format, ctx->err);
+}
+
+int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!desc || !out) return fail(ctx, PTW_ERR_INVALID_ARG, "null descriptor or output");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: walks need a HIP device");
+    const double t0 = now_ms();
+    const ptw_walk_desc &d = *desc;
+    const uint32_t V = ctx->dev.V;
+    if (d.app < PTW_APP_PAGERANK || d.app > PTW_APP_DEEPWALK) return fail(ctx, PTW_ERR_INVALID_ARG, "unknown app");
+    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
+    if (d.start_mode < PTW_START_UNIFORM || d.start_mode > PTW_START_ROUNDROBIN)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown start mode");
+    if (d.dangling_policy != PTW_DANGLING_ERROR && d.dangling_policy != PTW_DANGLING_STOP)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown dangling policy");
+    const bool count_app = d.app == PTW_APP_PAGERANK || d.app == PTW_APP_PPR;
+    if (d.app != PTW_APP_DEEPWALK && !(d.tp > 0.0 && d.tp <= 1.0))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "termination probability must be in (0, 1]");
+    if (d.app == PTW_APP_DEEPWALK && d.walk_len == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "walk length must be >= 1");
+    if (d.start_mode == PTW_START_SOURCE && d.source >= V)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "source vertex " + std::to_string(d.source) + " out of range");
+    if (count_app && !out->counts) return fail(ctx, PTW_ERR_INVALID_ARG, "counts output required");
+    if (d.app == PTW_APP_RWR && !out->ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
+    if (d.app == PTW_APP_DEEPWALK && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
+    if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
+    const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
+    const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
+    out->steps = out->visits = out->dangling = 0;
+    out->kernel_ms = 0.0;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+
+    // device-side output buffers
+    uint32_t *ends_dev = nullptr, *paths_dev = nullptr;
+    uint64_t *counts_out_dev = nullptr;
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } t_ends, t_paths, t_counts;
+    const uint64_t W = d.walkers;
+    if (d.app == PTW_APP_RWR) {
+        if (on_dev) {
+            ends_dev = out->ends;
+        } else if (W) {
+            HIP_TRY(ctx, hipMalloc(&t_ends.p, W * 4));
+            ends_dev = (uint32_t *)t_ends.p;
+        }
+    }
+    if (d.app == PTW_APP_DEEPWALK) {
+        if (W > UINT64_MAX / 4 / d.walk_len) return fail(ctx, PTW_ERR_INVALID_ARG, "path buffer size overflows");
+        if (on_dev) {
+            paths_dev = out->paths;
+        } else if (W) {
+            HIP_TRY(ctx, hipMalloc(&t_paths.p, W * d.walk_len * 4));
+            paths_dev = (uint32_t *)t_paths.p;
+        }
+    }
+    if (count_app) {
+        if (on_dev) {
+            counts_out_dev = out->counts;
+        } else {
+            HIP_TRY(ctx, hipMalloc(&t_counts.p, (size_t)V * 8));
+            counts_out_dev = (uint64_t *)t_counts.p;
+            if (accumulate) HIP_TRY(ctx, hipMemcpyAsync(counts_out_dev, out->counts, (size_t)V * 8, hipMemcpyHostToDevice, s));
+        }
+    }
+
+    uint32_t source_dev = 0;
+    if (d.start_mode == PTW_START_SOURCE) {
+        source_dev = d.source;
+        if (ctx->dev.relabelled) {
+            // perm lives on the device; the host copy of the relabel is not kept
+            HIP_TRY(ctx, hipMemcpy(&source_dev, ctx->dev.perm + d.source, 4, hipMemcpyDeviceToHost));
+        }
+    }
+
+    ptw::WalkLaunch p{};
+    p.row_ptr = ctx->dev.row_ptr;
+    p.col = ctx->dev.col;
+    p.perm = ctx->dev.perm;
+    p.inv = ctx->dev.inv;
+    p.V = V;
+    p.hubs = ctx->dev.hubs;
+    p.app = d.app;
+    p.semantics = d.semantics;
+    p.start_mode = d.start_mode;
+    p.source_dev = source_dev;
+    p.walk_len = d.walk_len;
+    {
+        double c = std::floor((1.0 - d.tp) * 4294967296.0);
+        if (c < 0.0) c = 0.0;
+        if (c > 4294967295.0) c = 4294967295.0;
+        p.thr = (uint32_t)c;
+    }
+    p.seed_lo = (uint32_t)d.seed;
+    p.seed_hi = (uint32_t)(d.seed >> 32);
+    p.stream = d.stream;
+    p.wid_base = d.wid_base;
+    p.walkers = W;
+    p.counts_dev = ctx->counts_dev;
+    p.ends = ends_dev;
+    p.paths = paths_dev;
+    p.scratch = ctx->scratch;
+
+    HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
+    if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    if (W) HIP_TRY(ctx, ptw::launch_walk(p, ctx->shape, ctx->num_cus, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    if (count_app) {
+        const bool from_source = d.start_mode == PTW_START_SOURCE;
+        HIP_TRY(ctx, ptw::launch_counts_to_graph_ids(ctx->counts_dev, ctx->dev.perm, V, counts_out_dev, accumulate,
+                                                     from_source ? d.source : PTW_INVALID_VERTEX,
+                                                     from_source ? W : 0, s));
+    }
+    unsigned long long sc[8];
+    HIP_TRY(ctx, hipMemcpyAsync(sc, ctx->scratch, sizeof(sc), hipMemcpyDeviceToHost, s));
+    if (!on_dev) {
+        if (count_app) HIP_TRY(ctx, hipMemcpyAsync(out->counts, counts_out_dev, (size_t)V * 8, hipMemcpyDeviceToHost, s));
+        if (ends_dev && W) HIP_TRY(ctx, hipMemcpyAsync(out->ends, ends_dev, W * 4, hipMemcpyDeviceToHost, s));
+        if (paths_dev && W)
+            HIP_TRY(ctx, hipMemcpyAsync(out->paths, paths_dev, W * d.walk_len * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->kernel_ms = ms;
+    out->steps = sc[1];
+    out->dangling = sc[2];
+    out->visits = count_app ? W + sc[1] : 0;
+    out->total_ms = now_ms() - t0;
+    if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
+        return fail(ctx, PTW_ERR_DANGLING,
+                    std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
+    return PTW_OK;
+}
+
+int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed, double *gathers_per_s,
+                     double *kernel_ms)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (table_bytes < 4 || gathers == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "empty gather benchmark");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } table, sink;
+    const uint64_t words = table_bytes / 4;
+    HIP_TRY(ctx, hipMalloc(&table.p, words * 4));
+    HIP_TRY(ctx, hipMalloc(&sink.p, 8));
+    HIP_TRY(ctx, ptw::launch_fill_u64((unsigned long long *)table.p, words / 2, 0x0123456789ABCDEFull, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_fill_u64((unsigned long long *)sink.p, 1, 0ull, ctx->stream));
+    // warm-up launch, then the timed one
+    HIP_TRY(ctx, ptw::launch_gather_bench((const uint32_t *)table.p, words, gathers / 4 + 1, (uint32_t)seed,
+                                          (uint32_t)(seed >> 32) ^ 1u, (unsigned long long *)sink.p, ctx->num_cus,
+                                          ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_gather_bench((const uint32_t *)table.p, words, gathers, (uint32_t)seed,
+                                          (uint32_t)(seed >> 32), (unsigned long long *)sink.p, ctx->num_cus,
+                                          ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (gathers_per_s) *gathers_per_s = ms > 0 ? (double)gathers / (ms * 1e-3) : 0.0;
+    return PTW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Internal declarations shared by the host graph builder, the C ABI and the
+// kernel launchers.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace ptw {
+
+// CSR in graph ids, host resident.  Adjacency order is significant: the
+// neighbour pick indexes into it.
+struct HostCsr {
+    uint32_t V = 0;
+    std::vector<uint32_t> row_ptr;  // V + 1
+    std::vector<uint32_t> col;      // E
+    uint64_t E() const { return col.size(); }
+};
+
+// Device copy of the graph under the degree-descending relabel.
+struct DeviceGraph {
+    uint32_t V = 0;
+    uint64_t E = 0;
+    uint32_t *row_ptr = nullptr;  // V + 1, device ids
+    uint32_t *col = nullptr;      // E, device ids, per-vertex order kept
+    uint32_t *perm = nullptr;     // graph id -> device id (nullptr: identity)
+    uint32_t *inv = nullptr;      // device id -> graph id (nullptr: identity)
+    uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
+    bool relabelled = false;
+    uint64_t bytes = 0;
+};
+
+// Host-side parallel loop over [0, n) in contiguous chunks.
+void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &body,
+                  unsigned threads = 0);
+unsigned host_threads();
+
+// Edge-list ingest with the reference's load semantics.  Returns 0 or a
+// PTW_ERR_* code with `err` filled.
+int load_edge_list(const char *path, int format, bool as_undirected, HostCsr &g, std::string &err);
+int write_edge_list(const HostCsr &g, const char *path, int format, std::string &err);
+int build_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+               bool keep_isolated, HostCsr &g, std::string &err);
+int validate_csr(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
+                 std::string &err);
+
+// Degree-descending relabel (stable by graph id).  perm[g] = d, inv[d] = g;
+// d_row_ptr / d_col are the relabelled CSR.
+void relabel_by_degree(const HostCsr &g, std::vector<uint32_t> &perm, std::vector<uint32_t> &inv,
+                       std::vector<uint32_t> &d_row_ptr, std::vector<uint32_t> &d_col);
+
+// ---- kernel launchers (walk.hip) -------------------------------------------
+
+struct WalkLaunch {
+    // graph
+    const uint32_t *row_ptr;
+    const uint32_t *col;
+    const uint32_t *perm;  // nullptr: identity
+    const uint32_t *inv;   // nullptr: identity
+    uint32_t V;
+    uint32_t hubs;         // LDS-privatized counters for device ids < hubs
+    // process
+    int app;
+    int semantics;
+    int start_mode;
+    uint32_t source_dev;   // device id of the source
+    uint32_t walk_len;
+    uint32_t thr;          // continue iff r.z < thr
+    uint32_t seed_lo, seed_hi, stream;
+    uint64_t wid_base;
+    uint64_t walkers;
+    // outputs (device)
+    unsigned long long *counts_dev;  // device-id counters (count apps)
+    uint32_t *ends;                  // graph ids
+    uint32_t *paths;                 // graph ids
+    // scratch (device): [0] queue head, [1] steps, [2] dangling
+    unsigned long long *scratch;
+};
+
+struct LaunchShape {
+    int block = 1024;      // threads per workgroup
+    int blocks = 0;        // 0: fill the device from the occupancy query
+    int chunk = 4096;      // walker ids a wave takes from the queue at once
+    int lds_hubs = 8192;   // u64 LDS hub counters per workgroup (count apps)
+};
+
+hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s);
+hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
+                                      uint32_t V, uint64_t *out, bool accumulate,
+                                      uint32_t extra_vertex, uint64_t extra, hipStream_t s);
+hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,
+                               uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink,
+                               int num_cus, hipStream_t s);
+
+}  // namespace ptw

@@ -1,0 +1,289 @@
+// Walk kernels for gfx950 (CDNA4, wave64).
+//
+// walk_kernel is persistent: the grid is sized to the device's resident
+// capacity and every lane owns one walker at a time.  When a walker ends,
+// its lane is refilled from the wave's private slice of walker ids; a wave
+// takes a new slice of `chunk` ids from one global queue counter when its
+// slice runs dry, so the queue sees one atomic per `chunk` walkers.  Lanes
+// needing a walker are ranked with ballot + mbcnt (wave-level prefix sum),
+// so the wave stays fully occupied until the queue drains; a wave exits
+// only when the queue is empty and none of its lanes is active, which
+// every wave reaches because every walk ends.
+//
+// Every random decision comes from one Philox4x32-10 block keyed by the
+// seed and counted by (walker id, step, stream), so results do not depend
+// on lane, slot, wave or launch order.
+//
+// Count apps privatize the `lds_hubs` highest-degree vertices (device ids
+// below `hubs` after the degree-descending relabel) in LDS u64 counters,
+// flushed once per workgroup; other visits are device-scope u64 atomics.
+#include "../../include/ptw.h"
+#include "engine.hpp"
+#include "philox.hpp"
+
+namespace ptw {
+
+namespace {
+
+enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2 };
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{
+    // number of set bits of `mask` below this lane
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_broadcast_first(uint64_t x)
+{
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk, uint32_t lds_hubs)
+{
+    extern __shared__ unsigned long long lds_cnt[];
+    if (KIND == KIND_COUNT) {
+        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) lds_cnt[i] = 0;
+        __syncthreads();
+    }
+    const uint32_t lane = __lane_id();
+    uint64_t q_next = 0, q_end = 0;  // wave-uniform slice of local walker indices
+    bool exhausted = false;
+    bool active = false;
+    uint64_t w = 0;      // local walker index in [0, walkers)
+    uint32_t wlo = 0, whi = 0;  // global walker id words
+    uint32_t v = 0, step = 0;
+    uint64_t my_steps = 0;
+    uint32_t my_dangling = 0;
+
+    auto count_visit = [&](uint32_t x) {
+        if (x < lds_hubs)
+            atomicAdd(&lds_cnt[x], 1ull);
+        else
+            atomicAdd(&p.counts_dev[x], 1ull);
+    };
+
+    for (;;) {
+        // ---- refill idle lanes (wave-uniform control flow) ----
+        uint64_t idle = __ballot(!active);
+        while (idle != 0 && !exhausted) {
+            if (q_next == q_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&p.scratch[0], (unsigned long long)chunk);
+                base = wave_broadcast_first(base);
+                if (base >= p.walkers) {
+                    exhausted = true;
+                    break;
+                }
+                q_next = base;
+                q_end = base + chunk < p.walkers ? base + chunk : p.walkers;
+            }
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            const uint64_t avail = q_end - q_next;
+            const uint32_t take = avail < n_idle ? (uint32_t)avail : n_idle;
+            const uint32_t rank = lane_rank(idle);
+            if (!active && rank < take) {
+                w = q_next + rank;
+                const uint64_t gw = p.wid_base + w;
+                wlo = (uint32_t)gw;
+                whi = (uint32_t)(gw >> 32);
+                uint32_t g;
+                if (p.start_mode == PTW_START_SOURCE) {
+                    v = p.source_dev;
+                } else {
+                    if (p.start_mode == PTW_START_ROUNDROBIN) {
+                        g = (uint32_t)(gw % p.V);
+                    } else {
+                        u32x4 r = philox4x32_10(wlo, whi, kInitStep, p.stream, p.seed_lo, p.seed_hi);
+                        g = mulhi64(r.x, r.y, p.V);
+                    }
+                    v = p.perm ? p.perm[g] : g;
+                    if (KIND == KIND_COUNT) count_visit(v);
+                }
+                if (KIND == KIND_PATH) {
+                    p.paths[w * p.walk_len] = p.inv ? p.inv[v] : v;
+                }
+                step = 0;
+                active = true;
+            }
+            q_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+
+        // ---- one step of every active walker ----
+        if (active) {
+            const u32x4 r = philox4x32_10(wlo, whi, step, p.stream, p.seed_lo, p.seed_hi);
+            bool stop;
+            if (KIND == KIND_PATH) {
+                stop = step + 1 >= p.walk_len;
+            } else {
+                const bool coin = (p.semantics == PTW_SEM_TEXTBOOK) || step >= 1;
+                stop = coin && r.z >= p.thr;
+            }
+            bool dangled = false;
+            if (!stop) {
+                const uint32_t beg = p.row_ptr[v];
+                const uint32_t deg = p.row_ptr[v + 1] - beg;
+                if (deg == 0) {
+                    dangled = true;
+                    stop = true;
+                } else {
+                    v = p.col[beg + mulhi64(r.x, r.y, deg)];
+                    ++step;
+                    if (KIND == KIND_COUNT) count_visit(v);
+                    if (KIND == KIND_PATH) p.paths[w * p.walk_len + step] = p.inv ? p.inv[v] : v;
+                }
+            }
+            if (stop) {
+                my_steps += step;
+                my_dangling += dangled;
+                if (KIND == KIND_END) p.ends[w] = p.inv ? p.inv[v] : v;
+                if (KIND == KIND_PATH && dangled) {
+                    for (uint32_t k = step + 1; k < p.walk_len; ++k)
+                        p.paths[w * p.walk_len + k] = PTW_INVALID_VERTEX;
+                }
+                active = false;
+            }
+        }
+    }
+
+    // ---- per-wave totals, then the LDS hub flush ----
+    for (int off = 32; off > 0; off >>= 1) {
+        my_steps += __shfl_xor(my_steps, off);
+        my_dangling += __shfl_xor(my_dangling, off);
+    }
+    if (lane == 0) {
+        if (my_steps) atomicAdd(&p.scratch[1], (unsigned long long)my_steps);
+        if (my_dangling) atomicAdd(&p.scratch[2], (unsigned long long)my_dangling);
+    }
+    if (KIND == KIND_COUNT) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) {
+            const unsigned long long c = lds_cnt[i];
+            if (c) atomicAdd(&p.counts_dev[i], c);
+        }
+    }
+}
+
+__global__ void counts_to_graph_ids_kernel(const unsigned long long *counts_dev, const uint32_t *perm,
+                                           uint32_t V, uint64_t *out, int accumulate,
+                                           uint32_t extra_vertex, uint64_t extra)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < V; u += stride) {
+        uint64_t c = counts_dev[perm ? perm[u] : u];
+        if (u == extra_vertex) c += extra;
+        out[u] = accumulate ? out[u] + c : c;
+    }
+}
+
+// Independent random u32 gathers: 8 loads in flight per lane per iteration.
+__global__ __launch_bounds__(256) void gather_bench_kernel(const uint32_t *table, uint64_t words,
+                                                          uint64_t gathers, uint32_t k0, uint32_t k1,
+                                                          unsigned long long *sink)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t rounds = (gathers + 8 * nthreads - 1) / (8 * nthreads);
+    uint32_t acc = 0;
+    const uint32_t wl = (uint32_t)(words > 0xFFFFFFFFull ? 0xFFFFFFFFull : words);
+    for (uint64_t it = 0; it < rounds; ++it) {
+        const u32x4 a = philox4x32_10((uint32_t)tid, (uint32_t)it, 0, 0x47415448u, k0, k1);
+        const u32x4 b = philox4x32_10((uint32_t)tid, (uint32_t)it, 1, 0x47415448u, k0, k1);
+        const uint32_t idx[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t val[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t g = (it * nthreads + tid) * 8 + j;
+            const uint32_t i = (uint32_t)(((uint64_t)idx[j] * wl) >> 32);
+            val[j] = g < gathers ? __builtin_nontemporal_load(&table[i]) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc ^= val[j];
+    }
+    if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);  // keeps the loads live
+}
+
+template <int KIND>
+int occupancy_blocks(int block, size_t lds)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_kernel<KIND>, block, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
+}
+
+}  // namespace
+
+hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s)
+{
+    const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
+                     : p.app == PTW_APP_RWR                            ? KIND_END
+                                                                       : KIND_PATH;
+    const int block = shape.block;
+    uint32_t hubs = 0;
+    if (kind == KIND_COUNT) hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
+    const size_t lds = (size_t)hubs * sizeof(unsigned long long);
+    int blocks = shape.blocks;
+    if (blocks <= 0) {
+        int per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
+                     : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
+                                        : occupancy_blocks<KIND_PATH>(block, lds);
+        blocks = per_cu * num_cus;
+    }
+    const uint32_t chunk = (uint32_t)shape.chunk;
+    if (kind == KIND_COUNT)
+        hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs);
+    else if (kind == KIND_END)
+        hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u);
+    else
+        hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
+                                      uint32_t V, uint64_t *out, bool accumulate,
+                                      uint32_t extra_vertex, uint64_t extra, hipStream_t s)
+{
+    uint64_t blocks = ((uint64_t)V + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(counts_to_graph_ids_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, counts_dev,
+                       perm, V, out, accumulate ? 1 : 0, extra_vertex, extra);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,
+                               uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink,
+                               int num_cus, hipStream_t s)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gather_bench_kernel, 256, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    hipLaunchKernelGGL(gather_bench_kernel, dim3(per_cu * num_cus), dim3(256), 0, s, table, table_words,
+                       gathers, seed_lo, seed_hi, sink);
+    return hipGetLastError();
+}
+
+}  // namespace ptw

@@ -1,0 +1,259 @@
+"""GPU parity: the HIP walk kernels against the oracle's Philox replay, bit for bit
+(visit counts, endpoints, paths, step totals), through the C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import ptwalker_amd as ptw
+from walkstats import random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def replay(rp, col, desc: ptw.WalkDesc, threads=0):
+    return oracle.walk_replay(rp, col, desc.app, desc.walkers, tp=desc.tp, seed=desc.seed, source=desc.source,
+                              start_mode=desc.start_mode, semantics=desc.semantics, walk_len=desc.walk_len,
+                              stream=desc.stream, wid_base=desc.wid_base, threads=threads)
+
+
+def check(ctx, rp, col, app, W, **kw):
+    d = ptw.make_desc(app, W, **kw)
+    got = ptw.walk(ctx, W, app, **kw)
+    ref = replay(rp, col, d)
+    assert got.steps == ref["steps"], (got.steps, ref["steps"])
+    if got.counts is not None:
+        assert got.visits == W + got.steps
+        assert np.array_equal(got.counts, ref["counts"]), np.nonzero(got.counts != ref["counts"])
+    if got.ends is not None:
+        assert np.array_equal(got.ends, ref["ends"])
+    if got.paths is not None:
+        assert np.array_equal(got.paths, ref["paths"])
+    assert got.dangling == ref["dangling"]
+    return got
+
+
+@pytest.fixture(scope="module")
+def rmat12(ctx):
+    gi = ctx.rmat(scale=12, edge_factor=16, seed=1)
+    rp, col = ctx.export_csr()
+    return gi, rp, col
+
+
+def apps_for(gi):
+    hub = int(gi.max_degree_vertex)
+    return [
+        ptw.PPR(0.15, hub),
+        ptw.PPR(0.2, gi.num_vertices - 1),
+        ptw.PageRank(0.15),
+        ptw.PageRank(0.3, semantics=ptw.SEM_TEXTBOOK),
+        ptw.RWR(0.2, hub),
+        ptw.RWR(0.5),
+        ptw.DeepWalk(20),
+        ptw.DeepWalk(7, start=ptw.START_ROUNDROBIN),
+    ]
+
+
+def test_parity_rmat12_all_apps(ctx, rmat12):
+    gi, rp, col = rmat12
+    for app in apps_for(gi):
+        print("case", app, flush=True)
+        W = 3000 if isinstance(app, ptw.DeepWalk) else 200000
+        check(ctx, rp, col, app, W, seed=1)
+
+
+def test_parity_seed_stream_and_wid_base(ctx, rmat12):
+    gi, rp, col = rmat12
+    app = ptw.PPR(0.15, int(gi.max_degree_vertex))
+    for seed, stream, base in [(2, 0, 0), (1, 7, 0), (1, 0, (1 << 33) + 5), (0xDEADBEEFCAFE, 3, 123)]:
+        check(ctx, rp, col, app, 50000, seed=seed, stream=stream, wid_base=base)
+
+
+def test_split_runs_add_up(ctx, rmat12):
+    gi, rp, col = rmat12
+    app = ptw.PageRank(0.15)
+    whole = ptw.walk(ctx, 100000, app, seed=4)
+    a = ptw.walk(ctx, 37000, app, seed=4, wid_base=0)
+    b = ptw.walk(ctx, 63000, app, seed=4, wid_base=37000)
+    assert np.array_equal(a.counts + b.counts, whole.counts)
+    assert a.steps + b.steps == whole.steps
+
+
+@pytest.mark.parametrize("relabel", [True, False])
+def test_parity_unsorted_adjacency(ctx, relabel):
+    rp, col = random_graph(60, 0.08, seed=3, symmetric=False)
+    ctx.from_csr(rp, col, flags=0 if relabel else ptw.GRAPH_NO_RELABEL)
+    assert ctx.info().relabelled == (1 if relabel else 0)
+    for app in [ptw.PPR(0.15, 5), ptw.PageRank(0.25), ptw.RWR(0.2, 9), ptw.DeepWalk(15)]:
+        check(ctx, rp, col, app, 20000 if not isinstance(app, ptw.DeepWalk) else 2000, seed=9)
+
+
+def test_edge_cases(ctx):
+    rp, col = random_graph(30, 0.1, seed=5)
+    ctx.from_csr(rp, col)
+    # tp = 1: every walker takes exactly one step
+    g = check(ctx, rp, col, ptw.PPR(1.0, 2), 10000, seed=3)
+    assert g.steps == 10000
+    # a single walker, and zero walkers
+    check(ctx, rp, col, ptw.RWR(0.3, 4), 1, seed=3)
+    z = ptw.walk(ctx, 0, ptw.PPR(0.2, 1), seed=3)
+    assert z.steps == 0 and int(z.counts.sum()) == 0
+    # path of one vertex: no steps
+    g = check(ctx, rp, col, ptw.DeepWalk(1), 100, seed=3)
+    assert g.steps == 0
+    # K2
+    ctx.from_csr(np.array([0, 1, 2], dtype=np.uint32), np.array([1, 0], dtype=np.uint32))
+    check(ctx, np.array([0, 1, 2]), np.array([1, 0]), ptw.PPR(0.15, 0), 100000, seed=8)
+
+
+def test_invalid_arguments(ctx):
+    rp, col = random_graph(10, 0.3, seed=1)
+    ctx.from_csr(rp, col)
+    with pytest.raises(ptw.PtwError) as e:
+        ptw.walk(ctx, 10, ptw.PPR(0.2, 10))
+    assert e.value.code == ptw.ERR_INVALID_ARG
+    for tp in (0.0, -0.1, 1.5, float("nan")):
+        with pytest.raises(ptw.PtwError) as e:
+            ptw.walk(ctx, 10, ptw.PageRank(tp))
+        assert e.value.code == ptw.ERR_INVALID_ARG
+    with pytest.raises(ptw.PtwError) as e:
+        ptw.walk(ctx, 10, ptw.DeepWalk(0))
+    assert e.value.code == ptw.ERR_INVALID_ARG
+
+
+def test_dangling_policies(ctx):
+    rp, col = random_graph(20, 0.2, seed=2, dangling=4)
+    ctx.from_csr(rp, col)
+    assert ctx.info().num_dangling == 4
+    with pytest.raises(ptw.PtwError) as e:
+        ptw.walk(ctx, 5000, ptw.PageRank(0.15))
+    assert e.value.code == ptw.ERR_DANGLING
+    for app in [ptw.PageRank(0.15), ptw.RWR(0.1, 0), ptw.DeepWalk(25)]:
+        g = check(ctx, rp, col, app, 5000, seed=2, dangling=ptw.DANGLING_STOP)
+        assert g.dangling > 0
+
+
+def test_lds_and_global_counters_scale16(ctx):
+    # V > the LDS hub capacity, so both counter paths are exercised
+    gi = ctx.rmat(scale=16, edge_factor=16, seed=1)
+    assert gi.num_vertices > 8192
+    rp, col = ctx.export_csr()
+    check(ctx, rp, col, ptw.PPR(0.15, int(gi.max_degree_vertex)), 2_000_000, seed=1)
+    check(ctx, rp, col, ptw.PageRank(0.15), 1_000_000, seed=1)
+    check(ctx, rp, col, ptw.RWR(0.2, int(gi.max_degree_vertex)), 1_000_000, seed=1)
+
+
+def test_device_outputs_and_accumulate(ctx):
+    import torch
+
+    gi = ctx.rmat(scale=12, edge_factor=8, seed=3)
+    rp, col = ctx.export_csr()
+    V = gi.num_vertices
+    W = 100000
+    counts = torch.zeros(V, dtype=torch.int64, device="cuda")
+    d = ptw.make_desc(ptw.PPR(0.2, 3), W, seed=5, flags=ptw.FLAG_OUT_DEVICE)
+    ctx.walk(d, counts=counts.data_ptr(), device_ptrs=True)
+    d2 = ptw.make_desc(ptw.PPR(0.2, 3), W, seed=5, wid_base=W, flags=ptw.FLAG_OUT_DEVICE | ptw.FLAG_ACCUMULATE)
+    ctx.walk(d2, counts=counts.data_ptr(), device_ptrs=True)
+    torch.cuda.synchronize()
+    ref = oracle.walk_replay(rp, col, oracle.APP_PPR, 2 * W, tp=0.2, seed=5, source=3,
+                             start_mode=oracle.START_SOURCE)
+    assert np.array_equal(counts.cpu().numpy().astype(np.uint64), ref["counts"])
+    ends = torch.zeros(W, dtype=torch.int32, device="cuda")
+    d3 = ptw.make_desc(ptw.RWR(0.2, 3), W, seed=5, flags=ptw.FLAG_OUT_DEVICE)
+    ctx.walk(d3, ends=ends.data_ptr(), device_ptrs=True)
+    torch.cuda.synchronize()
+    ref = oracle.walk_replay(rp, col, oracle.APP_RWR, W, tp=0.2, seed=5, source=3, start_mode=oracle.START_SOURCE)
+    assert np.array_equal(ends.cpu().numpy().view(np.uint32), ref["ends"])
+
+
+def test_full_size_properties_scale20(ctx):
+    """Size-independent properties at a large size: sum of counts, mean length,
+    split invariance, run-to-run determinism."""
+    gi = ctx.rmat(scale=20, edge_factor=16, seed=1)
+    src = int(gi.max_degree_vertex)
+    W, tp = 20_000_000, 0.2
+    a = ptw.walk(ctx, W, ptw.PPR(tp, src), seed=1)
+    assert int(a.counts.sum()) == W + a.steps
+    assert a.counts[src] >= W
+    sd = np.sqrt(W * (1 - tp)) / tp
+    assert abs(a.steps - W / tp) < 6 * sd
+    b = ptw.walk(ctx, W, ptw.PPR(tp, src), seed=1)
+    assert np.array_equal(a.counts, b.counts)
+    h1 = ptw.walk(ctx, W // 2, ptw.PPR(tp, src), seed=1)
+    h2 = ptw.walk(ctx, W // 2, ptw.PPR(tp, src), seed=1, wid_base=W // 2)
+    assert np.array_equal(h1.counts + h2.counts, a.counts)
+
+
+def test_fp64_l1_scale14(ctx):
+    """Monte-Carlo estimate against the FP64 power iteration (L1 within a multiple of its
+    expected size under the multinomial-like noise)."""
+    gi = ctx.rmat(scale=14, edge_factor=16, seed=2)
+    rp, col = ctx.export_csr()
+    src = int(gi.max_degree_vertex)
+    tp, W = 0.2, 5_000_000
+    r = ptw.walk(ctx, W, ptw.PPR(tp, src), seed=1)
+    est = r.counts / W
+    exact = oracle.power_iteration(rp, col, tp=tp, start=src, mode="visits")
+    l1 = np.abs(est - exact).sum() / exact.sum()
+    # per-vertex sd <= sqrt(E[visits^2]/W); sum of sds bounds E[L1] up to a constant
+    bound = np.sqrt((2 / tp ** 2 + 3 / tp + 1) * exact / exact.sum() / W).sum()
+    assert l1 < 3 * bound, (l1, bound)
+
+
+def test_cli_pagerank_end_to_end(ctx, tmp_path):
+    ctx.rmat(scale=10, edge_factor=8, seed=4)
+    g = str(tmp_path / "g.txt")
+    ctx.write(g, ptw.FORMAT_TEXT)
+    rp, col, _ = oracle.load_edge_list(g)
+    out = str(tmp_path / "counts.txt")
+    exe = os.path.join(ptw.BIN_DIR, "pagerank")
+    r = subprocess.run([exe, "-f", "text", "-g", g, "-w", "65536", "--sp", "0.15", "-s", "1", "-t", "20",
+                        "--seed", "1", "--out", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("Partition time ")
+    assert any(l.startswith("start to walk ") for l in lines)
+    assert any(l.startswith("total time (including init start walker) ") for l in lines)
+    got = np.loadtxt(out, dtype=np.uint64)[:, 1]
+    ref = oracle.walk_replay(rp, col, oracle.APP_PAGERANK, 65536, tp=0.15, seed=1)
+    assert np.array_equal(got, ref["counts"])
+    assert lines[-1] == f"ppr value {int(ref['counts'][0]) & 0xFFFFFFFF}"
+    steps = int(lines[-2].rsplit(" ", 1)[1])
+    assert steps == ref["steps"]
+
+
+def test_cli_ppr_rwr_deepwalk(ctx, tmp_path):
+    ctx.rmat(scale=9, edge_factor=8, seed=6)
+    g = str(tmp_path / "g.bin")
+    ctx.write(g, ptw.FORMAT_BINARY)
+    rp, col, _ = oracle.load_edge_list(g, "binary")
+    deg = np.diff(rp.astype(np.int64))
+    hub = int(np.argmax(deg))
+    b = ptw.BIN_DIR
+    r = subprocess.run([f"{b}/PPR", "-f", "binary", "-g", g, "-w", "5000", "--sp", "0.2", "--out",
+                        str(tmp_path / "p.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"Root vertex {hub}" in r.stdout
+    ref = oracle.walk_replay(rp, col, oracle.APP_PPR, 5000, tp=0.2, seed=1, source=hub, start_mode=oracle.START_SOURCE)
+    assert np.array_equal(np.loadtxt(tmp_path / "p.txt", dtype=np.uint64)[:, 1], ref["counts"])
+    r = subprocess.run([f"{b}/RWR", "-f", "binary", "-g", g, "-w", "5000", "-l", "10", "--out",
+                        str(tmp_path / "r.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = oracle.walk_replay(rp, col, oracle.APP_RWR, 5000, tp=0.5, seed=1)
+    assert np.array_equal(np.loadtxt(tmp_path / "r.txt", dtype=np.uint32)[:, 1], ref["ends"])
+    r = subprocess.run([f"{b}/deepwalk", "-f", "binary", "-g", g, "-e", "2", "-l", "8", "--out",
+                        str(tmp_path / "d.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    V = len(rp) - 1
+    ref = oracle.walk_replay(rp, col, oracle.APP_DEEPWALK, 2 * V, walk_len=8, seed=1)
+    got = np.fromfile(tmp_path / "d.bin", dtype=np.uint32).reshape(2 * V, 8)
+    assert np.array_equal(got, ref["paths"])
+    assert r.stdout.splitlines()[-1] == "ppr value 0"
+
+
+def test_gather_bench_runs(ctx):
+    gps, ms = ctx.bench_gather(1 << 28, 1 << 26)
+    assert gps > 1e9 and ms > 0
