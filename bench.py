@@ -49,7 +49,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import ptwalker_amd as ptw
 
@@ -63,20 +62,20 @@ def main():
     rnd = int(rng.integers(0, gi.num_vertices))
     sources = [hub, rnd]
     V = gi.num_vertices
-    counts = torch.zeros(V, dtype=torch.int64, device="cuda")
-    ends = torch.zeros(W, dtype=torch.int32, device="cuda")
+    counts = ctx.alloc(8 * V)
+    ends = ctx.alloc(4 * W)
 
     def one_step(record):
         steps = 0
         for s in sources:
             d = ptw.make_desc(ptw.PPR(args.tp, s), W, seed=1, flags=ptw.FLAG_OUT_DEVICE)
-            o = ctx.walk(d, counts=counts.data_ptr(), device_ptrs=True)
+            o = ctx.walk(d, counts=counts.ptr, device_ptrs=True)
             steps += o.steps
             if record is not None:
                 record["ppr_ms"].append(o.kernel_ms)
                 record["ppr_steps"].append(o.steps)
             d = ptw.make_desc(ptw.RWR(args.tp, s), W, seed=1, flags=ptw.FLAG_OUT_DEVICE)
-            o = ctx.walk(d, ends=ends.data_ptr(), device_ptrs=True)
+            o = ctx.walk(d, ends=ends.ptr, device_ptrs=True)
             steps += o.steps
             if record is not None:
                 record["rwr_ms"].append(o.kernel_ms)
@@ -86,12 +85,11 @@ def main():
     for _ in range(args.warmup):
         one_step(None)
     rec = {"ppr_ms": [], "ppr_steps": [], "rwr_ms": [], "rwr_steps": []}
-    torch.cuda.synchronize()
+    # ptw_walk returns after its stream has drained, so the region is bracketed
     t1 = time.perf_counter()
     total_steps = 0
     for _ in range(args.steps):
         total_steps += one_step(rec)
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
     value = total_steps / elapsed
 
@@ -108,9 +106,11 @@ def main():
         with open(args.traffic) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
-    # measured random-gather roofline at the CSR's col[] footprint
+    # measured random-gather roofline at the CSR's col[] footprint, and the
+    # stream-copy rate (also the known-size kernel for counter calibration)
     col_bytes = 4 * gi.num_edges
     gps, _ = ctx.bench_gather(col_bytes, 1 << 30, seed=1)
+    copy_gbs, copy_ms = ctx.bench_copy(1 << 30)
     ppr_sps = ppr_steps / (ppr_ms * 1e-3)
     gather_frac = 2.0 * ppr_sps / gps
 
@@ -160,10 +160,13 @@ def main():
             "frac": gather_frac,
             "note": "(PPR steps/s x 2 random gathers per step) / measured random u32 gathers/s",
         },
+        "stream_copy": {"GBs": copy_gbs, "bytes": 2 * (1 << 30), "kernel_ms": copy_ms},
         "rwr": {"kernel_ms": rwr_ms, "achieved_GBs": rwr_gbs, "bytes_per_step": 12},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    counts.free()
+    ends.free()
     ctx.close()
 
 

@@ -69,6 +69,7 @@ extern "C" {
 /* ptw_walk_desc.flags */
 #define PTW_FLAG_OUT_DEVICE 1u /* output pointers are device pointers */
 #define PTW_FLAG_ACCUMULATE 2u /* counts += instead of counts = */
+#define PTW_FLAG_WEIGHTED 4u   /* neighbour choice by edge weight (alias tables) */
 
 /* ptw_graph_rmat flags */
 #define PTW_RMAT_KEEP_ISOLATED 1u /* keep degree-0 ids (default: compact them away) */
@@ -91,6 +92,8 @@ typedef struct ptw_graph_info {
     uint32_t hub_vertices;      /* device ids below this count in LDS hub counters */
     uint32_t relabelled;        /* 1 if the device holds a degree relabel */
     uint64_t device_bytes;      /* HBM held by the resident graph */
+    uint32_t weighted;          /* 1 if edge weights and alias tables are resident */
+    uint32_t reserved;
 } ptw_graph_info;
 
 /* Plain-data replacement for the reference's Alg template concept
@@ -152,6 +155,20 @@ int ptw_graph_from_csr(ptw_ctx *ctx, const uint32_t *row_ptr, const uint32_t *co
 int ptw_graph_rmat(ptw_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a, double b,
                    double c, uint64_t seed, uint32_t flags);
 
+/* Edge weights for weighted neighbour choice, one per CSR position in graph
+ * ids (finite, > 0).  The reference samples uniformly only (AdjUnit holds no
+ * weight, src/core/graph.hpp:29-31); weighted walks are an addition.
+ * Weights are quantized per vertex to integers q = floor(w / w_max 2^24 +
+ * 0.5) clamped to [1, 2^24], and each vertex gets an alias table built by
+ * Vose's method in exact integer arithmetic: a step picks slot
+ * j = mulhi64(r.xy, deg) and takes neighbour j if r.w < thr[j], else
+ * alias[j]. */
+int ptw_graph_set_weights(ptw_ctx *ctx, const float *weights, uint64_t num_edges);
+/* U(0, 1] weights on a 2^-24 grid from Philox(seed, edge index). */
+int ptw_graph_random_weights(ptw_ctx *ctx, uint64_t seed);
+/* Copy weights and alias tables (graph-id CSR order); any pointer may be NULL. */
+int ptw_graph_export_weights(ptw_ctx *ctx, float *weights, uint32_t *alias_thr, uint32_t *alias_idx);
+
 int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info);
 
 /* Copy the resident graph's CSR in graph ids to host memory. */
@@ -176,6 +193,36 @@ int ptw_graph_write(ptw_ctx *ctx, const char *path, int format);
  * dangling stop).  Synchronous: returns when outputs are complete. */
 int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
 
+/* ---- launch tuning ---------------------------------------------------- */
+/* Launch shape of the walk kernels.  The reference's counterparts are its
+ * walker container size --ws and hub degree boundary --db
+ * (include/option.hpp:243-300) and the sampler cost catalogue of
+ * src/core/mini_bmk.hpp:29-118; here the knobs are GPU launch parameters,
+ * chosen by the autotuner (ptwalker_amd.tune). */
+typedef struct ptw_tuning {
+    int32_t block;         /* threads per workgroup: multiple of 64, 64..1024 */
+    int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query */
+    int32_t chunk;         /* walker ids a wavefront takes from the queue at once (>= 64) */
+    int32_t lds_hubs;      /* LDS counters for the top-degree vertices, 0..16384 */
+    int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 8192; 0 = off) */
+} ptw_tuning;
+int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
+int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);
+
+/* ---- device buffers --------------------------------------------------- */
+/* Device memory on the context's device for callers without a HIP
+ * allocator of their own (outputs with PTW_FLAG_OUT_DEVICE).  No counterpart
+ * in the reference, whose outputs are host arrays owned by the app
+ * (src/core/pagerank.cpp:23, src/core/RWR.cpp:29). */
+int ptw_buffer_alloc(ptw_ctx *ctx, uint64_t bytes, void **dptr);
+int ptw_buffer_free(ptw_ctx *ctx, void *dptr);
+int ptw_buffer_upload(ptw_ctx *ctx, void *dptr, const void *host, uint64_t bytes);
+int ptw_buffer_download(ptw_ctx *ctx, const void *dptr, void *host, uint64_t bytes);
+/* Zero `bytes` (a multiple of 4) with a fill kernel on the context's stream. */
+int ptw_buffer_zero(ptw_ctx *ctx, void *dptr, uint64_t bytes);
+/* Free and total HBM of the context's device. */
+int ptw_device_memory(ptw_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* ---- measurement ------------------------------------------------------ */
 /* Random-gather microbenchmark: `gathers` dependent-free random u32 loads
  * from a table of `table_bytes`, Philox indices.  Reports gathers/s and the
@@ -183,6 +230,10 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
  * kernel is measured against. */
 int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed,
                      double *gathers_per_s, double *kernel_ms);
+/* Streaming copy of `bytes` (multiple of 16) between two device buffers:
+ * the stream roofline, and the known-size kernel that calibrates HBM byte
+ * counters.  No counterpart in the reference. */
+int ptw_bench_copy(ptw_ctx *ctx, uint64_t bytes, double *gb_per_s, double *kernel_ms);
 
 #ifdef __cplusplus
 }

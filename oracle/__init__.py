@@ -84,6 +84,11 @@ def lib():
         L.orc_walk_replay.argtypes = [_P, _P, ctypes.c_uint32, ctypes.POINTER(WalkDesc), _P, _P, _P,
                                       ctypes.POINTER(WalkStats), ctypes.c_int]
         L.orc_walk_replay.restype = ctypes.c_int
+        L.orc_walk_replay_weighted.argtypes = [_P, _P, ctypes.c_uint32, _P, _P, ctypes.POINTER(WalkDesc), _P, _P,
+                                               _P, ctypes.POINTER(WalkStats), ctypes.c_int]
+        L.orc_walk_replay_weighted.restype = ctypes.c_int
+        L.orc_build_alias.argtypes = [_P, ctypes.c_uint32, _P, _P, _P]
+        L.orc_build_alias.restype = ctypes.c_int
         L.orc_walk_length.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]
         L.orc_walk_length.restype = ctypes.c_uint32
         L.orc_ref_walk.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
@@ -128,8 +133,9 @@ def walk_length(seed: int, stream: int, wid: int, tp: float, semantics: int = SE
 
 def walk_replay(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, seed: int = 1, source: int = 0,
                 start_mode: int = START_UNIFORM, semantics: int = SEM_REFERENCE, walk_len: int = 0,
-                stream: int = 0, wid_base: int = 0, threads: int = 0):
+                stream: int = 0, wid_base: int = 0, threads: int = 0, alias=None):
     """Philox replay of the engine's process (oracle.c orc_walk_replay).
+    alias: optional (thr, idx) tables for weighted neighbour choice.
 
     Returns dict(counts, ends, paths, steps, visits, dangling)."""
     rp, cl = _u32(row_ptr), _u32(col)
@@ -139,10 +145,15 @@ def walk_replay(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, seed:
     ends = np.zeros(max(walkers, 1), dtype=np.uint32) if app == APP_RWR else None
     paths = np.zeros(max(walkers * walk_len, 1), dtype=np.uint32) if app == APP_DEEPWALK else None
     st = WalkStats()
-    rc = lib().orc_walk_replay(rp.ctypes.data, cl.ctypes.data if len(cl) else None, V, ctypes.byref(d),
-                               counts.ctypes.data if counts is not None else None,
-                               ends.ctypes.data if ends is not None else None,
-                               paths.ctypes.data if paths is not None else None, ctypes.byref(st), threads)
+    at = ai = None
+    if alias is not None:
+        at, ai = _u32(alias[0]), _u32(alias[1])
+    rc = lib().orc_walk_replay_weighted(rp.ctypes.data, cl.ctypes.data if len(cl) else None, V,
+                                        at.ctypes.data if at is not None else None,
+                                        ai.ctypes.data if ai is not None else None, ctypes.byref(d),
+                                        counts.ctypes.data if counts is not None else None,
+                                        ends.ctypes.data if ends is not None else None,
+                                        paths.ctypes.data if paths is not None else None, ctypes.byref(st), threads)
     if rc != 0:
         raise ValueError("invalid oracle walk arguments")
     if ends is not None:
@@ -150,6 +161,31 @@ def walk_replay(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, seed:
     if paths is not None:
         paths = paths[: walkers * walk_len].reshape(walkers, walk_len)
     return dict(counts=counts, ends=ends, paths=paths, steps=st.steps, visits=st.visits, dangling=st.dangling)
+
+
+def build_alias(row_ptr, w):
+    """Integer alias tables from float weights (oracle.c orc_build_alias)."""
+    rp = _u32(row_ptr)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    E = int(rp[-1])
+    thr = np.zeros(max(E, 1), dtype=np.uint32)
+    ali = np.zeros(max(E, 1), dtype=np.uint32)
+    if lib().orc_build_alias(rp.ctypes.data, len(rp) - 1, w.ctypes.data, thr.ctypes.data, ali.ctypes.data) != 0:
+        raise ValueError("weights must be finite and > 0")
+    return thr[:E], ali[:E]
+
+
+def alias_distribution(row_ptr, thr, ali, v):
+    """Exact (rational) neighbour distribution implied by vertex v's alias table."""
+    from fractions import Fraction
+    b, e = int(row_ptr[v]), int(row_ptr[v + 1])
+    n = e - b
+    p = [Fraction(0)] * n
+    for j in range(n):
+        own = Fraction(int(thr[b + j]), 2 ** 32)
+        p[j] += own / n
+        p[int(ali[b + j])] += (1 - own) / n
+    return p
 
 
 def ref_walk(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, source: int = 0, start_uniform: bool = True,

@@ -87,6 +87,14 @@ int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
                     const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
                     uint32_t *paths, orc_walk_stats *st, int threads)
 {
+    return orc_walk_replay_weighted(row_ptr, col, V, NULL, NULL, d, counts, ends, paths, st, threads);
+}
+
+int orc_walk_replay_weighted(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                             const uint32_t *alias_thr, const uint32_t *alias_idx,
+                             const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
+                             uint32_t *paths, orc_walk_stats *st, int threads)
+{
     if (V == 0 || d->walkers == 0) {
         if (st) memset(st, 0, sizeof(*st));
         return 0;
@@ -132,7 +140,9 @@ int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
             }
             uint32_t beg = row_ptr[v], deg = row_ptr[v + 1] - beg;
             if (deg == 0) { tot_dangling++; break; }
-            v = col[beg + orc_mulhi64(r[0], r[1], deg)];
+            uint32_t j = orc_mulhi64(r[0], r[1], deg);
+            if (alias_thr && r[3] >= alias_thr[beg + j]) j = alias_idx[beg + j];
+            v = col[beg + j];
             if (count_app && counts) __atomic_fetch_add(&counts[v], 1, __ATOMIC_RELAXED);
             tot_visits += count_app;
             if (path) path[i + 1] = v;
@@ -148,6 +158,52 @@ int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
         st->visits = tot_visits;
         st->dangling = tot_dangling;
     }
+    return 0;
+}
+
+int orc_build_alias(const uint32_t *row_ptr, uint32_t V, const float *w, uint32_t *thr, uint32_t *alias)
+{
+    uint64_t E = row_ptr[V];
+    for (uint64_t e = 0; e < E; ++e)
+        if (!(w[e] > 0.0f) || !isfinite(w[e])) return -1;
+    uint32_t maxdeg = 0;
+    for (uint32_t v = 0; v < V; ++v)
+        if (row_ptr[v + 1] - row_ptr[v] > maxdeg) maxdeg = row_ptr[v + 1] - row_ptr[v];
+    uint64_t *mass = (uint64_t *)malloc(sizeof(uint64_t) * (maxdeg + 1));
+    uint32_t *sm = (uint32_t *)malloc(sizeof(uint32_t) * (maxdeg + 1));
+    uint32_t *lg = (uint32_t *)malloc(sizeof(uint32_t) * (maxdeg + 1));
+    for (uint32_t v = 0; v < V; ++v) {
+        uint32_t b = row_ptr[v], n = row_ptr[v + 1] - b;
+        if (n == 0) continue;
+        double wmax = 0.0;
+        for (uint32_t k = 0; k < n; ++k)
+            if ((double)w[b + k] > wmax) wmax = (double)w[b + k];
+        uint64_t Q = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            double qd = floor((double)w[b + k] / wmax * 16777216.0 + 0.5);
+            uint64_t q = qd < 1.0 ? 1 : (qd > 16777216.0 ? 16777216 : (uint64_t)qd);
+            mass[k] = q;
+            Q += q;
+        }
+        uint32_t ns = 0, nl = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            mass[k] *= n;
+            if (mass[k] < Q) sm[ns++] = k; else lg[nl++] = k;
+        }
+        while (ns > 0 && nl > 0) {
+            uint32_t s = sm[--ns];
+            uint32_t l = lg[--nl];
+            thr[b + s] = (uint32_t)(((unsigned __int128)mass[s] << 32) / Q);
+            alias[b + s] = l;
+            mass[l] -= Q - mass[s];
+            if (mass[l] < Q) sm[ns++] = l; else lg[nl++] = l;
+        }
+        for (uint32_t i = 0; i < nl; ++i) { thr[b + lg[i]] = 0xFFFFFFFFu; alias[b + lg[i]] = lg[i]; }
+        for (uint32_t i = 0; i < ns; ++i) { thr[b + sm[i]] = 0xFFFFFFFFu; alias[b + sm[i]] = sm[i]; }
+    }
+    free(mass);
+    free(sm);
+    free(lg);
     return 0;
 }
 

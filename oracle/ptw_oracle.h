@@ -93,6 +93,24 @@ int orc_walk_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
                     const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
                     uint32_t *paths, orc_walk_stats *st, int threads);
 
+/* Same, with weighted neighbour choice through per-vertex alias tables
+ * (thr, alias in CSR order): slot j = mulhi64(r.x | r.y << 32, deg), then
+ * neighbour j if r.w < thr[j] else alias[j]. */
+int orc_walk_replay_weighted(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
+                             const uint32_t *alias_thr, const uint32_t *alias_idx,
+                             const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
+                             uint32_t *paths, orc_walk_stats *st, int threads);
+
+/* Integer alias tables (Vose's method, exact integer masses) from float
+ * weights in CSR order.  Per vertex: q_k = clamp(floor(w_k / w_max 2^24 +
+ * 0.5), 1, 2^24), Q = sum q, mass m_k = q_k deg; slots with m < Q go on a
+ * "small" stack, others on a "large" stack, both filled in index order and
+ * popped from the top; pairing small s with large l sets
+ * thr[s] = floor(m_s 2^32 / Q), alias[s] = l and m_l -= Q - m_s; slots
+ * left over get thr = 2^32 - 1, alias = self.  Returns -1 on a weight that
+ * is not finite and > 0. */
+int orc_build_alias(const uint32_t *row_ptr, uint32_t V, const float *w, uint32_t *thr, uint32_t *alias);
+
 /* Walk length of one walker under the Philox process (for closed-form checks). */
 uint32_t orc_walk_length(uint64_t seed, uint32_t stream, uint64_t wid, double tp, int semantics);
 

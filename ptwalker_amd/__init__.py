@@ -62,6 +62,7 @@ DANGLING_STOP = 1
 
 FLAG_OUT_DEVICE = 1
 FLAG_ACCUMULATE = 2
+FLAG_WEIGHTED = 4
 
 RMAT_KEEP_ISOLATED = 1
 GRAPH_NO_RELABEL = 2
@@ -85,6 +86,8 @@ class GraphInfo(ctypes.Structure):
         ("hub_vertices", ctypes.c_uint32),
         ("relabelled", ctypes.c_uint32),
         ("device_bytes", ctypes.c_uint64),
+        ("weighted", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -118,6 +121,19 @@ class WalkOut(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("lds_hubs", ctypes.c_int32),
+        ("cache_slots", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every symbol include/ptw.h declares, with its ctypes signature
 _P = ctypes.c_void_p
 _SIGNATURES = {
@@ -129,12 +145,25 @@ _SIGNATURES = {
     "ptw_graph_from_csr": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32]),
     "ptw_graph_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32]),
+    "ptw_graph_set_weights": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "ptw_graph_random_weights": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "ptw_graph_export_weights": (ctypes.c_int, [_P, _P, _P, _P]),
     "ptw_graph_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
     "ptw_graph_export_csr": (ctypes.c_int, [_P, _P, _P]),
     "ptw_graph_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "ptw_walk": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
+    "ptw_tuning_get": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "ptw_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "ptw_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "ptw_buffer_free": (ctypes.c_int, [_P, _P]),
+    "ptw_buffer_upload": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
+    "ptw_buffer_download": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
+    "ptw_buffer_zero": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "ptw_device_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_bench_gather": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "ptw_bench_copy": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -227,6 +256,21 @@ class Context:
         self.check(self.lib.ptw_graph_export_csr(self.h, _ptr(row_ptr), _ptr(col)))
         return row_ptr, col[: gi.num_edges]
 
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self.check(self.lib.ptw_graph_set_weights(self.h, _ptr(w), len(w)))
+
+    def random_weights(self, seed: int = 1):
+        self.check(self.lib.ptw_graph_random_weights(self.h, seed))
+
+    def export_weights(self):
+        E = self.info().num_edges
+        w = np.empty(max(E, 1), dtype=np.float32)
+        thr = np.empty(max(E, 1), dtype=np.uint32)
+        ali = np.empty(max(E, 1), dtype=np.uint32)
+        self.check(self.lib.ptw_graph_export_weights(self.h, _ptr(w), _ptr(thr), _ptr(ali)))
+        return w[:E], thr[:E], ali[:E]
+
     def write(self, path: str, fmt: int = FORMAT_TEXT):
         self.check(self.lib.ptw_graph_write(self.h, path.encode(), fmt))
 
@@ -240,11 +284,79 @@ class Context:
         self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
         return out
 
+    # ---- launch tuning ---------------------------------------------------
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self.check(self.lib.ptw_tuning_get(self.h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> Tuning:
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self.check(self.lib.ptw_tuning_set(self.h, ctypes.byref(t)))
+        return t
+
+    # ---- device buffers --------------------------------------------------
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def device_memory(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(self.lib.ptw_device_memory(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def bench_copy(self, nbytes: int):
+        gbs = ctypes.c_double()
+        ms = ctypes.c_double()
+        self.check(self.lib.ptw_bench_copy(self.h, nbytes, ctypes.byref(gbs), ctypes.byref(ms)))
+        return gbs.value, ms.value
+
     def bench_gather(self, table_bytes: int, gathers: int, seed: int = 1):
         gps = ctypes.c_double()
         ms = ctypes.c_double()
         self.check(self.lib.ptw_bench_gather(self.h, table_bytes, gathers, seed, ctypes.byref(gps), ctypes.byref(ms)))
         return gps.value, ms.value
+
+
+class DeviceBuffer:
+    """Device memory on a context's device (ptw_buffer_*)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _P()
+        ctx.check(ctx.lib.ptw_buffer_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def zero(self):
+        self.ctx.check(self.ctx.lib.ptw_buffer_zero(self.ctx.h, self.ptr, self.nbytes))
+        return self
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        assert host.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.ptw_buffer_upload(self.ctx.h, self.ptr, host.ctypes.data, host.nbytes))
+        return self
+
+    def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        self.ctx.check(self.ctx.lib.ptw_buffer_download(self.ctx.h, self.ptr, out.ctypes.data, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            self.ctx.lib.ptw_buffer_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ---- the reference's apps as plain data --------------------------------------

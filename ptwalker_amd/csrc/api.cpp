@@ -24,6 +24,12 @@ struct ptw_ctx {
     ptw::DeviceGraph dev;
     uint32_t max_degree = 0, max_degree_vertex = 0, num_dangling = 0;
 
+    // host copies of the relabel (empty without one)
+    std::vector<uint32_t> h_perm, h_inv, h_drow;
+    // weighted sampling: weights and alias tables in graph-id CSR order
+    std::vector<float> weights;
+    std::vector<uint32_t> alias_thr, alias_idx;
+
     unsigned long long *counts_dev = nullptr;  // V device-id counters
     unsigned long long *scratch = nullptr;     // 8 words
     ptw::LaunchShape shape;
@@ -47,6 +53,14 @@ int fail(ptw_ctx *ctx, int code, const std::string &msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));                       \
     } while (0)
 
+template <typename T>
+int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
+{
+    HIP_TRY(ctx, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
+    if (n) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return PTW_OK;
+}
+
 void free_graph(ptw_ctx *ctx)
 {
     auto &d = ctx->dev;
@@ -54,18 +68,46 @@ void free_graph(ptw_ctx *ctx)
     if (d.col) (void)hipFree(d.col);
     if (d.perm) (void)hipFree(d.perm);
     if (d.inv) (void)hipFree(d.inv);
+    if (d.alias) (void)hipFree(d.alias);
     if (ctx->counts_dev) (void)hipFree(ctx->counts_dev);
     d = ptw::DeviceGraph{};
     ctx->counts_dev = nullptr;
     ctx->has_graph = false;
     ctx->host = HostCsr{};
+    ctx->h_perm.clear();
+    ctx->h_inv.clear();
+    ctx->h_drow.clear();
+    ctx->weights.clear();
+    ctx->alias_thr.clear();
+    ctx->alias_idx.clear();
 }
 
-template <typename T>
-int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
+// Builds alias tables from ctx->weights and uploads them in device order.
+int install_weights(ptw_ctx *ctx)
 {
-    HIP_TRY(ctx, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
-    if (n) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    int rc = ptw::build_alias_tables(ctx->host, ctx->weights, ctx->alias_thr, ctx->alias_idx, ctx->err);
+    if (rc) return rc;
+    if (ctx->device < 0) return PTW_OK;
+    auto &d = ctx->dev;
+    const HostCsr &h = ctx->host;
+    std::vector<uint2> tab(h.E());
+    if (d.relabelled) {
+        ptw::parallel_for(h.V, [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t dv = lo; dv < hi; ++dv) {
+                const uint32_t u = ctx->h_inv[dv];
+                const uint32_t src = h.row_ptr[u], n = h.row_ptr[u + 1] - src, dst = ctx->h_drow[dv];
+                for (uint32_t k = 0; k < n; ++k) tab[dst + k] = uint2{ctx->alias_thr[src + k], ctx->alias_idx[src + k]};
+            }
+        });
+    } else {
+        ptw::parallel_for(h.E(), [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t e = lo; e < hi; ++e) tab[e] = uint2{ctx->alias_thr[e], ctx->alias_idx[e]};
+        });
+    }
+    if (d.alias) (void)hipFree(d.alias);
+    d.alias = nullptr;
+    if ((rc = upload(ctx, &d.alias, tab.data(), tab.size()))) return rc;
+    d.bytes += 8ull * h.E();
     return PTW_OK;
 }
 
@@ -104,12 +146,12 @@ int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
         d.relabelled = false;
         d.hubs = 0;
     } else {
-        std::vector<uint32_t> perm, inv, rp, col;
-        ptw::relabel_by_degree(h, perm, inv, rp, col);
-        if ((rc = upload(ctx, &d.row_ptr, rp.data(), rp.size()))) return rc;
+        std::vector<uint32_t> col;
+        ptw::relabel_by_degree(h, ctx->h_perm, ctx->h_inv, ctx->h_drow, col);
+        if ((rc = upload(ctx, &d.row_ptr, ctx->h_drow.data(), ctx->h_drow.size()))) return rc;
         if ((rc = upload(ctx, &d.col, col.data(), col.size()))) return rc;
-        if ((rc = upload(ctx, &d.perm, perm.data(), perm.size()))) return rc;
-        if ((rc = upload(ctx, &d.inv, inv.data(), inv.size()))) return rc;
+        if ((rc = upload(ctx, &d.perm, ctx->h_perm.data(), ctx->h_perm.size()))) return rc;
+        if ((rc = upload(ctx, &d.inv, ctx->h_inv.data(), ctx->h_inv.size()))) return rc;
         d.relabelled = true;
         d.hubs = V;
     }
@@ -241,6 +283,47 @@ int ptw_graph_rmat(ptw_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a,
     }
 }
 
+int ptw_graph_set_weights(ptw_ctx *ctx, const float *weights, uint64_t num_edges)
+{
+    if (!ctx || (num_edges && !weights)) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (num_edges != ctx->host.E()) return fail(ctx, PTW_ERR_INVALID_ARG, "weight count differs from the edge count");
+    try {
+        if (ctx->device >= 0) HIP_TRY(ctx, hipSetDevice(ctx->device));
+        ctx->weights.assign(weights, weights + num_edges);
+        int rc = install_weights(ctx);
+        if (rc) ctx->weights.clear();
+        return rc;
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed while building alias tables");
+    }
+}
+
+int ptw_graph_random_weights(ptw_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    try {
+        if (ctx->device >= 0) HIP_TRY(ctx, hipSetDevice(ctx->device));
+        ptw::random_weights(ctx->host.E(), seed, ctx->weights);
+        return install_weights(ctx);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed while building alias tables");
+    }
+}
+
+int ptw_graph_export_weights(ptw_ctx *ctx, float *weights, uint32_t *alias_thr, uint32_t *alias_idx)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->weights.empty() && ctx->host.E()) return fail(ctx, PTW_ERR_NO_GRAPH, "graph has no weights");
+    const size_t E = ctx->host.E();
+    if (weights) memcpy(weights, ctx->weights.data(), E * 4);
+    if (alias_thr) memcpy(alias_thr, ctx->alias_thr.data(), E * 4);
+    if (alias_idx) memcpy(alias_idx, ctx->alias_idx.data(), E * 4);
+    return PTW_OK;
+}
+
 int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info)
 {
     if (!ctx || !info) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
@@ -253,6 +336,7 @@ int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info)
     info->hub_vertices = ctx->dev.hubs < (uint32_t)ctx->shape.lds_hubs ? ctx->dev.hubs : (uint32_t)ctx->shape.lds_hubs;
     info->relabelled = ctx->dev.relabelled ? 1 : 0;
     info->device_bytes = ctx->dev.bytes;
+    info->weighted = ctx->weights.empty() ? 0 : 1;
     return PTW_OK;
 }
 
@@ -301,6 +385,8 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     if (d.app == PTW_APP_RWR && !out->ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
     if (d.app == PTW_APP_DEEPWALK && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
     if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
+    const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
+    if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
     const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
     const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
     out->steps = out->visits = out->dangling = 0;
@@ -345,11 +431,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
 
     uint32_t source_dev = 0;
     if (d.start_mode == PTW_START_SOURCE) {
-        source_dev = d.source;
-        if (ctx->dev.relabelled) {
-            // perm lives on the device; the host copy of the relabel is not kept
-            HIP_TRY(ctx, hipMemcpy(&source_dev, ctx->dev.perm + d.source, 4, hipMemcpyDeviceToHost));
-        }
+        source_dev = ctx->dev.relabelled ? ctx->h_perm[d.source] : d.source;
     }
 
     ptw::WalkLaunch p{};
@@ -357,6 +439,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     p.col = ctx->dev.col;
     p.perm = ctx->dev.perm;
     p.inv = ctx->dev.inv;
+    p.alias = weighted ? ctx->dev.alias : nullptr;
     p.V = V;
     p.hubs = ctx->dev.hubs;
     p.app = d.app;
@@ -410,6 +493,126 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
         return fail(ctx, PTW_ERR_DANGLING,
                     std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
+    return PTW_OK;
+}
+
+int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)
+{
+    if (!ctx || !t) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    t->block = ctx->shape.block;
+    t->blocks_per_cu = ctx->shape.blocks_per_cu;
+    t->chunk = ctx->shape.chunk;
+    t->lds_hubs = ctx->shape.lds_hubs;
+    t->cache_slots = ctx->shape.cache_slots;
+    return PTW_OK;
+}
+
+int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
+{
+    if (!ctx || !t) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (t->block < 64 || t->block > 1024 || t->block % 64)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "block must be a multiple of 64 in [64, 1024]");
+    if (t->blocks_per_cu < 0 || t->blocks_per_cu > 32)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "blocks_per_cu must be in [0, 32]");
+    if (t->chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "chunk must be >= 64");
+    if (t->lds_hubs < 0 || t->lds_hubs > 16384) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 16384]");
+    if (t->cache_slots < 0 || t->cache_slots > 8192 || (t->cache_slots & (t->cache_slots - 1)))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "cache_slots must be 0 or a power of two <= 8192");
+    ctx->shape.block = t->block;
+    ctx->shape.blocks_per_cu = t->blocks_per_cu;
+    ctx->shape.chunk = t->chunk;
+    ctx->shape.lds_hubs = t->lds_hubs;
+    ctx->shape.cache_slots = t->cache_slots;
+    return PTW_OK;
+}
+
+int ptw_buffer_alloc(ptw_ctx *ctx, uint64_t bytes, void **dptr)
+{
+    if (!ctx || !dptr) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    *dptr = nullptr;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc(dptr, bytes ? bytes : 4));
+    return PTW_OK;
+}
+
+int ptw_buffer_free(ptw_ctx *ctx, void *dptr)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!dptr) return PTW_OK;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipFree(dptr));
+    return PTW_OK;
+}
+
+int ptw_buffer_upload(ptw_ctx *ctx, void *dptr, const void *host, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!dptr || !host))) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (!bytes) return PTW_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PTW_OK;
+}
+
+int ptw_buffer_download(ptw_ctx *ctx, const void *dptr, void *host, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!dptr || !host))) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (!bytes) return PTW_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PTW_OK;
+}
+
+int ptw_buffer_zero(ptw_ctx *ctx, void *dptr, uint64_t bytes)
+{
+    if (!ctx || (bytes && !dptr) || (bytes & 3)) return fail(ctx, PTW_ERR_INVALID_ARG, "bad buffer or size");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, ptw::launch_fill_u32((uint32_t *)dptr, bytes / 4, 0u, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PTW_OK;
+}
+
+int ptw_device_memory(ptw_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return PTW_OK;
+}
+
+int ptw_bench_copy(ptw_ctx *ctx, uint64_t bytes, double *gb_per_s, double *kernel_ms)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (bytes == 0 || bytes % 16) return fail(ctx, PTW_ERR_INVALID_ARG, "copy size must be a positive multiple of 16");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } a, b;
+    HIP_TRY(ctx, hipMalloc(&a.p, bytes));
+    HIP_TRY(ctx, hipMalloc(&b.p, bytes));
+    HIP_TRY(ctx, ptw::launch_fill_u32((uint32_t *)a.p, bytes / 4, 0x5A5A5A5Au, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_copy_bench((const uint4 *)a.p, (uint4 *)b.p, bytes / 16, ctx->num_cus, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_copy_bench((const uint4 *)a.p, (uint4 *)b.p, bytes / 16, ctx->num_cus, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (gb_per_s) *gb_per_s = ms > 0 ? 2.0 * (double)bytes / (ms * 1e-3) / 1e9 : 0.0;
     return PTW_OK;
 }
 

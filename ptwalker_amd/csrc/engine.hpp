@@ -28,6 +28,7 @@ struct DeviceGraph {
     uint32_t *col = nullptr;      // E, device ids, per-vertex order kept
     uint32_t *perm = nullptr;     // graph id -> device id (nullptr: identity)
     uint32_t *inv = nullptr;      // device id -> graph id (nullptr: identity)
+    uint2 *alias = nullptr;       // E, {threshold, local alias index}, device order
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
     bool relabelled = false;
     uint64_t bytes = 0;
@@ -47,6 +48,18 @@ int build_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double 
 int validate_csr(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
                  std::string &err);
 
+// Edge weights -> per-vertex integer alias tables (weighted neighbour choice).
+// Weights are quantized per vertex to q = clamp(floor(w / w_max * 2^24 + 0.5),
+// 1, 2^24); the table is built by Vose's method in exact integer arithmetic
+// with LIFO small/large stacks filled in index order:
+//   slot s (mass m_s = q_s * deg < Q = sum q): thr = floor(m_s 2^32 / Q), alias = l
+//   slot left full:                             thr = 2^32 - 1,          alias = self
+// A pick j then takes neighbour j if coin < thr[j], else alias[j].
+int build_alias_tables(const HostCsr &g, const std::vector<float> &w, std::vector<uint32_t> &thr,
+                       std::vector<uint32_t> &alias, std::string &err);
+// U(0, 1] weights on 2^-24 steps from Philox(seed, edge index).
+void random_weights(uint64_t E, uint64_t seed, std::vector<float> &w);
+
 // Degree-descending relabel (stable by graph id).  perm[g] = d, inv[d] = g;
 // d_row_ptr / d_col are the relabelled CSR.
 void relabel_by_degree(const HostCsr &g, std::vector<uint32_t> &perm, std::vector<uint32_t> &inv,
@@ -60,6 +73,7 @@ struct WalkLaunch {
     const uint32_t *col;
     const uint32_t *perm;  // nullptr: identity
     const uint32_t *inv;   // nullptr: identity
+    const uint2 *alias;    // nullptr: uniform neighbour choice
     uint32_t V;
     uint32_t hubs;         // LDS-privatized counters for device ids < hubs
     // process
@@ -81,17 +95,20 @@ struct WalkLaunch {
 };
 
 struct LaunchShape {
-    int block = 1024;      // threads per workgroup
-    int blocks = 0;        // 0: fill the device from the occupancy query
-    int chunk = 4096;      // walker ids a wave takes from the queue at once
-    int lds_hubs = 8192;   // u64 LDS hub counters per workgroup (count apps)
+    int block = 1024;        // threads per workgroup
+    int blocks_per_cu = 0;   // 0: fill the device from the occupancy query
+    int chunk = 4096;        // walker ids a wave takes from the queue at once
+    int lds_hubs = 8192;     // u32 LDS counters for device ids below this (count apps)
+    int cache_slots = 4096;  // LDS visit-cache slots per workgroup (power of two, 0 = off)
 };
 
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s);
 hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
                                       uint32_t V, uint64_t *out, bool accumulate,
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);
+hipError_t launch_copy_bench(const uint4 *src, uint4 *dst, uint64_t n16, int num_cus, hipStream_t s);
 hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,
                                uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink,
                                int num_cus, hipStream_t s);

@@ -458,6 +458,83 @@ int build_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double 
     return PTW_OK;
 }
 
+void random_weights(uint64_t E, uint64_t seed, std::vector<float> &w)
+{
+    w.resize(E);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    parallel_for(E, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t e = lo; e < hi; ++e) {
+            u32x4 r = philox4x32_10((uint32_t)e, (uint32_t)(e >> 32), 0, 0x57474854u, k0, k1);
+            w[e] = (float)((r.x >> 8) + 1) * (1.0f / 16777216.0f);  // exact in f32
+        }
+    });
+}
+
+int build_alias_tables(const HostCsr &g, const std::vector<float> &w, std::vector<uint32_t> &thr,
+                       std::vector<uint32_t> &alias, std::string &err)
+{
+    const uint64_t E = g.E();
+    if (w.size() != E) {
+        err = "weight count differs from the edge count";
+        return PTW_ERR_INVALID_ARG;
+    }
+    std::atomic<int> bad{0};
+    parallel_for(E, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t e = lo; e < hi; ++e)
+            if (!(w[e] > 0.0f) || !std::isfinite(w[e])) bad = 1;
+    });
+    if (bad) {
+        err = "edge weights must be finite and > 0";
+        return PTW_ERR_INVALID_ARG;
+    }
+    thr.assign(E, 0);
+    alias.assign(E, 0);
+    parallel_for(g.V, [&](uint64_t lo, uint64_t hi) {
+        std::vector<uint64_t> m;
+        std::vector<uint32_t> small, large;
+        for (uint64_t v = lo; v < hi; ++v) {
+            const uint32_t beg = g.row_ptr[v], deg = g.row_ptr[v + 1] - beg;
+            if (deg == 0) continue;
+            double wmax = 0.0;
+            for (uint32_t k = 0; k < deg; ++k) wmax = std::max(wmax, (double)w[beg + k]);
+            m.resize(deg);
+            uint64_t Q = 0;
+            for (uint32_t k = 0; k < deg; ++k) {
+                double qd = std::floor((double)w[beg + k] / wmax * 16777216.0 + 0.5);
+                uint64_t q = qd < 1.0 ? 1 : (qd > 16777216.0 ? 16777216 : (uint64_t)qd);
+                m[k] = q;
+                Q += q;
+            }
+            small.clear();
+            large.clear();
+            for (uint32_t k = 0; k < deg; ++k) {
+                m[k] *= deg;
+                (m[k] < Q ? small : large).push_back(k);
+            }
+            uint32_t *T = thr.data() + beg, *A = alias.data() + beg;
+            while (!small.empty() && !large.empty()) {
+                const uint32_t s = small.back();
+                small.pop_back();
+                const uint32_t l = large.back();
+                large.pop_back();
+                T[s] = (uint32_t)(((unsigned __int128)m[s] << 32) / Q);
+                A[s] = l;
+                m[l] -= Q - m[s];
+                (m[l] < Q ? small : large).push_back(l);
+            }
+            for (uint32_t k : large) {
+                T[k] = 0xFFFFFFFFu;
+                A[k] = k;
+            }
+            for (uint32_t k : small) {  // unreachable with exact sums; kept total
+                T[k] = 0xFFFFFFFFu;
+                A[k] = k;
+            }
+        }
+    });
+    return PTW_OK;
+}
+
 void relabel_by_degree(const HostCsr &g, std::vector<uint32_t> &perm, std::vector<uint32_t> &inv,
                        std::vector<uint32_t> &d_row_ptr, std::vector<uint32_t> &d_col)
 {

@@ -47,12 +47,76 @@ __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk, uint32_t lds_hubs)
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v)
 {
-    extern __shared__ unsigned long long lds_cnt[];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Workgroup-private visit counters in LDS.
+//   direct: u32 counters for device ids [0, hubs) (the top-degree vertices)
+//   cache:  open-addressing table of `slots` (power of two) {u32 key, u32 count}
+//           that admits any vertex first come, first served -- the vertices a
+//           workgroup visits first and most (a source's neighbourhood, the
+//           hubs of a walk) end up resident; a vertex that finds no slot in
+//           kProbes probes goes to a device-scope u64 atomic.
+// Counts are u32; the increment that takes a counter to 2^31 moves 2^31 to
+// the device counter, so no LDS counter can wrap.
+struct LdsCounters {
+    uint32_t *direct;
+    uint32_t *keys;
+    uint32_t *vals;
+    uint32_t hubs;
+    uint32_t mask;  // slots - 1, or 0xFFFFFFFF when the cache is off
+};
+
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr int kProbes = 8;
+
+__device__ __forceinline__ void lds_add1(uint32_t *c, unsigned long long *dev_counter)
+{
+    const uint32_t old = atomicAdd(c, 1u);
+    if (old == 0x7FFFFFFFu) {  // exactly one increment crosses 2^31
+        atomicSub(c, 0x80000000u);
+        atomicAdd(dev_counter, 0x80000000ull);
+    }
+}
+
+__device__ __forceinline__ void count_in_lds(const LdsCounters &L, uint32_t x, unsigned long long *counts)
+{
+    if (x < L.hubs) {
+        lds_add1(&L.direct[x], &counts[x]);
+        return;
+    }
+    if (L.mask != 0xFFFFFFFFu) {
+        uint32_t h = (x * 0x9E3779B1u) & L.mask;
+#pragma unroll 1
+        for (int probe = 0; probe < kProbes; ++probe) {
+            uint32_t k = L.keys[h];
+            if (k == kEmptyKey) k = atomicCAS(&L.keys[h], kEmptyKey, x);
+            if (k == kEmptyKey || k == x) {
+                lds_add1(&L.vals[h], &counts[x]);
+                return;
+            }
+            h = (h + 1) & L.mask;
+        }
+    }
+    atomicAdd(&counts[x], 1ull);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk, uint32_t lds_hubs,
+                                                   uint32_t cache_slots)
+{
+    extern __shared__ uint32_t lds_words[];
+    LdsCounters L{lds_words, lds_words + lds_hubs, lds_words + lds_hubs + cache_slots, lds_hubs,
+                  cache_slots ? cache_slots - 1 : 0xFFFFFFFFu};
     if (KIND == KIND_COUNT) {
-        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) lds_cnt[i] = 0;
+        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) L.direct[i] = 0;
+        for (uint32_t i = threadIdx.x; i < cache_slots; i += blockDim.x) {
+            L.keys[i] = kEmptyKey;
+            L.vals[i] = 0;
+        }
         __syncthreads();
     }
     const uint32_t lane = __lane_id();
@@ -64,12 +128,16 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     uint32_t v = 0, step = 0;
     uint64_t my_steps = 0;
     uint32_t my_dangling = 0;
+    // Single-source walks return to the source often: its visits are counted
+    // in a register and reduced once per wave instead of hammering one address.
+    const uint32_t hot = p.start_mode == PTW_START_SOURCE ? p.source_dev : PTW_INVALID_VERTEX;
+    uint64_t my_hot = 0;
 
     auto count_visit = [&](uint32_t x) {
-        if (x < lds_hubs)
-            atomicAdd(&lds_cnt[x], 1ull);
+        if (x == hot)
+            ++my_hot;
         else
-            atomicAdd(&p.counts_dev[x], 1ull);
+            count_in_lds(L, x, p.counts_dev);
     };
 
     for (;;) {
@@ -138,7 +206,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     dangled = true;
                     stop = true;
                 } else {
-                    v = p.col[beg + mulhi64(r.x, r.y, deg)];
+                    uint32_t j = mulhi64(r.x, r.y, deg);
+                    if (p.alias) {  // weighted: alias coin from the 4th word
+                        const uint2 a = p.alias[beg + j];
+                        if (r.w >= a.x) j = a.y;
+                    }
+                    v = p.col[beg + j];
                     ++step;
                     if (KIND == KIND_COUNT) count_visit(v);
                     if (KIND == KIND_PATH) p.paths[w * p.walk_len + step] = p.inv ? p.inv[v] : v;
@@ -158,19 +231,26 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     }
 
     // ---- per-wave totals, then the LDS hub flush ----
+    uint64_t hot64 = my_hot;
     for (int off = 32; off > 0; off >>= 1) {
         my_steps += __shfl_xor(my_steps, off);
         my_dangling += __shfl_xor(my_dangling, off);
+        hot64 += __shfl_xor(hot64, off);
     }
     if (lane == 0) {
         if (my_steps) atomicAdd(&p.scratch[1], (unsigned long long)my_steps);
         if (my_dangling) atomicAdd(&p.scratch[2], (unsigned long long)my_dangling);
+        if (KIND == KIND_COUNT && hot64) atomicAdd(&p.counts_dev[hot], (unsigned long long)hot64);
     }
     if (KIND == KIND_COUNT) {
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) {
-            const unsigned long long c = lds_cnt[i];
-            if (c) atomicAdd(&p.counts_dev[i], c);
+            const uint32_t c = L.direct[i];
+            if (c) atomicAdd(&p.counts_dev[i], (unsigned long long)c);
+        }
+        for (uint32_t i = threadIdx.x; i < cache_slots; i += blockDim.x) {
+            const uint32_t k = L.keys[i], c = L.vals[i];
+            if (k != kEmptyKey && c) atomicAdd(&p.counts_dev[k], (unsigned long long)c);
         }
     }
 }
@@ -214,6 +294,14 @@ __global__ __launch_bounds__(256) void gather_bench_kernel(const uint32_t *table
     if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);  // keeps the loads live
 }
 
+// Plain streaming copy, 16 B per lane per iteration: the HBM stream roofline
+// and the byte-count calibration for the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void copy_bench_kernel(const uint4 *src, uint4 *dst, uint64_t n16)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 template <int KIND>
 int occupancy_blocks(int block, size_t lds)
 {
@@ -235,29 +323,43 @@ hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long
     return hipGetLastError();
 }
 
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s)
 {
     const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
                      : p.app == PTW_APP_RWR                            ? KIND_END
                                                                        : KIND_PATH;
     const int block = shape.block;
-    uint32_t hubs = 0;
-    if (kind == KIND_COUNT) hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
-    const size_t lds = (size_t)hubs * sizeof(unsigned long long);
-    int blocks = shape.blocks;
-    if (blocks <= 0) {
-        int per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
-                     : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
-                                        : occupancy_blocks<KIND_PATH>(block, lds);
-        blocks = per_cu * num_cus;
+    uint32_t hubs = 0, slots = 0;
+    if (kind == KIND_COUNT) {
+        hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
+        slots = (uint32_t)shape.cache_slots;
     }
+    const size_t lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
+    // Workgroups beyond the resident capacity are harmless (no workgroup waits
+    // on another): they start late and find the queue drained.
+    int per_cu = shape.blocks_per_cu;
+    if (per_cu <= 0) {
+        per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
+                 : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
+                                    : occupancy_blocks<KIND_PATH>(block, lds);
+    }
+    const int blocks = per_cu * num_cus;
     const uint32_t chunk = (uint32_t)shape.chunk;
     if (kind == KIND_COUNT)
-        hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs);
+        hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);
     else if (kind == KIND_END)
-        hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u);
+        hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
     else
-        hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u);
+        hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
     return hipGetLastError();
 }
 
@@ -270,6 +372,12 @@ hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, cons
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(counts_to_graph_ids_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, counts_dev,
                        perm, V, out, accumulate ? 1 : 0, extra_vertex, extra);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_bench(const uint4 *src, uint4 *dst, uint64_t n16, int num_cus, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_bench_kernel, dim3(num_cus * 16), dim3(256), 0, s, src, dst, n16);
     return hipGetLastError();
 }
 

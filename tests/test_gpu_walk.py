@@ -146,27 +146,27 @@ def test_lds_and_global_counters_scale16(ctx):
 
 
 def test_device_outputs_and_accumulate(ctx):
-    import torch
-
     gi = ctx.rmat(scale=12, edge_factor=8, seed=3)
     rp, col = ctx.export_csr()
     V = gi.num_vertices
     W = 100000
-    counts = torch.zeros(V, dtype=torch.int64, device="cuda")
+    counts = ctx.alloc(8 * V).zero()
     d = ptw.make_desc(ptw.PPR(0.2, 3), W, seed=5, flags=ptw.FLAG_OUT_DEVICE)
-    ctx.walk(d, counts=counts.data_ptr(), device_ptrs=True)
+    ctx.walk(d, counts=counts.ptr, device_ptrs=True)
     d2 = ptw.make_desc(ptw.PPR(0.2, 3), W, seed=5, wid_base=W, flags=ptw.FLAG_OUT_DEVICE | ptw.FLAG_ACCUMULATE)
-    ctx.walk(d2, counts=counts.data_ptr(), device_ptrs=True)
-    torch.cuda.synchronize()
+    ctx.walk(d2, counts=counts.ptr, device_ptrs=True)
     ref = oracle.walk_replay(rp, col, oracle.APP_PPR, 2 * W, tp=0.2, seed=5, source=3,
                              start_mode=oracle.START_SOURCE)
-    assert np.array_equal(counts.cpu().numpy().astype(np.uint64), ref["counts"])
-    ends = torch.zeros(W, dtype=torch.int32, device="cuda")
+    assert np.array_equal(counts.download(np.uint64), ref["counts"])
+    ends = ctx.alloc(4 * W)
     d3 = ptw.make_desc(ptw.RWR(0.2, 3), W, seed=5, flags=ptw.FLAG_OUT_DEVICE)
-    ctx.walk(d3, ends=ends.data_ptr(), device_ptrs=True)
-    torch.cuda.synchronize()
+    ctx.walk(d3, ends=ends.ptr, device_ptrs=True)
     ref = oracle.walk_replay(rp, col, oracle.APP_RWR, W, tp=0.2, seed=5, source=3, start_mode=oracle.START_SOURCE)
-    assert np.array_equal(ends.cpu().numpy().view(np.uint32), ref["ends"])
+    assert np.array_equal(ends.download(np.uint32), ref["ends"])
+    free, total = ctx.device_memory()
+    assert 0 < free <= total
+    counts.free()
+    ends.free()
 
 
 def test_full_size_properties_scale20(ctx):
