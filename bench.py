@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk"],
+                    help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus")
+    ap.add_argument("--walks-per-vertex", type=int, default=10)
+    ap.add_argument("--walk-len", type=int, default=80)
     return ap.parse_args()
 
 
@@ -47,6 +51,8 @@ def main():
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
               "(walkers shard by id range with one counter all-reduce; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "deepwalk":
+        return bench_deepwalk(args)
 
     import numpy as np
 
@@ -167,6 +173,79 @@ def main():
     print(json.dumps(line), flush=True)
     counts.free()
     ends.free()
+    ctx.close()
+
+
+def bench_deepwalk(args):
+    """DeepWalk corpus: R-MAT scale 24 (default), walks_per_vertex walks from every
+    vertex, walk_len vertices each, uniform and alias-weighted neighbour choice;
+    paths stay in HBM.  One bench step = the unweighted corpus + the weighted one."""
+    import numpy as np
+
+    import ptwalker_amd as ptw
+
+    scale = args.scale if args.scale != 22 else 24
+    ctx = ptw.Context(0)
+    t0 = time.time()
+    gi = ctx.rmat(scale=scale, edge_factor=16, seed=1)
+    ctx.random_weights(seed=1)
+    build_s = time.time() - t0
+    V, L = gi.num_vertices, args.walk_len
+    W = args.walks_per_vertex * V
+    paths = ctx.alloc(4 * W * L)
+
+    def run(weighted, rec):
+        flags = ptw.FLAG_OUT_DEVICE | (ptw.FLAG_WEIGHTED if weighted else 0)
+        d = ptw.make_desc(ptw.DeepWalk(L, start=ptw.START_ROUNDROBIN), W, seed=1, flags=flags)
+        o = ctx.walk(d, paths=paths.ptr, device_ptrs=True)
+        if rec is not None:
+            rec.append((weighted, o.kernel_ms, o.steps))
+        return o.steps
+
+    for _ in range(args.warmup):
+        run(False, None)
+        run(True, None)
+    rec = []
+    t1 = time.perf_counter()
+    total = 0
+    for _ in range(args.steps):
+        total += run(False, rec) + run(True, rec)
+    elapsed = time.perf_counter() - t1
+    uw = [r for r in rec if not r[0]]
+    wt = [r for r in rec if r[0]]
+    uw_ms = float(np.mean([r[1] for r in uw]))
+    wt_ms = float(np.mean([r[1] for r in wt]))
+    steps = float(uw[0][2])
+    gps, _ = ctx.bench_gather(4 * gi.num_edges, 1 << 30, seed=1)
+    line = {
+        "metric": "walk-steps/s",
+        "value": total / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT graph and U(0,1] weights generated in-process (seeded Philox); no dataset",
+        "config": {"workload": f"DeepWalk corpus, R-MAT scale {scale} ef16, {args.walks_per_vertex} walks per vertex "
+                               f"x {L} vertices, uniform + alias-weighted, paths resident in HBM",
+                   "vertices": V, "edges": gi.num_edges, "walkers": W, "path_bytes": 4 * W * L,
+                   "graph_and_alias_build_s": round(build_s, 1)},
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel<path> (unweighted)",
+                     "achieved": 16.0 * steps / (uw_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": 16.0 * steps / (uw_ms * 1e-3) / 1e9 / 8000.0, "traffic": None, "bytes_per_step": 16,
+                     "kernel_ms": uw_ms, "steps_per_launch": steps},
+        "weighted": {"kernel_ms": wt_ms, "bytes_per_step": 24,
+                     "achieved": 24.0 * steps / (wt_ms * 1e-3) / 1e9},
+        "random_gather": {"gathers_per_s": gps, "frac_unweighted": 2.0 * steps / (uw_ms * 1e-3) / gps,
+                          "frac_weighted": 3.0 * steps / (wt_ms * 1e-3) / gps},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    paths.free()
     ctx.close()
 
 

@@ -339,12 +339,16 @@ class DeviceBuffer:
         self.ctx.check(self.ctx.lib.ptw_buffer_upload(self.ctx.h, self.ptr, host.ctypes.data, host.nbytes))
         return self
 
-    def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
+    def download(self, dtype, count: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        """Copy `count` elements starting at element `offset` to the host."""
         dtype = np.dtype(dtype)
         if count is None:
-            count = self.nbytes // dtype.itemsize
+            count = self.nbytes // dtype.itemsize - offset
+        if offset < 0 or count < 0 or (offset + count) * dtype.itemsize > self.nbytes:
+            raise ValueError("range outside the buffer")
         out = np.empty(count, dtype=dtype)
-        self.ctx.check(self.ctx.lib.ptw_buffer_download(self.ctx.h, self.ptr, out.ctypes.data, out.nbytes))
+        self.ctx.check(self.ctx.lib.ptw_buffer_download(self.ctx.h, self.ptr + offset * dtype.itemsize,
+                                                        out.ctypes.data, out.nbytes))
         return out
 
     def free(self):

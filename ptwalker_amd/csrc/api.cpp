@@ -388,6 +388,8 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
     if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
     const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
+    if (on_dev && (((uintptr_t)out->paths & 15) || ((uintptr_t)out->ends & 3) || ((uintptr_t)out->counts & 7)))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "device outputs must be aligned: paths 16 B, ends 4 B, counts 8 B");
     const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
     out->steps = out->visits = out->dangling = 0;
     out->kernel_ms = 0.0;

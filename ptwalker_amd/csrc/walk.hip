@@ -140,6 +140,37 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             count_in_lds(L, x, p.counts_dev);
     };
 
+    // Path rows are written through a 4-entry register queue: every entry that
+    // completes an aligned 16 B chunk of the flat path array flushes the chunk
+    // with one dwordx4 store; a row's unaligned head and tail go out as dwords.
+    uint64_t pos = 0;  // flat index of the next path entry
+    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nbuf = 0;
+    auto path_append = [&](uint32_t x) {
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = x;
+        ++nbuf;
+        if ((pos & 3) == 3) {
+            if (nbuf >= 4) {
+                *reinterpret_cast<uint4 *>(p.paths + (pos - 3)) = uint4{b0, b1, b2, b3};
+            } else {
+                if (nbuf >= 3) p.paths[pos - 2] = b1;
+                if (nbuf >= 2) p.paths[pos - 1] = b2;
+                p.paths[pos] = b3;
+            }
+            nbuf = 0;
+        }
+        ++pos;
+    };
+    auto path_flush_tail = [&]() {
+        // the last nbuf (< 4) entries sit at pos - nbuf .. pos - 1
+        if (nbuf >= 3) p.paths[pos - 3] = b1;
+        if (nbuf >= 2) p.paths[pos - 2] = b2;
+        if (nbuf >= 1) p.paths[pos - 1] = b3;
+        nbuf = 0;
+    };
+
     for (;;) {
         // ---- refill idle lanes (wave-uniform control flow) ----
         uint64_t idle = __ballot(!active);
@@ -178,7 +209,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     if (KIND == KIND_COUNT) count_visit(v);
                 }
                 if (KIND == KIND_PATH) {
-                    p.paths[w * p.walk_len] = p.inv ? p.inv[v] : v;
+                    pos = w * p.walk_len;
+                    nbuf = 0;
+                    path_append(p.inv ? p.inv[v] : v);
                 }
                 step = 0;
                 active = true;
@@ -214,16 +247,19 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     v = p.col[beg + j];
                     ++step;
                     if (KIND == KIND_COUNT) count_visit(v);
-                    if (KIND == KIND_PATH) p.paths[w * p.walk_len + step] = p.inv ? p.inv[v] : v;
+                    if (KIND == KIND_PATH) path_append(p.inv ? p.inv[v] : v);
                 }
             }
             if (stop) {
                 my_steps += step;
                 my_dangling += dangled;
                 if (KIND == KIND_END) p.ends[w] = p.inv ? p.inv[v] : v;
-                if (KIND == KIND_PATH && dangled) {
-                    for (uint32_t k = step + 1; k < p.walk_len; ++k)
-                        p.paths[w * p.walk_len + k] = PTW_INVALID_VERTEX;
+                if (KIND == KIND_PATH) {
+                    path_flush_tail();
+                    if (dangled) {
+                        for (uint32_t k = step + 1; k < p.walk_len; ++k)
+                            p.paths[w * p.walk_len + k] = PTW_INVALID_VERTEX;
+                    }
                 }
                 active = false;
             }
