@@ -38,8 +38,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk"],
-                    help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus")
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch"],
+                    help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
+                         "batch: 4096 PPR queries in one launch on R-MAT scale 24")
+    ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--walks-per-vertex", type=int, default=10)
     ap.add_argument("--walk-len", type=int, default=80)
     return ap.parse_args()
@@ -53,6 +55,8 @@ def main():
         sys.exit(2)
     if args.workload == "deepwalk":
         return bench_deepwalk(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import numpy as np
 
@@ -246,6 +250,77 @@ def bench_deepwalk(args):
     }
     print(json.dumps(line), flush=True)
     paths.free()
+    ctx.close()
+
+
+def batch_queries(gi, rp, n, seed=1):
+    """Sources drawn with probability proportional to degree, walker budgets
+    log-uniform in [1e3, 1e6], termination probabilities uniform in [0.15, 0.3]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
+    src = rng.choice(gi.num_vertices, size=n, p=deg / deg.sum())
+    W = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n)).astype(np.int64)
+    tp = rng.uniform(0.15, 0.3, n)
+    return src, W, tp
+
+
+def bench_batch(args):
+    """Batched PPR: `queries` single-source queries in one persistent launch with
+    per-query counter slices (dense above 1e5 walkers, hash below) and top-100
+    per query.  One bench step = one batch; run eagerly and as a replayed hipGraph."""
+    import numpy as np
+
+    import ptwalker_amd as ptw
+
+    scale = args.scale if args.scale != 22 else 24
+    ctx = ptw.Context(0)
+    t0 = time.time()
+    gi = ctx.rmat(scale=scale, edge_factor=16, seed=1)
+    build_s = time.time() - t0
+    rp, _ = ctx.export_csr()
+    src, W, tp = batch_queries(gi, rp, args.queries, seed=1)
+    res = {}
+    for mode in ("eager", "graph"):
+        for _ in range(args.warmup):
+            ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"))
+        t1 = time.perf_counter()
+        ks = []
+        for _ in range(args.steps):
+            r = ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"))
+            ks.append(r.kernel_ms)
+        el = (time.perf_counter() - t1) / args.steps
+        res[mode] = (el, float(np.mean(ks)), r)
+    el, kms, r = res["graph"]
+    line = {
+        "metric": "walk-steps/s",
+        "value": r.total_steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT graph and query mix generated in-process (seeded); no dataset",
+        "config": {"workload": f"{args.queries} PPR queries, R-MAT scale {scale} ef16, sources degree-weighted, "
+                               "walkers log-U[1e3,1e6], tp U[0.15,0.3], top-100 per query, one captured launch",
+                   "vertices": gi.num_vertices, "edges": gi.num_edges, "walkers": int(W.sum()),
+                   "dense_queries": r.dense_queries, "hash_queries": r.hash_queries,
+                   "slice_bytes": r.slice_bytes, "graph_build_s": round(build_s, 1)},
+        "queries_per_s": args.queries / el,
+        "kernel_ms": kms,
+        "eager": {"ms_per_batch": 1e3 * res["eager"][0], "kernel_ms": res["eager"][1]},
+        "roofline": {"bound": "hbm", "kernel": "batch_walk_kernel + batch_topk_kernel",
+                     "achieved": 20.0 * r.total_steps / (kms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": 20.0 * r.total_steps / (kms * 1e-3) / 1e9 / 8000.0, "traffic": None,
+                     "bytes_per_step": 20},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
     ctx.close()
 
 

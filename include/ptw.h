@@ -193,6 +193,55 @@ int ptw_graph_write(ptw_ctx *ctx, const char *path, int format);
  * dangling stop).  Synchronous: returns when outputs are complete. */
 int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
 
+/* ---- batched queries -------------------------------------------------- */
+/* Many single-source PPR queries (visit counts, type 0) in one persistent
+ * launch.  The reference runs one query per process (src/core/ppr.cpp:121-139,
+ * one start vertex, src/core/ppr.cpp:54-57); batching is an addition.
+ * Query i's walker w draws from Philox(seed; w, step, queries[i].stream), so
+ * its counts equal ptw_walk(PPR, source, walkers, tp, stream) alone, in any
+ * mix.  Per-query counters live in u32 slices: dense (V counters) for
+ * walkers > dense_threshold, a hash table otherwise.  Walkers that reach a
+ * vertex without out-edges stop there. */
+typedef struct ptw_query {
+    uint32_t source;  /* graph id */
+    uint32_t stream;  /* 4th Philox counter word of this query */
+    uint64_t walkers; /* at most 2^32 / (4 (1 + 1/tp)) so u32 slices cannot wrap */
+    double tp;
+} ptw_query;
+
+#define PTW_BATCH_GRAPH 1u    /* run the device sequence as a captured hipGraph (replayed across batches) */
+#define PTW_BATCH_WEIGHTED 2u /* neighbour choice by edge weight */
+
+typedef struct ptw_batch_desc {
+    const ptw_query *queries;
+    uint32_t num_queries;
+    uint32_t topk;            /* results per query, <= 1024 (0: none) */
+    uint64_t seed;
+    int32_t semantics;        /* PTW_SEM_* */
+    uint32_t flags;           /* PTW_BATCH_* */
+    uint64_t dense_threshold; /* walkers above this use a dense slice; 0 = 100000 */
+} ptw_batch_desc;
+
+typedef struct ptw_batch_out {
+    uint32_t *topk_vertex; /* host [num_queries * topk]: graph ids, count desc (ties: degree desc, id asc) */
+    uint64_t *topk_count;  /* host [num_queries * topk] visit counts (PTW_INVALID_VERTEX / 0 padding) */
+    uint64_t *steps;       /* host [num_queries] or NULL */
+    /* filled on return */
+    uint64_t total_steps;
+    uint64_t slice_bytes; /* HBM held by the counter slices of this batch */
+    uint32_t dense_queries;
+    uint32_t hash_queries;
+    double kernel_ms; /* device time: counter setup, walk, top-k */
+    double total_ms;
+} ptw_batch_out;
+
+int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out);
+/* HBM the batch would need beyond what the context already holds (admission). */
+int ptw_batch_plan_bytes(ptw_ctx *ctx, const ptw_batch_desc *desc, uint64_t *bytes);
+/* Full visit-count vector (graph ids, host u64[V]) of query `index` of the
+ * last batch. */
+int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts);
+
 /* ---- launch tuning ---------------------------------------------------- */
 /* Launch shape of the walk kernels.  The reference's counterparts are its
  * walker container size --ws and hub degree boundary --db
@@ -205,6 +254,8 @@ typedef struct ptw_tuning {
     int32_t chunk;         /* walker ids a wavefront takes from the queue at once (>= 64) */
     int32_t lds_hubs;      /* LDS counters for the top-degree vertices, 0..16384 */
     int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 8192; 0 = off) */
+    int32_t batch_chunk;   /* walker ids per queue entry in batched launches (>= 64) */
+    int32_t topk_tile;     /* LDS candidate slots of the top-k kernel (power of two, 256..4096) */
 } ptw_tuning;
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
 int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);

@@ -302,3 +302,30 @@ def load_edge_list(path: str, fmt: str = "text", as_undirected: bool = False):
     row_ptr[1:] = np.cumsum([len(x) for x in adj])
     col = np.array([x for lst in adj for x in lst], dtype=np.uint32)
     return row_ptr, col, name2id
+
+
+def topk(counts, row_ptr, k):
+    """Top-k of a count vector: count descending, ties by degree descending then
+    vertex id ascending; zero counts never listed.  Returns (vertices, counts)
+    padded with (INVALID, 0) to length k."""
+    counts = np.asarray(counts, dtype=np.uint64)
+    deg = np.diff(np.asarray(row_ptr, dtype=np.int64))
+    ids = np.arange(len(counts))
+    order = np.lexsort((ids, -deg, -counts.astype(np.float64)))
+    order = order[counts[order] > 0][:k]
+    v = np.full(k, INVALID, dtype=np.uint32)
+    c = np.zeros(k, dtype=np.uint64)
+    v[: len(order)] = order
+    c[: len(order)] = counts[order]
+    return v, c
+
+
+def topk_l1(estimate, exact, k=100):
+    """Top-k L1 error: sum over the k largest entries of the normalized exact vector of
+    |estimate/sum(estimate) - exact/sum(exact)|."""
+    est = np.asarray(estimate, dtype=np.float64)
+    ex = np.asarray(exact, dtype=np.float64)
+    est = est / est.sum()
+    ex = ex / ex.sum()
+    top = np.argsort(-ex, kind="stable")[:k]
+    return float(np.abs(est[top] - ex[top]).sum())

@@ -121,6 +121,45 @@ class WalkOut(ctypes.Structure):
     ]
 
 
+class Query(ctypes.Structure):
+    _fields_ = [
+        ("source", ctypes.c_uint32),
+        ("stream", ctypes.c_uint32),
+        ("walkers", ctypes.c_uint64),
+        ("tp", ctypes.c_double),
+    ]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("queries", ctypes.c_void_p),
+        ("num_queries", ctypes.c_uint32),
+        ("topk", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("semantics", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("dense_threshold", ctypes.c_uint64),
+    ]
+
+
+class BatchOut(ctypes.Structure):
+    _fields_ = [
+        ("topk_vertex", ctypes.c_void_p),
+        ("topk_count", ctypes.c_void_p),
+        ("steps", ctypes.c_void_p),
+        ("total_steps", ctypes.c_uint64),
+        ("slice_bytes", ctypes.c_uint64),
+        ("dense_queries", ctypes.c_uint32),
+        ("hash_queries", ctypes.c_uint32),
+        ("kernel_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+
+BATCH_GRAPH = 1
+BATCH_WEIGHTED = 2
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [
         ("block", ctypes.c_int32),
@@ -128,6 +167,8 @@ class Tuning(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("lds_hubs", ctypes.c_int32),
         ("cache_slots", ctypes.c_int32),
+        ("batch_chunk", ctypes.c_int32),
+        ("topk_tile", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -152,6 +193,9 @@ _SIGNATURES = {
     "ptw_graph_export_csr": (ctypes.c_int, [_P, _P, _P]),
     "ptw_graph_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "ptw_walk": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
+    "ptw_batch_ppr": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchOut)]),
+    "ptw_batch_plan_bytes": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_uint64)]),
+    "ptw_batch_counts": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "ptw_tuning_get": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -282,6 +326,55 @@ class Context:
         else:
             out.counts, out.ends, out.paths = _ptr(counts), _ptr(ends), _ptr(paths)
         self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
+        return out
+
+    # ---- batched queries -------------------------------------------------
+    @staticmethod
+    def _batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold):
+        n = len(sources)
+        qs = (Query * max(n, 1))()
+        streams = range(n) if streams is None else streams
+        for i, (s, w, tp, st) in enumerate(zip(sources, walkers, tps, streams)):
+            qs[i].source, qs[i].walkers, qs[i].tp, qs[i].stream = int(s), int(w), float(tp), int(st)
+        d = BatchDesc()
+        d.queries = ctypes.cast(qs, ctypes.c_void_p)
+        d.num_queries = n
+        d.topk = topk
+        d.seed = seed
+        d.semantics = semantics
+        d.flags = flags
+        d.dense_threshold = dense_threshold
+        return d, qs
+
+    def batch_ppr(self, sources, walkers, tps, *, streams=None, topk: int = 100, seed: int = 1,
+                  semantics: int = SEM_REFERENCE, graph: bool = False, weighted: bool = False,
+                  dense_threshold: int = 0) -> "BatchResult":
+        """Many single-source PPR queries in one launch (ptw_batch_ppr).  Query i uses Philox
+        stream `streams[i]` (default i)."""
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_WEIGHTED if weighted else 0)
+        d, qs = self._batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold)
+        n = len(sources)
+        tv = np.zeros(max(n * topk, 1), dtype=np.uint32)
+        tc = np.zeros(max(n * topk, 1), dtype=np.uint64)
+        steps = np.zeros(max(n, 1), dtype=np.uint64)
+        out = BatchOut()
+        out.topk_vertex, out.topk_count, out.steps = _ptr(tv), _ptr(tc), _ptr(steps)
+        self.check(self.lib.ptw_batch_ppr(self.h, ctypes.byref(d), ctypes.byref(out)))
+        del qs
+        return BatchResult(tv[: n * topk].reshape(n, topk), tc[: n * topk].reshape(n, topk), steps[:n],
+                           out.total_steps, out.slice_bytes, out.dense_queries, out.hash_queries, out.kernel_ms,
+                           out.total_ms)
+
+    def batch_plan_bytes(self, sources, walkers, tps, *, topk: int = 100, dense_threshold: int = 0) -> int:
+        d, qs = self._batch_desc(sources, walkers, tps, None, topk, 1, SEM_REFERENCE, 0, dense_threshold)
+        b = ctypes.c_uint64()
+        self.check(self.lib.ptw_batch_plan_bytes(self.h, ctypes.byref(d), ctypes.byref(b)))
+        del qs
+        return b.value
+
+    def batch_counts(self, index: int) -> np.ndarray:
+        out = np.zeros(self.info().num_vertices, dtype=np.uint64)
+        self.check(self.lib.ptw_batch_counts(self.h, index, _ptr(out)))
         return out
 
     # ---- launch tuning ---------------------------------------------------
@@ -418,6 +511,19 @@ def make_desc(app, walkers: int, seed: int = 1, wid_base: int = 0, stream: int =
     else:
         raise TypeError(f"unknown app {app!r}")
     return d
+
+
+@dataclass
+class BatchResult:
+    topk_vertex: np.ndarray  # (queries, k) graph ids
+    topk_count: np.ndarray   # (queries, k)
+    steps: np.ndarray        # (queries,)
+    total_steps: int
+    slice_bytes: int
+    dense_queries: int
+    hash_queries: int
+    kernel_ms: float
+    total_ms: float
 
 
 @dataclass

@@ -10,59 +10,18 @@
 #include <string>
 
 #include "../../include/ptw.h"
+#include "ctx.hpp"
 #include "engine.hpp"
-
-struct ptw_ctx {
-    int device = 0;
-    int num_cus = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-
-    bool has_graph = false;
-    ptw::HostCsr host;
-    ptw::DeviceGraph dev;
-    uint32_t max_degree = 0, max_degree_vertex = 0, num_dangling = 0;
-
-    // host copies of the relabel (empty without one)
-    std::vector<uint32_t> h_perm, h_inv, h_drow;
-    // weighted sampling: weights and alias tables in graph-id CSR order
-    std::vector<float> weights;
-    std::vector<uint32_t> alias_thr, alias_idx;
-
-    unsigned long long *counts_dev = nullptr;  // V device-id counters
-    unsigned long long *scratch = nullptr;     // 8 words
-    ptw::LaunchShape shape;
-};
 
 namespace {
 
+using ptw::fail;
 using ptw::HostCsr;
-
-int fail(ptw_ctx *ctx, int code, const std::string &msg)
-{
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIP_TRY(ctx, call)                                                                        \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? PTW_ERR_OOM : PTW_ERR_HIP,             \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                       \
-    } while (0)
-
-template <typename T>
-int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
-{
-    HIP_TRY(ctx, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
-    if (n) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-    return PTW_OK;
-}
+using ptw::upload;
 
 void free_graph(ptw_ctx *ctx)
 {
+    ptw::free_batch(ctx);
     auto &d = ctx->dev;
     if (d.row_ptr) (void)hipFree(d.row_ptr);
     if (d.col) (void)hipFree(d.col);
@@ -104,6 +63,7 @@ int install_weights(ptw_ctx *ctx)
             for (uint64_t e = lo; e < hi; ++e) tab[e] = uint2{ctx->alias_thr[e], ctx->alias_idx[e]};
         });
     }
+    ptw::drop_batch_graph(ctx);
     if (d.alias) (void)hipFree(d.alias);
     d.alias = nullptr;
     if ((rc = upload(ctx, &d.alias, tab.data(), tab.size()))) return rc;
@@ -506,6 +466,8 @@ int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)
     t->chunk = ctx->shape.chunk;
     t->lds_hubs = ctx->shape.lds_hubs;
     t->cache_slots = ctx->shape.cache_slots;
+    t->batch_chunk = ctx->shape.batch_chunk;
+    t->topk_tile = ctx->shape.topk_tile;
     return PTW_OK;
 }
 
@@ -520,6 +482,12 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
     if (t->lds_hubs < 0 || t->lds_hubs > 16384) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 16384]");
     if (t->cache_slots < 0 || t->cache_slots > 8192 || (t->cache_slots & (t->cache_slots - 1)))
         return fail(ctx, PTW_ERR_INVALID_ARG, "cache_slots must be 0 or a power of two <= 8192");
+    if (t->batch_chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "batch_chunk must be >= 64");
+    if (t->topk_tile < 256 || t->topk_tile > 4096 || (t->topk_tile & (t->topk_tile - 1)))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "topk_tile must be a power of two in [256, 4096]");
+    ptw::drop_batch_graph(ctx);
+    ctx->shape.batch_chunk = t->batch_chunk;
+    ctx->shape.topk_tile = t->topk_tile;
     ctx->shape.block = t->block;
     ctx->shape.blocks_per_cu = t->blocks_per_cu;
     ctx->shape.chunk = t->chunk;

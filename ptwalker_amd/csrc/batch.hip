@@ -1,0 +1,565 @@
+// Batched single-source walks (PPR visit counts) for gfx950.
+//
+// All queries of a batch run in one persistent launch.  The host cuts every
+// query into chunks of walker ids; wavefronts take chunks from one queue
+// counter and refill idle lanes from their current chunk, so lanes of one
+// wavefront may hold walkers of different queries.  Query q's walker w draws
+// from Philox(key = seed, ctr = (w, step, stream_q)): a query's counts are
+// the same alone (ptw_walk with stream = stream_q) and in any mix.
+//
+// Counts go to per-query slices:
+//   dense slice: u32[V] in device ids; after the walk a compaction pass lists
+//                each slice's nonzero counters, so top-k and the cleanup read
+//                only touched entries
+//   hash slice:  open addressing over {u32 key, u32 count}, capacity a power
+//                of two sized from the query's expected visits
+// A query's visits to its own source are counted in a register and flushed
+// when the lane moves to another query; other visits go through a
+// per-wavefront LDS cache of (query, vertex) counts.
+//
+// Top-k: one workgroup per query narrows the value range that holds the
+// k-th count with LDS histograms, gathers the candidates into LDS, and
+// bitonic-sorts them by (count desc, device id asc); ties that do not fit
+// the tile are cut by an exact radix select on the device id.
+#include "../../include/ptw.h"
+#include "engine.hpp"
+#include "philox.hpp"
+#include "wave.hpp"
+
+namespace ptw {
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kBins = 2048;
+
+constexpr unsigned long long kEmpty64 = ~0ull;
+constexpr uint32_t kWaveSlots = 256;  // per-wavefront LDS cache slots
+constexpr int kCacheProbes = 8;
+constexpr uint32_t kSeg = 65536;  // dense-slice compaction segment
+
+struct LaneQuery {
+    uint32_t q = kEmpty;
+    uint32_t src = 0, thr = 0, kind = 0, mask = 0, stream = 0;
+    uint64_t off = 0;
+};
+
+__device__ __forceinline__ void slice_add(const BatchLaunch &p, uint32_t q, uint32_t kind, uint64_t off,
+                                          uint32_t mask, uint32_t v, uint32_t n)
+{
+    if (kind == SLICE_DENSE) {
+        atomicAdd(&p.dense[off + v], n);
+        return;
+    }
+    uint32_t h = (v * 0x9E3779B1u) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        uint2 *slot = &p.hash[off + h];
+        uint32_t k = slot->x;
+        if (k == kEmpty) k = atomicCAS(&slot->x, kEmpty, v);
+        if (k == kEmpty || k == v) {
+            atomicAdd(&slot->y, n);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    atomicAdd(&p.overflow[q], n);
+}
+
+// Per-wavefront LDS cache of (query, vertex) counts in front of the slices:
+// the lanes of a wavefront walk one chunk (one query) at a time, whose hot
+// vertices -- the source's neighbourhood -- would otherwise collide in global
+// atomics.  Each wavefront owns its region, so no barrier is needed; the
+// region is flushed whenever the wavefront moves to a chunk of another query.
+struct WaveCache {
+    unsigned long long *keys;
+    uint32_t *cnt;
+    __device__ __forceinline__ bool add(uint32_t q, uint32_t v) const
+    {
+        const unsigned long long key = ((unsigned long long)q << 32) | v;
+        uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 56) & (kWaveSlots - 1);
+#pragma unroll 1
+        for (int probe = 0; probe < kCacheProbes; ++probe) {
+            unsigned long long k = keys[h];
+            if (k == kEmpty64) k = atomicCAS(&keys[h], kEmpty64, key);
+            if (k == kEmpty64 || k == key) {
+                atomicAdd(&cnt[h], 1u);
+                return true;
+            }
+            h = (h + 1) & (kWaveSlots - 1);
+        }
+        return false;
+    }
+    // Wavefront-uniform: every lane of the wavefront calls it.
+    __device__ __forceinline__ void flush(const BatchLaunch &p, uint32_t lane) const
+    {
+        for (uint32_t s = lane; s < kWaveSlots; s += 64) {
+            const unsigned long long k = keys[s];
+            if (k != kEmpty64) {
+                const uint32_t q = (uint32_t)(k >> 32);
+                const BatchQuery bq = p.queries[q];
+                slice_add(p, q, bq.kind, bq.slice_off, bq.mask, (uint32_t)k, cnt[s]);
+                keys[s] = kEmpty64;
+                cnt[s] = 0;
+            }
+        }
+    }
+};
+
+__global__ void batch_begin_kernel(BatchLaunch p, uint32_t max_queries)
+{
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.queue[0] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < max_queries && i < nq; i += gridDim.x * blockDim.x) {
+        p.touched_count[i] = 0;
+        p.overflow[i] = 0;
+        p.steps[i] = 0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
+{
+    extern __shared__ unsigned long long wave_keys[];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const WaveCache C{wave_keys + (size_t)wave * kWaveSlots,
+                      reinterpret_cast<uint32_t *>(wave_keys + (size_t)waves * kWaveSlots) + wave * kWaveSlots};
+    for (uint32_t s = lane; s < kWaveSlots; s += 64) {
+        C.keys[s] = kEmpty64;
+        C.cnt[s] = 0;
+    }
+    const uint64_t num_chunks = p.hdr[0];
+    const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
+    const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
+    const uint2 *alias = p.hdr[5] ? p.alias : nullptr;
+    uint64_t c_next = 0, c_end = 0;  // wave-uniform current chunk
+    uint32_t cq = kEmpty;
+    bool exhausted = false;
+    bool active = false;
+    LaneQuery Q;
+    uint64_t w = 0;
+    uint32_t v = 0, step = 0;
+    uint32_t my_src = 0;
+    uint64_t my_steps = 0;
+
+    auto flush = [&]() {
+        if (Q.q == kEmpty) return;
+        if (my_src) slice_add(p, Q.q, Q.kind, Q.off, Q.mask, Q.src, my_src);
+        if (my_steps) atomicAdd(&p.steps[Q.q], (unsigned long long)my_steps);
+        my_src = 0;
+        my_steps = 0;
+    };
+
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        while (idle != 0 && !exhausted) {
+            if (c_next == c_end) {
+                unsigned long long ci = 0;
+                if (lane == 0) ci = atomicAdd(&p.queue[0], 1ull);
+                ci = wave_broadcast_first(ci);
+                if (ci >= num_chunks) {
+                    exhausted = true;
+                    break;
+                }
+                const BatchChunk ch = p.chunks[ci];
+                if (ch.q != cq && cq != kEmpty) C.flush(p, lane);  // wave-uniform
+                cq = ch.q;
+                c_next = ch.start;
+                c_end = ch.start + ch.len;
+            }
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            const uint64_t avail = c_end - c_next;
+            const uint32_t take = avail < n_idle ? (uint32_t)avail : n_idle;
+            const uint32_t rank = lane_rank(idle);
+            if (!active && rank < take) {
+                if (Q.q != cq) {
+                    flush();
+                    const BatchQuery bq = p.queries[cq];
+                    Q.q = cq;
+                    Q.src = bq.source_dev;
+                    Q.thr = bq.thr;
+                    Q.kind = bq.kind;
+                    Q.mask = bq.mask;
+                    Q.stream = bq.stream;
+                    Q.off = bq.slice_off;
+                }
+                w = c_next + rank;
+                v = Q.src;
+                step = 0;
+                ++my_src;  // the start visit
+                active = true;
+            }
+            c_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+
+        if (active) {
+            const u32x4 r = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), step, Q.stream, seed_lo, seed_hi);
+            const bool coin = textbook || step >= 1;
+            bool stop = coin && r.z >= Q.thr;
+            if (!stop) {
+                const uint32_t beg = p.row_ptr[v];
+                const uint32_t deg = p.row_ptr[v + 1] - beg;
+                if (deg == 0) {
+                    stop = true;  // batches end walkers at vertices without out-edges
+                } else {
+                    uint32_t j = mulhi64(r.x, r.y, deg);
+                    if (alias) {
+                        const uint2 a = alias[beg + j];
+                        if (r.w >= a.x) j = a.y;
+                    }
+                    v = p.col[beg + j];
+                    ++step;
+                    if (v == Q.src)
+                        ++my_src;
+                    else if (!C.add(Q.q, v))
+                        slice_add(p, Q.q, Q.kind, Q.off, Q.mask, v, 1u);
+                }
+            }
+            if (stop) {
+                my_steps += step;
+                active = false;
+            }
+        }
+    }
+    flush();
+    C.flush(p, lane);
+}
+
+// After the walk: the nonzero counters of each dense slice, compacted into
+// the query's touched list (one reservation per segment), so that top-k and
+// the cleanup read only touched entries.  A list that would overflow its
+// capacity marks the query for full-slice scans (touched_count > capacity).
+__global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
+{
+    __shared__ uint32_t s_count[256];
+    __shared__ uint32_t s_base;
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    const uint32_t segs = (p.V + kSeg - 1) / kSeg;
+    const uint64_t tasks = (uint64_t)nq * segs;
+    for (uint64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
+        const uint32_t q = (uint32_t)(t / segs);
+        const BatchQuery bq = p.queries[q];
+        if (bq.kind != SLICE_DENSE) continue;  // block-uniform
+        const uint32_t lo = (uint32_t)(t % segs) * kSeg;
+        const uint32_t hi = lo + kSeg < p.V ? lo + kSeg : p.V;
+        const uint32_t *slice = p.dense + bq.slice_off;
+        uint32_t mine = 0;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) mine += slice[i] != 0;
+        s_count[threadIdx.x] = mine;
+        __syncthreads();
+        // inclusive scan over the 256 per-thread counts
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            const uint32_t x = threadIdx.x >= off ? s_count[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s_count[threadIdx.x] += x;
+            __syncthreads();
+        }
+        const uint32_t total = s_count[255];
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(&p.touched_count[q], total) : 0u;
+        __syncthreads();
+        uint32_t idx = s_base + s_count[threadIdx.x] - mine;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            if (slice[i] != 0) {
+                if (idx < bq.touched_cap) p.touched[bq.touched_off + idx] = i;
+                ++idx;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- top-k -------------------------------------------------------------------
+
+struct Entries {
+    const BatchLaunch *p;
+    BatchQuery Q;
+    uint32_t mode;  // 0 dense list, 1 dense full slice, 2 hash
+    uint32_t n;
+    uint32_t V;
+    __device__ __forceinline__ void get(uint32_t i, uint32_t &v, uint32_t &c) const
+    {
+        if (mode == 0) {
+            v = p->touched[Q.touched_off + i];
+            c = p->dense[Q.slice_off + v];
+        } else if (mode == 1) {
+            v = i;
+            c = p->dense[Q.slice_off + i];
+        } else {
+            const uint2 s = p->hash[Q.slice_off + i];
+            v = s.x;
+            c = s.x == kEmpty ? 0u : s.y;
+        }
+    }
+};
+
+__device__ void block_bitonic_desc(unsigned long long *a, uint32_t n)
+{
+    for (uint32_t size = 2; size <= n; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const bool desc = (i & size) == 0;
+                    const unsigned long long x = a[i], y = a[j];
+                    if (desc ? (x < y) : (x > y)) {
+                        a[i] = y;
+                        a[j] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const uint32_t *inv, uint32_t *out_v,
+                                                         unsigned long long *out_c, uint32_t tile)
+{
+    __shared__ uint32_t hist[kBins];
+    __shared__ uint32_t s_u32[8];
+    __shared__ unsigned long long s_u64[4];
+    extern __shared__ unsigned long long cand[];
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    const uint32_t k = (uint32_t)p.hdr[2];
+    if (k == 0) return;
+
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        Entries E;
+        E.p = &p;
+        E.Q = p.queries[q];
+        E.V = p.V;
+        if (E.Q.kind == SLICE_DENSE) {
+            const uint32_t t = p.touched_count[q];
+            if (t <= E.Q.touched_cap) {
+                E.mode = 0;
+                E.n = t;
+            } else {
+                E.mode = 1;
+                E.n = p.V;
+            }
+        } else {
+            E.mode = 2;
+            E.n = E.Q.mask + 1;
+        }
+        // max count
+        if (threadIdx.x == 0) s_u32[0] = 0;
+        __syncthreads();
+        uint32_t local_max = 0;
+        for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+            uint32_t v, c;
+            E.get(i, v, c);
+            local_max = c > local_max ? c : local_max;
+        }
+        atomicMax(&s_u32[0], local_max);
+        __syncthreads();
+        const uint32_t maxc = s_u32[0];
+        // narrow [lo, hi] until the candidates fit the tile or one value remains
+        uint64_t lo = 1, hi = maxc;
+        uint32_t above = 0;  // entries with count > hi
+        uint32_t inbin = 0;
+        bool overflow_ties = false;
+        while (maxc > 0) {
+            const uint64_t range = hi - lo + 1;
+            const uint64_t width = (range + kBins - 1) / kBins;
+            for (uint32_t b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                uint32_t v, c;
+                E.get(i, v, c);
+                if (c >= lo && c <= hi) atomicAdd(&hist[(uint32_t)((c - lo) / width)], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t cum = above, b = 0;
+                const uint32_t nb = (uint32_t)((range + width - 1) / width);
+                for (int bb = (int)nb - 1; bb >= 0; --bb) {
+                    if (cum + hist[bb] >= k || bb == 0) {
+                        b = (uint32_t)bb;
+                        break;
+                    }
+                    cum += hist[bb];
+                }
+                s_u32[1] = b;
+                s_u32[2] = cum;        // entries above bin b
+                s_u32[3] = hist[b];    // entries in bin b
+            }
+            __syncthreads();
+            const uint32_t b = s_u32[1];
+            const uint64_t nlo = lo + (uint64_t)b * width;
+            uint64_t nhi = nlo + width - 1;
+            if (nhi > hi) nhi = hi;
+            above = s_u32[2];
+            inbin = s_u32[3];
+            lo = nlo;
+            hi = nhi;
+            __syncthreads();
+            if (above + inbin <= tile) break;
+            if (lo == hi) {
+                overflow_ties = true;
+                break;
+            }
+        }
+        // collect candidates: everything above hi, and the bin [lo, hi]
+        uint32_t tie_limit = 0xFFFFFFFFu;  // ties at count lo == hi keep ids <= tie_limit
+        if (overflow_ties) {
+            // radix select of the (k - above) smallest device ids among count == lo
+            const uint32_t need = k - above;
+            uint32_t prefix = 0, pmask = 0;
+            const int shifts[3] = {21, 10, 0};
+            const uint32_t widths[3] = {11, 11, 10};
+            uint32_t rank_left = need;
+            for (int pass = 0; pass < 3; ++pass) {
+                for (uint32_t bb = threadIdx.x; bb < kBins; bb += blockDim.x) hist[bb] = 0;
+                __syncthreads();
+                const uint32_t dmask = (1u << widths[pass]) - 1;
+                for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                    uint32_t v, c;
+                    E.get(i, v, c);
+                    if (c == (uint32_t)lo && (v & pmask) == prefix) atomicAdd(&hist[(v >> shifts[pass]) & dmask], 1u);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    uint32_t cum = 0, d = 0;
+                    for (uint32_t dd = 0; dd <= dmask; ++dd) {
+                        if (cum + hist[dd] >= rank_left) {
+                            d = dd;
+                            break;
+                        }
+                        cum += hist[dd];
+                    }
+                    s_u32[4] = d;
+                    s_u32[5] = cum;
+                }
+                __syncthreads();
+                prefix |= s_u32[4] << shifts[pass];
+                pmask |= dmask << shifts[pass];
+                rank_left -= s_u32[5];
+                __syncthreads();
+            }
+            tie_limit = prefix;
+        }
+        if (threadIdx.x == 0) s_u32[6] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+            uint32_t v, c;
+            E.get(i, v, c);
+            if (c == 0) continue;
+            bool take = c > hi;
+            if (!take && c >= lo && c <= hi) take = !overflow_ties || v <= tie_limit;
+            if (take) {
+                const uint32_t slot = atomicAdd(&s_u32[6], 1u);
+                if (slot < tile) cand[slot] = ((unsigned long long)c << 32) | (uint32_t)~v;
+            }
+        }
+        __syncthreads();
+        uint32_t nc = s_u32[6];
+        if (nc > tile) nc = tile;  // cannot happen: the narrowing bounds the count
+        uint32_t n2 = 1;
+        while (n2 < nc) n2 <<= 1;
+        for (uint32_t i = nc + threadIdx.x; i < n2; i += blockDim.x) cand[i] = 0;
+        __syncthreads();
+        block_bitonic_desc(cand, n2);
+        for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+            uint32_t gv = PTW_INVALID_VERTEX;
+            unsigned long long c = 0;
+            if (i < nc) {
+                const unsigned long long key = cand[i];
+                const uint32_t dv = ~(uint32_t)key;
+                c = key >> 32;
+                gv = inv ? inv[dv] : dv;
+            }
+            out_v[(uint64_t)q * k + i] = gv;
+            out_c[(uint64_t)q * k + i] = c;
+        }
+        __syncthreads();
+        (void)s_u64;
+    }
+}
+
+// ---- cleanup and export --------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void batch_reset_kernel(BatchLaunch p)
+{
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const BatchQuery Q = p.queries[q];
+        if (Q.kind == SLICE_DENSE) {
+            const uint32_t t = p.touched_count[q];
+            if (t <= Q.touched_cap) {
+                for (uint32_t i = threadIdx.x; i < t; i += blockDim.x) p.dense[Q.slice_off + p.touched[Q.touched_off + i]] = 0;
+            } else {
+                for (uint32_t i = threadIdx.x; i < p.V; i += blockDim.x) p.dense[Q.slice_off + i] = 0;
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i <= Q.mask; i += blockDim.x) p.hash[Q.slice_off + i] = uint2{kEmpty, 0u};
+        }
+    }
+}
+
+__global__ void batch_export_kernel(BatchLaunch p, uint32_t q, const uint32_t *perm, const uint32_t *inv,
+                                    uint64_t *out, int phase)
+{
+    const BatchQuery Q = p.queries[q];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (Q.kind == SLICE_DENSE) {
+        if (phase != 0) return;
+        for (uint64_t g = t0; g < p.V; g += stride) out[g] = p.dense[Q.slice_off + (perm ? perm[g] : g)];
+    } else if (phase == 0) {
+        for (uint64_t g = t0; g < p.V; g += stride) out[g] = 0;
+    } else {
+        for (uint64_t i = t0; i <= Q.mask; i += stride) {
+            const uint2 s = p.hash[Q.slice_off + i];
+            if (s.x != kEmpty) out[inv ? inv[s.x] : s.x] = s.y;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStream_t s)
+{
+    uint32_t blocks = (max_queries + 255) / 256;
+    if (blocks == 0) blocks = 1;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(batch_begin_kernel, dim3(blocks), dim3(256), 0, s, p, max_queries);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, hipStream_t s)
+{
+    const size_t lds = (size_t)(block / 64) * kWaveSlots * 12;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * num_cus), dim3(block), lds, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v, unsigned long long *out_c,
+                             int tile, int num_cus, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_topk_kernel, dim3(num_cus * 2), dim3(1024), (size_t)tile * 8, s, p, inv, out_v, out_c,
+                       (uint32_t)tile);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(num_cus * 2), dim3(1024), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_export(const BatchLaunch &p, uint32_t q, const uint32_t *perm, const uint32_t *inv,
+                               uint64_t *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_export_kernel, dim3(1024), dim3(256), 0, s, p, q, perm, inv, out, 0);
+    hipLaunchKernelGGL(batch_export_kernel, dim3(1024), dim3(256), 0, s, p, q, perm, inv, out, 1);
+    return hipGetLastError();
+}
+
+}  // namespace ptw

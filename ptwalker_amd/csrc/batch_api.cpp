@@ -1,0 +1,407 @@
+// C ABI of batched queries: planning, pools, the launch sequence (eager or
+// as one captured hipGraph replayed across batches), top-k and export.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/ptw.h"
+#include "ctx.hpp"
+#include "engine.hpp"
+
+namespace ptw {
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+uint64_t next_pow2(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+template <typename T>
+void hfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void drop_exec(BatchState &b)
+{
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    b.exec = nullptr;
+}
+
+struct Plan {
+    std::vector<BatchQuery> q;
+    std::vector<BatchChunk> chunks;
+    uint64_t dense = 0, hash = 0, touched = 0;  // elements needed
+    uint32_t n_dense = 0, n_hash = 0;
+};
+
+int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
+{
+    const uint32_t V = ctx->dev.V;
+    const uint64_t thr_dense = d.dense_threshold ? d.dense_threshold : 100000ull;
+    const uint64_t chunk = (uint64_t)ctx->shape.batch_chunk;
+    P.q.resize(d.num_queries);
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
+        const ptw_query &qq = d.queries[i];
+        if (qq.source >= V) return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": source out of range");
+        if (!(qq.tp > 0.0 && qq.tp <= 1.0))
+            return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": tp must be in (0, 1]");
+        if (qq.walkers == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": no walkers");
+        const double per_walker = 1.0 + 1.0 / qq.tp;  // expected visits per walker (upper: reference semantics)
+        if ((double)qq.walkers * per_walker * 4.0 > 4294967296.0)
+            return fail(ctx, PTW_ERR_INVALID_ARG,
+                        "query " + std::to_string(i) + ": too many walkers for u32 slices; use ptw_walk");
+        BatchQuery &B = P.q[i];
+        B.source_dev = ctx->dev.relabelled ? ctx->h_perm[qq.source] : qq.source;
+        B.thr = continue_threshold(qq.tp);
+        B.walkers = qq.walkers;
+        B.stream = qq.stream;
+        const uint64_t upper = (uint64_t)(2.0 * (double)qq.walkers * per_walker) + 64;
+        if (qq.walkers > thr_dense) {
+            B.kind = SLICE_DENSE;
+            B.slice_off = P.dense;
+            P.dense += V;
+            B.touched_cap = (uint32_t)std::min<uint64_t>(V, upper);
+            B.touched_off = P.touched;
+            P.touched += B.touched_cap;
+            B.mask = 0;
+            ++P.n_dense;
+        } else {
+            const uint64_t cap = std::min<uint64_t>(next_pow2(2 * upper), next_pow2(2ull * V));
+            B.kind = SLICE_HASH;
+            B.slice_off = P.hash;
+            P.hash += cap;
+            B.mask = (uint32_t)(cap - 1);
+            B.touched_cap = 0;
+            B.touched_off = 0;
+            ++P.n_hash;
+        }
+        for (uint64_t s = 0; s < qq.walkers; s += chunk)
+            P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(chunk, qq.walkers - s), s});
+    }
+    return PTW_OK;
+}
+
+uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint32_t k)
+{
+    uint64_t need = 0;
+    if (P.dense > b.dense_cap) need += 4 * P.dense;
+    if (P.hash > b.hash_cap) need += 8 * P.hash;
+    if (P.touched > b.touched_cap) need += 4 * P.touched;
+    if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + 16);
+    if (P.chunks.size() > b.chunk_cap) need += P.chunks.size() * sizeof(BatchChunk);
+    if ((uint64_t)nq * k > b.topk_cap) need += (uint64_t)nq * k * 12;
+    return need;
+}
+
+// Kernel arguments hold addresses only; per-batch values live in hdr.
+BatchLaunch launch_args(ptw_ctx *ctx)
+{
+    BatchState &b = ctx->batch;
+    BatchLaunch p{};
+    p.row_ptr = ctx->dev.row_ptr;
+    p.col = ctx->dev.col;
+    p.alias = ctx->dev.alias;
+    p.V = ctx->dev.V;
+    p.queries = b.queries;
+    p.chunks = b.chunks;
+    p.hdr = b.hdr;
+    p.dense = b.dense;
+    p.hash = b.hash;
+    p.touched = b.touched;
+    p.touched_count = b.touched_count;
+    p.overflow = b.overflow;
+    p.steps = b.steps;
+    p.queue = b.queue;
+    return p;
+}
+
+// Grows the pools to fit the plan; new slice memory is filled clean.
+int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
+{
+    BatchState &b = ctx->batch;
+    bool moved = false;
+    if (!b.hdr) {
+        HIP_TRY(ctx, hipMalloc((void **)&b.hdr, 8 * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMalloc((void **)&b.queue, sizeof(unsigned long long)));
+        moved = true;
+    }
+    if (P.dense > b.dense_cap) {
+        hfree(b.dense);
+        const uint64_t cap = std::max<uint64_t>(P.dense, b.dense_cap + b.dense_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.dense, cap * 4));
+        HIP_TRY(ctx, launch_fill_u32(b.dense, cap, 0u, ctx->stream));
+        b.dense_cap = cap;
+        moved = true;
+    }
+    if (P.hash > b.hash_cap) {
+        hfree(b.hash);
+        const uint64_t cap = std::max<uint64_t>(P.hash, b.hash_cap + b.hash_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.hash, cap * 8));
+        // {key = empty, count = 0}: 0x00000000FFFFFFFF little-endian per slot
+        HIP_TRY(ctx, launch_fill_u64((unsigned long long *)b.hash, cap, 0x00000000FFFFFFFFull, ctx->stream));
+        b.hash_cap = cap;
+        moved = true;
+    }
+    if (P.touched > b.touched_cap || !b.touched) {
+        hfree(b.touched);
+        const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(P.touched, 1), b.touched_cap + b.touched_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.touched, cap * 4));
+        b.touched_cap = cap;
+        moved = true;
+    }
+    if (nq > b.q_cap) {
+        hfree(b.queries);
+        hfree(b.touched_count);
+        hfree(b.overflow);
+        hfree(b.steps);
+        const uint32_t cap = std::max<uint32_t>(nq, b.q_cap + b.q_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.queries, (size_t)cap * sizeof(BatchQuery)));
+        HIP_TRY(ctx, hipMalloc((void **)&b.touched_count, (size_t)cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.overflow, (size_t)cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.steps, (size_t)cap * 8));
+        b.q_cap = cap;
+        moved = true;
+    }
+    if (P.chunks.size() > b.chunk_cap) {
+        hfree(b.chunks);
+        const uint64_t cap = std::max<uint64_t>(P.chunks.size(), b.chunk_cap + b.chunk_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.chunks, cap * sizeof(BatchChunk)));
+        b.chunk_cap = cap;
+        moved = true;
+    }
+    const uint64_t tk = std::max<uint64_t>((uint64_t)nq * k, 1);
+    if (tk > b.topk_cap) {
+        hfree(b.topk_v);
+        hfree(b.topk_c);
+        const uint64_t cap = std::max<uint64_t>(tk, b.topk_cap + b.topk_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.topk_v, cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.topk_c, cap * 8));
+        b.topk_cap = cap;
+        moved = true;
+    }
+    if (moved) drop_exec(b);
+    return PTW_OK;
+}
+
+// Counter setup + walk + top-k for the batch described by the device tables.
+int enqueue_run(ptw_ctx *ctx, const BatchLaunch &p)
+{
+    HIP_TRY(ctx, launch_batch_begin(p, ctx->batch.q_cap, ctx->stream));
+    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->stream));
+    HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, ctx->batch.topk_v, ctx->batch.topk_c, ctx->shape.topk_tile,
+                                   ctx->num_cus, ctx->stream));
+    return PTW_OK;
+}
+
+int clean_previous(ptw_ctx *ctx)
+{
+    BatchState &b = ctx->batch;
+    if (!b.dirty) return PTW_OK;
+    BatchLaunch p = launch_args(ctx);
+    HIP_TRY(ctx, launch_batch_reset(p, ctx->num_cus, ctx->stream));
+    b.dirty = false;
+    return PTW_OK;
+}
+
+}  // namespace
+
+void free_batch(ptw_ctx *ctx)
+{
+    BatchState &b = ctx->batch;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_exec(b);
+    hfree(b.dense);
+    hfree(b.hash);
+    hfree(b.touched);
+    hfree(b.queries);
+    hfree(b.touched_count);
+    hfree(b.overflow);
+    hfree(b.steps);
+    hfree(b.chunks);
+    hfree(b.topk_v);
+    hfree(b.topk_c);
+    hfree(b.hdr);
+    hfree(b.queue);
+    b = BatchState{};
+}
+
+}  // namespace ptw
+
+using ptw::fail;
+
+extern "C" {
+
+int ptw_batch_plan_bytes(ptw_ctx *ctx, const ptw_batch_desc *desc, uint64_t *bytes)
+{
+    if (!ctx || !desc || !bytes) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (desc->num_queries && !desc->queries) return fail(ctx, PTW_ERR_INVALID_ARG, "null query array");
+    ptw::Plan P;
+    int rc = ptw::make_plan(ctx, *desc, P);
+    if (rc) return rc;
+    *bytes = ptw::plan_bytes_beyond(ctx->batch, P, desc->num_queries, desc->topk);
+    return PTW_OK;
+}
+
+int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!desc || !out) return fail(ctx, PTW_ERR_INVALID_ARG, "null descriptor or output");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: batches need a HIP device");
+    const ptw_batch_desc &d = *desc;
+    if (d.num_queries && !d.queries) return fail(ctx, PTW_ERR_INVALID_ARG, "null query array");
+    if (d.topk > 1024 || 2 * d.topk > (uint32_t)ctx->shape.topk_tile)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "topk must be <= 1024 and <= topk_tile / 2");
+    if (d.topk && (!out->topk_vertex || !out->topk_count))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "top-k output arrays required");
+    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
+    const bool weighted = (d.flags & PTW_BATCH_WEIGHTED) != 0;
+    if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted batch on a graph without weights");
+    const double t0 = ptw::now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ptw::BatchState &b = ctx->batch;
+    out->total_steps = 0;
+    out->slice_bytes = 0;
+    out->dense_queries = out->hash_queries = 0;
+    out->kernel_ms = 0.0;
+
+    ptw::Plan P;
+    int rc = ptw::make_plan(ctx, d, P);
+    if (rc) return rc;
+    // admission: what the plan needs beyond the pools must fit in free HBM
+    const uint64_t extra = ptw::plan_bytes_beyond(b, P, d.num_queries, d.topk);
+    if (extra) {
+        size_t f = 0, t = 0;
+        HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+        if (extra > f)
+            return fail(ctx, PTW_ERR_ADMISSION,
+                        "batch needs " + std::to_string(extra) + " more bytes of HBM, " + std::to_string(f) + " free");
+    }
+    if ((rc = ptw::clean_previous(ctx))) return rc;
+    if ((rc = ptw::ensure_pools(ctx, P, d.num_queries, d.topk))) return rc;
+
+    b.h_queries = std::move(P.q);
+    b.h_chunks = std::move(P.chunks);
+    b.h_hdr[0] = b.h_chunks.size();
+    b.h_hdr[1] = d.num_queries;
+    b.h_hdr[2] = d.topk;
+    b.h_hdr[3] = d.seed;
+    b.h_hdr[4] = (unsigned long long)d.semantics;
+    b.h_hdr[5] = weighted ? 1 : 0;
+    b.last_nq = d.num_queries;
+    b.weighted = weighted;
+    hipStream_t s = ctx->stream;
+    if (d.num_queries) {
+        HIP_TRY(ctx, hipMemcpyAsync(b.queries, b.h_queries.data(), b.h_queries.size() * sizeof(ptw::BatchQuery),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(b.chunks, b.h_chunks.data(), b.h_chunks.size() * sizeof(ptw::BatchChunk),
+                                    hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(b.hdr, b.h_hdr, sizeof(b.h_hdr), hipMemcpyHostToDevice, s));
+    b.dirty = true;
+
+    const ptw::BatchLaunch p = ptw::launch_args(ctx);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    if (d.flags & PTW_BATCH_GRAPH) {
+        // The sequence reads sizes, seed and semantics from hdr, so one capture
+        // serves every batch until a pool or the graph moves.
+        if (!b.exec) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rc = ptw::enqueue_run(ctx, p);
+            hipError_t ec = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(ctx, ec);
+            hipError_t ei = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_TRY(ctx, ei);
+        }
+        HIP_TRY(ctx, hipGraphLaunch(b.exec, s));
+    } else {
+        if ((rc = ptw::enqueue_run(ctx, p))) return rc;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+
+    std::vector<unsigned long long> steps(d.num_queries);
+    std::vector<uint32_t> over(d.num_queries);
+    std::vector<unsigned long long> tc;
+    if (d.num_queries) {
+        HIP_TRY(ctx, hipMemcpyAsync(steps.data(), b.steps, d.num_queries * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(over.data(), b.overflow, d.num_queries * 4, hipMemcpyDeviceToHost, s));
+        if (d.topk) {
+            tc.resize((size_t)d.num_queries * d.topk);
+            HIP_TRY(ctx, hipMemcpyAsync(out->topk_vertex, b.topk_v, tc.size() * 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(ctx, hipMemcpyAsync(tc.data(), b.topk_c, tc.size() * 8, hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->kernel_ms = ms;
+    for (size_t i = 0; i < tc.size(); ++i) out->topk_count[i] = tc[i];
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
+        total += steps[i];
+        if (out->steps) out->steps[i] = steps[i];
+    }
+    out->total_steps = total;
+    out->dense_queries = 0;
+    out->hash_queries = 0;
+    uint64_t bytes = 0;
+    for (const auto &q : b.h_queries) {
+        if (q.kind == ptw::SLICE_DENSE) {
+            ++out->dense_queries;
+            bytes += 4ull * ctx->dev.V + 4ull * q.touched_cap;
+        } else {
+            ++out->hash_queries;
+            bytes += 8ull * ((uint64_t)q.mask + 1);
+        }
+    }
+    out->slice_bytes = bytes;
+    out->total_ms = ptw::now_ms() - t0;
+    for (uint32_t i = 0; i < d.num_queries; ++i)
+        if (over[i])
+            return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
+    return PTW_OK;
+}
+
+int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts)
+{
+    if (!ctx || !counts) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context");
+    ptw::BatchState &b = ctx->batch;
+    if (!b.dirty || index >= b.last_nq) return fail(ctx, PTW_ERR_INVALID_ARG, "no such query in the last batch");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint32_t V = ctx->dev.V;
+    uint64_t *tmp = nullptr;
+    HIP_TRY(ctx, hipMalloc((void **)&tmp, (size_t)V * 8));
+    const ptw::BatchLaunch p = ptw::launch_args(ctx);
+    hipError_t e = ptw::launch_batch_export(p, index, ctx->dev.perm, ctx->dev.inv, tmp, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, tmp, (size_t)V * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    HIP_TRY(ctx, e);
+    return PTW_OK;
+}
+
+}  // extern "C"

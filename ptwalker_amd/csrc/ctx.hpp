@@ -1,0 +1,118 @@
+// The context behind the C ABI's opaque ptw_ctx, shared by api.cpp (graphs,
+// single walks, buffers) and batch_api.cpp (batched queries).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ptw.h"
+#include "engine.hpp"
+
+namespace ptw {
+
+// Device pools and tables of the batched-query path.  The slice pools are
+// kept clean between batches (dense counters zero, hash slots empty), so a
+// batch only has to clean what the previous one touched.
+struct BatchState {
+    uint32_t *dense = nullptr;
+    uint64_t dense_cap = 0;  // u32 counters
+    uint2 *hash = nullptr;
+    uint64_t hash_cap = 0;  // slots
+    uint32_t *touched = nullptr;
+    uint64_t touched_cap = 0;
+    BatchQuery *queries = nullptr;
+    uint32_t *touched_count = nullptr, *overflow = nullptr;
+    unsigned long long *steps = nullptr;
+    uint32_t q_cap = 0;
+    BatchChunk *chunks = nullptr;
+    uint64_t chunk_cap = 0;
+    uint32_t *topk_v = nullptr;
+    unsigned long long *topk_c = nullptr;
+    uint64_t topk_cap = 0;
+    unsigned long long *hdr = nullptr;    // chunks, queries, k, seed, semantics, weighted
+    unsigned long long *queue = nullptr;  // chunk queue head
+    bool dirty = false;                   // the last batch's slices are not yet cleaned
+    bool weighted = false;
+    uint32_t last_nq = 0;
+    std::vector<BatchQuery> h_queries;
+    std::vector<BatchChunk> h_chunks;
+    unsigned long long h_hdr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // captured launch sequence, valid while the pools and the graph keep their addresses
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace ptw
+
+struct ptw_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    bool has_graph = false;
+    ptw::HostCsr host;
+    ptw::DeviceGraph dev;
+    uint32_t max_degree = 0, max_degree_vertex = 0, num_dangling = 0;
+
+    // host copies of the relabel (empty without one)
+    std::vector<uint32_t> h_perm, h_inv, h_drow;
+    // weighted sampling: weights and alias tables in graph-id CSR order
+    std::vector<float> weights;
+    std::vector<uint32_t> alias_thr, alias_idx;
+
+    unsigned long long *counts_dev = nullptr;  // V device-id counters
+    unsigned long long *scratch = nullptr;     // 8 words
+    ptw::LaunchShape shape;
+    ptw::BatchState batch;
+};
+
+namespace ptw {
+
+inline int fail(ptw_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return ::ptw::fail((ctx), e_ == hipErrorOutOfMemory ? PTW_ERR_OOM : PTW_ERR_HIP,      \
+                               std::string(#call) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <typename T>
+int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
+{
+    HIP_TRY(ctx, hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
+    if (n) HIP_TRY(ctx, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return PTW_OK;
+}
+
+// Frees the batch pools (graph change or context teardown).
+void free_batch(ptw_ctx *ctx);
+
+// Drops the captured batch graph (its kernels' arguments or launch shape changed).
+inline void drop_batch_graph(ptw_ctx *ctx)
+{
+    if (ctx->batch.exec) {
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(ctx->batch.exec);
+    }
+    ctx->batch.exec = nullptr;
+}
+
+// Continuation threshold of a termination probability: continue iff r.z < thr.
+inline uint32_t continue_threshold(double tp)
+{
+    double c = __builtin_floor((1.0 - tp) * 4294967296.0);
+    if (c < 0.0) c = 0.0;
+    if (c > 4294967295.0) c = 4294967295.0;
+    return (uint32_t)c;
+}
+
+}  // namespace ptw

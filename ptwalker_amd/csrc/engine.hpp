@@ -100,6 +100,8 @@ struct LaunchShape {
     int chunk = 4096;        // walker ids a wave takes from the queue at once
     int lds_hubs = 8192;     // u32 LDS counters for device ids below this (count apps)
     int cache_slots = 4096;  // LDS visit-cache slots per workgroup (power of two, 0 = off)
+    int batch_chunk = 1024;  // walker ids per chunk of a batched launch
+    int topk_tile = 2048;    // top-k LDS candidate slots
 };
 
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
@@ -108,6 +110,63 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
 hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
                                       uint32_t V, uint64_t *out, bool accumulate,
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);
+// ---- batched queries (batch.hip) -------------------------------------------
+
+enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1 };
+
+// One query of a batch, as the device sees it.
+struct BatchQuery {
+    uint32_t source_dev;
+    uint32_t thr;          // continue iff r.z < thr
+    uint64_t walkers;
+    uint64_t slice_off;    // dense: u32 element offset; hash: uint2 slot offset
+    uint32_t kind;         // SliceKind
+    uint32_t mask;         // hash: capacity - 1
+    uint64_t touched_off;  // dense: u32 offset of the first-touch list
+    uint32_t touched_cap;  // dense: list capacity
+    uint32_t stream;       // 4th Philox counter word
+};
+
+struct BatchChunk {
+    uint32_t q;
+    uint32_t len;
+    uint64_t start;  // first local walker index of the chunk
+};
+
+struct BatchLaunch {
+    const uint32_t *row_ptr;
+    const uint32_t *col;
+    const uint2 *alias;  // used when hdr[5] != 0
+    uint32_t V;
+    const BatchQuery *queries;
+    const BatchChunk *chunks;
+    // per-batch values live on the device so one captured graph serves every
+    // batch: hdr[0] chunks, [1] queries, [2] top-k length, [3] seed,
+    // [4] semantics, [5] weighted
+    const unsigned long long *hdr;
+    uint32_t *dense;         // dense slice pool (u32 counters, device ids)
+    uint2 *hash;             // hash slice pool {key, count}
+    uint32_t *touched;       // first-touch lists
+    uint32_t *touched_count; // per query
+    uint32_t *overflow;      // per query: hash insertions that found no slot
+    unsigned long long *steps;  // per query
+    unsigned long long *queue;  // [0]: next chunk
+};
+
+// Zeroes the per-query counters and the chunk queue of the batch in hdr.
+hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStream_t s);
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, hipStream_t s);
+// Per-query top-k (count desc, device id asc) into out_v (graph ids) / out_c,
+// one workgroup per query, `tile` candidate slots of LDS per workgroup.
+hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v,
+                             unsigned long long *out_c, int tile, int num_cus, hipStream_t s);
+// Zero what a batch left in its slices (dense: the touched entries or the whole
+// slice on list overflow; hash: every slot), so the pools stay clean.
+hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s);
+// Query q's full count vector in graph ids.
+hipError_t launch_batch_export(const BatchLaunch &p, uint32_t q, const uint32_t *perm, const uint32_t *inv,
+                               uint64_t *out, hipStream_t s);
+
 hipError_t launch_copy_bench(const uint4 *src, uint4 *dst, uint64_t n16, int num_cus, hipStream_t s);
 hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,
                                uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink,

@@ -20,26 +20,13 @@
 #include "../../include/ptw.h"
 #include "engine.hpp"
 #include "philox.hpp"
+#include "wave.hpp"
 
 namespace ptw {
 
 namespace {
 
 enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2 };
-
-__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
-{
-    // number of set bits of `mask` below this lane
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ uint64_t wave_broadcast_first(uint64_t x)
-{
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
 __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
 {

@@ -1,0 +1,165 @@
+"""Batched PPR queries in one persistent launch: every query's full count
+vector bit-exact against the oracle's replay of that query alone (its own
+Philox stream), top-k exact under the stated tie order, the same results
+alone and in any mix, and the captured hipGraph replayed several times with
+eager batches in between, every replay equal to the eager result."""
+import numpy as np
+import pytest
+
+import oracle
+import ptwalker_amd as ptw
+from walkstats import random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def replay_query(rp, col, src, W, tp, stream, seed=1, alias=None):
+    return oracle.walk_replay(rp, col, oracle.APP_PPR, W, tp=tp, seed=seed, source=src,
+                              start_mode=oracle.START_SOURCE, stream=stream, alias=alias)
+
+
+def make_queries(gi, n, seed, wmin=100, wmax=200000):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, gi.num_vertices, n)
+    src[0] = gi.max_degree_vertex
+    W = np.exp(rng.uniform(np.log(wmin), np.log(wmax), n)).astype(np.int64)
+    tp = rng.uniform(0.15, 0.3, n)
+    return src, W, tp
+
+
+@pytest.fixture(scope="module")
+def g12(ctx):
+    gi = ctx.rmat(scale=12, edge_factor=16, seed=2)
+    rp, col = ctx.export_csr()
+    return gi, rp, col
+
+
+def test_batch_counts_and_topk_match_replay(ctx, g12):
+    gi, rp, col = g12
+    src, W, tp = make_queries(gi, 40, seed=1)
+    k = 50
+    r = ctx.batch_ppr(src, W, tp, topk=k, seed=3, dense_threshold=20000)
+    assert r.dense_queries == int(np.sum(W > 20000)) and r.hash_queries == 40 - r.dense_queries
+    total = 0
+    for i in range(40):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=3)
+        got = ctx.batch_counts(i)
+        assert np.array_equal(got, ref["counts"]), i
+        assert r.steps[i] == ref["steps"]
+        v, c = oracle.topk(ref["counts"], rp, k)
+        assert np.array_equal(r.topk_vertex[i], v), i
+        assert np.array_equal(r.topk_count[i], c), i
+        total += ref["steps"]
+    assert r.total_steps == total
+
+
+def test_batch_alone_equals_in_mix(ctx, g12):
+    gi, rp, col = g12
+    src, W, tp = make_queries(gi, 12, seed=5)
+    mix = ctx.batch_ppr(src, W, tp, topk=20, seed=9)
+    counts_mix = [ctx.batch_counts(i) for i in range(12)]
+    for j in (0, 5, 11):
+        alone = ctx.batch_ppr([src[j]], [W[j]], [tp[j]], streams=[j], topk=20, seed=9)
+        assert np.array_equal(ctx.batch_counts(0), counts_mix[j])
+        assert np.array_equal(alone.topk_vertex[0], mix.topk_vertex[j])
+        # and the single-walk entry point with the same stream
+        d = ptw.make_desc(ptw.PPR(float(tp[j]), int(src[j])), int(W[j]), seed=9, stream=j)
+        single = np.zeros(gi.num_vertices, dtype=np.uint64)
+        ctx.walk(d, counts=single)
+        assert np.array_equal(single, counts_mix[j])
+
+
+def test_batch_graph_replays_match_eager(ctx, g12):
+    gi, rp, col = g12
+    a = make_queries(gi, 30, seed=7)
+    b = make_queries(gi, 17, seed=8)
+    eager_a = ctx.batch_ppr(*a, topk=40, seed=4)
+    ca = [ctx.batch_counts(i) for i in (0, 13, 29)]
+    eager_b = ctx.batch_ppr(*b, topk=40, seed=4)
+    for rep in range(3):
+        ga = ctx.batch_ppr(*a, topk=40, seed=4, graph=True)
+        assert np.array_equal(ga.topk_vertex, eager_a.topk_vertex), rep
+        assert np.array_equal(ga.topk_count, eager_a.topk_count), rep
+        assert np.array_equal(ga.steps, eager_a.steps), rep
+        for i, c in zip((0, 13, 29), ca):
+            assert np.array_equal(ctx.batch_counts(i), c)
+        # eager work between replays
+        e = ctx.batch_ppr(*b, topk=40, seed=4)
+        assert np.array_equal(e.topk_vertex, eager_b.topk_vertex)
+        gb = ctx.batch_ppr(*b, topk=40, seed=4, graph=True)
+        assert np.array_equal(gb.topk_count, eager_b.topk_count), rep
+
+
+def test_batch_weighted(ctx):
+    gi = ctx.rmat(scale=11, edge_factor=16, seed=4)
+    ctx.random_weights(seed=3)
+    rp, col = ctx.export_csr()
+    w, _, _ = ctx.export_weights()
+    alias = oracle.build_alias(rp, w)
+    src, W, tp = make_queries(gi, 10, seed=2, wmax=50000)
+    r = ctx.batch_ppr(src, W, tp, topk=10, seed=6, weighted=True, dense_threshold=5000)
+    for i in range(10):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=6, alias=alias)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+
+
+def test_topk_tie_overflow_path(ctx):
+    # star: hub 0 joined to 5000 leaves; tp = 1 makes each walker visit the hub then one
+    # leaf, so leaf counts are small and tied far beyond the 256-slot tile
+    n = 5000
+    rp = np.zeros(n + 2, dtype=np.uint32)
+    rp[1] = n
+    rp[2:] = n + np.arange(1, n + 1)
+    col = np.concatenate([np.arange(1, n + 1), np.zeros(n)]).astype(np.uint32)
+    ctx.from_csr(rp, col)
+    t = ctx.tuning()
+    ctx.set_tuning(topk_tile=256)
+    try:
+        r = ctx.batch_ppr([0, 0], [2000, 3000], [1.0, 1.0], topk=128, seed=2)
+        for i, Wq in enumerate((2000, 3000)):
+            ref = replay_query(rp, col, 0, Wq, 1.0, i, seed=2)
+            v, c = oracle.topk(ref["counts"], rp, 128)
+            assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], c)
+    finally:
+        ctx.set_tuning(topk_tile=t.topk_tile)
+
+
+def test_batch_small_graph_dangling_and_edges(ctx):
+    rp, col = random_graph(30, 0.15, seed=3, dangling=3)
+    ctx.from_csr(rp, col)
+    r = ctx.batch_ppr([1, 2], [5000, 200], [0.2, 0.25], topk=100, seed=1)
+    for i, (s, Wq, tp) in enumerate(((1, 5000, 0.2), (2, 200, 0.25))):
+        ref = replay_query(rp, col, s, Wq, tp, i)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"])
+        v, c = oracle.topk(ref["counts"], rp, 100)
+        assert np.array_equal(r.topk_vertex[i], v)
+    empty = ctx.batch_ppr([], [], [], topk=10)
+    assert empty.total_steps == 0
+
+
+def test_batch_invalid(ctx):
+    rp, col = random_graph(10, 0.3, seed=1)
+    ctx.from_csr(rp, col)
+    for kw in [dict(sources=[10], walkers=[10], tps=[0.2]), dict(sources=[1], walkers=[0], tps=[0.2]),
+               dict(sources=[1], walkers=[10], tps=[0.0]), dict(sources=[1], walkers=[2 ** 31], tps=[0.2])]:
+        with pytest.raises(ptw.PtwError) as e:
+            ctx.batch_ppr(kw["sources"], kw["walkers"], kw["tps"], topk=10)
+        assert e.value.code == ptw.ERR_INVALID_ARG
+    with pytest.raises(ptw.PtwError):
+        ctx.batch_ppr([1], [10], [0.2], topk=2000)
+    with pytest.raises(ptw.PtwError):
+        ctx.batch_counts(5)
+    assert ctx.batch_plan_bytes([1, 2], [10, 10 ** 6], [0.2, 0.2]) >= 0
+
+
+def test_batch_scale20_properties(ctx):
+    gi = ctx.rmat(scale=20, edge_factor=16, seed=1)
+    src, W, tp = make_queries(gi, 512, seed=11, wmin=1000, wmax=1000000)
+    r = ctx.batch_ppr(src, W, tp, topk=100, seed=1)
+    for i in (0, 100, 511):
+        c = ctx.batch_counts(i)
+        assert int(c.sum()) == int(W[i]) + int(r.steps[i])
+        assert c[src[i]] >= W[i]
+        order = np.argsort(-c.astype(np.float64), kind="stable")[:100]
+        assert np.array_equal(np.sort(r.topk_count[i])[::-1], np.sort(c[order])[::-1])
+    assert abs(r.total_steps - float(np.sum(W / tp))) < 6 * np.sqrt(np.sum(W * (1 - tp) / tp ** 2))
