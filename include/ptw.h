@@ -242,6 +242,21 @@ int ptw_batch_plan_bytes(ptw_ctx *ctx, const ptw_batch_desc *desc, uint64_t *byt
  * last batch. */
 int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts);
 
+/* ---- FP64 solver ------------------------------------------------------ */
+/* Expected visits per walker of single-source walks from each of n sources,
+ * by FP64 power iteration on the device (pull over the transposed CSR, up to
+ * 64 sources per pass):  x = e_s + sum_{i>=1} P(L >= i) e_s P^i  with
+ * P(L >= i) = (1-tp)^(i-1) (PTW_SEM_REFERENCE) or (1-tp)^i (TEXTBOOK),
+ * iterated until the remaining mass bound (1-tp)^i / tp is below tol.
+ * Mass that reaches a vertex without out-edges stops there, as walkers do.
+ * out: host double[n * V] in graph ids, or NULL.  topk_*: host [n * k] of
+ * the k largest entries per source (value desc, id asc), or k = 0.
+ * This is the FP64 oracle the walk estimates are scored against on the
+ * device; no counterpart in the reference. */
+int ptw_ppr_exact(ptw_ctx *ctx, const uint32_t *sources, const double *tps, uint32_t n, int32_t semantics,
+                  double tol, double *out, uint32_t k, uint32_t *topk_vertex, double *topk_value,
+                  uint32_t *iterations);
+
 /* ---- launch tuning ---------------------------------------------------- */
 /* Launch shape of the walk kernels.  The reference's counterparts are its
  * walker container size --ws and hub degree boundary --db

@@ -196,6 +196,8 @@ _SIGNATURES = {
     "ptw_batch_ppr": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchOut)]),
     "ptw_batch_plan_bytes": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_batch_counts": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "ptw_ppr_exact": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, _P,
+                                     ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "ptw_tuning_get": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -376,6 +378,23 @@ class Context:
         out = np.zeros(self.info().num_vertices, dtype=np.uint64)
         self.check(self.lib.ptw_batch_counts(self.h, index, _ptr(out)))
         return out
+
+    # ---- FP64 solver ------------------------------------------------------
+    def ppr_exact(self, sources, tps, *, semantics: int = SEM_REFERENCE, tol: float = 1e-10,
+                  full: bool = True, k: int = 0):
+        """Expected visits per walker from each source (FP64, on the device).
+        Returns (vectors [n, V] or None, topk_vertex [n, k] or None, topk_value, iterations)."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        tp = np.ascontiguousarray(tps, dtype=np.float64)
+        n = len(src)
+        V = self.info().num_vertices
+        out = np.zeros((n, V), dtype=np.float64) if full else None
+        tv = np.zeros((n, k), dtype=np.uint32) if k else None
+        tval = np.zeros((n, k), dtype=np.float64) if k else None
+        it = ctypes.c_uint32()
+        self.check(self.lib.ptw_ppr_exact(self.h, _ptr(src), _ptr(tp), n, semantics, tol, _ptr(out), k, _ptr(tv),
+                                          _ptr(tval), ctypes.byref(it)))
+        return out, tv, tval, it.value
 
     # ---- launch tuning ---------------------------------------------------
     def tuning(self) -> Tuning:

@@ -29,14 +29,17 @@ struct DeviceGraph {
     uint32_t *perm = nullptr;     // graph id -> device id (nullptr: identity)
     uint32_t *inv = nullptr;      // device id -> graph id (nullptr: identity)
     uint2 *alias = nullptr;       // E, {threshold, local alias index}, device order
+    // transpose (in-edges) and out-degrees for the FP64 solver, built on first use
+    uint32_t *t_row = nullptr, *t_col = nullptr, *out_deg = nullptr;
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
     bool relabelled = false;
     uint64_t bytes = 0;
 };
 
-// Host-side parallel loop over [0, n) in contiguous chunks.
+// Host-side parallel loop over [0, n) in contiguous chunks of at least
+// min_per_thread items.
 void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &body,
-                  unsigned threads = 0);
+                  unsigned threads = 0, uint64_t min_per_thread = 4096);
 unsigned host_threads();
 
 // Edge-list ingest with the reference's load semantics.  Returns 0 or a
@@ -166,6 +169,14 @@ hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s);
 // Query q's full count vector in graph ids.
 hipError_t launch_batch_export(const BatchLaunch &p, uint32_t q, const uint32_t *perm, const uint32_t *inv,
                                uint64_t *out, hipStream_t s);
+
+// ---- FP64 power iteration (exact.hip) --------------------------------------
+hipError_t launch_exact_init(uint32_t V, uint32_t B, const uint32_t *src_dev, double *y, double *x, hipStream_t s);
+hipError_t launch_exact_step(const uint32_t *t_row, const uint32_t *t_col, const uint32_t *out_deg, uint32_t V,
+                             uint32_t B, const double *y, double *y_next, double *x, const double *weight,
+                             int num_cus, hipStream_t s);
+hipError_t launch_exact_export(uint32_t V, uint32_t B, const uint32_t *perm, const double *x, double *out,
+                               hipStream_t s);
 
 hipError_t launch_copy_bench(const uint4 *src, uint4 *dst, uint64_t n16, int num_cus, hipStream_t s);
 hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,

@@ -43,14 +43,17 @@ unsigned host_threads()
     return n;
 }
 
-void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &body, unsigned threads)
+void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &body, unsigned threads,
+                  uint64_t min_per_thread)
 {
     if (threads == 0) threads = host_threads();
-    if (n < 4096 || threads == 1) {
+    if (min_per_thread == 0) min_per_thread = 1;
+    const uint64_t useful = (n + min_per_thread - 1) / min_per_thread;
+    if (useful < threads) threads = (unsigned)useful;
+    if (threads <= 1) {
         if (n) body(0, n);
         return;
     }
-    if (threads > n) threads = (unsigned)n;
     std::vector<std::thread> pool;
     pool.reserve(threads);
     for (unsigned t = 0; t < threads; ++t) {
