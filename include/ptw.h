@@ -226,6 +226,7 @@ typedef struct ptw_batch_out {
     uint32_t *topk_vertex; /* host [num_queries * topk]: graph ids, count desc (ties: degree desc, id asc) */
     uint64_t *topk_count;  /* host [num_queries * topk] visit counts (PTW_INVALID_VERTEX / 0 padding) */
     uint64_t *steps;       /* host [num_queries] or NULL */
+    double *entropy;       /* host [num_queries] or NULL: entropy (nats) of each query's visit distribution */
     /* filled on return */
     uint64_t total_steps;
     uint64_t slice_bytes; /* HBM held by the counter slices of this batch */
@@ -241,6 +242,10 @@ int ptw_batch_plan_bytes(ptw_ctx *ctx, const ptw_batch_desc *desc, uint64_t *byt
 /* Full visit-count vector (graph ids, host u64[V]) of query `index` of the
  * last batch. */
 int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts);
+/* Counts of n (query index, graph id) pairs of the last batch, on the device
+ * in one pass: out[i] = count of vertices[i] in query query_index[i]. */
+int ptw_batch_gather(ptw_ctx *ctx, uint64_t n, const uint32_t *query_index, const uint32_t *vertices,
+                     uint64_t *out);
 
 /* ---- FP64 solver ------------------------------------------------------ */
 /* Expected visits per walker of single-source walks from each of n sources,
@@ -256,6 +261,32 @@ int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts);
 int ptw_ppr_exact(ptw_ctx *ctx, const uint32_t *sources, const double *tps, uint32_t n, int32_t semantics,
                   double tol, double *out, uint32_t k, uint32_t *topk_vertex, double *topk_value,
                   uint32_t *iterations);
+
+/* ---- budget predictor ------------------------------------------------- */
+/* A 6-64-64-1 ReLU MLP that predicts log(walker budget) of a PPR query from
+ * six features (log degree, log mean neighbour degree, log 2-hop degree sum,
+ * pilot top-10 mass, pilot entropy, log target eps).  BF16 weights (round to
+ * nearest even) with FP32 accumulation; forward, backward and Adam run as
+ * HIP kernels; gradients are summed in a fixed order (deterministic).
+ * Inputs and targets are standardized with the model's normalization.
+ * No counterpart in the reference (its budget is the -w flag). */
+typedef struct ptw_mlp ptw_mlp;
+int ptw_mlp_create(ptw_ctx *ctx, uint64_t seed, ptw_mlp **out);
+void ptw_mlp_destroy(ptw_mlp *mlp);
+/* Number of FP32 parameters: W1[64][6], b1[64], W2[64][64], b2[64], W3[64], b3. */
+int ptw_mlp_num_params(void);
+int ptw_mlp_set_normalization(ptw_mlp *mlp, const float *mean6, const float *std6, float y_mean, float y_std);
+/* params: FP32 master copy (host, ptw_mlp_num_params floats); norm: 14 floats
+ * (mean6, std6, y_mean, y_std); either may be NULL. */
+int ptw_mlp_get_params(ptw_mlp *mlp, float *params, float *norm);
+int ptw_mlp_set_params(ptw_mlp *mlp, const float *params);
+/* y[i] = predicted log budget of x[i * 6 .. i * 6 + 5] (host arrays). */
+int ptw_mlp_forward(ptw_mlp *mlp, const float *x, uint32_t n, float *y);
+/* Gradient of the mean squared error (standardized target) over all n rows. */
+int ptw_mlp_gradients(ptw_mlp *mlp, const float *x, const float *t, uint32_t n, float *grad, float *loss);
+/* Adam (beta 0.9 / 0.999) over shuffled mini-batches of `batch` rows. */
+int ptw_mlp_train(ptw_mlp *mlp, const float *x, const float *t, uint32_t n, uint32_t epochs, uint32_t batch,
+                  float lr, uint64_t seed, float *final_loss);
 
 /* ---- launch tuning ---------------------------------------------------- */
 /* Launch shape of the walk kernels.  The reference's counterparts are its

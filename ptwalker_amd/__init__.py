@@ -147,6 +147,7 @@ class BatchOut(ctypes.Structure):
         ("topk_vertex", ctypes.c_void_p),
         ("topk_count", ctypes.c_void_p),
         ("steps", ctypes.c_void_p),
+        ("entropy", ctypes.c_void_p),
         ("total_steps", ctypes.c_uint64),
         ("slice_bytes", ctypes.c_uint64),
         ("dense_queries", ctypes.c_uint32),
@@ -196,8 +197,19 @@ _SIGNATURES = {
     "ptw_batch_ppr": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchOut)]),
     "ptw_batch_plan_bytes": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_batch_counts": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "ptw_batch_gather": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, _P]),
     "ptw_ppr_exact": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, _P,
                                      ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
+    "ptw_mlp_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "ptw_mlp_destroy": (None, [_P]),
+    "ptw_mlp_num_params": (ctypes.c_int, []),
+    "ptw_mlp_set_normalization": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, ctypes.c_float]),
+    "ptw_mlp_get_params": (ctypes.c_int, [_P, _P, _P]),
+    "ptw_mlp_set_params": (ctypes.c_int, [_P, _P]),
+    "ptw_mlp_forward": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "ptw_mlp_gradients": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)]),
+    "ptw_mlp_train": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
     "ptw_tuning_get": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -359,13 +371,22 @@ class Context:
         tv = np.zeros(max(n * topk, 1), dtype=np.uint32)
         tc = np.zeros(max(n * topk, 1), dtype=np.uint64)
         steps = np.zeros(max(n, 1), dtype=np.uint64)
+        ent = np.zeros(max(n, 1), dtype=np.float64)
         out = BatchOut()
-        out.topk_vertex, out.topk_count, out.steps = _ptr(tv), _ptr(tc), _ptr(steps)
+        out.topk_vertex, out.topk_count, out.steps, out.entropy = _ptr(tv), _ptr(tc), _ptr(steps), _ptr(ent)
         self.check(self.lib.ptw_batch_ppr(self.h, ctypes.byref(d), ctypes.byref(out)))
         del qs
         return BatchResult(tv[: n * topk].reshape(n, topk), tc[: n * topk].reshape(n, topk), steps[:n],
                            out.total_steps, out.slice_bytes, out.dense_queries, out.hash_queries, out.kernel_ms,
-                           out.total_ms)
+                           out.total_ms, ent[:n])
+
+    def batch_gather(self, query_index, vertices) -> np.ndarray:
+        """Counts of (query, graph id) pairs of the last batch."""
+        q = np.ascontiguousarray(query_index, dtype=np.uint32)
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        out = np.zeros(max(len(q), 1), dtype=np.uint64)
+        self.check(self.lib.ptw_batch_gather(self.h, len(q), _ptr(q), _ptr(v), _ptr(out)))
+        return out[: len(q)]
 
     def batch_plan_bytes(self, sources, walkers, tps, *, topk: int = 100, dense_threshold: int = 0) -> int:
         d, qs = self._batch_desc(sources, walkers, tps, None, topk, 1, SEM_REFERENCE, 0, dense_threshold)
@@ -429,6 +450,67 @@ class Context:
         ms = ctypes.c_double()
         self.check(self.lib.ptw_bench_gather(self.h, table_bytes, gathers, seed, ctypes.byref(gps), ctypes.byref(ms)))
         return gps.value, ms.value
+
+
+class BudgetMLP:
+    """The 6-64-64-1 budget predictor (ptw_mlp_*): BF16 weights, FP32 accumulation,
+    trained on the device."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = _P()
+        ctx.check(ctx.lib.ptw_mlp_create(ctx.h, seed, ctypes.byref(h)))
+        self.h = h
+        self.num_params = ctx.lib.ptw_mlp_num_params()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.ptw_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_normalization(self, mean, std, y_mean: float, y_std: float):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(std, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.ptw_mlp_set_normalization(self.h, _ptr(m), _ptr(s), y_mean, y_std))
+
+    def params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        n = np.zeros(14, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.ptw_mlp_get_params(self.h, _ptr(p), _ptr(n)))
+        return p, n
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert len(p) == self.num_params
+        self.ctx.check(self.ctx.lib.ptw_mlp_set_params(self.h, _ptr(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        y = np.zeros(max(len(x), 1), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.ptw_mlp_forward(self.h, _ptr(x), len(x), _ptr(y)))
+        return y[: len(x)]
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx.lib.ptw_mlp_gradients(self.h, _ptr(x), _ptr(t), len(x), _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, *, epochs: int = 50, batch: int = 256, lr: float = 1e-3, seed: int = 1) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx.lib.ptw_mlp_train(self.h, _ptr(x), _ptr(t), len(x), epochs, batch, lr, seed,
+                                                  ctypes.byref(loss)))
+        return loss.value
 
 
 class DeviceBuffer:
@@ -543,6 +625,7 @@ class BatchResult:
     hash_queries: int
     kernel_ms: float
     total_ms: float
+    entropy: np.ndarray      # (queries,) nats
 
 
 @dataclass

@@ -322,7 +322,6 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
     extern __shared__ unsigned long long cand[];
     const uint32_t nq = (uint32_t)p.hdr[1];
     const uint32_t k = (uint32_t)p.hdr[2];
-    if (k == 0) return;
 
     for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
         Entries E;
@@ -342,18 +341,37 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             E.mode = 2;
             E.n = E.Q.mask + 1;
         }
-        // max count
-        if (threadIdx.x == 0) s_u32[0] = 0;
+        // max count, and sum c and sum c ln c for the entropy of the visit
+        // distribution; c ln c is summed in 2^-20 fixed point so the result
+        // does not depend on the (nondeterministic) order of the entries
+        if (threadIdx.x == 0) {
+            s_u32[0] = 0;
+            s_u64[0] = 0;
+            s_u64[1] = 0;
+        }
         __syncthreads();
         uint32_t local_max = 0;
+        unsigned long long local_sum = 0, local_clogc = 0;
         for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
             uint32_t v, c;
             E.get(i, v, c);
             local_max = c > local_max ? c : local_max;
+            local_sum += c;
+            if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
         }
         atomicMax(&s_u32[0], local_max);
+        atomicAdd(&s_u64[0], local_sum);
+        atomicAdd(&s_u64[1], local_clogc);
         __syncthreads();
         const uint32_t maxc = s_u32[0];
+        if (threadIdx.x == 0) {
+            const double N = (double)s_u64[0];
+            p.entropy[q] = N > 0 ? log(N) - ((double)s_u64[1] / 1048576.0) / N : 0.0;
+        }
+        if (k == 0) {  // entropy only
+            __syncthreads();
+            continue;
+        }
         // narrow [lo, hi] until the candidates fit the tile or one value remains
         uint64_t lo = 1, hi = maxc;
         uint32_t above = 0;  // entries with count > hi
@@ -473,7 +491,6 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             out_c[(uint64_t)q * k + i] = c;
         }
         __syncthreads();
-        (void)s_u64;
     }
 }
 
@@ -494,6 +511,32 @@ __global__ __launch_bounds__(1024) void batch_reset_kernel(BatchLaunch p)
         } else {
             for (uint32_t i = threadIdx.x; i <= Q.mask; i += blockDim.x) p.hash[Q.slice_off + i] = uint2{kEmpty, 0u};
         }
+    }
+}
+
+__global__ void batch_gather_kernel(BatchLaunch p, uint64_t n, const uint32_t *qidx, const uint32_t *vertices,
+                                    const uint32_t *perm, uint64_t *out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const BatchQuery Q = p.queries[qidx[i]];
+        const uint32_t v = perm ? perm[vertices[i]] : vertices[i];
+        uint64_t c = 0;
+        if (Q.kind == SLICE_DENSE) {
+            c = p.dense[Q.slice_off + v];
+        } else {
+            uint32_t h = (v * 0x9E3779B1u) & Q.mask;
+            for (uint32_t probe = 0; probe <= Q.mask; ++probe) {
+                const uint2 s = p.hash[Q.slice_off + h];
+                if (s.x == v) {
+                    c = s.y;
+                    break;
+                }
+                if (s.x == kEmpty) break;
+                h = (h + 1) & Q.mask;
+            }
+        }
+        out[i] = c;
     }
 }
 
@@ -551,6 +594,16 @@ hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t
 hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(num_cus * 2), dim3(1024), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_gather(const BatchLaunch &p, uint64_t n, const uint32_t *qidx, const uint32_t *vertices,
+                               const uint32_t *perm, uint64_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(batch_gather_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, qidx, vertices, perm, out);
     return hipGetLastError();
 }
 

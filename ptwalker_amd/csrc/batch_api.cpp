@@ -126,6 +126,7 @@ BatchLaunch launch_args(ptw_ctx *ctx)
     p.touched_count = b.touched_count;
     p.overflow = b.overflow;
     p.steps = b.steps;
+    p.entropy = b.entropy;
     p.queue = b.queue;
     return p;
 }
@@ -169,11 +170,13 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
         hfree(b.touched_count);
         hfree(b.overflow);
         hfree(b.steps);
+        hfree(b.entropy);
         const uint32_t cap = std::max<uint32_t>(nq, b.q_cap + b.q_cap / 2);
         HIP_TRY(ctx, hipMalloc((void **)&b.queries, (size_t)cap * sizeof(BatchQuery)));
         HIP_TRY(ctx, hipMalloc((void **)&b.touched_count, (size_t)cap * 4));
         HIP_TRY(ctx, hipMalloc((void **)&b.overflow, (size_t)cap * 4));
         HIP_TRY(ctx, hipMalloc((void **)&b.steps, (size_t)cap * 8));
+        HIP_TRY(ctx, hipMalloc((void **)&b.entropy, (size_t)cap * 8));
         b.q_cap = cap;
         moved = true;
     }
@@ -232,6 +235,7 @@ void free_batch(ptw_ctx *ctx)
     hfree(b.touched_count);
     hfree(b.overflow);
     hfree(b.steps);
+    hfree(b.entropy);
     hfree(b.chunks);
     hfree(b.topk_v);
     hfree(b.topk_c);
@@ -348,6 +352,8 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     if (d.num_queries) {
         HIP_TRY(ctx, hipMemcpyAsync(steps.data(), b.steps, d.num_queries * 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(over.data(), b.overflow, d.num_queries * 4, hipMemcpyDeviceToHost, s));
+        if (out->entropy)
+            HIP_TRY(ctx, hipMemcpyAsync(out->entropy, b.entropy, d.num_queries * 8, hipMemcpyDeviceToHost, s));
         if (d.topk) {
             tc.resize((size_t)d.num_queries * d.topk);
             HIP_TRY(ctx, hipMemcpyAsync(out->topk_vertex, b.topk_v, tc.size() * 4, hipMemcpyDeviceToHost, s));
@@ -382,6 +388,36 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     for (uint32_t i = 0; i < d.num_queries; ++i)
         if (over[i])
             return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
+    return PTW_OK;
+}
+
+int ptw_batch_gather(ptw_ctx *ctx, uint64_t n, const uint32_t *query_index, const uint32_t *vertices,
+                     uint64_t *out)
+{
+    if (!ctx || (n && (!query_index || !vertices || !out))) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context");
+    ptw::BatchState &b = ctx->batch;
+    if (!b.dirty) return fail(ctx, PTW_ERR_INVALID_ARG, "no batch results to gather from");
+    for (uint64_t i = 0; i < n; ++i)
+        if (query_index[i] >= b.last_nq || vertices[i] >= ctx->dev.V)
+            return fail(ctx, PTW_ERR_INVALID_ARG, "gather pair " + std::to_string(i) + " out of range");
+    if (n == 0) return PTW_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } tq, tv, to;
+    HIP_TRY(ctx, hipMalloc(&tq.p, n * 4));
+    HIP_TRY(ctx, hipMalloc(&tv.p, n * 4));
+    HIP_TRY(ctx, hipMalloc(&to.p, n * 8));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(tq.p, query_index, n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(tv.p, vertices, n * 4, hipMemcpyHostToDevice, s));
+    const ptw::BatchLaunch p = ptw::launch_args(ctx);
+    HIP_TRY(ctx, ptw::launch_batch_gather(p, n, (const uint32_t *)tq.p, (const uint32_t *)tv.p, ctx->dev.perm,
+                                          (uint64_t *)to.p, s));
+    HIP_TRY(ctx, hipMemcpyAsync(out, to.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
     return PTW_OK;
 }
 

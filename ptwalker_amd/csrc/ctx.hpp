@@ -25,6 +25,7 @@ struct BatchState {
     BatchQuery *queries = nullptr;
     uint32_t *touched_count = nullptr, *overflow = nullptr;
     unsigned long long *steps = nullptr;
+    double *entropy = nullptr;
     uint32_t q_cap = 0;
     BatchChunk *chunks = nullptr;
     uint64_t chunk_cap = 0;

@@ -153,6 +153,7 @@ struct BatchLaunch {
     uint32_t *touched_count; // per query
     uint32_t *overflow;      // per query: hash insertions that found no slot
     unsigned long long *steps;  // per query
+    double *entropy;            // per query, written by the top-k kernel
     unsigned long long *queue;  // [0]: next chunk
 };
 
@@ -166,6 +167,9 @@ hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t
 // Zero what a batch left in its slices (dense: the touched entries or the whole
 // slice on list overflow; hash: every slot), so the pools stay clean.
 hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s);
+// out[i] = count of (graph id) vertices[i] in query qidx[i].
+hipError_t launch_batch_gather(const BatchLaunch &p, uint64_t n, const uint32_t *qidx, const uint32_t *vertices,
+                               const uint32_t *perm, uint64_t *out, hipStream_t s);
 // Query q's full count vector in graph ids.
 hipError_t launch_batch_export(const BatchLaunch &p, uint32_t q, const uint32_t *perm, const uint32_t *inv,
                                uint64_t *out, hipStream_t s);

@@ -90,6 +90,24 @@ def test_batch_graph_replays_match_eager(ctx, g12):
         assert np.array_equal(gb.topk_count, eager_b.topk_count), rep
 
 
+def test_batch_gather_and_entropy(ctx, g12):
+    gi, rp, col = g12
+    src, W, tp = make_queries(gi, 10, seed=3, wmax=300000)
+    r = ctx.batch_ppr(src, W, tp, topk=0, seed=2, dense_threshold=50000)
+    rng = np.random.default_rng(0)
+    qi = rng.integers(0, 10, 5000)
+    vs = rng.integers(0, gi.num_vertices, 5000)
+    got = ctx.batch_gather(qi, vs)
+    refs = [replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=2)["counts"] for i in range(10)]
+    assert np.array_equal(got, np.array([refs[q][v] for q, v in zip(qi, vs)], dtype=np.uint64))
+    for i in range(10):
+        c = refs[i][refs[i] > 0].astype(np.float64)
+        p = c / c.sum()
+        assert abs(r.entropy[i] - float(-(p * np.log(p)).sum())) < 1e-6, i
+    again = ctx.batch_ppr(src, W, tp, topk=0, seed=2, dense_threshold=50000)
+    assert np.array_equal(again.entropy, r.entropy)  # order-independent sums
+
+
 def test_batch_weighted(ctx):
     gi = ctx.rmat(scale=11, edge_factor=16, seed=4)
     ctx.random_weights(seed=3)
