@@ -38,9 +38,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch"],
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
-                         "batch: 4096 PPR queries in one launch on R-MAT scale 24")
+                         "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
+                         "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22")
+    ap.add_argument("--requests", type=int, default=3000)
+    ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
+    ap.add_argument("--audit-rate", type=float, default=0.02)
+    ap.add_argument("--eval", type=int, default=64, help="serve: requests per mode scored against FP64")
     ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--walks-per-vertex", type=int, default=10)
     ap.add_argument("--walk-len", type=int, default=80)
@@ -57,6 +62,8 @@ def main():
         return bench_deepwalk(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import numpy as np
 
@@ -321,6 +328,119 @@ def bench_batch(args):
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def serve_requests(rp, n, rate, seed=1):
+    """Half degree-weighted, half uniform sources; eps uniform over {1e-3, 3e-4};
+    Poisson arrivals at `rate` queries/s."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
+    src = np.concatenate([rng.choice(len(deg), n // 2, p=deg / deg.sum()), rng.integers(0, len(deg), n - n // 2)])
+    rng.shuffle(src)
+    eps = rng.choice([1e-3, 3e-4], n)
+    arr = np.cumsum(rng.exponential(1.0 / rate, n))
+    return src, eps, arr
+
+
+def score_served(ctx, res, src, eps, tp, seed, sample, k=100):
+    """Top-k L1 of the served answers against the FP64 solver: each sampled
+    request's answering walk (main, or its 10x audit when topped up) is rerun
+    alone -- bit-identical to the served one -- and its counts are read at
+    the exact top-k vertices."""
+    import numpy as np
+
+    idx = np.linspace(0, len(src) - 1, sample).astype(np.int64)
+    s = src[idx]
+    w = np.where(res.topped_up[idx], 10 * res.budget[idx], res.budget[idx]).astype(np.int64)
+    streams = np.where(res.topped_up[idx], (1 << 31) + idx, idx)
+    _, tv, tval, _ = ctx.ppr_exact(s, [tp] * len(s), tol=1e-7, full=False, k=k)
+    r = ctx.batch_ppr(s, w, [tp] * len(s), streams=streams, topk=0, seed=seed)
+    cnt = ctx.batch_gather(np.repeat(np.arange(len(s), dtype=np.uint32), k), tv.reshape(-1)).reshape(len(s), k)
+    est = cnt.astype(np.float64) / (w + r.steps.astype(np.float64))[:, None]
+    l1 = np.abs(est - tval / (1.0 + 1.0 / tp)).sum(axis=1)
+    e = eps[idx]
+    return {"requests": int(len(s)), "mean_l1_over_eps": float(np.mean(l1 / e)),
+            "p95_l1_over_eps": float(np.quantile(l1 / e, 0.95)), "frac_within_eps": float(np.mean(l1 <= e)),
+            "max_l1_at_1e-3": float(l1[e == 1e-3].max()) if np.any(e == 1e-3) else None}
+
+
+def bench_serve(args):
+    """Serving: requests (source, eps) arrive by a Poisson process and are served by
+    continuous batching (ptw_serve_replay) with learned budgets (committed policy,
+    pilot walks, MLP on the device) and with the fixed budget fixed_c2 / eps^2 that
+    covers the same share of training queries.  One bench step = one full replay."""
+    import numpy as np
+
+    import ptwalker_amd as ptw
+    from ptwalker_amd.budget import BudgetPolicy
+
+    policy = BudgetPolicy.load(os.path.join(ROOT, "ptwalker_amd", "models", "budget_policy.json"))
+    tp = float(policy.meta["tp"])
+    ctx = ptw.Context(0)
+    t0 = time.time()
+    gi = ctx.rmat(scale=args.scale, edge_factor=16, seed=1)
+    build_s = time.time() - t0
+    rp, _ = ctx.export_csr()
+    net = policy.network(ctx)
+    common = dict(tp=tp, seed=1, topk=100, w_min=policy.w_min, w_max=policy.w_max, audit_rate=args.audit_rate,
+                  max_batch=512, fixed_c2=policy.fixed_c2)
+    modes = {"learned": dict(mlp=net, budget_mode=ptw.BUDGET_LEARNED, margin=policy.margin,
+                             feat_lo=policy.feat_lo, feat_hi=policy.feat_hi),
+             "fixed": dict(budget_mode=ptw.BUDGET_FIXED)}
+    rates = [float(x) for x in args.rates.split(",")]
+    runs = {}
+    for rate in rates:
+        src, eps, arr = serve_requests(rp, args.requests, rate, seed=1)
+        for name, kw in modes.items():
+            for _ in range(args.warmup):
+                ctx.serve(src[:256], eps[:256], np.zeros(256), **common, **kw)
+            res = ctx.serve(src, eps, arr, **common, **kw)
+            st = res.stats
+            rec = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+            rec["mean_budget"] = float(res.budget.mean())
+            if rate == rates[-1] and args.eval:
+                rec["accuracy"] = score_served(ctx, res, src, eps, tp, 1, args.eval)
+            runs[f"{name}@{int(rate)}"] = rec
+            print(f"# {name} at {rate:.0f} qps: {st['queries_per_s']:.0f} q/s, p50 {st['p50_ms']:.1f} ms, "
+                  f"p99 {st['p99_ms']:.1f} ms, {st['launches']} launches", file=sys.stderr, flush=True)
+    top = int(rates[-1])
+    L, F = runs[f"learned@{top}"], runs[f"fixed@{top}"]
+    ms = 1e3 * L["makespan_s"]
+    gather_gps, _ = ctx.bench_gather(int(rp[-1]) * 4 + 4 * len(rp), 1 << 28)
+    line = {
+        "metric": "queries/s",
+        "value": L["queries_per_s"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT graph and request stream generated in-process (seeded); no dataset",
+        "config": {"workload": f"PPR serving, R-MAT scale {args.scale} ef16, {args.requests} requests, Poisson "
+                               f"arrivals at {args.rates} q/s, eps U{{1e-3, 3e-4}}, top-100, tp {tp}, "
+                               f"audit rate {args.audit_rate}, continuous batching (max 512 per launch)",
+                   "vertices": gi.num_vertices, "edges": gi.num_edges, "graph_build_s": round(build_s, 1),
+                   "policy": {"margin": policy.margin, "fixed_c2": policy.fixed_c2,
+                              "val_rmse_log": policy.meta.get("val_rmse_log")}},
+        "learned_over_fixed_queries_per_s": L["queries_per_s"] / F["queries_per_s"],
+        "runs": runs,
+        "walk_steps_per_s": L["walk_steps"] / L["makespan_s"],
+        "gather_frac": L["walk_steps"] / L["makespan_s"] / gather_gps,
+        "roofline": {"bound": "hbm", "kernel": "serving loop (batch_walk_kernel + top-k per launch)",
+                     "achieved": 20.0 * L["walk_steps"] / L["makespan_s"] / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": 20.0 * L["walk_steps"] / L["makespan_s"] / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_step": 20},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    net.close()
     ctx.close()
 
 

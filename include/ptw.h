@@ -205,7 +205,8 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
 typedef struct ptw_query {
     uint32_t source;  /* graph id */
     uint32_t stream;  /* 4th Philox counter word of this query */
-    uint64_t walkers; /* at most 2^32 / (4 (1 + 1/tp)) so u32 slices cannot wrap */
+    uint64_t walkers; /* at most 2^32 / (1.5 (1 + 1/tp)); a query whose visit total
+                         reaches 2^32 (u32 counters may have wrapped) fails with PTW_ERR_OOM */
     double tp;
 } ptw_query;
 
@@ -287,6 +288,69 @@ int ptw_mlp_gradients(ptw_mlp *mlp, const float *x, const float *t, uint32_t n, 
 /* Adam (beta 0.9 / 0.999) over shuffled mini-batches of `batch` rows. */
 int ptw_mlp_train(ptw_mlp *mlp, const float *x, const float *t, uint32_t n, uint32_t epochs, uint32_t batch,
                   float lr, uint64_t seed, float *final_loss);
+
+/* ---- serving ---------------------------------------------------------- */
+/* A request loop over the resident graph: requests (source, target eps)
+ * arrive at given times and are served by continuous batching -- every launch
+ * carries the pilot walks of the requests that arrived since the previous
+ * launch and the main walks of requests whose budgets are known, admitted
+ * while their counter slices fit the HBM allowance; the rest wait for the
+ * next launch.  A request with id r walks Philox streams r (main),
+ * r + 2^30 (pilot) and r + 2^31 (audit), so its answer does not depend on
+ * what it was batched with.  No counterpart in the reference (one query per
+ * process). */
+#define PTW_BUDGET_FIXED 0   /* budget = fixed_c2 / eps^2 for every request */
+#define PTW_BUDGET_LEARNED 1 /* budget = margin * exp(mlp(features)); fixed on out-of-range features */
+
+typedef struct ptw_request {
+    uint32_t source;  /* graph id */
+    uint32_t id;      /* < 2^30; selects the request's Philox streams and audit draw */
+    float eps;        /* target top-k L1 error */
+    uint32_t reserved;
+    double arrival_s; /* seconds after the call starts */
+} ptw_request;
+
+typedef struct ptw_serve_config {
+    double tp;
+    uint64_t seed;
+    int32_t budget_mode;   /* PTW_BUDGET_* */
+    uint32_t topk;         /* results per request, <= 1024 */
+    double fixed_c2;       /* fixed budget numerator (also the fallback) */
+    double margin;         /* learned budget safety factor */
+    uint64_t w_min, w_max; /* budget clip */
+    float feat_lo[6];      /* learned mode: a feature outside [lo, hi] selects the fixed budget */
+    float feat_hi[6];
+    double audit_rate;     /* fraction of requests re-walked at 10x budget; a short one is replaced by it */
+    uint32_t max_batch;    /* queries per launch (pilots + mains + audits) */
+    uint32_t reserved;
+    uint64_t hbm_limit;    /* bytes of counter slices a launch may hold; 0 = the device's free HBM */
+} ptw_serve_config;
+
+typedef struct ptw_request_result {
+    uint64_t budget;   /* walkers of the main walk */
+    double admit_s;    /* launch start of the main walk (s since the loop started) */
+    double done_s;     /* completion (s since the loop started) */
+    uint32_t fallback; /* 1 if the fixed budget was used in learned mode */
+    uint32_t audited;
+    uint32_t topped_up;
+    uint32_t launches; /* launches the request took part in */
+} ptw_request_result;
+
+typedef struct ptw_serve_stats {
+    double makespan_s;
+    double queries_per_s;
+    double p50_ms, p99_ms, mean_ms; /* latency: done - arrival */
+    uint64_t walk_steps;
+    uint32_t launches, fallbacks, audited, topped_up;
+    uint32_t admission_waits; /* launches that left a ready request waiting for HBM */
+    uint32_t reserved;
+} ptw_serve_stats;
+
+/* Serves n requests in real time (arrival_s relative to the call); mlp may be
+ * NULL in fixed mode.  topk_vertex / topk_count: host [n * topk]. */
+int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, const ptw_request *reqs, uint32_t n,
+                     ptw_request_result *results, uint32_t *topk_vertex, uint64_t *topk_count,
+                     ptw_serve_stats *stats);
 
 /* ---- launch tuning ---------------------------------------------------- */
 /* Launch shape of the walk kernels.  The reference's counterparts are its

@@ -176,6 +176,71 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+BUDGET_FIXED = 0
+BUDGET_LEARNED = 1
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("source", ctypes.c_uint32),
+        ("id", ctypes.c_uint32),
+        ("eps", ctypes.c_float),
+        ("reserved", ctypes.c_uint32),
+        ("arrival_s", ctypes.c_double),
+    ]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("tp", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("budget_mode", ctypes.c_int32),
+        ("topk", ctypes.c_uint32),
+        ("fixed_c2", ctypes.c_double),
+        ("margin", ctypes.c_double),
+        ("w_min", ctypes.c_uint64),
+        ("w_max", ctypes.c_uint64),
+        ("feat_lo", ctypes.c_float * 6),
+        ("feat_hi", ctypes.c_float * 6),
+        ("audit_rate", ctypes.c_double),
+        ("max_batch", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("hbm_limit", ctypes.c_uint64),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("budget", ctypes.c_uint64),
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("fallback", ctypes.c_uint32),
+        ("audited", ctypes.c_uint32),
+        ("topped_up", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("makespan_s", ctypes.c_double),
+        ("queries_per_s", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("walk_steps", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+        ("fallbacks", ctypes.c_uint32),
+        ("audited", ctypes.c_uint32),
+        ("topped_up", ctypes.c_uint32),
+        ("admission_waits", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 # every symbol include/ptw.h declares, with its ctypes signature
 _P = ctypes.c_void_p
 _SIGNATURES = {
@@ -210,6 +275,8 @@ _SIGNATURES = {
     "ptw_mlp_gradients": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)]),
     "ptw_mlp_train": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
+    "ptw_serve_replay": (ctypes.c_int, [_P, _P, ctypes.POINTER(ServeConfig), _P, ctypes.c_uint32, _P, _P, _P,
+                                        ctypes.POINTER(ServeStats)]),
     "ptw_tuning_get": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "ptw_buffer_alloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -416,6 +483,38 @@ class Context:
         self.check(self.lib.ptw_ppr_exact(self.h, _ptr(src), _ptr(tp), n, semantics, tol, _ptr(out), k, _ptr(tv),
                                           _ptr(tval), ctypes.byref(it)))
         return out, tv, tval, it.value
+
+    # ---- serving -----------------------------------------------------------
+    def serve(self, sources, eps, arrival_s, *, ids=None, mlp: Optional["BudgetMLP"] = None, tp: float = 0.2,
+              seed: int = 1, budget_mode: int = BUDGET_FIXED, topk: int = 100, fixed_c2: float = 2.0,
+              margin: float = 1.0, w_min: int = 1000, w_max: int = 10 ** 7, feat_lo=None, feat_hi=None,
+              audit_rate: float = 0.0, max_batch: int = 512, hbm_limit: int = 0) -> "ServeResult":
+        """Serve requests arriving at `arrival_s` (seconds after the call) by continuous
+        batching (ptw_serve_replay).  Request i has id `ids[i]` (default i)."""
+        n = len(sources)
+        reqs = (Request * max(n, 1))()
+        ids = range(n) if ids is None else ids
+        for i, (s_, e_, a_, id_) in enumerate(zip(sources, eps, arrival_s, ids)):
+            reqs[i].source, reqs[i].eps, reqs[i].arrival_s, reqs[i].id = int(s_), float(e_), float(a_), int(id_)
+        c = ServeConfig()
+        c.tp, c.seed, c.budget_mode, c.topk = tp, seed, budget_mode, topk
+        c.fixed_c2, c.margin, c.w_min, c.w_max = fixed_c2, margin, w_min, w_max
+        lo = [-3.0e38] * 6 if feat_lo is None else feat_lo
+        hi = [3.0e38] * 6 if feat_hi is None else feat_hi
+        for k in range(6):
+            c.feat_lo[k], c.feat_hi[k] = float(lo[k]), float(hi[k])
+        c.audit_rate, c.max_batch, c.hbm_limit = audit_rate, max_batch, hbm_limit
+        res = (RequestResult * max(n, 1))()
+        tv = np.zeros(max(n * topk, 1), dtype=np.uint32)
+        tc = np.zeros(max(n * topk, 1), dtype=np.uint64)
+        st = ServeStats()
+        self.check(self.lib.ptw_serve_replay(self.h, mlp.h if mlp is not None else None, ctypes.byref(c), reqs, n,
+                                             res, _ptr(tv), _ptr(tc), ctypes.byref(st)))
+        field = lambda f, dt: np.array([getattr(res[i], f) for i in range(n)], dtype=dt)
+        return ServeResult(tv[: n * topk].reshape(n, topk), tc[: n * topk].reshape(n, topk),
+                           field("budget", np.uint64), field("admit_s", np.float64), field("done_s", np.float64),
+                           field("fallback", bool), field("audited", bool), field("topped_up", bool),
+                           field("launches", np.uint32), st.as_dict())
 
     # ---- launch tuning ---------------------------------------------------
     def tuning(self) -> Tuning:
@@ -626,6 +725,20 @@ class BatchResult:
     kernel_ms: float
     total_ms: float
     entropy: np.ndarray      # (queries,) nats
+
+
+@dataclass
+class ServeResult:
+    topk_vertex: np.ndarray  # (requests, k) graph ids
+    topk_count: np.ndarray   # (requests, k) visit counts of the answering walk
+    budget: np.ndarray       # walkers of each main walk
+    admit_s: np.ndarray
+    done_s: np.ndarray
+    fallback: np.ndarray
+    audited: np.ndarray
+    topped_up: np.ndarray
+    launches: np.ndarray
+    stats: dict
 
 
 @dataclass

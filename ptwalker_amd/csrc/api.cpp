@@ -28,9 +28,10 @@ void free_graph(ptw_ctx *ctx)
     if (d.perm) (void)hipFree(d.perm);
     if (d.inv) (void)hipFree(d.inv);
     if (d.alias) (void)hipFree(d.alias);
-    if (d.t_row) (void)hipFree(d.t_row);
     if (d.t_col) (void)hipFree(d.t_col);
     if (d.out_deg) (void)hipFree(d.out_deg);
+    if (d.piece_begin) (void)hipFree(d.piece_begin);
+    if (d.vertex_piece) (void)hipFree(d.vertex_piece);
     if (ctx->counts_dev) (void)hipFree(ctx->counts_dev);
     d = ptw::DeviceGraph{};
     ctx->counts_dev = nullptr;

@@ -62,7 +62,9 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
             return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": tp must be in (0, 1]");
         if (qq.walkers == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": no walkers");
         const double per_walker = 1.0 + 1.0 / qq.tp;  // expected visits per walker (upper: reference semantics)
-        if ((double)qq.walkers * per_walker * 4.0 > 4294967296.0)
+        // No u32 counter can pass the query's visit total; plan for 1.5x the
+        // expected total here and check the actual total after the walk.
+        if ((double)qq.walkers * per_walker * 1.5 > 4294967296.0)
             return fail(ctx, PTW_ERR_INVALID_ARG,
                         "query " + std::to_string(i) + ": too many walkers for u32 slices; use ptw_walk");
         BatchQuery &B = P.q[i];
@@ -385,9 +387,13 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     }
     out->slice_bytes = bytes;
     out->total_ms = ptw::now_ms() - t0;
-    for (uint32_t i = 0; i < d.num_queries; ++i)
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
         if (over[i])
             return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
+        if (d.queries[i].walkers + steps[i] >= 4294967296ull)
+            return fail(ctx, PTW_ERR_OOM,
+                        "query " + std::to_string(i) + ": visit total reached 2^32, u32 counters may have wrapped");
+    }
     return PTW_OK;
 }
 

@@ -29,8 +29,10 @@ struct DeviceGraph {
     uint32_t *perm = nullptr;     // graph id -> device id (nullptr: identity)
     uint32_t *inv = nullptr;      // device id -> graph id (nullptr: identity)
     uint2 *alias = nullptr;       // E, {threshold, local alias index}, device order
-    // transpose (in-edges) and out-degrees for the FP64 solver, built on first use
-    uint32_t *t_row = nullptr, *t_col = nullptr, *out_deg = nullptr;
+    // transpose (in-edges, cut into pieces) and out-degrees for the FP64
+    // solver, built on first use
+    uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
+    uint64_t pieces = 0;
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
     bool relabelled = false;
     uint64_t bytes = 0;
@@ -175,10 +177,19 @@ hipError_t launch_batch_export(const BatchLaunch &p, uint32_t q, const uint32_t 
                                uint64_t *out, hipStream_t s);
 
 // ---- FP64 power iteration (exact.hip) --------------------------------------
+// Transpose CSR cut into pieces of at most kExactPiece in-edges.
+constexpr uint32_t kExactPiece = 512;
+struct ExactGraph {
+    uint32_t V;
+    const uint32_t *t_col;         // in-neighbours, grouped by destination
+    const uint32_t *out_deg;       // out-degree of every vertex
+    const uint32_t *piece_begin;   // pieces + 1 edge offsets into t_col
+    const uint32_t *vertex_piece;  // V + 1 piece offsets per destination
+    uint64_t pieces;
+};
 hipError_t launch_exact_init(uint32_t V, uint32_t B, const uint32_t *src_dev, double *y, double *x, hipStream_t s);
-hipError_t launch_exact_step(const uint32_t *t_row, const uint32_t *t_col, const uint32_t *out_deg, uint32_t V,
-                             uint32_t B, const double *y, double *y_next, double *x, const double *weight,
-                             int num_cus, hipStream_t s);
+hipError_t launch_exact_step(const ExactGraph &g, uint32_t B, const double *y, double *y_next, double *x,
+                             double *partial, const double *weight, int num_cus, hipStream_t s);
 hipError_t launch_exact_export(uint32_t V, uint32_t B, const uint32_t *perm, const double *x, double *out,
                                hipStream_t s);
 

@@ -17,11 +17,12 @@ namespace ptw {
 
 namespace {
 
-// Transpose of the device-id CSR plus out-degrees, built once per graph.
+// Transpose of the device-id CSR, cut into pieces of at most kExactPiece
+// in-edges, plus out-degrees; built once per graph.
 int ensure_transpose(ptw_ctx *ctx)
 {
     DeviceGraph &d = ctx->dev;
-    if (d.t_row) return PTW_OK;
+    if (d.t_col) return PTW_OK;
     const uint32_t V = d.V;
     std::vector<uint32_t> row(V + 1), col;
     // device-id CSR on the host
@@ -41,10 +42,19 @@ int ensure_transpose(ptw_ctx *ctx)
         const uint32_t u = dev_of(g);
         for (uint32_t k = h.row_ptr[g]; k < h.row_ptr[g + 1]; ++k) col[fill[dev_of(h.col[k])]++] = u;
     }
+    std::vector<uint32_t> piece_begin, vertex_piece(V + 1);
+    for (uint32_t v = 0; v < V; ++v) {
+        vertex_piece[v] = (uint32_t)piece_begin.size();
+        for (uint32_t b = row[v]; b < row[v + 1]; b += kExactPiece) piece_begin.push_back(b);
+    }
+    vertex_piece[V] = (uint32_t)piece_begin.size();
+    d.pieces = piece_begin.size();
+    piece_begin.push_back(row[V]);
     int rc;
-    if ((rc = upload(ctx, &d.t_row, row.data(), row.size()))) return rc;
     if ((rc = upload(ctx, &d.t_col, col.data(), col.size()))) return rc;
     if ((rc = upload(ctx, &d.out_deg, outdeg.data(), outdeg.size()))) return rc;
+    if ((rc = upload(ctx, &d.piece_begin, piece_begin.data(), piece_begin.size()))) return rc;
+    if ((rc = upload(ctx, &d.vertex_piece, vertex_piece.data(), vertex_piece.size()))) return rc;
     return PTW_OK;
 }
 
@@ -82,9 +92,12 @@ int ptw_ppr_exact(ptw_ctx *ctx, const uint32_t *sources, const double *tps, uint
     struct Tmp {
         void *p = nullptr;
         ~Tmp() { if (p) (void)hipFree(p); }
-    } ty, tz, tx, tw, ts, to;
+    } ty, tz, tx, tw, ts, to, tp_;
     const uint32_t Bcap = std::min(n, BMAX);
     if (n == 0) return PTW_OK;
+    const ptw::ExactGraph eg{V, ctx->dev.t_col, ctx->dev.out_deg, ctx->dev.piece_begin, ctx->dev.vertex_piece,
+                             ctx->dev.pieces};
+    HIP_TRY(ctx, hipMalloc(&tp_.p, (size_t)std::max<uint64_t>(ctx->dev.pieces, 1) * Bcap * 8));
     HIP_TRY(ctx, hipMalloc(&ty.p, (size_t)V * Bcap * 8));
     HIP_TRY(ctx, hipMalloc(&tz.p, (size_t)V * Bcap * 8));
     HIP_TRY(ctx, hipMalloc(&tx.p, (size_t)V * Bcap * 8));
@@ -121,8 +134,8 @@ int ptw_ppr_exact(ptw_ctx *ctx, const uint32_t *sources, const double *tps, uint
         double *y = (double *)ty.p, *z = (double *)tz.p, *x = (double *)tx.p;
         HIP_TRY(ctx, ptw::launch_exact_init(V, B, (const uint32_t *)ts.p, y, x, s));
         for (uint32_t i = 0; i < iters; ++i) {
-            HIP_TRY(ctx, ptw::launch_exact_step(ctx->dev.t_row, ctx->dev.t_col, ctx->dev.out_deg, V, B, y, z, x,
-                                                (const double *)tw.p + (size_t)i * B, ctx->num_cus, s));
+            HIP_TRY(ctx, ptw::launch_exact_step(eg, B, y, z, x, (double *)tp_.p, (const double *)tw.p + (size_t)i * B,
+                                                ctx->num_cus, s));
             std::swap(y, z);
         }
         HIP_TRY(ctx, ptw::launch_exact_export(V, B, ctx->dev.perm, x, (double *)to.p, s));

@@ -1,0 +1,52 @@
+"""What the trained budget predictor gains over the method without it.
+
+The committed policy (network trained on the device on engine-generated
+pairs from two R-MAT scale-20 graphs, seeds 101 and 102) is scored on a graph
+it never saw (R-MAT scale 20, seed 7): for 600 sources and eps in {1e-3,
+3e-4}, the required budget (c_s / eps)^2 comes from 8 independent
+16384-walker sub-walks per source scored against the device FP64 solver.
+The learned budgets must cover the requirement about as often as the fixed
+budget does, with less total walk work."""
+import os
+
+import numpy as np
+import pytest
+
+import ptwalker_amd as ptw
+from ptwalker_amd.budget import BudgetPolicy, error_constants, features, labels_from_constants
+
+pytestmark = pytest.mark.gpu
+
+POLICY = os.path.join(os.path.dirname(ptw.__file__), "models", "budget_policy.json")
+
+
+def test_learned_budget_beats_fixed_on_unseen_graph(ctx):
+    policy = BudgetPolicy.load(POLICY)
+    tp = policy.meta["tp"]
+    ctx.rmat(scale=20, edge_factor=16, seed=7)
+    rp, col = ctx.export_csr()
+    rng = np.random.default_rng(3)
+    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
+    src = np.concatenate([rng.choice(len(deg), 300, p=deg / deg.sum()), rng.integers(0, len(deg), 300)])
+    f = features(ctx, rp, col, src, tp, 1e-3, stream_base=(1 << 30) + 7 * (1 << 20))
+    c, _ = error_constants(ctx, src, tp, seed=7)
+    net = policy.network(ctx)
+    learned, fixed, need, fallback = [], [], [], []
+    for eps in (1e-3, 3e-4):
+        x = f.copy()
+        x[:, 5] = np.log(eps)
+        w, ood = policy.budgets(net, x)
+        learned.append(w)
+        fallback.append(ood)
+        fixed.append(np.clip(policy.fixed_budget(np.full(len(x), eps)), policy.w_min, policy.w_max))
+        need.append(np.exp(labels_from_constants(c, eps)))
+    net.close()
+    learned, fixed, need, fallback = map(np.concatenate, (learned, fixed, need, fallback))
+    cov_l = float(np.mean(learned >= need))
+    cov_f = float(np.mean(fixed >= need))
+    ratio = float(fixed.sum() / learned.sum())
+    print(f"coverage learned {cov_l:.3f} fixed {cov_f:.3f}; fixed/learned walk work {ratio:.2f}; "
+          f"fallbacks {fallback.mean():.3f}")
+    assert fallback.mean() < 0.05
+    assert cov_l >= 0.90 and cov_l >= cov_f - 0.03
+    assert ratio > 1.25
