@@ -14,8 +14,11 @@
 //   hash slice:  open addressing over {u32 key, u32 count}, capacity a power
 //                of two sized from the query's expected visits
 // A query's visits to its own source are counted in a register and flushed
-// when the lane moves to another query; other visits go through a
-// per-wavefront LDS cache of (query, vertex) counts.
+// when the lane moves to another query.  Each wavefront keeps direct LDS
+// counters for the kWaveHubs highest-degree vertices (device ids below
+// kWaveHubs after the degree relabel) of the query of its current chunk, and
+// a small LDS hash cache of (query, vertex) counts for the rest; both are
+// flushed to the slices when the wavefront moves to a chunk of another query.
 //
 // Top-k: one workgroup per query narrows the value range that holds the
 // k-th count with LDS histograms, gathers the candidates into LDS, and
@@ -35,6 +38,7 @@ constexpr uint32_t kBins = 2048;
 
 constexpr unsigned long long kEmpty64 = ~0ull;
 constexpr uint32_t kWaveSlots = 256;  // per-wavefront LDS cache slots
+constexpr uint32_t kWaveHubs = 1024;  // per-wavefront direct hub counters
 constexpr int kCacheProbes = 8;
 constexpr uint32_t kSeg = 65536;  // dense-slice compaction segment
 
@@ -121,12 +125,14 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     extern __shared__ unsigned long long wave_keys[];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x / 64, waves = blockDim.x / 64;
-    const WaveCache C{wave_keys + (size_t)wave * kWaveSlots,
-                      reinterpret_cast<uint32_t *>(wave_keys + (size_t)waves * kWaveSlots) + wave * kWaveSlots};
+    uint32_t *const wave_cnt = reinterpret_cast<uint32_t *>(wave_keys + (size_t)waves * kWaveSlots);
+    const WaveCache C{wave_keys + (size_t)wave * kWaveSlots, wave_cnt + wave * kWaveSlots};
+    uint32_t *const hub = wave_cnt + (size_t)waves * kWaveSlots + (size_t)wave * kWaveHubs;
     for (uint32_t s = lane; s < kWaveSlots; s += 64) {
         C.keys[s] = kEmpty64;
         C.cnt[s] = 0;
     }
+    for (uint32_t s = lane; s < kWaveHubs; s += 64) hub[s] = 0;
     const uint64_t num_chunks = p.hdr[0];
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
@@ -141,6 +147,17 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     uint32_t my_src = 0;
     uint64_t my_steps = 0;
 
+    // hub counters belong to cq, the query of the wavefront's current chunk
+    auto flush_hubs = [&]() {
+        if (cq == kEmpty) return;
+        const BatchQuery bq = p.queries[cq];
+        for (uint32_t s = lane; s < kWaveHubs; s += 64) {
+            if (hub[s]) {
+                slice_add(p, cq, bq.kind, bq.slice_off, bq.mask, s, hub[s]);
+                hub[s] = 0;
+            }
+        }
+    };
     auto flush = [&]() {
         if (Q.q == kEmpty) return;
         if (my_src) slice_add(p, Q.q, Q.kind, Q.off, Q.mask, Q.src, my_src);
@@ -161,7 +178,10 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                     break;
                 }
                 const BatchChunk ch = p.chunks[ci];
-                if (ch.q != cq && cq != kEmpty) C.flush(p, lane);  // wave-uniform
+                if (ch.q != cq && cq != kEmpty) {  // wave-uniform
+                    C.flush(p, lane);
+                    flush_hubs();
+                }
                 cq = ch.q;
                 c_next = ch.start;
                 c_end = ch.start + ch.len;
@@ -212,6 +232,8 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                     ++step;
                     if (v == Q.src)
                         ++my_src;
+                    else if (v < kWaveHubs && Q.q == cq)
+                        atomicAdd(&hub[v], 1u);
                     else if (!C.add(Q.q, v))
                         slice_add(p, Q.q, Q.kind, Q.off, Q.mask, v, 1u);
                 }
@@ -224,6 +246,7 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     }
     flush();
     C.flush(p, lane);
+    flush_hubs();
 }
 
 // After the walk: the nonzero counters of each dense slice, compacted into
@@ -572,7 +595,7 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, hipStream_t s)
 {
-    const size_t lds = (size_t)(block / 64) * kWaveSlots * 12;
+    const size_t lds = (size_t)(block / 64) * (kWaveSlots * 12 + kWaveHubs * 4);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;

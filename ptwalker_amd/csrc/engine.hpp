@@ -105,7 +105,7 @@ struct LaunchShape {
     int chunk = 4096;        // walker ids a wave takes from the queue at once
     int lds_hubs = 8192;     // u32 LDS counters for device ids below this (count apps)
     int cache_slots = 4096;  // LDS visit-cache slots per workgroup (power of two, 0 = off)
-    int batch_chunk = 1024;  // walker ids per chunk of a batched launch
+    int batch_chunk = 16384;  // walker ids per chunk of a batched launch
     int topk_tile = 2048;    // top-k LDS candidate slots
 };
 
