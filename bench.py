@@ -42,6 +42,7 @@ def parse():
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
                          "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22")
+    ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
     ap.add_argument("--audit-rate", type=float, default=0.02)
@@ -288,17 +289,33 @@ def bench_batch(args):
     build_s = time.time() - t0
     rp, _ = ctx.export_csr()
     src, W, tp = batch_queries(gi, rp, args.queries, seed=1)
-    res = {}
-    for mode in ("eager", "graph"):
-        for _ in range(args.warmup):
-            ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"))
-        t1 = time.perf_counter()
-        ks = []
-        for _ in range(args.steps):
-            r = ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"))
-            ks.append(r.kernel_ms)
-        el = (time.perf_counter() - t1) / args.steps
-        res[mode] = (el, float(np.mean(ks)), r)
+
+    def measure(dense_threshold=0):
+        res = {}
+        for mode in ("eager", "graph"):
+            for _ in range(args.warmup):
+                ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"), dense_threshold=dense_threshold)
+            t1 = time.perf_counter()
+            ks = []
+            for _ in range(args.steps):
+                r = ctx.batch_ppr(src, W, tp, topk=100, seed=1, graph=(mode == "graph"),
+                                  dense_threshold=dense_threshold)
+                ks.append(r.kernel_ms)
+            el = (time.perf_counter() - t1) / args.steps
+            res[mode] = (el, float(np.mean(ks)), r)
+        return res
+
+    ctx_default_shape = ctx.tuning().as_dict()
+    untuned = measure()
+    tuned_cfg = None
+    res = untuned
+    if not args.no_tune:
+        from ptwalker_amd import tune
+
+        t0 = time.time()
+        tr = tune.tune_batch(ctx, src, W, tp, topk=100, repeats=1, rounds=1, cache_path=None)
+        tuned_cfg = {"choice": tr.best, "tuning_s": round(time.time() - t0, 1), "candidates": len(tr.trials)}
+        res = measure(dense_threshold=tr.best.get("dense_threshold", 0))
     el, kms, r = res["graph"]
     line = {
         "metric": "walk-steps/s",
@@ -321,6 +338,9 @@ def bench_batch(args):
         "queries_per_s": args.queries / el,
         "kernel_ms": kms,
         "eager": {"ms_per_batch": 1e3 * res["eager"][0], "kernel_ms": res["eager"][1]},
+        "autotune": tuned_cfg,
+        "untuned": {"steps_per_s": untuned["graph"][2].total_steps / untuned["graph"][0],
+                    "kernel_ms": untuned["graph"][1], "shape": ctx_default_shape},
         "roofline": {"bound": "hbm", "kernel": "batch_walk_kernel + batch_topk_kernel",
                      "achieved": 20.0 * r.total_steps / (kms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": 20.0 * r.total_steps / (kms * 1e-3) / 1e9 / 8000.0, "traffic": None,
