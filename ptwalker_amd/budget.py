@@ -100,6 +100,16 @@ def error_constants(ctx: Context, sources, tp: float, *, sub_walks: int = 8, sub
     return c, l1_all
 
 
+def hoeffding_c2(k: int = TOP, delta: float = 0.05) -> float:
+    """Numerator of the distribution-free fixed budget W = c2 / eps^2: the L1
+    concentration bound for an empirical distribution over m = k + 1 cells
+    (the top-k vertices and the rest) from n independent samples,
+    P(|p_hat - p|_1 >= eps) <= (2^m - 2) exp(-n eps^2 / 2)  (Weissman et al.,
+    HP Labs tech. report HPL-2003-97), solved for n at failure probability
+    delta with one sample per walker."""
+    return 2.0 * ((k + 1) * math.log(2.0) + math.log(1.0 / delta))
+
+
 def labels_from_constants(c, eps) -> np.ndarray:
     """log W with W = (c / eps)^2."""
     return 2.0 * (np.log(c) - np.log(eps))

@@ -89,8 +89,10 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
     if (C.budget_mode == PTW_BUDGET_LEARNED && C.topk < 10)
         return fail(ctx, PTW_ERR_INVALID_ARG, "learned mode needs topk >= 10 (pilot top-10 mass)");
     if (!(C.audit_rate >= 0.0 && C.audit_rate <= 1.0)) return fail(ctx, PTW_ERR_INVALID_ARG, "audit_rate must be in [0, 1]");
-    // largest walk (an audit of the largest budget) must fit the u32 counter slices
-    if (10.0 * (double)C.w_max * (1.0 + 1.0 / C.tp) * 1.5 > 4294967296.0)
+    // the largest walk (the largest budget, or its 10x audit when audits are on)
+    // must fit the u32 counter slices
+    const double largest = (C.audit_rate > 0.0 ? 10.0 : 1.0) * (double)C.w_max;
+    if (largest * (1.0 + 1.0 / C.tp) * 1.5 > 4294967296.0)
         return fail(ctx, PTW_ERR_INVALID_ARG, "w_max too large for u32 counter slices at this tp");
     const uint32_t V = ctx->dev.V, K = C.topk;
     for (uint32_t i = 0; i < n; ++i) {

@@ -2,7 +2,7 @@
 itself, train the budget predictor on the device, calibrate and save the
 policy.
 
-    python -m ptwalker_amd.train_budget --scale 20 --graphs 2 --sources 10000 \
+    python -m ptwalker_amd.train_budget --scale 20 --graphs 5 --sources 20000 \
         --out ptwalker_amd/models/budget_policy.json
 """
 from __future__ import annotations
@@ -51,8 +51,8 @@ def generate(ctx: Context, scale: int, graph_seed: int, n_sources: int, tp: floa
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--graphs", type=int, default=2)
-    ap.add_argument("--sources", type=int, default=10000, help="sources per graph (2 labels each)")
+    ap.add_argument("--graphs", type=int, default=5)
+    ap.add_argument("--sources", type=int, default=20000, help="sources per graph (2 labels each)")
     ap.add_argument("--tp", type=float, default=0.2)
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "models", "budget_policy.json"))
