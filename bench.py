@@ -38,10 +38,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve"],
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
-                         "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22")
+                         "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22; "
+                         "cli: the reference's pagerank command on R-MAT scale 16, binary vs CPU path")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
@@ -67,6 +68,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "cli":
+        return bench_cli(args)
 
     import numpy as np
 
@@ -493,6 +496,90 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
     net.close()
     ctx.close()
+
+
+def bench_cli(args):
+    """The reference's own command on its CPU-sized case: R-MAT scale 16 (ef 16,
+    symmetrized, seed 1) written as a text edge list, then
+    `pagerank -f text -g G -w 655360 --sp 0.15 -s 1 -t 20` -- each run a fresh
+    process of the GPU binary, timed over the reference's printed scope (walker
+    init + walk, graph load excluded).  The CPU baseline is the reference
+    process (xorshift*, quantized geometric, OpenMP) on the same graph, at the
+    command's 20 threads and at all usable cores.  This process never touches
+    the GPU itself: the graph is generated by a host-only context."""
+    import statistics
+    import subprocess
+    import tempfile
+
+    import oracle
+    import ptwalker_amd as ptw
+
+    h = ptw.Context(-1)
+    gi = h.rmat(scale=16, edge_factor=16, seed=1)
+    rp, col = h.export_csr()
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "rmat16.txt")
+    h.write(path, ptw.FORMAT_TEXT)
+    h.close()
+    binp = os.path.join(ROOT, "ptwalker_amd", "bin", "pagerank")
+    cmd = [binp, "-f", "text", "-g", path, "-w", "655360", "--sp", "0.15", "-s", "1", "-t", "20", "--json"]
+    recs = []
+    for i in range(args.warmup + args.steps):
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        if out.returncode != 0:
+            print(out.stdout + out.stderr, file=sys.stderr)
+            sys.exit(out.returncode)
+        rec = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1])
+        if i >= args.warmup:
+            recs.append(rec)
+    walk_s = statistics.median(r["walk_s"] for r in recs)
+    kernel_ms = statistics.median(r["kernel_ms"] for r in recs)
+    steps = recs[0]["steps"]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    cpu = {}
+    for th in sorted({20, cores}):
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            r = oracle.ref_walk(rp, col, oracle.APP_PAGERANK, 655360, tp=0.15, start_uniform=True, seed=1, threads=th)
+            ts.append(time.perf_counter() - t)
+        cpu[th] = (r["steps"] / statistics.median(ts), statistics.median(ts), r["steps"])
+    best_th = max(cpu, key=lambda k: cpu[k][0])
+    line = {
+        "metric": "walk-steps/s",
+        "value": steps / walk_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * walk_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic R-MAT graph generated in-process (seeded) and written as a text edge list; no dataset",
+        "config": {"workload": "pagerank -f text -g rmat16.txt -w 655360 --sp 0.15 -s 1 -t 20 (R-MAT scale 16 ef16 "
+                               "symmetrized, seed 1), one process per run, reference timing scope",
+                   "vertices": gi.num_vertices, "edges": gi.num_edges, "walk_steps": steps},
+        "kernel_ms": kernel_ms,
+        "kernel_steps_per_s": steps / (kernel_ms * 1e-3),
+        "walk_s_runs": [r["walk_s"] for r in recs],
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel<count>", "achieved": 20.0 * steps / (kernel_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": 20.0 * steps / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_step": 20, "note": "the graph (4 MB CSR) is cache-resident; launch-bound"},
+        "cpu_baseline": {"value": cpu[best_th][0], "unit": "steps/s", "cores": best_th, "kind": "port",
+                         "sample": f"the whole command: 655360 walkers, {cpu[best_th][2]} steps, "
+                                   f"{cpu[best_th][1] * 1e3:.1f} ms (median of 3), reference process "
+                                   f"(xorshift*, quantized geometric), {best_th} threads",
+                         "by_threads": {str(k): v[0] for k, v in cpu.items()}},
+        "vs_cpu": (steps / walk_s) / cpu[best_th][0],
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(ctx, hub, args):

@@ -112,7 +112,10 @@ struct WaveCache {
 __global__ void batch_begin_kernel(BatchLaunch p, uint32_t max_queries)
 {
     const uint32_t nq = (uint32_t)p.hdr[1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.queue[0] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.queue[0] = 0;
+        p.queue[1] = 0;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < max_queries && i < nq; i += gridDim.x * blockDim.x) {
         p.touched_count[i] = 0;
         p.overflow[i] = 0;
@@ -134,6 +137,7 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     }
     for (uint32_t s = lane; s < kWaveHubs; s += 64) hub[s] = 0;
     const uint64_t num_chunks = p.hdr[0];
+    const uint64_t first_chunk = p.hdr[6];  // the workgroup chunks of dense queries come first
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
     const uint2 *alias = p.hdr[5] ? p.alias : nullptr;
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
             if (c_next == c_end) {
                 unsigned long long ci = 0;
                 if (lane == 0) ci = atomicAdd(&p.queue[0], 1ull);
-                ci = wave_broadcast_first(ci);
+                ci = first_chunk + wave_broadcast_first(ci);
                 if (ci >= num_chunks) {
                     exhausted = true;
                     break;
@@ -247,6 +251,180 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     flush();
     C.flush(p, lane);
     flush_hubs();
+}
+
+// Dense queries, workgroup-coherent.  A workgroup takes one chunk of one
+// query at a time from the workgroup queue, and its wavefronts refill idle
+// lanes from the chunk through an LDS offset.  Visits of the workgroup's
+// query go to a register (the query's own source), to LDS direct counters for
+// device ids below `hubs` (the top-degree vertices after the relabel), to an
+// LDS first-come hash cache of `slots` other vertices, and otherwise to the
+// query's dense slice.  The LDS counts are flushed to the slice at a barrier
+// when the workgroup's next chunk belongs to another query, and at the end.
+// Every wavefront reaches each barrier: a chunk runs dry and every walk ends.
+// LDS counts are u32 and cannot wrap: none exceeds its query's visit total,
+// which the host checks is below 2^32.
+__global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32_t hubs, uint32_t slots)
+{
+    extern __shared__ uint32_t dense_lds[];
+    uint32_t *const direct = dense_lds;
+    uint32_t *const keys = dense_lds + hubs;
+    uint32_t *const vals = keys + slots;
+    const uint32_t mask = slots ? slots - 1 : 0u;
+    __shared__ uint32_t s_q, s_len, s_off, s_done;
+    __shared__ unsigned long long s_start;
+
+    for (uint32_t i = threadIdx.x; i < hubs; i += blockDim.x) direct[i] = 0;
+    for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) {
+        keys[i] = kEmpty;
+        vals[i] = 0;
+    }
+    const uint32_t lane = __lane_id();
+    const uint64_t n_dense = p.hdr[6];
+    const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
+    const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
+    const uint2 *alias = p.hdr[5] ? p.alias : nullptr;
+    uint32_t cq = kEmpty;  // workgroup-uniform: the query whose counts the LDS holds
+    uint32_t src = 0, thr = 0, stream = 0;
+    uint64_t off = 0;
+    uint32_t my_src = 0;
+    uint64_t my_steps = 0;
+
+    // Workgroup-uniform: every thread calls it, between barriers.
+    auto flush = [&]() {
+        if (cq == kEmpty) return;
+        uint64_t hot = my_src, st = my_steps;
+        for (int o = 32; o > 0; o >>= 1) {
+            hot += __shfl_xor(hot, o);
+            st += __shfl_xor(st, o);
+        }
+        if (lane == 0) {
+            if (hot) atomicAdd(&p.dense[off + src], (uint32_t)hot);
+            if (st) atomicAdd(&p.steps[cq], (unsigned long long)st);
+        }
+        my_src = 0;
+        my_steps = 0;
+        for (uint32_t i = threadIdx.x; i < hubs; i += blockDim.x) {
+            const uint32_t c = direct[i];
+            if (c) {
+                atomicAdd(&p.dense[off + i], c);
+                direct[i] = 0;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) {
+            const uint32_t k = keys[i];
+            if (k != kEmpty) {
+                atomicAdd(&p.dense[off + k], vals[i]);
+                keys[i] = kEmpty;
+                vals[i] = 0;
+            }
+        }
+    };
+
+    for (;;) {
+        __syncthreads();  // the previous chunk is walked; the chunk state is free
+        if (threadIdx.x == 0) {
+            const unsigned long long ci = atomicAdd(&p.queue[1], 1ull);
+            if (ci >= n_dense) {
+                s_done = 1;
+            } else {
+                const BatchChunk ch = p.chunks[ci];
+                s_q = ch.q;
+                s_start = ch.start;
+                s_len = ch.len;
+                s_off = 0;
+                s_done = 0;
+            }
+        }
+        __syncthreads();
+        if (s_done) break;
+        const uint32_t q = s_q;
+        if (q != cq) {
+            flush();
+            __syncthreads();  // LDS is clear before counting for q
+            cq = q;
+            const BatchQuery bq = p.queries[q];
+            src = bq.source_dev;
+            thr = bq.thr;
+            stream = bq.stream;
+            off = bq.slice_off;
+        }
+        const uint64_t start = s_start;
+        const uint32_t len = s_len;
+        bool exhausted = false, active = false;
+        uint64_t w = 0;
+        uint32_t v = 0, step = 0;
+        for (;;) {
+            const uint64_t idle = __ballot(!active);
+            if (idle != 0 && !exhausted) {  // wave-uniform
+                const uint32_t n_idle = (uint32_t)__popcll(idle);
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&s_off, n_idle);
+                base = (uint32_t)wave_broadcast_first(base);
+                if (base >= len) {
+                    exhausted = true;
+                } else {
+                    const uint32_t take = len - base < n_idle ? len - base : n_idle;
+                    const uint32_t rank = lane_rank(idle);
+                    if (!active && rank < take) {
+                        w = start + base + rank;
+                        v = src;
+                        step = 0;
+                        ++my_src;  // the start visit
+                        active = true;
+                    }
+                }
+            }
+            if (__ballot(active) == 0) break;
+            if (active) {
+                const u32x4 r = philox4x32_10((uint32_t)w, (uint32_t)(w >> 32), step, stream, seed_lo, seed_hi);
+                const bool coin = textbook || step >= 1;
+                bool stop = coin && r.z >= thr;
+                if (!stop) {
+                    const uint32_t beg = p.row_ptr[v];
+                    const uint32_t deg = p.row_ptr[v + 1] - beg;
+                    if (deg == 0) {
+                        stop = true;  // batches end walkers at vertices without out-edges
+                    } else {
+                        uint32_t j = mulhi64(r.x, r.y, deg);
+                        if (alias) {
+                            const uint2 a = alias[beg + j];
+                            if (r.w >= a.x) j = a.y;
+                        }
+                        v = p.col[beg + j];
+                        ++step;
+                        if (v == src) {
+                            ++my_src;
+                        } else if (v < hubs) {
+                            atomicAdd(&direct[v], 1u);
+                        } else {
+                            bool done = false;
+                            if (slots) {
+                                uint32_t h = (v * 0x9E3779B1u) & mask;
+#pragma unroll 1
+                                for (int probe = 0; probe < kCacheProbes; ++probe) {
+                                    uint32_t k = keys[h];
+                                    if (k == kEmpty) k = atomicCAS(&keys[h], kEmpty, v);
+                                    if (k == kEmpty || k == v) {
+                                        atomicAdd(&vals[h], 1u);
+                                        done = true;
+                                        break;
+                                    }
+                                    h = (h + 1) & mask;
+                                }
+                            }
+                            if (!done) atomicAdd(&p.dense[off + v], 1u);
+                        }
+                    }
+                }
+                if (stop) {
+                    my_steps += step;
+                    active = false;
+                }
+            }
+        }
+    }
+    flush();
 }
 
 // After the walk: the nonzero counters of each dense slice, compacted into
@@ -593,14 +771,24 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, hipStream_t s)
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
+                             hipStream_t s)
 {
-    const size_t lds = (size_t)(block / 64) * (kWaveSlots * 12 + kWaveHubs * 4);
+    const size_t dlds = ((size_t)lds_hubs + 2 * (size_t)cache_slots) * 4;
     int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_dense_kernel, block, dlds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    hipLaunchKernelGGL(batch_dense_kernel, dim3(per_cu * num_cus), dim3(block), dlds, s, p, (uint32_t)lds_hubs,
+                       (uint32_t)cache_slots);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = (size_t)(block / 64) * (kWaveSlots * 12 + kWaveHubs * 4);
+    per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * num_cus), dim3(block), lds, s, p);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(256), 0, s, p);
     return hipGetLastError();

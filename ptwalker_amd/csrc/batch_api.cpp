@@ -44,10 +44,22 @@ void drop_exec(BatchState &b)
 
 struct Plan {
     std::vector<BatchQuery> q;
-    std::vector<BatchChunk> chunks;
+    std::vector<BatchChunk> chunks;  // workgroup chunks of dense queries, then wavefront chunks of hash queries
     uint64_t dense = 0, hash = 0, touched = 0;  // elements needed
     uint32_t n_dense = 0, n_hash = 0;
+    uint64_t dense_chunks = 0;
 };
+
+// Walker ids per workgroup chunk of a dense query: about eight chunks per
+// resident workgroup over the batch's dense walkers, so the last chunks
+// balance the tail, and at least 32768 (32 per lane of a 1024-thread
+// workgroup) so that the barrier at a change of query stays rare.
+uint64_t dense_chunk_size(uint64_t dense_walkers, int num_cus)
+{
+    const uint64_t target = dense_walkers / ((uint64_t)std::max(num_cus, 1) * 8);
+    uint64_t c = (target + 4095) / 4096 * 4096;
+    return std::min<uint64_t>(std::max<uint64_t>(c, 32768), 1u << 20);
+}
 
 int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
 {
@@ -92,8 +104,21 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
             B.touched_off = 0;
             ++P.n_hash;
         }
-        for (uint64_t s = 0; s < qq.walkers; s += chunk)
-            P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(chunk, qq.walkers - s), s});
+    }
+    uint64_t dense_walkers = 0;
+    for (const BatchQuery &B : P.q)
+        if (B.kind == SLICE_DENSE) dense_walkers += B.walkers;
+    const uint64_t dchunk = dense_chunk_size(dense_walkers, ctx->num_cus);
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
+        if (P.q[i].kind != SLICE_DENSE) continue;
+        for (uint64_t s = 0; s < P.q[i].walkers; s += dchunk)
+            P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(dchunk, P.q[i].walkers - s), s});
+    }
+    P.dense_chunks = P.chunks.size();
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
+        if (P.q[i].kind == SLICE_DENSE) continue;
+        for (uint64_t s = 0; s < P.q[i].walkers; s += chunk)
+            P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(chunk, P.q[i].walkers - s), s});
     }
     return PTW_OK;
 }
@@ -140,7 +165,7 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
     bool moved = false;
     if (!b.hdr) {
         HIP_TRY(ctx, hipMalloc((void **)&b.hdr, 8 * sizeof(unsigned long long)));
-        HIP_TRY(ctx, hipMalloc((void **)&b.queue, sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMalloc((void **)&b.queue, 2 * sizeof(unsigned long long)));
         moved = true;
     }
     if (P.dense > b.dense_cap) {
@@ -207,7 +232,8 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
 int enqueue_run(ptw_ctx *ctx, const BatchLaunch &p)
 {
     HIP_TRY(ctx, launch_batch_begin(p, ctx->batch.q_cap, ctx->stream));
-    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->stream));
+    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->shape.lds_hubs, ctx->shape.cache_slots,
+                                   ctx->stream));
     HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, ctx->batch.topk_v, ctx->batch.topk_c, ctx->shape.topk_tile,
                                    ctx->num_cus, ctx->stream));
     return PTW_OK;
@@ -311,6 +337,7 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     b.h_hdr[3] = d.seed;
     b.h_hdr[4] = (unsigned long long)d.semantics;
     b.h_hdr[5] = weighted ? 1 : 0;
+    b.h_hdr[6] = P.dense_chunks;
     b.last_nq = d.num_queries;
     b.weighted = weighted;
     hipStream_t s = ctx->stream;

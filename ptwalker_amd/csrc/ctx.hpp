@@ -65,6 +65,8 @@ struct ptw_ctx {
     std::vector<uint32_t> alias_thr, alias_idx;
 
     unsigned long long *counts_dev = nullptr;  // V device-id counters
+    uint64_t *counts_stage = nullptr;          // V graph-id counts staged for a host output
+    uint64_t *pinned = nullptr;                // host, page-locked: [0, 8) scratch words, [8, 8 + V) counts
     unsigned long long *scratch = nullptr;     // 8 words
     ptw::LaunchShape shape;
     ptw::BatchState batch;

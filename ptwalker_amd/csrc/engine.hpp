@@ -147,7 +147,8 @@ struct BatchLaunch {
     const BatchChunk *chunks;
     // per-batch values live on the device so one captured graph serves every
     // batch: hdr[0] chunks, [1] queries, [2] top-k length, [3] seed,
-    // [4] semantics, [5] weighted
+    // [4] semantics, [5] weighted, [6] workgroup chunks of dense queries (they
+    // come first in the chunk table; the wavefront chunks of hash queries follow)
     const unsigned long long *hdr;
     uint32_t *dense;         // dense slice pool (u32 counters, device ids)
     uint2 *hash;             // hash slice pool {key, count}
@@ -156,12 +157,16 @@ struct BatchLaunch {
     uint32_t *overflow;      // per query: hash insertions that found no slot
     unsigned long long *steps;  // per query
     double *entropy;            // per query, written by the top-k kernel
-    unsigned long long *queue;  // [0]: next chunk
+    unsigned long long *queue;  // [0]: next wavefront chunk, [1]: next workgroup chunk
 };
 
 // Zeroes the per-query counters and the chunk queue of the batch in hdr.
 hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStream_t s);
-hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, hipStream_t s);
+// Dense queries (workgroup chunks, LDS hub counters + cache of `lds_hubs` /
+// `cache_slots`), then hash queries (wavefront chunks), then the compaction
+// of dense slices.
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
+                             hipStream_t s);
 // Per-query top-k (count desc, device id asc) into out_v (graph ids) / out_c,
 // one workgroup per query, `tile` candidate slots of LDS per workgroup.
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v,

@@ -17,6 +17,8 @@
 // Count apps privatize the `lds_hubs` highest-degree vertices (device ids
 // below `hubs` after the degree-descending relabel) in LDS u64 counters,
 // flushed once per workgroup; other visits are device-scope u64 atomics.
+#include <algorithm>
+
 #include "../../include/ptw.h"
 #include "engine.hpp"
 #include "philox.hpp"
@@ -360,13 +362,13 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
                      : p.app == PTW_APP_RWR                            ? KIND_END
                                                                        : KIND_PATH;
-    const int block = shape.block;
+    int block = shape.block;
     uint32_t hubs = 0, slots = 0;
     if (kind == KIND_COUNT) {
         hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
         slots = (uint32_t)shape.cache_slots;
     }
-    const size_t lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
+    size_t lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
@@ -374,9 +376,23 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
         per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
                  : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
                                     : occupancy_blocks<KIND_PATH>(block, lds);
+        // A small walk (under 8 walkers per resident lane) takes as long as its
+        // longest walker chains plus each workgroup's LDS setup and flush: one
+        // 512-thread workgroup per CU and no visit cache measured fastest.
+        if (p.walkers < 8ull * (uint64_t)per_cu * (uint64_t)num_cus * (uint64_t)block) {
+            block = block < 512 ? block : 512;
+            per_cu = 1;
+            slots = 0;
+            lds = (size_t)hubs * sizeof(uint32_t);
+        }
     }
     const int blocks = per_cu * num_cus;
-    const uint32_t chunk = (uint32_t)shape.chunk;
+    // Walker ids per queue take: the configured chunk, cut down for small
+    // walks so that every resident wavefront gets about four takes.
+    const uint64_t waves = (uint64_t)blocks * (uint64_t)(block / 64);
+    uint64_t fair = p.walkers / (waves * 4);
+    fair = (fair + 63) / 64 * 64;
+    const uint32_t chunk = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(fair, (uint64_t)shape.chunk));
     if (kind == KIND_COUNT)
         hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);
     else if (kind == KIND_END)

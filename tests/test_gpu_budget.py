@@ -1,7 +1,7 @@
 """What the trained budget predictor gains over the method without it.
 
 The committed policy (network trained on the device on engine-generated
-pairs from two R-MAT scale-20 graphs, seeds 101 and 102) is scored on a graph
+pairs -- 200,000 of them, from five R-MAT scale-20 graphs, seeds 101-105) is scored on a graph
 it never saw (R-MAT scale 20, seed 7): for 600 sources and eps in {1e-3,
 3e-4}, the required budget (c_s / eps)^2 comes from 8 independent
 16384-walker sub-walks per source scored against the device FP64 solver.

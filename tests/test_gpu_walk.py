@@ -145,6 +145,23 @@ def test_lds_and_global_counters_scale16(ctx):
     check(ctx, rp, col, ptw.RWR(0.2, int(gi.max_degree_vertex)), 1_000_000, seed=1)
 
 
+@pytest.mark.parametrize("hubs,slots", [(0, 0), (256, 0), (0, 4096), (256, 1024), (8192, 4096)])
+def test_counter_paths_explicit_shapes(ctx, hubs, slots):
+    """Every counting path, pinned by an explicit launch shape (blocks_per_cu set,
+    so the small-walk rule does not apply): direct LDS hub counters, the LDS
+    visit cache, both, and device atomics only."""
+    gi = ctx.rmat(scale=14, edge_factor=16, seed=2)
+    rp, col = ctx.export_csr()
+    default = ctx.tuning().as_dict()
+    try:
+        ctx.set_tuning(lds_hubs=hubs, cache_slots=slots, blocks_per_cu=1, block=1024)
+        check(ctx, rp, col, ptw.PPR(0.15, int(gi.max_degree_vertex)), 600_000, seed=3)
+        check(ctx, rp, col, ptw.PPR(0.2, 777), 300_000, seed=4)
+        check(ctx, rp, col, ptw.PageRank(0.15), 300_000, seed=5)
+    finally:
+        ctx.set_tuning(**default)
+
+
 def test_device_outputs_and_accumulate(ctx):
     gi = ctx.rmat(scale=12, edge_factor=8, seed=3)
     rp, col = ctx.export_csr()
