@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
     ap.add_argument("--audit-rate", type=float, default=0.02)
     ap.add_argument("--eval", type=int, default=64, help="serve: requests per mode scored against FP64")
+    ap.add_argument("--calibration-sources", type=int, default=512,
+                    help="serve: sources of the serving graph labelled against FP64 to calibrate both budgets")
     ap.add_argument("--hoeffding-requests", type=int, default=200,
                     help="serve: requests of stream A served with the Hoeffding budget (146e6 walkers each)")
     ap.add_argument("--queries", type=int, default=4096)
@@ -406,16 +408,25 @@ def bench_serve(args):
     import numpy as np
 
     import ptwalker_amd as ptw
-    from ptwalker_amd.budget import BudgetPolicy, hoeffding_c2
+    from ptwalker_amd.budget import BudgetPolicy, calibrate, hoeffding_c2
 
-    policy = BudgetPolicy.load(os.path.join(ROOT, "ptwalker_amd", "models", "budget_policy.json"))
-    tp = float(policy.meta["tp"])
+    trained = BudgetPolicy.load(os.path.join(ROOT, "ptwalker_amd", "models", "budget_policy.json"))
+    tp = float(trained.meta["tp"])
     ctx = ptw.Context(0)
     t0 = time.time()
     gi = ctx.rmat(scale=args.scale, edge_factor=16, seed=1)
     build_s = time.time() - t0
     rp, _ = ctx.export_csr()
-    net = policy.network(ctx)
+    net = trained.network(ctx)
+    # calibrate both budgets to 95 % coverage on this graph (sources disjoint
+    # in seed from the served stream), outside the timed region
+    t0 = time.time()
+    cal_src, _, _ = serve_requests(rp, args.calibration_sources, 1.0, seed=99)
+    policy, cal_stats = calibrate(ctx, trained, net, cal_src)
+    cal_stats["seconds"] = round(time.time() - t0, 1)
+    print(f"# calibration on {len(cal_src)} sources: margin {trained.margin:.3f} -> {policy.margin:.3f}, fixed_c2 "
+          f"{trained.fixed_c2:.3f} -> {policy.fixed_c2:.3f}, work ratio {cal_stats['work_ratio_fixed_over_learned']:.3f}",
+          file=sys.stderr, flush=True)
     h_c2 = hoeffding_c2()
     common = dict(tp=tp, seed=1, topk=100, max_batch=512)
     modes = {
@@ -479,8 +490,10 @@ def bench_serve(args):
                                f"audit rate {args.audit_rate}",
                    "vertices": gi.num_vertices, "edges": gi.num_edges, "graph_build_s": round(build_s, 1),
                    "policy": {"margin": policy.margin, "fixed_c2": policy.fixed_c2, "hoeffding_c2": h_c2,
-                              "train_pairs": policy.meta.get("pairs"),
-                              "val_rmse_log": policy.meta.get("val_rmse_log")}},
+                              "trained_margin": trained.margin, "trained_fixed_c2": trained.fixed_c2,
+                              "train_pairs": trained.meta.get("pairs"),
+                              "val_rmse_log": trained.meta.get("val_rmse_log")},
+                   "calibration": cal_stats},
         "learned_over_fixed": round(L["queries_per_s"] / stream_a["fixed"]["queries_per_s"], 3),
         "learned_over_hoeffding": round(L["queries_per_s"] / stream_a["hoeffding"]["queries_per_s"], 3),
         "stream_a_eps_1e-3": stream_a,

@@ -157,6 +157,81 @@ class BudgetPolicy:
         return np.clip(np.ceil(w), self.w_min, self.w_max).astype(np.int64), ood
 
 
+def realized_coverage(budgets, c, shapes, eps) -> float:
+    """Share of queries whose realized top-k L1 stays within eps at the given
+    budgets.  A query's realized error at W walkers is c_s * r / sqrt(W), with
+    r drawn from its sub-walk errors divided by their mean (`shapes`, one row
+    per query): the Monte-Carlo scatter around the expected error."""
+    w = np.asarray(budgets, dtype=np.float64)
+    return float(np.mean(np.asarray(c)[:, None] * shapes / np.sqrt(w)[:, None] <= np.asarray(eps)[:, None]))
+
+
+def _smallest(f, target, lo=1e-4, hi=1e4, iters=60) -> float:
+    """Smallest x in [lo, hi] with f(x) >= target, for f non-decreasing."""
+    if f(hi) < target:
+        return hi
+    for _ in range(iters):
+        mid = math.sqrt(lo * hi)
+        if f(mid) >= target:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+def calibrate(ctx: Context, policy: "BudgetPolicy", net: BudgetMLP, sources, *, eps=(1e-3, 3e-4),
+              coverage: float = 0.95, seed: int = 97, stream_base: int = (1 << 30) + (1 << 29)):
+    """Re-fit the policy's two scale constants on the graph it will serve.
+
+    The network ranks queries by difficulty; how many walkers a unit of
+    predicted difficulty needs depends on the graph (its size, its degree
+    tail).  On `sources` of the resident graph this measures each source's
+    error constant c_s and its Monte-Carlo scatter (8 independent sub-walks
+    scored against the FP64 solver, as in training), then sets
+      fixed_c2 to the smallest constant whose fixed budgets fixed_c2 / eps^2, and
+      margin   to the smallest factor whose learned budgets margin * exp(net)
+               (fixed budgets for out-of-range features)
+    keep the *realized* top-k L1 within eps for `coverage` of the queries, so
+    that both methods are compared at the same realized accuracy.
+    Returns (calibrated policy, calibration statistics)."""
+    rp, col = ctx.export_csr()
+    tp = float(policy.meta.get("tp", 0.2))
+    sources = np.asarray(sources)
+    f = features(ctx, rp, col, sources, tp, eps[0], seed=seed, stream_base=stream_base)
+    c, l1 = error_constants(ctx, sources, tp, seed=seed)
+    shape = l1 / l1.mean(axis=1, keepdims=True)
+    xs, cs, shapes, es = [], [], [], []
+    for e in eps:
+        x = f.copy()
+        x[:, 5] = np.log(e)
+        xs.append(x)
+        cs.append(c)
+        shapes.append(shape)
+        es.append(np.full(len(c), e))
+    x, cc, sh, ee = map(np.concatenate, (xs, cs, shapes, es))
+    pred = np.exp(net.forward(x.astype(np.float32)).astype(np.float64))
+    ood = np.any((x < np.asarray(policy.feat_lo)) | (x > np.asarray(policy.feat_hi)), axis=1)
+    clip = lambda w: np.clip(np.ceil(w), policy.w_min, policy.w_max)
+    fixed_of = lambda k: clip(k / ee ** 2)
+    fixed_c2 = _smallest(lambda k: realized_coverage(fixed_of(k), cc, sh, ee), coverage)
+    learned_of = lambda m: np.where(ood, fixed_of(fixed_c2), clip(m * pred))
+    margin = _smallest(lambda m: realized_coverage(learned_of(m), cc, sh, ee), coverage)
+    meta = dict(policy.meta)
+    meta.update({"calibration_sources": int(len(sources)), "trained_margin": policy.margin,
+                 "trained_fixed_c2": policy.fixed_c2})
+    cal = BudgetPolicy(params=policy.params, norm=policy.norm, margin=margin, fixed_c2=fixed_c2,
+                       w_min=policy.w_min, w_max=policy.w_max, feat_lo=policy.feat_lo, feat_hi=policy.feat_hi,
+                       meta=meta)
+    learned, fixed = learned_of(margin), fixed_of(fixed_c2)
+    stats = {"sources": int(len(sources)), "margin": margin, "fixed_c2": fixed_c2,
+             "c_median": float(np.median(c)), "scatter_cv": float(np.mean(l1.std(1) / l1.mean(1))),
+             "ood_frac": float(ood.mean()),
+             "coverage_learned": realized_coverage(learned, cc, sh, ee),
+             "coverage_fixed": realized_coverage(fixed, cc, sh, ee),
+             "work_ratio_fixed_over_learned": float(fixed.sum() / learned.sum())}
+    return cal, stats
+
+
 def fit_policy(ctx: Context, x: np.ndarray, y: np.ndarray, *, coverage: float = 0.95, seed: int = 1,
                epochs: int = 200, batch: int = 512, lr: float = 2e-3, val_frac: float = 0.15):
     """Train the network on (x, y = log budget) and calibrate the policy on a held-out split."""

@@ -50,3 +50,43 @@ def test_learned_budget_beats_fixed_on_unseen_graph(ctx):
     assert fallback.mean() < 0.05
     assert cov_l >= 0.90 and cov_l >= cov_f - 0.03
     assert ratio > 1.25
+
+
+def test_calibration_on_the_serving_graph(ctx):
+    """On a graph of another scale (R-MAT 18), calibrating both budget constants
+    on 400 labelled sources brings held-out coverage of both methods to about
+    the target, so that they are compared at equal accuracy."""
+    from ptwalker_amd.budget import calibrate, realized_coverage
+
+    policy = BudgetPolicy.load(POLICY)
+    tp = policy.meta["tp"]
+    ctx.rmat(scale=18, edge_factor=16, seed=11)
+    rp, col = ctx.export_csr()
+    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
+    rng = np.random.default_rng(5)
+    pick = lambda n: np.concatenate([rng.choice(len(deg), n // 2, p=deg / deg.sum()), rng.integers(0, len(deg), n // 2)])
+    cal_src, test_src = pick(400), pick(400)
+    net = policy.network(ctx)
+    cal, stats = calibrate(ctx, policy, net, cal_src)
+    assert stats["coverage_learned"] >= 0.949 and stats["coverage_fixed"] >= 0.949
+    f = features(ctx, rp, col, test_src, tp, 1e-3, stream_base=(1 << 30) + 11 * (1 << 20))
+    c, l1 = error_constants(ctx, test_src, tp, seed=11)
+    shape = l1 / l1.mean(axis=1, keepdims=True)
+    cov = {"learned": [], "fixed": []}
+    work = {"learned": 0.0, "fixed": 0.0}
+    for eps in (1e-3, 3e-4):
+        x = f.copy()
+        x[:, 5] = np.log(eps)
+        w, _ = cal.budgets(net, x)
+        fx = np.clip(cal.fixed_budget(np.full(len(x), eps)), cal.w_min, cal.w_max)
+        e = np.full(len(x), eps)
+        cov["learned"].append(realized_coverage(w, c, shape, e))
+        cov["fixed"].append(realized_coverage(fx, c, shape, e))
+        work["learned"] += float(w.sum())
+        work["fixed"] += float(fx.sum())
+    net.close()
+    cl, cf = float(np.mean(cov["learned"])), float(np.mean(cov["fixed"]))
+    print(f"calibrated margin {cal.margin:.3f} fixed_c2 {cal.fixed_c2:.3f}; held-out realized coverage learned "
+          f"{cl:.3f} fixed {cf:.3f}; fixed/learned work {work['fixed'] / work['learned']:.2f}; {stats}")
+    assert cl >= 0.9 and cf >= 0.9 and abs(cl - cf) < 0.05
+    assert work["fixed"] / work["learned"] > 1.0
