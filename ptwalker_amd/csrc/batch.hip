@@ -427,46 +427,74 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     flush();
 }
 
-// After the walk: the nonzero counters of each dense slice, compacted into
-// the query's touched list (one reservation per segment), so that top-k and
-// the cleanup read only touched entries.  A list that would overflow its
+// After the walk: the nonzero counters of each dense slice, listed as
+// (device id, count) pairs in the query's touched list, so that top-k streams
+// the pairs instead of gathering from the slice.  One read of each slice:
+// a workgroup takes a segment of kSeg counters, each thread loads 16 of them
+// per round with 16 B loads, and one block scan plus one reservation on the
+// query's count place the round's pairs.  A list that would overflow its
 // capacity marks the query for full-slice scans (touched_count > capacity).
 __global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
 {
-    __shared__ uint32_t s_count[256];
+    __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_base;
     const uint32_t nq = (uint32_t)p.hdr[1];
-    const uint32_t segs = (p.V + kSeg - 1) / kSeg;
+    const uint32_t pitch = dense_pitch(p.V);
+    const uint32_t segs = (pitch + kSeg - 1) / kSeg;
     const uint64_t tasks = (uint64_t)nq * segs;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
         const uint32_t q = (uint32_t)(t / segs);
         const BatchQuery bq = p.queries[q];
         if (bq.kind != SLICE_DENSE) continue;  // block-uniform
         const uint32_t lo = (uint32_t)(t % segs) * kSeg;
-        const uint32_t hi = lo + kSeg < p.V ? lo + kSeg : p.V;
-        const uint32_t *slice = p.dense + bq.slice_off;
-        uint32_t mine = 0;
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) mine += slice[i] != 0;
-        s_count[threadIdx.x] = mine;
-        __syncthreads();
-        // inclusive scan over the 256 per-thread counts
-        for (uint32_t off = 1; off < 256; off <<= 1) {
-            const uint32_t x = threadIdx.x >= off ? s_count[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s_count[threadIdx.x] += x;
-            __syncthreads();
-        }
-        const uint32_t total = s_count[255];
-        if (threadIdx.x == 0) s_base = total ? atomicAdd(&p.touched_count[q], total) : 0u;
-        __syncthreads();
-        uint32_t idx = s_base + s_count[threadIdx.x] - mine;
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-            if (slice[i] != 0) {
-                if (idx < bq.touched_cap) p.touched[bq.touched_off + idx] = i;
-                ++idx;
+        const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // a multiple of 64
+        const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + bq.slice_off);
+        for (uint32_t base = lo; base < hi; base += 256 * 16) {
+            const uint32_t my0 = base + threadIdx.x * 16;  // my0 < hi implies my0 + 16 <= hi
+            uint32_t c[16];
+            if (my0 < hi) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint4 x = slice4[(my0 >> 2) + j];
+                    c[4 * j] = x.x;
+                    c[4 * j + 1] = x.y;
+                    c[4 * j + 2] = x.z;
+                    c[4 * j + 3] = x.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) c[j] = 0;
             }
+            uint32_t n = 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) n += c[j] != 0;
+            uint32_t incl = n;  // inclusive prefix over the wavefront
+#pragma unroll
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) s_wave[wave] = incl;
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < 4; ++w) {
+                before += w < wave ? s_wave[w] : 0u;
+                total += s_wave[w];
+            }
+            if (threadIdx.x == 0) s_base = total ? atomicAdd(&p.touched_count[q], total) : 0u;
+            __syncthreads();
+            uint32_t idx = s_base + before + incl - n;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (c[j]) {
+                    if (idx < bq.touched_cap) p.touched[bq.touched_off + idx] = uint2{my0 + (uint32_t)j, c[j]};
+                    ++idx;
+                }
+            }
+            __syncthreads();  // s_wave and s_base are reused by the next round
         }
-        __syncthreads();
     }
 }
 
@@ -481,8 +509,9 @@ struct Entries {
     __device__ __forceinline__ void get(uint32_t i, uint32_t &v, uint32_t &c) const
     {
         if (mode == 0) {
-            v = p->touched[Q.touched_off + i];
-            c = p->dense[Q.slice_off + v];
+            const uint2 e = p->touched[Q.touched_off + i];
+            v = e.x;
+            c = e.y;
         } else if (mode == 1) {
             v = i;
             c = p->dense[Q.slice_off + i];
@@ -583,10 +612,17 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             const uint64_t width = (range + kBins - 1) / kBins;
             for (uint32_t b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
             __syncthreads();
+            // The lowest bin is not counted: it holds most entries (counts of
+            // one or two visits) and would serialize the LDS atomics, yet it
+            // matters only when fewer than k entries lie above it -- then the
+            // next pass narrows into it.
             for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
                 uint32_t v, c;
                 E.get(i, v, c);
-                if (c >= lo && c <= hi) atomicAdd(&hist[(uint32_t)((c - lo) / width)], 1u);
+                if (c >= lo && c <= hi) {
+                    const uint32_t bin = (uint32_t)((c - lo) / width);
+                    if (bin) atomicAdd(&hist[bin], 1u);
+                }
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -601,7 +637,7 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
                 }
                 s_u32[1] = b;
                 s_u32[2] = cum;        // entries above bin b
-                s_u32[3] = hist[b];    // entries in bin b
+                s_u32[3] = hist[b];    // entries in bin b (bin 0: not counted)
             }
             __syncthreads();
             const uint32_t b = s_u32[1];
@@ -613,14 +649,32 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             lo = nlo;
             hi = nhi;
             __syncthreads();
-            if (above + inbin <= tile) break;
+            if (b != 0 && (uint64_t)above + inbin <= tile) break;
             if (lo == hi) {
+                // one value left whose count is not known to fit: the exact
+                // id select below takes the k - above smallest ids among it
                 overflow_ties = true;
                 break;
             }
         }
         // collect candidates: everything above hi, and the bin [lo, hi]
         uint32_t tie_limit = 0xFFFFFFFFu;  // ties at count lo == hi keep ids <= tie_limit
+        if (overflow_ties) {
+            // the narrowing can end on an uncounted value: count its ties, and
+            // take them all when they fit beside the entries above
+            if (threadIdx.x == 0) s_u32[7] = 0;
+            __syncthreads();
+            uint32_t ties = 0;
+            for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                uint32_t v, c;
+                E.get(i, v, c);
+                ties += c == (uint32_t)lo;
+            }
+            if (ties) atomicAdd(&s_u32[7], ties);
+            __syncthreads();
+            if ((uint64_t)above + s_u32[7] <= tile) overflow_ties = false;
+            __syncthreads();
+        }
         if (overflow_ties) {
             // radix select of the (k - above) smallest device ids among count == lo
             const uint32_t need = k - above;
@@ -703,11 +757,15 @@ __global__ __launch_bounds__(1024) void batch_reset_kernel(BatchLaunch p)
     for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
         const BatchQuery Q = p.queries[q];
         if (Q.kind == SLICE_DENSE) {
+            // few touched entries: clear them one by one; otherwise stream
+            // zeros over the whole slice (a scattered 4 B store moves a line)
             const uint32_t t = p.touched_count[q];
-            if (t <= Q.touched_cap) {
-                for (uint32_t i = threadIdx.x; i < t; i += blockDim.x) p.dense[Q.slice_off + p.touched[Q.touched_off + i]] = 0;
+            const uint32_t pitch = dense_pitch(p.V);
+            if (t <= Q.touched_cap && t < pitch / 16) {
+                for (uint32_t i = threadIdx.x; i < t; i += blockDim.x) p.dense[Q.slice_off + p.touched[Q.touched_off + i].x] = 0;
             } else {
-                for (uint32_t i = threadIdx.x; i < p.V; i += blockDim.x) p.dense[Q.slice_off + i] = 0;
+                uint4 *d4 = reinterpret_cast<uint4 *>(p.dense + Q.slice_off);
+                for (uint32_t i = threadIdx.x; i < pitch / 4; i += blockDim.x) d4[i] = uint4{0u, 0u, 0u, 0u};
             }
         } else {
             for (uint32_t i = threadIdx.x; i <= Q.mask; i += blockDim.x) p.hash[Q.slice_off + i] = uint2{kEmpty, 0u};

@@ -88,7 +88,7 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         if (qq.walkers > thr_dense) {
             B.kind = SLICE_DENSE;
             B.slice_off = P.dense;
-            P.dense += V;
+            P.dense += dense_pitch(V);
             B.touched_cap = (uint32_t)std::min<uint64_t>(V, upper);
             B.touched_off = P.touched;
             P.touched += B.touched_cap;
@@ -128,7 +128,7 @@ uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint
     uint64_t need = 0;
     if (P.dense > b.dense_cap) need += 4 * P.dense;
     if (P.hash > b.hash_cap) need += 8 * P.hash;
-    if (P.touched > b.touched_cap) need += 4 * P.touched;
+    if (P.touched > b.touched_cap) need += 8 * P.touched;
     if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + 16);
     if (P.chunks.size() > b.chunk_cap) need += P.chunks.size() * sizeof(BatchChunk);
     if ((uint64_t)nq * k > b.topk_cap) need += (uint64_t)nq * k * 12;
@@ -188,7 +188,7 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
     if (P.touched > b.touched_cap || !b.touched) {
         hfree(b.touched);
         const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(P.touched, 1), b.touched_cap + b.touched_cap / 2);
-        HIP_TRY(ctx, hipMalloc((void **)&b.touched, cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.touched, cap * 8));
         b.touched_cap = cap;
         moved = true;
     }
@@ -406,7 +406,7 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     for (const auto &q : b.h_queries) {
         if (q.kind == ptw::SLICE_DENSE) {
             ++out->dense_queries;
-            bytes += 4ull * ctx->dev.V + 4ull * q.touched_cap;
+            bytes += 4ull * ptw::dense_pitch(ctx->dev.V) + 8ull * q.touched_cap;
         } else {
             ++out->hash_queries;
             bytes += 8ull * ((uint64_t)q.mask + 1);

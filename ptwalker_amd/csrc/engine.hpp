@@ -119,6 +119,10 @@ hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, cons
 
 enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1 };
 
+// Dense slices are laid out every dense_pitch(V) counters: a multiple of 64,
+// so every slice starts on a 256 B boundary and can be read with 16 B loads.
+__host__ __device__ inline uint32_t dense_pitch(uint32_t V) { return (V + 63u) & ~63u; }
+
 // One query of a batch, as the device sees it.
 struct BatchQuery {
     uint32_t source_dev;
@@ -152,7 +156,7 @@ struct BatchLaunch {
     const unsigned long long *hdr;
     uint32_t *dense;         // dense slice pool (u32 counters, device ids)
     uint2 *hash;             // hash slice pool {key, count}
-    uint32_t *touched;       // first-touch lists
+    uint2 *touched;          // per dense query: (device id, count) of its nonzero entries
     uint32_t *touched_count; // per query
     uint32_t *overflow;      // per query: hash insertions that found no slot
     unsigned long long *steps;  // per query
