@@ -38,11 +38,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli"],
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
                          "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22; "
-                         "cli: the reference's pagerank command on R-MAT scale 16, binary vs CPU path")
+                         "cli: the reference's pagerank command on R-MAT scale 16, binary vs CPU path; "
+                         "ingest: loading an R-MAT scale 22 edge list, device vs host parsing")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
@@ -72,6 +73,8 @@ def main():
         return bench_serve(args)
     if args.workload == "cli":
         return bench_cli(args)
+    if args.workload == "ingest":
+        return bench_ingest(args)
 
     import numpy as np
 
@@ -593,6 +596,79 @@ def bench_cli(args):
         "vs_cpu": (steps / walk_s) / cpu[best_th][0],
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_ingest(args):
+    """Graph load (ptw_graph_load) of an R-MAT edge list written as text and as
+    binary: the file parsed and the CSR built on the device against the same on
+    the host threads (mmap + parallel parse), both followed by the same degree
+    relabel and upload.  The reference reads the text with one thread
+    (include/io.hpp:76-90).  Files are written to a temporary directory first
+    (not timed); each load is timed from the call to the graph being resident."""
+    import shutil
+    import statistics
+    import tempfile
+
+    import ptwalker_amd as ptw
+
+    tmp = tempfile.mkdtemp()
+    try:
+        h = ptw.Context(-1)
+        gi = h.rmat(scale=args.scale, edge_factor=16, seed=1)
+        paths = {"text": os.path.join(tmp, "g.txt"), "binary": os.path.join(tmp, "g.bin")}
+        h.write(paths["text"], ptw.FORMAT_TEXT)
+        h.write(paths["binary"], ptw.FORMAT_BINARY)
+        h.close()
+        ctx = ptw.Context(0)
+        res = {}
+        for fmt_name, fmt in (("text", ptw.FORMAT_TEXT), ("binary", ptw.FORMAT_BINARY)):
+            for where, flag in (("device", ptw.GRAPH_INGEST_DEVICE), ("host", ptw.GRAPH_INGEST_HOST)):
+                ts = []
+                for i in range(args.warmup + args.steps):
+                    t = time.perf_counter()
+                    g2 = ctx.load(paths[fmt_name], fmt, flags=flag)
+                    if i >= args.warmup:
+                        ts.append(time.perf_counter() - t)
+                assert g2.num_edges == gi.num_edges
+                res[f"{fmt_name}_{where}_s"] = statistics.median(ts)
+                print(f"# {fmt_name} {where}: {statistics.median(ts):.3f} s", file=sys.stderr, flush=True)
+        size = os.path.getsize(paths["text"])
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        line = {
+            "metric": "edges/s loaded",
+            "value": gi.num_edges / res["text_device_s"],
+            "unit": "edges/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": 1e3 * res["text_device_s"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "u32",
+            "data": "synthetic R-MAT graph written as text and binary edge lists in-process (seeded); no dataset",
+            "config": {"workload": f"ptw_graph_load of R-MAT scale {args.scale} ef16 symmetrized "
+                                   f"({gi.num_edges} lines, {size / 1e9:.2f} GB of text), load to resident graph",
+                       "vertices": gi.num_vertices, "edges": gi.num_edges, "text_bytes": size},
+            "seconds": {k: round(v, 3) for k, v in res.items()},
+            "device_over_host": {"text": res["text_host_s"] / res["text_device_s"],
+                                 "binary": res["binary_host_s"] / res["binary_device_s"]},
+            "roofline": {"bound": "hbm", "kernel": "whole load (file copy, parse, sorts, download, relabel, upload)",
+                         "achieved": size / res["text_device_s"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": size / res["text_device_s"] / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "note": "end-to-end input bytes per second; PCIe and the host relabel bound it"},
+            "cpu_baseline": {"value": gi.num_edges / res["text_host_s"], "unit": "edges/s",
+                             "cores": min(cores, 16), "kind": "port",
+                             "sample": "the same load with the file parsed and the CSR built by host threads "
+                                       "(mmap + parallel parse), same relabel and upload"},
+        }
+        print(json.dumps(line), flush=True)
+        ctx.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def cpu_baseline(ctx, hub, args):

@@ -74,6 +74,8 @@ extern "C" {
 /* ptw_graph_rmat flags */
 #define PTW_RMAT_KEEP_ISOLATED 1u /* keep degree-0 ids (default: compact them away) */
 #define PTW_GRAPH_NO_RELABEL 2u   /* keep graph ids on the device (no degree relabel) */
+#define PTW_GRAPH_INGEST_HOST 4u  /* ptw_graph_load: parse and build the CSR on the host */
+#define PTW_GRAPH_INGEST_DEVICE 8u /* ptw_graph_load: parse and build the CSR on the device */
 
 #define PTW_INVALID_VERTEX 0xFFFFFFFFu
 
@@ -139,7 +141,11 @@ int ptw_version(void);
 /* Graph::load(path, GraphFormat, as_undirected, bodary),
  * src/core/graph.hpp:277-366, with read_text_graph / read_binary_graph,
  * include/io.hpp:39-53,76-90.  Ids are compacted by first appearance and
- * adjacency keeps file order.  flags: PTW_GRAPH_NO_RELABEL. */
+ * adjacency keeps file order.  flags: PTW_GRAPH_NO_RELABEL, and where the
+ * file is parsed and the CSR built: PTW_GRAPH_INGEST_HOST (threads over an
+ * mmap) or PTW_GRAPH_INGEST_DEVICE (the file is copied to HBM, parsed and
+ * compacted there, and the CSR comes back); by default the device for files
+ * of 16 MiB and more on a device context.  Both give the same graph. */
 int ptw_graph_load(ptw_ctx *ctx, const char *path, int format, int as_undirected, uint32_t flags);
 
 /* A CSR in graph ids (row_ptr[V + 1], col[E]); replaces the graph build of

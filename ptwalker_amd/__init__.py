@@ -66,6 +66,8 @@ FLAG_WEIGHTED = 4
 
 RMAT_KEEP_ISOLATED = 1
 GRAPH_NO_RELABEL = 2
+GRAPH_INGEST_HOST = 4
+GRAPH_INGEST_DEVICE = 8
 
 INVALID_VERTEX = 0xFFFFFFFF
 

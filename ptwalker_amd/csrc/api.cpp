@@ -1,6 +1,7 @@
 // C ABI of the walk engine (include/ptw.h).  Host orchestration only: the
 // walk itself runs in the HIP kernels of walk.hip; there is no CPU path.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <chrono>
 #include <cmath>
@@ -246,10 +247,20 @@ int ptw_graph_load(ptw_ctx *ctx, const char *path, int format, int as_undirected
     if (!ctx) return PTW_ERR_INVALID_ARG;
     try {
         if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+        if ((flags & PTW_GRAPH_INGEST_HOST) && (flags & PTW_GRAPH_INGEST_DEVICE))
+            return fail(ctx, PTW_ERR_INVALID_ARG, "PTW_GRAPH_INGEST_HOST and PTW_GRAPH_INGEST_DEVICE exclude each other");
+        bool on_device = (flags & PTW_GRAPH_INGEST_DEVICE) != 0;
+        if (on_device && ctx->device < 0)
+            return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: device ingest needs a HIP device");
+        if (!on_device && !(flags & PTW_GRAPH_INGEST_HOST) && ctx->device >= 0 && path) {
+            struct stat st;
+            on_device = stat(path, &st) == 0 && (uint64_t)st.st_size >= (16ull << 20);
+        }
         HostCsr g;
-        int rc = ptw::load_edge_list(path, format, as_undirected != 0, g, ctx->err);
+        int rc = on_device ? ptw::device_load_edge_list(ctx->stream, path, format, as_undirected != 0, g, ctx->err)
+                           : ptw::load_edge_list(path, format, as_undirected != 0, g, ctx->err);
         if (rc) return rc;
-        if (g.V == 0) return fail(ctx, PTW_ERR_FORMAT, std::string("graph file '") + path + "' holds no edges");
+        if (g.V == 0) return fail(ctx, PTW_ERR_FORMAT, std::string("graph file '") + (path ? path : "") + "' holds no edges");
         return install_graph(ctx, std::move(g), flags);
     } catch (const std::bad_alloc &) {
         return fail(ctx, PTW_ERR_OOM, "host allocation failed while loading the graph");

@@ -112,6 +112,10 @@ struct LaunchShape {
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s);
+
+// Graph ingest on the device with the host loader's semantics (ingest.hip).
+int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, HostCsr &g,
+                          std::string &err);
 hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
                                       uint32_t V, uint64_t *out, bool accumulate,
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);

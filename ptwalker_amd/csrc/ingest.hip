@@ -1,0 +1,503 @@
+// Graph ingest on the device: the edge-list file is copied to HBM, split
+// into lines, parsed, compacted to first-appearance ids and turned into a
+// CSR there; only the finished CSR comes back to the host.
+//
+// The result is the host loader's, bit for bit (load_edge_list in
+// graph_host.cpp, which follows Graph::load, src/core/graph.hpp:293-366, and
+// read_text_graph / read_binary_graph, include/io.hpp:39-53,76-90):
+//   text   lines end at '\n'; an empty line or one starting with '#' is
+//          skipped; a line holds an edge if it starts with two unsigned
+//          decimal numbers (blanks, an optional '+' before each; values
+//          saturate at 2^32 - 1), anything after them is ignored
+//   binary u32 (src, dst) pairs; a trailing partial pair is dropped
+//   ids    assigned in order of first appearance over src0, dst0, src1, ...
+//   CSR    adjacency in edge order; with as_undirected every reversed edge
+//          follows all forward edges
+//
+// Steps:
+//   1. newlines per 64 KiB tile, an exclusive scan, and the newline
+//      positions (u64: files may exceed 4 GiB)
+//   2. one thread per line parses it into a (src, dst) pair and a flag;
+//      an order-preserving select keeps the edges
+//   3. a stable radix sort of (raw id, position) over all 2m endpoints: the
+//      first element of each run holds the id's first position; sorting the
+//      runs by that position gives the new ids, scattered back to positions
+//   4. a stable radix sort of the edges by new source id gives the adjacency
+//      in edge order; a degree histogram and a scan give row_ptr
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ptw.h"
+#include "engine.hpp"
+
+namespace ptw {
+
+namespace {
+
+constexpr uint64_t kTile = 65536;  // bytes per newline-count tile
+constexpr int kTileThreads = 256;  // each thread scans kTile / kTileThreads = 256 bytes
+
+__device__ __forceinline__ uint32_t newlines_in(uint4 x)
+{
+    uint32_t n = 0;
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        // bytes equal to 0x0A are the zero bytes of y = w ^ 0x0A0A0A0A; the
+        // carry-free form flags exactly those (the shorter borrow form can
+        // also flag a 0x01 byte above a zero byte)
+        const uint32_t y = w[k] ^ 0x0A0A0A0Au;
+        const uint32_t t = (y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+        n += __popc(~(t | y | 0x7F7F7F7Fu));
+    }
+    return n;
+}
+
+// Bytes are read as uint4 from a buffer padded with zeros to a multiple of
+// kTile, so every tile is whole.
+__global__ __launch_bounds__(kTileThreads) void count_newlines_kernel(const uint4 *data, uint64_t tiles,
+                                                                     uint64_t *tile_count)
+{
+    __shared__ uint32_t s_sum;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        if (threadIdx.x == 0) s_sum = 0;
+        __syncthreads();
+        const uint4 *p = data + t * (kTile / 16) + (uint64_t)threadIdx.x * (kTile / 16 / kTileThreads);
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < kTile / 16 / kTileThreads; ++k) n += newlines_in(p[k]);
+        atomicAdd(&s_sum, n);
+        __syncthreads();
+        if (threadIdx.x == 0) tile_count[t] = s_sum;
+        __syncthreads();
+    }
+}
+
+// Positions of the newlines, in file order: tile base + the thread's
+// exclusive prefix inside the tile + order within the thread's bytes.
+__global__ __launch_bounds__(kTileThreads) void newline_positions_kernel(const uint4 *data, uint64_t tiles,
+                                                                        const uint64_t *tile_base, uint64_t *pos)
+{
+    __shared__ uint32_t s_count[kTileThreads];
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t first = t * kTile + (uint64_t)threadIdx.x * (kTile / kTileThreads);
+        const uint4 *p = data + first / 16;
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < kTile / 16 / kTileThreads; ++k) n += newlines_in(p[k]);
+        s_count[threadIdx.x] = n;
+        __syncthreads();
+        for (uint32_t off = 1; off < kTileThreads; off <<= 1) {  // inclusive scan
+            const uint32_t x = threadIdx.x >= off ? s_count[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s_count[threadIdx.x] += x;
+            __syncthreads();
+        }
+        uint64_t out = tile_base[t] + s_count[threadIdx.x] - n;
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(p);
+        for (uint32_t k = 0; k < kTile / kTileThreads && n; ++k) {
+            if (b[k] == '\n') {
+                pos[out++] = first + k;
+                --n;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool is_blank(unsigned char c)
+{
+    return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+
+__device__ bool parse_pair_dev(const unsigned char *p, const unsigned char *end, uint32_t &a, uint32_t &b)
+{
+    uint64_t v[2];
+    for (int k = 0; k < 2; ++k) {
+        while (p < end && is_blank(*p)) ++p;
+        if (p < end && *p == '+') ++p;
+        if (p >= end || *p < '0' || *p > '9') return false;
+        uint64_t x = 0;
+        while (p < end && *p >= '0' && *p <= '9') {
+            x = x * 10 + (uint64_t)(*p - '0');
+            if (x > 0xFFFFFFFFull) x = 0xFFFFFFFFull;
+            ++p;
+        }
+        v[k] = x;
+    }
+    a = (uint32_t)v[0];
+    b = (uint32_t)v[1];
+    return true;
+}
+
+// Line i runs from just after newline i - 1 (or the file start) to newline
+// i (or the file end).
+__global__ void parse_lines_kernel(const unsigned char *file, uint64_t size, const uint64_t *nl, uint64_t n_nl,
+                                   uint64_t lines, uint2 *pairs, uint8_t *flags)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lines; i += stride) {
+        const uint64_t s = i == 0 ? 0 : nl[i - 1] + 1;
+        const uint64_t e = i < n_nl ? nl[i] : size;
+        uint32_t a = 0, b = 0;
+        const bool ok = e > s && file[s] != '#' && parse_pair_dev(file + s, file + e, a, b);
+        pairs[i] = uint2{a, b};
+        flags[i] = ok ? 1 : 0;
+    }
+}
+
+__global__ void endpoints_kernel(const uint2 *pairs, uint64_t m, uint32_t *keys, uint32_t *vals)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const uint2 e = pairs[i];
+        keys[2 * i] = e.x;
+        keys[2 * i + 1] = e.y;
+        vals[2 * i] = (uint32_t)(2 * i);
+        vals[2 * i + 1] = (uint32_t)(2 * i + 1);
+    }
+}
+
+__global__ void run_heads_kernel(const uint32_t *skeys, uint64_t n, uint32_t *head)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        head[k] = (k == 0 || skeys[k] != skeys[k - 1]) ? 1u : 0u;
+}
+
+// run_of = exclusive scan of head: a head's run index; first_pos[run] is the
+// head's position (the smallest of its run: the sort is stable and positions
+// were ascending).
+__global__ void first_positions_kernel(const uint32_t *head, const uint32_t *run_excl, const uint32_t *svals,
+                                       uint64_t n, uint32_t *first_pos, uint32_t *run_idx)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        if (head[k]) {
+            const uint32_t r = run_excl[k];
+            first_pos[r] = svals[k];
+            run_idx[r] = r;
+        }
+    }
+}
+
+__global__ void rank_runs_kernel(const uint32_t *order, uint32_t runs, uint32_t *new_id)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += stride)
+        new_id[order[r]] = (uint32_t)r;
+}
+
+// Every endpoint position gets its new id: the run of sorted element k is
+// head-inclusive-scan - 1, i.e. run_excl[k] + head[k] - 1.
+__global__ void scatter_ids_kernel(const uint32_t *head, const uint32_t *run_excl, const uint32_t *svals,
+                                   const uint32_t *new_id, uint64_t n, uint32_t *mapped)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        mapped[svals[k]] = new_id[run_excl[k] + head[k] - 1];
+}
+
+__global__ void edges_kernel(const uint32_t *mapped, uint64_t m, int undirected, uint32_t *esrc, uint32_t *edst,
+                             uint32_t *deg)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const uint32_t s = mapped[2 * i], d = mapped[2 * i + 1];
+        esrc[i] = s;
+        edst[i] = d;
+        atomicAdd(&deg[s], 1u);
+        if (undirected) {
+            esrc[m + i] = d;
+            edst[m + i] = s;
+            atomicAdd(&deg[d], 1u);
+        }
+    }
+}
+
+uint32_t grid_for(uint64_t n)
+{
+    uint64_t b = (n + 255) / 256;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 65536));
+}
+
+// Device buffers released on every exit path.
+struct Buffers {
+    std::vector<void *> p;
+    ~Buffers()
+    {
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t get(T **out, uint64_t count)
+    {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<uint64_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) p.push_back(q);
+        *out = (T *)q;
+        return e;
+    }
+};
+
+struct Pinned {
+    void *p = nullptr;
+    ~Pinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct Fd {
+    int fd = -1;
+    ~Fd()
+    {
+        if (fd >= 0) close(fd);
+    }
+};
+
+#define ING_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            err = std::string("device ingest: ") + #expr + ": " + hipGetErrorString(e_); \
+            return e_ == hipErrorOutOfMemory ? PTW_ERR_OOM : PTW_ERR_HIP;          \
+        }                                                                          \
+    } while (0)
+
+// Stable radix sort of (key, value) pairs over `bits` key bits, with its
+// temporary storage.
+template <typename K, typename V>
+int sort_pairs(Buffers &B, const K *kin, K *kout, const V *vin, V *vout, uint64_t n, int bits, hipStream_t s,
+               std::string &err)
+{
+    size_t tmp = 0;
+    ING_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, bits, s));
+    uint8_t *t = nullptr;
+    ING_TRY(B.get(&t, tmp));
+    ING_TRY(hipcub::DeviceRadixSort::SortPairs(t, tmp, kin, kout, vin, vout, n, 0, bits, s));
+    return PTW_OK;
+}
+
+template <typename T, typename U>
+int exclusive_sum(Buffers &B, const T *in, U *out, uint64_t n, hipStream_t s, std::string &err)
+{
+    size_t tmp = 0;
+    ING_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s));
+    uint8_t *t = nullptr;
+    ING_TRY(B.get(&t, tmp));
+    ING_TRY(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, s));
+    return PTW_OK;
+}
+
+int bits_for(uint64_t max_value)
+{
+    int b = 1;
+    while (b < 32 && (max_value >> b) != 0) ++b;
+    return b;
+}
+
+}  // namespace
+
+int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, HostCsr &g,
+                          std::string &err)
+{
+    if (!path) {
+        err = "graph path is NULL";
+        return PTW_ERR_INVALID_ARG;
+    }
+    if (format != PTW_FORMAT_TEXT && format != PTW_FORMAT_BINARY) {
+        err = "unknown graph format " + std::to_string(format);
+        return PTW_ERR_INVALID_ARG;
+    }
+    Fd f;
+    f.fd = open(path, O_RDONLY);
+    if (f.fd < 0) {
+        err = std::string("cannot open graph file '") + path + "': " + strerror(errno);
+        return PTW_ERR_IO;
+    }
+    struct stat st;
+    if (fstat(f.fd, &st) != 0) {
+        err = std::string("cannot stat '") + path + "'";
+        return PTW_ERR_IO;
+    }
+    const uint64_t size = (uint64_t)st.st_size;
+    Buffers B;
+    int rc;
+    // ---- 1. the file, padded with zeros to whole tiles ----
+    const uint64_t tiles = (size + kTile - 1) / kTile;
+    const uint64_t padded = std::max<uint64_t>(tiles, 1) * kTile;
+    unsigned char *file = nullptr;
+    ING_TRY(B.get(&file, padded));
+    ING_TRY(hipMemsetAsync(file + (padded - kTile), 0, kTile, s));
+    {
+        const uint64_t chunk = 64ull << 20;
+        Pinned stage;
+        ING_TRY(hipHostMalloc(&stage.p, 2 * chunk, hipHostMallocDefault));
+        hipEvent_t done[2];
+        ING_TRY(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+        ING_TRY(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+        struct Ev {
+            hipEvent_t *e;
+            ~Ev()
+            {
+                (void)hipEventDestroy(e[0]);
+                (void)hipEventDestroy(e[1]);
+            }
+        } ev{done};
+        ING_TRY(hipEventRecord(done[0], s));
+        ING_TRY(hipEventRecord(done[1], s));
+        uint64_t off = 0;
+        for (int slot = 0; off < size; slot ^= 1) {
+            ING_TRY(hipEventSynchronize(done[slot]));  // the slot's previous copy has left
+            char *buf = (char *)stage.p + slot * chunk;
+            const uint64_t want = std::min(chunk, size - off);
+            uint64_t got = 0;
+            while (got < want) {
+                const ssize_t r = pread(f.fd, buf + got, want - got, (off_t)(off + got));
+                if (r <= 0) {
+                    err = std::string("short read from '") + path + "'";
+                    (void)hipStreamSynchronize(s);
+                    return PTW_ERR_IO;
+                }
+                got += (uint64_t)r;
+            }
+            ING_TRY(hipMemcpyAsync(file + off, buf, want, hipMemcpyHostToDevice, s));
+            ING_TRY(hipEventRecord(done[slot], s));
+            off += want;
+        }
+        ING_TRY(hipStreamSynchronize(s));
+        // the tail of the last tile beyond the file must read as zeros
+        if (size % kTile) ING_TRY(hipMemsetAsync(file + size, 0, padded - size, s));
+    }
+    // ---- 2. edges ----
+    uint2 *pairs = nullptr;
+    uint64_t m = 0;
+    if (format == PTW_FORMAT_BINARY) {
+        m = size / 8;  // a trailing partial record is dropped
+        pairs = reinterpret_cast<uint2 *>(file);
+    } else {
+        uint64_t *tile_count = nullptr;  // u64 so that the scan accumulates in 64 bits
+        uint64_t *tile_base = nullptr;
+        ING_TRY(B.get(&tile_count, tiles));
+        ING_TRY(B.get(&tile_base, tiles));
+        if (tiles) {
+            hipLaunchKernelGGL(count_newlines_kernel, dim3(grid_for(tiles * 256)), dim3(kTileThreads), 0, s,
+                               reinterpret_cast<const uint4 *>(file), tiles, tile_count);
+            ING_TRY(hipGetLastError());
+            if ((rc = exclusive_sum(B, tile_count, tile_base, tiles, s, err))) return rc;
+        }
+        uint64_t last_base = 0, last_count = 0;
+        if (tiles) {
+            ING_TRY(hipMemcpyAsync(&last_base, tile_base + tiles - 1, 8, hipMemcpyDeviceToHost, s));
+            ING_TRY(hipMemcpyAsync(&last_count, tile_count + tiles - 1, 8, hipMemcpyDeviceToHost, s));
+        }
+        unsigned char last_byte = '\n';
+        if (size) ING_TRY(hipMemcpyAsync(&last_byte, file + size - 1, 1, hipMemcpyDeviceToHost, s));
+        ING_TRY(hipStreamSynchronize(s));
+        const uint64_t n_nl = last_base + last_count;
+        const uint64_t lines = n_nl + (size && last_byte != '\n' ? 1 : 0);
+        uint64_t *nl = nullptr;
+        ING_TRY(B.get(&nl, n_nl));
+        if (n_nl) {
+            hipLaunchKernelGGL(newline_positions_kernel, dim3(grid_for(tiles * 256)), dim3(kTileThreads), 0, s,
+                               reinterpret_cast<const uint4 *>(file), tiles, tile_base, nl);
+            ING_TRY(hipGetLastError());
+        }
+        uint2 *all = nullptr;
+        uint8_t *flags = nullptr;
+        uint64_t *n_sel = nullptr;
+        ING_TRY(B.get(&all, lines));
+        ING_TRY(B.get(&flags, lines));
+        ING_TRY(B.get(&pairs, lines));
+        ING_TRY(B.get(&n_sel, 1));
+        if (lines) {
+            hipLaunchKernelGGL(parse_lines_kernel, dim3(grid_for(lines)), dim3(256), 0, s, file, size, nl, n_nl,
+                               lines, all, flags);
+            ING_TRY(hipGetLastError());
+            size_t tmp = 0;
+            ING_TRY(hipcub::DeviceSelect::Flagged(nullptr, tmp, all, flags, pairs, n_sel, (int64_t)lines, s));
+            uint8_t *t = nullptr;
+            ING_TRY(B.get(&t, tmp));
+            ING_TRY(hipcub::DeviceSelect::Flagged(t, tmp, all, flags, pairs, n_sel, (int64_t)lines, s));
+            ING_TRY(hipMemcpyAsync(&m, n_sel, 8, hipMemcpyDeviceToHost, s));
+            ING_TRY(hipStreamSynchronize(s));
+        }
+    }
+    const uint64_t total = as_undirected ? 2 * m : m;
+    if (total >= 0xFFFFFFFFull || 2 * m >= 0xFFFFFFFFull) {
+        err = "graph has " + std::to_string(total) + " edges; u32 offsets hold at most 2^32 - 2";
+        return PTW_ERR_UNSUPPORTED;
+    }
+    if (m == 0) {
+        g = HostCsr{};
+        return PTW_OK;
+    }
+    // ---- 3. first-appearance ids ----
+    const uint64_t n = 2 * m;
+    uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *svals = nullptr;
+    ING_TRY(B.get(&keys, n));
+    ING_TRY(B.get(&vals, n));
+    ING_TRY(B.get(&skeys, n));
+    ING_TRY(B.get(&svals, n));
+    hipLaunchKernelGGL(endpoints_kernel, dim3(grid_for(m)), dim3(256), 0, s, pairs, m, keys, vals);
+    ING_TRY(hipGetLastError());
+    if ((rc = sort_pairs(B, keys, skeys, vals, svals, n, 32, s, err))) return rc;
+    uint32_t *head = keys, *run_excl = vals;  // reuse
+    hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n)), dim3(256), 0, s, skeys, n, head);
+    ING_TRY(hipGetLastError());
+    if ((rc = exclusive_sum(B, head, run_excl, n, s, err))) return rc;
+    uint32_t last_excl = 0, last_head = 0;
+    ING_TRY(hipMemcpyAsync(&last_excl, run_excl + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipMemcpyAsync(&last_head, head + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipStreamSynchronize(s));
+    const uint32_t V = last_excl + last_head;
+    uint32_t *first_pos = nullptr, *run_idx = nullptr, *sfirst = nullptr, *order = nullptr, *new_id = nullptr;
+    ING_TRY(B.get(&first_pos, V));
+    ING_TRY(B.get(&run_idx, V));
+    ING_TRY(B.get(&sfirst, V));
+    ING_TRY(B.get(&order, V));
+    ING_TRY(B.get(&new_id, V));
+    hipLaunchKernelGGL(first_positions_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, svals, n,
+                       first_pos, run_idx);
+    ING_TRY(hipGetLastError());
+    if ((rc = sort_pairs(B, first_pos, sfirst, run_idx, order, V, bits_for(n - 1), s, err))) return rc;
+    hipLaunchKernelGGL(rank_runs_kernel, dim3(grid_for(V)), dim3(256), 0, s, order, V, new_id);
+    ING_TRY(hipGetLastError());
+    uint32_t *mapped = skeys;  // reuse: position -> new id
+    hipLaunchKernelGGL(scatter_ids_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, svals, new_id, n,
+                       mapped);
+    ING_TRY(hipGetLastError());
+    // ---- 4. CSR ----
+    const uint64_t E = total;
+    uint32_t *esrc = nullptr, *edst = nullptr, *ssrc = nullptr, *col = nullptr, *deg = nullptr, *row = nullptr;
+    ING_TRY(B.get(&esrc, E));
+    ING_TRY(B.get(&edst, E));
+    ING_TRY(B.get(&ssrc, E));
+    ING_TRY(B.get(&col, E));
+    ING_TRY(B.get(&deg, (uint64_t)V + 1));
+    ING_TRY(B.get(&row, (uint64_t)V + 1));
+    ING_TRY(hipMemsetAsync(deg, 0, ((uint64_t)V + 1) * 4, s));
+    hipLaunchKernelGGL(edges_kernel, dim3(grid_for(m)), dim3(256), 0, s, mapped, m, as_undirected ? 1 : 0, esrc,
+                       edst, deg);
+    ING_TRY(hipGetLastError());
+    if ((rc = sort_pairs(B, esrc, ssrc, edst, col, E, bits_for(V - 1), s, err))) return rc;
+    if ((rc = exclusive_sum(B, deg, row, (uint64_t)V + 1, s, err))) return rc;
+    g.V = V;
+    g.row_ptr.resize((size_t)V + 1);
+    g.col.resize(E);
+    ING_TRY(hipMemcpyAsync(g.row_ptr.data(), row, ((uint64_t)V + 1) * 4, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipMemcpyAsync(g.col.data(), col, E * 4, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipStreamSynchronize(s));
+    return PTW_OK;
+}
+
+}  // namespace ptw

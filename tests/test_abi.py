@@ -173,3 +173,14 @@ def test_rmat_quadrant_probabilities(host_ctx):
     gi = host_ctx.rmat(scale=6, edge_factor=4, a=0.0, b=1.0, c=0.0, seed=1)
     rp, col = host_ctx.export_csr()
     assert gi.num_vertices == 2 and gi.num_edges == 2
+
+
+def test_ingest_flags_checked(host_ctx, tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n1 2\n")
+    with pytest.raises(ptw.PtwError):
+        host_ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_HOST | ptw.GRAPH_INGEST_DEVICE)
+    with pytest.raises(ptw.PtwError):  # a host-only context has no device to ingest on
+        host_ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_DEVICE)
+    gi = host_ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_HOST)
+    assert gi.num_vertices == 3 and gi.num_edges == 2
