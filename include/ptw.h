@@ -52,6 +52,17 @@ extern "C" {
 #define PTW_APP_PPR 1
 #define PTW_APP_RWR 2
 #define PTW_APP_DEEPWALK 3
+/* Second-order (node2vec) walks of walk_len vertices: from cur, reached
+ * from prev, a neighbour x is taken with weight 1/p if x == prev, 1 if x is
+ * a neighbour of prev, 1/q otherwise; the first step is uniform.  The
+ * reference declares the app (src/core/node2vec.cpp, flags -p -q in
+ * include/option.hpp:447-482) but its sampler is not wired in
+ * (src/core/solver.hpp:155-159).  Decisions: up to 16 proposals, attempt a
+ * drawing Philox(seed; wid, step + a * 2^20, stream), j = mulhi64(r.x, r.y,
+ * deg), accepted iff r.z < floor(2^32 w(x) / max(1/p, 1, 1/q)); then one
+ * exact draw from those integer weights (64-bit r.y:r.x, attempt 16).  With
+ * p = q = 1 the paths are DEEPWALK's. */
+#define PTW_APP_NODE2VEC 4
 
 /* step-budget semantics for the probabilistic-termination apps */
 #define PTW_SEM_REFERENCE 0 /* L = 1 + Geometric(tp), src/core/pagerank.cpp:44-67 */
@@ -107,19 +118,21 @@ typedef struct ptw_walk_desc {
     int32_t start_mode;      /* PTW_START_* */
     int32_t dangling_policy; /* PTW_DANGLING_* */
     uint32_t source;         /* graph id, PTW_START_SOURCE */
-    uint32_t walk_len;       /* DEEPWALK: vertices per path (reference -l) */
+    uint32_t walk_len;       /* DEEPWALK, NODE2VEC: vertices per path (reference -l) */
     double tp;               /* termination probability (reference --sp), (0, 1] */
     uint64_t seed;           /* Philox key */
     uint64_t wid_base;       /* first walker id (for split or resumed runs) */
     uint64_t walkers;        /* walker count (reference -w) */
     uint32_t stream;         /* 4th Philox counter word (query id) */
     uint32_t flags;          /* PTW_FLAG_* */
+    double p;                /* NODE2VEC return parameter, [2^-16, 2^16] (reference -p) */
+    double q;                /* NODE2VEC in-out parameter, [2^-16, 2^16] (reference -q) */
 } ptw_walk_desc;
 
 typedef struct ptw_walk_out {
     uint64_t *counts; /* PAGERANK/PPR: u64[V] visit counts, or NULL */
     uint32_t *ends;   /* RWR: u32[walkers] end vertex per walker, or NULL */
-    uint32_t *paths;  /* DEEPWALK: u32[walkers * walk_len], or NULL */
+    uint32_t *paths;  /* DEEPWALK, NODE2VEC: u32[walkers * walk_len], or NULL */
     /* filled on return */
     uint64_t steps;    /* sum of transitions over walkers */
     uint64_t visits;   /* count increments (count apps) */

@@ -27,7 +27,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-APP_PAGERANK, APP_PPR, APP_RWR, APP_DEEPWALK = 0, 1, 2, 3
+APP_PAGERANK, APP_PPR, APP_RWR, APP_DEEPWALK, APP_NODE2VEC = 0, 1, 2, 3, 4
 SEM_REFERENCE, SEM_TEXTBOOK = 0, 1
 START_UNIFORM, START_SOURCE, START_ROUNDROBIN = 0, 1, 2
 INVALID = 0xFFFFFFFF
@@ -87,6 +87,9 @@ def lib():
         L.orc_walk_replay_weighted.argtypes = [_P, _P, ctypes.c_uint32, _P, _P, ctypes.POINTER(WalkDesc), _P, _P,
                                                _P, ctypes.POINTER(WalkStats), ctypes.c_int]
         L.orc_walk_replay_weighted.restype = ctypes.c_int
+        L.orc_node2vec_replay.argtypes = [_P, _P, ctypes.c_uint32, ctypes.POINTER(WalkDesc), ctypes.c_double,
+                                          ctypes.c_double, _P, ctypes.POINTER(WalkStats), ctypes.c_int]
+        L.orc_node2vec_replay.restype = ctypes.c_int
         L.orc_build_alias.argtypes = [_P, ctypes.c_uint32, _P, _P, _P]
         L.orc_build_alias.restype = ctypes.c_int
         L.orc_walk_length.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]
@@ -129,6 +132,40 @@ def continue_threshold(tp: float) -> int:
 
 def walk_length(seed: int, stream: int, wid: int, tp: float, semantics: int = SEM_REFERENCE) -> int:
     return int(lib().orc_walk_length(seed, stream, wid, tp, semantics))
+
+
+def node2vec_replay(row_ptr, col, walkers: int, walk_len: int, p: float, q: float, *, seed: int = 1,
+                    source: int = 0, start_mode: int = START_UNIFORM, stream: int = 0, wid_base: int = 0,
+                    threads: int = 0):
+    """Philox replay of the engine's node2vec walks (oracle.c orc_node2vec_replay).
+    Returns dict(paths [walkers, walk_len], steps, dangling)."""
+    rp, cl = _u32(row_ptr), _u32(col)
+    V = len(rp) - 1
+    d = WalkDesc(APP_NODE2VEC, SEM_REFERENCE, start_mode, source, 0.5, walk_len, seed, stream, wid_base, walkers)
+    paths = np.zeros(max(walkers * walk_len, 1), dtype=np.uint32)
+    st = WalkStats()
+    rc = lib().orc_node2vec_replay(rp.ctypes.data, cl.ctypes.data if len(cl) else None, V, ctypes.byref(d),
+                                   ctypes.c_double(p), ctypes.c_double(q), paths.ctypes.data, ctypes.byref(st),
+                                   threads)
+    if rc != 0:
+        raise ValueError("invalid oracle node2vec arguments")
+    return dict(paths=paths[: walkers * walk_len].reshape(walkers, walk_len), steps=st.steps, dangling=st.dangling)
+
+
+def node2vec_law(row_ptr, col, prev: int, cur: int, p: float, q: float):
+    """The second-order transition law from the definition (KnightKing,
+    third_party/knightking/test_node2vec.hpp:47-79): over cur's adjacency
+    positions, weight 1/p for prev, 1 for a neighbour of prev, 1/q otherwise,
+    normalized.  Returns {vertex: probability} (exact real weights)."""
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    nb_prev = set(int(x) for x in col[rp[prev]:rp[prev + 1]])
+    law = {}
+    for x in col[rp[cur]:rp[cur + 1]]:
+        x = int(x)
+        w = 1.0 / p if x == prev else (1.0 if x in nb_prev else 1.0 / q)
+        law[x] = law.get(x, 0.0) + w
+    tot = sum(law.values())
+    return {k: v / tot for k, v in law.items()}
 
 
 def walk_replay(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, seed: int = 1, source: int = 0,

@@ -161,6 +161,124 @@ int orc_walk_replay_weighted(const uint32_t *row_ptr, const uint32_t *col, uint3
     return 0;
 }
 
+static int cmp_u32(const void *a, const void *b)
+{
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+static int n2v_has(const uint32_t *a, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == x;
+}
+
+static uint64_t n2v_thr(double w, double M)
+{
+    double t = floor(w / M * 4294967296.0);
+    return t >= 4294967296.0 ? 4294967296ull : (uint64_t)t;
+}
+
+typedef struct {
+    const uint32_t *pn;
+    uint32_t pdeg, prev;
+    uint64_t w_ret, w_in, w_out;
+} n2v_ctx;
+
+static uint64_t n2v_w(const n2v_ctx *c, uint32_t x)
+{
+    if (x == c->prev) return c->w_ret;
+    return n2v_has(c->pn, c->pdeg, x) ? c->w_in : c->w_out;
+}
+
+int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, const orc_walk_desc *d,
+                        double p, double q, uint32_t *paths, orc_walk_stats *st, int threads)
+{
+    if (st) memset(st, 0, sizeof(*st));
+    if (V == 0 || d->walkers == 0) return 0;
+    if (d->walk_len == 0 || d->walk_len > (1u << 20)) return -1;
+    if (!(p >= 1.0 / 65536 && p <= 65536) || !(q >= 1.0 / 65536 && q <= 65536)) return -1;
+    if (d->start_mode == ORC_START_SOURCE && d->source >= V) return -1;
+    const uint64_t E = row_ptr[V];
+    uint32_t *scol = (uint32_t *)malloc((E ? E : 1) * sizeof(uint32_t));
+    if (!scol) return -1;
+    memcpy(scol, col, E * sizeof(uint32_t));
+    for (uint32_t v = 0; v < V; ++v) qsort(scol + row_ptr[v], row_ptr[v + 1] - row_ptr[v], 4, cmp_u32);
+    const double wr = 1.0 / p, wo = 1.0 / q;
+    double M = wr > 1.0 ? wr : 1.0;
+    if (wo > M) M = wo;
+    const uint64_t w_ret = n2v_thr(wr, M), w_in = n2v_thr(1.0, M), w_out = n2v_thr(wo, M);
+    uint64_t tot_steps = 0, tot_dangling = 0;
+    const int64_t W = (int64_t)d->walkers;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+ : tot_steps, tot_dangling)
+#else
+    (void)threads;
+#endif
+    for (int64_t w = 0; w < W; ++w) {
+        const uint64_t wid = d->wid_base + (uint64_t)w;
+        uint32_t r[4];
+        uint32_t v;
+        if (d->start_mode == ORC_START_SOURCE) {
+            v = d->source;
+        } else if (d->start_mode == ORC_START_ROUNDROBIN) {
+            v = (uint32_t)(wid % V);
+        } else {
+            draw(d->seed, d->stream, wid, 0xFFFFFFFFu, r);
+            v = orc_mulhi64(r[0], r[1], V);
+        }
+        uint32_t *path = paths ? paths + (uint64_t)w * d->walk_len : NULL;
+        if (path) path[0] = v;
+        uint32_t prev = ORC_INVALID, i = 0;
+        for (; i + 1 < d->walk_len; ++i) {
+            const uint32_t beg = row_ptr[v], deg = row_ptr[v + 1] - beg;
+            if (deg == 0) { tot_dangling++; break; }
+            draw(d->seed, d->stream, wid, i, r);
+            uint32_t x = 0;
+            if (i == 0) {
+                x = col[beg + orc_mulhi64(r[0], r[1], deg)];
+            } else {
+                n2v_ctx c = {scol + row_ptr[prev], row_ptr[prev + 1] - row_ptr[prev], prev, w_ret, w_in, w_out};
+                int found = 0;
+                for (uint32_t a = 0; a < 16 && !found; ++a) {
+                    if (a) draw(d->seed, d->stream, wid, i + (a << 20), r);
+                    const uint32_t y = col[beg + orc_mulhi64(r[0], r[1], deg)];
+                    if ((uint64_t)r[2] < n2v_w(&c, y)) { x = y; found = 1; }
+                }
+                if (!found) {
+                    uint64_t total = 0;
+                    for (uint32_t k = 0; k < deg; ++k) total += n2v_w(&c, col[beg + k]);
+                    draw(d->seed, d->stream, wid, i + (16u << 20), r);
+                    const uint64_t u = ((uint64_t)r[1] << 32) | r[0];
+                    const uint64_t pick = (uint64_t)(((unsigned __int128)u * total) >> 64);
+                    uint64_t cum = 0;
+                    x = col[beg + deg - 1];
+                    for (uint32_t k = 0; k < deg; ++k) {
+                        cum += n2v_w(&c, col[beg + k]);
+                        if (pick < cum) { x = col[beg + k]; break; }
+                    }
+                }
+            }
+            prev = v;
+            v = x;
+            if (path) path[i + 1] = v;
+        }
+        tot_steps += i;
+        if (path) for (uint32_t k = i + 1; k < d->walk_len; ++k) path[k] = ORC_INVALID;
+    }
+    free(scol);
+    if (st) {
+        st->steps = tot_steps;
+        st->dangling = tot_dangling;
+    }
+    return 0;
+}
+
 int orc_build_alias(const uint32_t *row_ptr, uint32_t V, const float *w, uint32_t *thr, uint32_t *alias)
 {
     uint64_t E = row_ptr[V];

@@ -52,7 +52,7 @@ extern "C" {
 #endif
 
 /* app kinds and semantics: identical numbering to include/ptw.h */
-enum { ORC_APP_PAGERANK = 0, ORC_APP_PPR = 1, ORC_APP_RWR = 2, ORC_APP_DEEPWALK = 3 };
+enum { ORC_APP_PAGERANK = 0, ORC_APP_PPR = 1, ORC_APP_RWR = 2, ORC_APP_DEEPWALK = 3, ORC_APP_NODE2VEC = 4 };
 enum { ORC_SEM_REFERENCE = 0, ORC_SEM_TEXTBOOK = 1 };
 enum { ORC_START_UNIFORM = 0, ORC_START_SOURCE = 1, ORC_START_ROUNDROBIN = 2 };
 
@@ -100,6 +100,19 @@ int orc_walk_replay_weighted(const uint32_t *row_ptr, const uint32_t *col, uint3
                              const uint32_t *alias_thr, const uint32_t *alias_idx,
                              const orc_walk_desc *d, uint64_t *counts, uint32_t *ends,
                              uint32_t *paths, orc_walk_stats *st, int threads);
+
+/* Second-order (node2vec) walks of d->walk_len vertices, replayed with the
+ * engine's decisions (node2vec app of include/ptw.h).  From cur, reached
+ * from prev, neighbour x weighs 1/p (x == prev), 1 (x a neighbour of prev) or
+ * 1/q; the integer weights are floor(2^32 w / max(1/p, 1, 1/q)).  Step i
+ * (i >= 1): attempt a = 0..15 draws Philox(wid, i + a * 2^20), proposes
+ * col[beg + mulhi64(r.x, r.y, deg)] and accepts iff r.z < its weight; then
+ * Philox(wid, i + 16 * 2^20) picks exactly: floor((r.y:r.x) * total / 2^64)
+ * against the running sum of the weights in adjacency order.  Step 0 is a
+ * uniform pick.  Requires 1 <= walk_len <= 2^20 and p, q in [2^-16, 2^16].
+ * The membership test uses a sorted copy of each adjacency. */
+int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, const orc_walk_desc *d,
+                        double p, double q, uint32_t *paths, orc_walk_stats *st, int threads);
 
 /* Integer alias tables (Vose's method, exact integer masses) from float
  * weights in CSR order.  Per vertex: q_k = clamp(floor(w_k / w_max 2^24 +

@@ -49,6 +49,7 @@ APP_PAGERANK = 0
 APP_PPR = 1
 APP_RWR = 2
 APP_DEEPWALK = 3
+APP_NODE2VEC = 4
 
 SEM_REFERENCE = 0
 SEM_TEXTBOOK = 1
@@ -107,6 +108,8 @@ class WalkDesc(ctypes.Structure):
         ("walkers", ctypes.c_uint64),
         ("stream", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("p", ctypes.c_double),
+        ("q", ctypes.c_double),
     ]
 
 
@@ -687,6 +690,18 @@ class DeepWalk:
     """src/core/deepwalk.cpp:10-42: fixed walk_len vertices per path (walk_len - 1 steps)."""
     walk_len: int
     start: int = START_UNIFORM
+    source: Optional[int] = None
+
+
+@dataclass
+class Node2Vec:
+    """Second-order walks of walk_len vertices (src/core/node2vec.cpp, flags -p -q): from cur,
+    reached from prev, neighbour x weighs 1/p if x == prev, 1 if x neighbours prev, 1/q otherwise."""
+    walk_len: int
+    p: float
+    q: float
+    start: int = START_UNIFORM
+    source: Optional[int] = None
 
 
 def make_desc(app, walkers: int, seed: int = 1, wid_base: int = 0, stream: int = 0,
@@ -710,6 +725,12 @@ def make_desc(app, walkers: int, seed: int = 1, wid_base: int = 0, stream: int =
             d.start_mode, d.source = START_SOURCE, app.source
     elif isinstance(app, DeepWalk):
         d.app, d.walk_len, d.start_mode = APP_DEEPWALK, app.walk_len, app.start
+        if app.source is not None:
+            d.start_mode, d.source = START_SOURCE, app.source
+    elif isinstance(app, Node2Vec):
+        d.app, d.walk_len, d.start_mode, d.p, d.q = APP_NODE2VEC, app.walk_len, app.start, app.p, app.q
+        if app.source is not None:
+            d.start_mode, d.source = START_SOURCE, app.source
     else:
         raise TypeError(f"unknown app {app!r}")
     return d

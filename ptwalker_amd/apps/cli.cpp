@@ -44,6 +44,8 @@ struct Opts {
     uint64_t walkers = 0;
     long long walk_len = 0;
     double sp = 0.0;
+    bool have_p = false, have_q = false;
+    double n2v_p = 1.0, n2v_q = 1.0;
     long long walker_size = 4196, degree_bod = 32;
     // engine extras
     uint64_t seed = 1;
@@ -78,6 +80,10 @@ void usage(FILE *f, const char *prog, Parser parser)
         fprintf(f, "      --db=[degree_boundary]            the boundary to partition high-low vertex\n");
     } else {
         fprintf(f, "      --sp=[startprb]                   [optional, RWR] termination probability (default 0.5)\n");
+    }
+    if (parser == Parser::NODE2VEC) {
+        fprintf(f, "      -p[p]                             node2vec parameter p\n");
+        fprintf(f, "      -q[q]                             node2vec parameter q\n");
     }
     fprintf(f, "      --seed=[seed]                     [optional] Philox seed (default 1)\n");
     fprintf(f, "      --source=[vertex]                 [optional] start vertex (PPR, RWR)\n");
@@ -140,7 +146,7 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
 {
     Opts o;
     // short flags taking a value, long flags taking a value, long switches
-    const char *short_val = "fgtsewl";
+    const char *short_val = parser == Parser::NODE2VEC ? "fgtsewlpq" : "fgtsewl";
     const std::vector<std::string> long_val = {"socket-mapping", "mem", "sp", "ws", "db", "seed", "source",
                                                "device", "out", "semantics", "dangling", "start"};
     for (int i = 1; i < argc; ++i) {
@@ -174,7 +180,7 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
             }
             bool known = false;
             for (auto &k : long_val) known |= (k == key);
-            if (!known || (parser == Parser::WALK && (key == "ws" || key == "db")))
+            if (!known || (parser != Parser::RWPT && (key == "ws" || key == "db")))
                 parse_error(prog, parser, "Flag could not be matched: " + key);
             if (eq != std::string::npos) {
                 val = body.substr(eq + 1);
@@ -198,6 +204,8 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
         else if (key == "e") { if (!to_ll(val, ll)) bad(); o.epoch = ll; o.have_epoch = true; }
         else if (key == "w") { if (!to_u64(val, u)) bad(); o.walkers = u; o.have_walkers = true; }
         else if (key == "l") { if (!to_ll(val, ll)) bad(); o.walk_len = ll; o.have_len = true; }
+        else if (key == "p") { if (!to_double(val, dv)) bad(); o.n2v_p = dv; o.have_p = true; }
+        else if (key == "q") { if (!to_double(val, dv)) bad(); o.n2v_q = dv; o.have_q = true; }
         else if (key == "socket-mapping") { o.socket_mapping = val; }
         else if (key == "mem") { if (!to_u64(val, u)) bad(); o.mem_gib = u; }
         else if (key == "sp") { if (!to_double(val, dv)) bad(); o.sp = dv; o.have_sp = true; }
@@ -237,6 +245,10 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
     } else {
         if (!o.have_len) check_failed("walk_len_flag");
     }
+    if (parser == Parser::NODE2VEC) {
+        if (!o.have_p) check_failed("p_flag");
+        if (!o.have_q) check_failed("q_flag");
+    }
     return o;
 }
 
@@ -252,6 +264,7 @@ const char *app_name(int app)
     case PTW_APP_PAGERANK: return "pagerank";
     case PTW_APP_PPR: return "PPR";
     case PTW_APP_RWR: return "RWR";
+    case PTW_APP_NODE2VEC: return "node2vec";
     default: return "deepwalk";
     }
 }
@@ -274,8 +287,9 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
     };
     fprintf(stderr, "Load graph %s\n", o.graph.c_str());
     const double tl = now_s();
+    // node2vec loads its graph as undirected (src/core/node2vec.cpp:18)
     rc = ptw_graph_load(ctx, o.graph.c_str(), o.format == "text" ? PTW_FORMAT_TEXT : PTW_FORMAT_BINARY,
-                        o.undirected ? 1 : 0, 0);
+                        (o.undirected || app == PTW_APP_NODE2VEC) ? 1 : 0, 0);
     if (rc) return die("graph load");
     ptw_graph_info gi;
     ptw_graph_info_get(ctx, &gi);
@@ -303,7 +317,9 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
         d.start_mode = PTW_START_SOURCE;
         d.source = o.source;
     }
-    if (app == PTW_APP_DEEPWALK && o.walk_len < 1) check_failed("walk length >= 1");
+    if ((app == PTW_APP_DEEPWALK || app == PTW_APP_NODE2VEC) && o.walk_len < 1) check_failed("walk length >= 1");
+    d.p = o.n2v_p;
+    d.q = o.n2v_q;
     fprintf(stderr, "walker size %lld\n", o.walker_size);
 
     std::vector<uint64_t> counts;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -30,6 +31,7 @@ void free_graph(ptw_ctx *ctx)
     if (d.inv) (void)hipFree(d.inv);
     if (d.alias) (void)hipFree(d.alias);
     if (d.t_col) (void)hipFree(d.t_col);
+    if (d.scol) (void)hipFree(d.scol);
     if (d.out_deg) (void)hipFree(d.out_deg);
     if (d.piece_begin) (void)hipFree(d.piece_begin);
     if (d.vertex_piece) (void)hipFree(d.vertex_piece);
@@ -100,7 +102,7 @@ int warm_up(ptw_ctx *ctx)
     p.walkers = 0;
     p.counts_dev = ctx->counts_dev;
     p.scratch = ctx->scratch;
-    for (int app : {PTW_APP_PAGERANK, PTW_APP_RWR, PTW_APP_DEEPWALK}) {
+    for (int app : {PTW_APP_PAGERANK, PTW_APP_RWR, PTW_APP_DEEPWALK, PTW_APP_NODE2VEC}) {
         p.app = app;
         HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
         HIP_TRY(ctx, ptw::launch_walk(p, ctx->shape, ctx->num_cus, s));
@@ -114,6 +116,28 @@ int warm_up(ptw_ctx *ctx)
     HIP_TRY(ctx, hipMemcpyAsync(ctx->counts_stage, ctx->pinned + 8, (size_t)ctx->dev.V * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     return PTW_OK;
+}
+
+// Each vertex's neighbours sorted by device id, for the node2vec membership
+// test (is x a neighbour of the previous vertex); built once per graph.
+int ensure_sorted_adjacency(ptw_ctx *ctx)
+{
+    ptw::DeviceGraph &d = ctx->dev;
+    if (d.scol) return PTW_OK;
+    const HostCsr &h = ctx->host;
+    const uint32_t V = d.V;
+    std::vector<uint32_t> scol(h.E());
+    const bool rel = d.relabelled;
+    ptw::parallel_for(V, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t u = lo; u < hi; ++u) {
+            const uint32_t g = rel ? ctx->h_inv[u] : (uint32_t)u;
+            const uint32_t b = h.row_ptr[g], e = h.row_ptr[g + 1];
+            const uint32_t out = rel ? ctx->h_drow[u] : b;
+            for (uint32_t k = b; k < e; ++k) scol[out + (k - b)] = rel ? ctx->h_perm[h.col[k]] : h.col[k];
+            std::sort(scol.begin() + out, scol.begin() + out + (e - b));
+        }
+    });
+    return upload(ctx, &d.scol, scol.data(), scol.size());
 }
 
 int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
@@ -389,7 +413,8 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     const double t0 = now_ms();
     const ptw_walk_desc &d = *desc;
     const uint32_t V = ctx->dev.V;
-    if (d.app < PTW_APP_PAGERANK || d.app > PTW_APP_DEEPWALK) return fail(ctx, PTW_ERR_INVALID_ARG, "unknown app");
+    if (d.app < PTW_APP_PAGERANK || d.app > PTW_APP_NODE2VEC) return fail(ctx, PTW_ERR_INVALID_ARG, "unknown app");
+    const bool path_app = d.app == PTW_APP_DEEPWALK || d.app == PTW_APP_NODE2VEC;
     if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
         return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
     if (d.start_mode < PTW_START_UNIFORM || d.start_mode > PTW_START_ROUNDROBIN)
@@ -397,14 +422,23 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     if (d.dangling_policy != PTW_DANGLING_ERROR && d.dangling_policy != PTW_DANGLING_STOP)
         return fail(ctx, PTW_ERR_INVALID_ARG, "unknown dangling policy");
     const bool count_app = d.app == PTW_APP_PAGERANK || d.app == PTW_APP_PPR;
-    if (d.app != PTW_APP_DEEPWALK && !(d.tp > 0.0 && d.tp <= 1.0))
+    if (!path_app && !(d.tp > 0.0 && d.tp <= 1.0))
         return fail(ctx, PTW_ERR_INVALID_ARG, "termination probability must be in (0, 1]");
-    if (d.app == PTW_APP_DEEPWALK && d.walk_len == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "walk length must be >= 1");
+    if (path_app && d.walk_len == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "walk length must be >= 1");
+    if (d.app == PTW_APP_NODE2VEC) {
+        // attempts are counted in bits 20.. of the step word
+        if (d.walk_len > (1u << 20)) return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec walk length must be <= 2^20");
+        const double lo = 1.0 / 65536.0, hi = 65536.0;
+        if (!(d.p >= lo && d.p <= hi) || !(d.q >= lo && d.q <= hi))
+            return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec p and q must be in [2^-16, 2^16]");
+        if (d.flags & PTW_FLAG_WEIGHTED)
+            return fail(ctx, PTW_ERR_UNSUPPORTED, "node2vec on weighted graphs is not supported");
+    }
     if (d.start_mode == PTW_START_SOURCE && d.source >= V)
         return fail(ctx, PTW_ERR_INVALID_ARG, "source vertex " + std::to_string(d.source) + " out of range");
     if (count_app && !out->counts) return fail(ctx, PTW_ERR_INVALID_ARG, "counts output required");
     if (d.app == PTW_APP_RWR && !out->ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
-    if (d.app == PTW_APP_DEEPWALK && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
+    if (path_app && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
     if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
     if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
@@ -433,7 +467,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
             ends_dev = (uint32_t *)t_ends.p;
         }
     }
-    if (d.app == PTW_APP_DEEPWALK) {
+    if (path_app) {
         if (W > UINT64_MAX / 4 / d.walk_len) return fail(ctx, PTW_ERR_INVALID_ARG, "path buffer size overflows");
         if (on_dev) {
             paths_dev = out->paths;
@@ -487,6 +521,19 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     p.ends = ends_dev;
     p.paths = paths_dev;
     p.scratch = ctx->scratch;
+    if (d.app == PTW_APP_NODE2VEC) {
+        if (int rc = ensure_sorted_adjacency(ctx)) return rc;
+        p.scol = ctx->dev.scol;
+        // integer acceptance thresholds: floor(2^32 w / max(1/p, 1, 1/q)), in [2^-32.., 2^32]
+        const double wr = 1.0 / d.p, wo = 1.0 / d.q, M = std::max(std::max(wr, 1.0), wo);
+        auto thr = [M](double w) {
+            const double t = std::floor(w / M * 4294967296.0);
+            return t >= 4294967296.0 ? 4294967296ull : (uint64_t)t;
+        };
+        p.w_ret = thr(wr);
+        p.w_in = thr(1.0);
+        p.w_out = thr(wo);
+    }
 
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));

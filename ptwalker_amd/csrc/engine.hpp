@@ -32,6 +32,9 @@ struct DeviceGraph {
     // transpose (in-edges, cut into pieces) and out-degrees for the FP64
     // solver, built on first use
     uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
+    // each vertex's neighbours sorted by device id (node2vec membership
+    // tests), built on first use
+    uint32_t *scol = nullptr;
     uint64_t pieces = 0;
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
     bool relabelled = false;
@@ -91,6 +94,10 @@ struct WalkLaunch {
     uint32_t seed_lo, seed_hi, stream;
     uint64_t wid_base;
     uint64_t walkers;
+    // node2vec: sorted adjacency and the acceptance thresholds (in [0, 2^32])
+    // of a neighbour that is the previous vertex, a neighbour of it, or neither
+    const uint32_t *scol;
+    uint64_t w_ret, w_in, w_out;
     // outputs (device)
     unsigned long long *counts_dev;  // device-id counters (count apps)
     uint32_t *ends;                  // graph ids

@@ -28,7 +28,63 @@ namespace ptw {
 
 namespace {
 
-enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2 };
+enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2, KIND_N2V = 3 };
+
+// ---- node2vec: second-order neighbour choice ---------------------------------
+
+constexpr uint32_t kN2vAttempts = 16;  // proposals before the exact draw
+
+__device__ __forceinline__ bool sorted_has(const uint32_t *a, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && a[lo] == x;
+}
+
+// Integer weight of moving to x after coming from prev, whose neighbours are
+// pn[0, pdeg) sorted: the return, in and out thresholds.
+__device__ __forceinline__ uint64_t n2v_weight(uint32_t x, uint32_t prev, const uint32_t *pn, uint32_t pdeg,
+                                               uint64_t w_ret, uint64_t w_in, uint64_t w_out)
+{
+    if (x == prev) return w_ret;
+    return sorted_has(pn, pdeg, x) ? w_in : w_out;
+}
+
+// The next vertex from the row [beg, beg + deg) of the current vertex.
+// Attempt a proposes a uniform neighbour from Philox(wid, step + a * 2^20)
+// and accepts it iff r.z < its weight (weights are scaled so the largest is
+// 2^32); attempt 0's block is `r`, the step's own.  After kN2vAttempts
+// rejections one exact draw from the same integer weights follows, so the
+// law is exactly w(x) / sum w whichever attempt decides.
+__device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, u32x4 r,
+                             uint32_t beg, uint32_t deg, uint32_t prev)
+{
+    const uint32_t pb = p.row_ptr[prev];
+    const uint32_t pdeg = p.row_ptr[prev + 1] - pb;
+    const uint32_t *pn = p.scol + pb;
+    for (uint32_t a = 0; a < kN2vAttempts; ++a) {
+        if (a) r = philox4x32_10(wlo, whi, step + (a << 20), p.stream, p.seed_lo, p.seed_hi);
+        const uint32_t x = p.col[beg + mulhi64(r.x, r.y, deg)];
+        if ((uint64_t)r.z < n2v_weight(x, prev, pn, pdeg, p.w_ret, p.w_in, p.w_out)) return x;
+    }
+    uint64_t total = 0;  // < deg * 2^32 < 2^64
+    for (uint32_t k = 0; k < deg; ++k) total += n2v_weight(p.col[beg + k], prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);
+    r = philox4x32_10(wlo, whi, step + (kN2vAttempts << 20), p.stream, p.seed_lo, p.seed_hi);
+    const uint64_t pick = __umul64hi(((uint64_t)r.y << 32) | r.x, total);  // floor(u * total / 2^64)
+    uint64_t cum = 0;
+    for (uint32_t k = 0; k < deg; ++k) {
+        const uint32_t x = p.col[beg + k];
+        cum += n2v_weight(x, prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);
+        if (pick < cum) return x;
+    }
+    return p.col[beg + deg - 1];  // not reached: pick < total
+}
 
 __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
 {
@@ -115,6 +171,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     uint64_t w = 0;      // local walker index in [0, walkers)
     uint32_t wlo = 0, whi = 0;  // global walker id words
     uint32_t v = 0, step = 0;
+    uint32_t prev = 0;  // node2vec: the vertex before v
     uint64_t my_steps = 0;
     uint32_t my_dangling = 0;
     // Single-source walks return to the source often: its visits are counted
@@ -197,7 +254,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     v = p.perm ? p.perm[g] : g;
                     if (KIND == KIND_COUNT) count_visit(v);
                 }
-                if (KIND == KIND_PATH) {
+                if (KIND == KIND_PATH || KIND == KIND_N2V) {
                     pos = w * p.walk_len;
                     nbuf = 0;
                     path_append(p.inv ? p.inv[v] : v);
@@ -214,7 +271,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         if (active) {
             const u32x4 r = philox4x32_10(wlo, whi, step, p.stream, p.seed_lo, p.seed_hi);
             bool stop;
-            if (KIND == KIND_PATH) {
+            if (KIND == KIND_PATH || KIND == KIND_N2V) {
                 stop = step + 1 >= p.walk_len;
             } else {
                 const bool coin = (p.semantics == PTW_SEM_TEXTBOOK) || step >= 1;
@@ -228,22 +285,29 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     dangled = true;
                     stop = true;
                 } else {
-                    uint32_t j = mulhi64(r.x, r.y, deg);
-                    if (p.alias) {  // weighted: alias coin from the 4th word
-                        const uint2 a = p.alias[beg + j];
-                        if (r.w >= a.x) j = a.y;
+                    uint32_t nx;
+                    if (KIND == KIND_N2V && step > 0) {
+                        nx = n2v_pick(p, wlo, whi, step, r, beg, deg, prev);
+                    } else {
+                        uint32_t j = mulhi64(r.x, r.y, deg);
+                        if (p.alias) {  // weighted: alias coin from the 4th word
+                            const uint2 a = p.alias[beg + j];
+                            if (r.w >= a.x) j = a.y;
+                        }
+                        nx = p.col[beg + j];
                     }
-                    v = p.col[beg + j];
+                    if (KIND == KIND_N2V) prev = v;
+                    v = nx;
                     ++step;
                     if (KIND == KIND_COUNT) count_visit(v);
-                    if (KIND == KIND_PATH) path_append(p.inv ? p.inv[v] : v);
+                    if (KIND == KIND_PATH || KIND == KIND_N2V) path_append(p.inv ? p.inv[v] : v);
                 }
             }
             if (stop) {
                 my_steps += step;
                 my_dangling += dangled;
                 if (KIND == KIND_END) p.ends[w] = p.inv ? p.inv[v] : v;
-                if (KIND == KIND_PATH) {
+                if (KIND == KIND_PATH || KIND == KIND_N2V) {
                     path_flush_tail();
                     if (dangled) {
                         for (uint32_t k = step + 1; k < p.walk_len; ++k)
@@ -361,6 +425,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
 {
     const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
                      : p.app == PTW_APP_RWR                            ? KIND_END
+                     : p.app == PTW_APP_NODE2VEC                       ? KIND_N2V
                                                                        : KIND_PATH;
     int block = shape.block;
     uint32_t hubs = 0, slots = 0;
@@ -375,6 +440,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     if (per_cu <= 0) {
         per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
                  : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
+                 : kind == KIND_N2V ? occupancy_blocks<KIND_N2V>(block, lds)
                                     : occupancy_blocks<KIND_PATH>(block, lds);
         // A small walk (under 8 walkers per resident lane) takes as long as its
         // longest walker chains plus each workgroup's LDS setup and flush: one
@@ -397,6 +463,8 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
         hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);
     else if (kind == KIND_END)
         hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
+    else if (kind == KIND_N2V)
+        hipLaunchKernelGGL(walk_kernel<KIND_N2V>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
     else
         hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
     return hipGetLastError();
