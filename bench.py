@@ -38,12 +38,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest"],
+    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest", "node2vec"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
                          "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22; "
                          "cli: the reference's pagerank command on R-MAT scale 16, binary vs CPU path; "
-                         "ingest: loading an R-MAT scale 22 edge list, device vs host parsing")
+                         "ingest: loading an R-MAT scale 22 edge list, device vs host parsing; "
+                         "node2vec: second-order walks on R-MAT scale 22")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
@@ -75,6 +76,8 @@ def main():
         return bench_cli(args)
     if args.workload == "ingest":
         return bench_ingest(args)
+    if args.workload == "node2vec":
+        return bench_node2vec(args)
 
     import numpy as np
 
@@ -669,6 +672,90 @@ def bench_ingest(args):
         ctx.close()
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def bench_node2vec(args):
+    """Node2vec corpus: 10 walks per vertex of 80 vertices on R-MAT scale 22,
+    paths resident in HBM, for (p, q) = (0.5, 2), (2, 0.5) and (1, 1) (the
+    last equals DeepWalk), beside DeepWalk on the same graph.  The CPU
+    baseline is the oracle's node2vec replay (the same decisions, OpenMP) on
+    a bounded walker sample."""
+    import statistics
+
+    import numpy as np
+
+    import oracle
+    import ptwalker_amd as ptw
+
+    ctx = ptw.Context(0)
+    gi = ctx.rmat(scale=args.scale, edge_factor=16, seed=1)
+    W = 10 * gi.num_vertices
+    L = 80
+    paths = ctx.alloc(W * L * 4)
+    runs = {}
+    for name, app in (("p0.5_q2", ptw.Node2Vec(L, 0.5, 2.0)), ("p2_q0.5", ptw.Node2Vec(L, 2.0, 0.5)),
+                      ("p1_q1", ptw.Node2Vec(L, 1.0, 1.0)), ("deepwalk", ptw.DeepWalk(L))):
+        d = ptw.make_desc(app, W, seed=1, flags=ptw.FLAG_OUT_DEVICE)
+        ks, ts = [], []
+        for i in range(args.warmup + args.steps):
+            t = time.perf_counter()
+            o = ctx.walk(d, paths=paths.ptr, device_ptrs=True)
+            el = time.perf_counter() - t
+            if i >= args.warmup:
+                ks.append(o.kernel_ms)
+                ts.append(el)
+        runs[name] = {"kernel_ms": statistics.median(ks), "ms": 1e3 * statistics.median(ts), "steps": o.steps,
+                      "steps_per_s": o.steps / (statistics.median(ks) * 1e-3)}
+        print(f"# {name}: {runs[name]['kernel_ms']:.1f} ms, {runs[name]['steps_per_s']:.3g} steps/s",
+              file=sys.stderr, flush=True)
+    main = runs["p0.5_q2"]
+    cpu = None
+    if not args.no_cpu:
+        rp, col = ctx.export_csr()
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        cores = max(1, min(cores, 16))
+        n = 20000
+        t = time.perf_counter()
+        r = oracle.node2vec_replay(rp, col, n, L, 0.5, 2.0, seed=1, threads=cores)
+        dt = time.perf_counter() - t
+        n = int(min(W, max(n, n * args.cpu_seconds / max(dt, 1e-3))))
+        t = time.perf_counter()
+        r = oracle.node2vec_replay(rp, col, n, L, 0.5, 2.0, seed=1, threads=cores)
+        dt = time.perf_counter() - t
+        cpu = {"value": r["steps"] / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+               "sample": f"{n} node2vec walks (p 0.5, q 2) of {L} vertices, {r['steps']} steps, {dt:.1f} s, "
+                         f"the oracle's replay of the same decisions, {cores} threads"}
+    line = {
+        "metric": "walk-steps/s",
+        "value": main["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT graph generated in-process (seeded); no dataset",
+        "config": {"workload": f"node2vec corpus, R-MAT scale {args.scale} ef16, 10 walks per vertex x {L} vertices, "
+                               "p 0.5, q 2, paths resident in HBM",
+                   "vertices": gi.num_vertices, "edges": gi.num_edges, "walkers": W},
+        "runs": runs,
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel<node2vec>",
+                     "achieved": 16.0 * main["steps_per_s"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": 16.0 * main["steps_per_s"] / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_step": 16,
+                     "note": "16 B per accepted step as for DeepWalk; rejected proposals and membership "
+                             "searches add reads not counted here"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    paths.free()
+    ctx.close()
 
 
 def cpu_baseline(ctx, hub, args):

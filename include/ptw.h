@@ -57,10 +57,10 @@ extern "C" {
  * a neighbour of prev, 1/q otherwise; the first step is uniform.  The
  * reference declares the app (src/core/node2vec.cpp, flags -p -q in
  * include/option.hpp:447-482) but its sampler is not wired in
- * (src/core/solver.hpp:155-159).  Decisions: up to 16 proposals, attempt a
+ * (src/core/solver.hpp:155-159).  Decisions: up to 64 proposals, attempt a
  * drawing Philox(seed; wid, step + a * 2^20, stream), j = mulhi64(r.x, r.y,
  * deg), accepted iff r.z < floor(2^32 w(x) / max(1/p, 1, 1/q)); then one
- * exact draw from those integer weights (64-bit r.y:r.x, attempt 16).  With
+ * exact draw from those integer weights (64-bit r.y:r.x, attempt 64).  With
  * p = q = 1 the paths are DEEPWALK's. */
 #define PTW_APP_NODE2VEC 4
 

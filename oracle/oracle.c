@@ -245,7 +245,7 @@ int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
             } else {
                 n2v_ctx c = {scol + row_ptr[prev], row_ptr[prev + 1] - row_ptr[prev], prev, w_ret, w_in, w_out};
                 int found = 0;
-                for (uint32_t a = 0; a < 16 && !found; ++a) {
+                for (uint32_t a = 0; a < 64 && !found; ++a) {
                     if (a) draw(d->seed, d->stream, wid, i + (a << 20), r);
                     const uint32_t y = col[beg + orc_mulhi64(r[0], r[1], deg)];
                     if ((uint64_t)r[2] < n2v_w(&c, y)) { x = y; found = 1; }
@@ -253,7 +253,7 @@ int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
                 if (!found) {
                     uint64_t total = 0;
                     for (uint32_t k = 0; k < deg; ++k) total += n2v_w(&c, col[beg + k]);
-                    draw(d->seed, d->stream, wid, i + (16u << 20), r);
+                    draw(d->seed, d->stream, wid, i + (64u << 20), r);
                     const uint64_t u = ((uint64_t)r[1] << 32) | r[0];
                     const uint64_t pick = (uint64_t)(((unsigned __int128)u * total) >> 64);
                     uint64_t cum = 0;

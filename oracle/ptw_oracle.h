@@ -105,9 +105,9 @@ int orc_walk_replay_weighted(const uint32_t *row_ptr, const uint32_t *col, uint3
  * engine's decisions (node2vec app of include/ptw.h).  From cur, reached
  * from prev, neighbour x weighs 1/p (x == prev), 1 (x a neighbour of prev) or
  * 1/q; the integer weights are floor(2^32 w / max(1/p, 1, 1/q)).  Step i
- * (i >= 1): attempt a = 0..15 draws Philox(wid, i + a * 2^20), proposes
+ * (i >= 1): attempt a = 0..63 draws Philox(wid, i + a * 2^20), proposes
  * col[beg + mulhi64(r.x, r.y, deg)] and accepts iff r.z < its weight; then
- * Philox(wid, i + 16 * 2^20) picks exactly: floor((r.y:r.x) * total / 2^64)
+ * Philox(wid, i + 64 * 2^20) picks exactly: floor((r.y:r.x) * total / 2^64)
  * against the running sum of the weights in adjacency order.  Step 0 is a
  * uniform pick.  Requires 1 <= walk_len <= 2^20 and p, q in [2^-16, 2^16].
  * The membership test uses a sorted copy of each adjacency. */

@@ -32,7 +32,7 @@ enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2, KIND_N2V = 3 };
 
 // ---- node2vec: second-order neighbour choice ---------------------------------
 
-constexpr uint32_t kN2vAttempts = 16;  // proposals before the exact draw
+constexpr uint32_t kN2vAttempts = 64;  // proposals before the exact draw
 
 __device__ __forceinline__ bool sorted_has(const uint32_t *a, uint32_t n, uint32_t x)
 {
@@ -68,10 +68,22 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
     const uint32_t pb = p.row_ptr[prev];
     const uint32_t pdeg = p.row_ptr[prev + 1] - pb;
     const uint32_t *pn = p.scol + pb;
+    // The same decisions with fewer searches: a coin below every weight
+    // accepts whatever x is, a coin at or above both non-return weights
+    // rejects any x but prev; only coins in between need x's class.
+    const uint64_t w_min = p.w_ret < p.w_in ? (p.w_ret < p.w_out ? p.w_ret : p.w_out) : (p.w_in < p.w_out ? p.w_in : p.w_out);
+    const uint64_t w_io = p.w_in > p.w_out ? p.w_in : p.w_out;
     for (uint32_t a = 0; a < kN2vAttempts; ++a) {
         if (a) r = philox4x32_10(wlo, whi, step + (a << 20), p.stream, p.seed_lo, p.seed_hi);
         const uint32_t x = p.col[beg + mulhi64(r.x, r.y, deg)];
-        if ((uint64_t)r.z < n2v_weight(x, prev, pn, pdeg, p.w_ret, p.w_in, p.w_out)) return x;
+        const uint64_t z = r.z;
+        if (z < w_min) return x;
+        if (x == prev) {
+            if (z < p.w_ret) return x;
+            continue;
+        }
+        if (z >= w_io) continue;
+        if (z < (sorted_has(pn, pdeg, x) ? p.w_in : p.w_out)) return x;
     }
     uint64_t total = 0;  // < deg * 2^32 < 2^64
     for (uint32_t k = 0; k < deg; ++k) total += n2v_weight(p.col[beg + k], prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);

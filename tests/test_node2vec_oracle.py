@@ -1,8 +1,9 @@
 """The oracle's node2vec replay, pinned without the engine: with p = q = 1 it
 is DeepWalk bit for bit; its second-order transition frequencies match the
 law written from the definition (KnightKing's construction,
-third_party/knightking/test_node2vec.hpp:47-79), including where the exact
-fallback decides most steps; it does not depend on the thread count."""
+third_party/knightking/test_node2vec.hpp:47-79), including a graph where the
+exact fallback decides a fifth of the steps; it does not depend on the thread
+count."""
 import numpy as np
 import pytest
 
@@ -53,8 +54,9 @@ def transitions(paths):
 
 def check_law(rp, col, paths, p, q, min_count=400):
     """Per (prev, cur) pair with enough samples: chi-square of the next-vertex
-    counts against the law, summed into one statistic compared with its
-    degrees of freedom (a 6-sigma band)."""
+    counts against the law, cells with an expected count below 5 pooled into
+    one, summed over pairs and compared with its degrees of freedom (a
+    6-sigma band)."""
     chi2, dof, pairs = 0.0, 0, 0
     for (prev, cur), d in transitions(paths).items():
         n = sum(d.values())
@@ -62,12 +64,20 @@ def check_law(rp, col, paths, p, q, min_count=400):
             continue
         law = oracle.node2vec_law(rp, col, prev, cur, p, q)
         assert set(d) <= set(law), "a move off the adjacency"
+        cells, pool_o, pool_e = [], 0, 0.0
         for x, px in law.items():
             e = n * px
-            chi2 += (d.get(x, 0) - e) ** 2 / e
-        dof += len(law) - 1
+            if e < 5:
+                pool_o += d.get(x, 0)
+                pool_e += e
+            else:
+                cells.append((d.get(x, 0), e))
+        if pool_e > 0:
+            cells.append((pool_o, pool_e))
+        chi2 += sum((o - e) ** 2 / e for o, e in cells)
+        dof += len(cells) - 1
         pairs += 1
-    assert pairs >= 5
+    assert pairs >= 5 and dof > 0
     assert abs(chi2 - dof) < 6 * np.sqrt(2 * dof), (chi2, dof, pairs)
 
 
@@ -80,8 +90,8 @@ def test_transition_law_random_graph(p, q):
 
 def test_transition_law_where_the_fallback_decides():
     # p = 1/64, q = 64: from a leaf into the centre the acceptance per proposal
-    # is (64 + 2 + 37/64) / (64 * 40) ~ 2.6 %, so ~65 % of those steps end in
-    # the exact draw after 16 rejections
+    # is (64 + 2 + 37/64) / (64 * 40) ~ 2.6 %, so ~19 % of those steps end in
+    # the exact draw after 64 rejections
     p, q = 1.0 / 64, 64.0
     rp, col = star_ring(40)
     r = oracle.node2vec_replay(rp, col, 20000, 30, p, q, seed=3)
