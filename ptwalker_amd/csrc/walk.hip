@@ -456,12 +456,15 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
                                     : occupancy_blocks<KIND_PATH>(block, lds);
         // A small walk (under 8 walkers per resident lane) takes as long as its
         // longest walker chains plus each workgroup's LDS setup and flush: one
-        // 512-thread workgroup per CU and no visit cache measured fastest.
+        // 512-thread workgroup per CU with at most 8192 hub counters measured
+        // fastest.  A visit cache only pays for single-source walks, whose
+        // source neighbours are hot; uniform starts spread their visits.
         if (p.walkers < 8ull * (uint64_t)per_cu * (uint64_t)num_cus * (uint64_t)block) {
             block = block < 512 ? block : 512;
             per_cu = 1;
-            slots = 0;
-            lds = (size_t)hubs * sizeof(uint32_t);
+            hubs = hubs < 8192u ? hubs : 8192u;
+            slots = p.start_mode == PTW_START_SOURCE ? (slots < 1024u ? slots : 1024u) : 0u;
+            lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
         }
     }
     const int blocks = per_cu * num_cus;
