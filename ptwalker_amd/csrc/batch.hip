@@ -1,29 +1,32 @@
 // Batched single-source walks (PPR visit counts) for gfx950.
 //
-// All queries of a batch run in one persistent launch.  The host cuts every
-// query into chunks of walker ids; wavefronts take chunks from one queue
-// counter and refill idle lanes from their current chunk, so lanes of one
-// wavefront may hold walkers of different queries.  Query q's walker w draws
-// from Philox(key = seed, ctr = (w, step, stream_q)): a query's counts are
-// the same alone (ptw_walk with stream = stream_q) and in any mix.
+// All queries of a batch run in two persistent launches.  The host cuts
+// every query into chunks of walker ids, those of dense queries first.
+// batch_dense_kernel: workgroups take the dense chunks from one queue
+// counter; a workgroup's LDS holds direct counters for the highest-degree
+// vertices and a hash cache for the rest, both flushed to the query's slice
+// at a barrier when the workgroup moves to another query.
+// batch_walk_kernel: wavefronts take the remaining (hash) chunks and refill
+// idle lanes from their current chunk, so lanes of one wavefront may hold
+// walkers of different queries; each wavefront keeps kWaveHubs direct LDS
+// counters and a small LDS cache of (query, vertex) counts.
+// Query q's walker w draws from Philox(key = seed, ctr = (w, step, stream_q)):
+// a query's counts are the same alone (ptw_walk with stream = stream_q) and
+// in any mix.  A query's visits to its own source are counted in a register.
 //
 // Counts go to per-query slices:
-//   dense slice: u32[V] in device ids; after the walk a compaction pass lists
-//                each slice's nonzero counters, so top-k and the cleanup read
-//                only touched entries
+//   dense slice: u32[V] in device ids
 //   hash slice:  open addressing over {u32 key, u32 count}, capacity a power
 //                of two sized from the query's expected visits
-// A query's visits to its own source are counted in a register and flushed
-// when the lane moves to another query.  Each wavefront keeps direct LDS
-// counters for the kWaveHubs highest-degree vertices (device ids below
-// kWaveHubs after the degree relabel) of the query of its current chunk, and
-// a small LDS hash cache of (query, vertex) counts for the rest; both are
-// flushed to the slices when the wavefront moves to a chunk of another query.
+// After the walk a compaction pass lists each query's visited vertices as
+// (vertex, count) pairs, so top-k and the cleanup read only those.
 //
-// Top-k: one workgroup per query narrows the value range that holds the
-// k-th count with LDS histograms, gathers the candidates into LDS, and
-// bitonic-sorts them by (count desc, device id asc); ties that do not fit
-// the tile are cut by an exact radix select on the device id.
+// Top-k: one workgroup per query.  The pass that sums the entropy terms also
+// counts entries per power-of-two count bucket; the bucket holding the k-th
+// count starts a narrowing of the value range with LDS histograms until the
+// candidates fit the tile; they are gathered into LDS and bitonic-sorted by
+// (count desc, device id asc); ties that do not fit the tile are cut by an
+// exact radix select on the device id.
 #include "../../include/ptw.h"
 #include "engine.hpp"
 #include "philox.hpp"
@@ -427,13 +430,42 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     flush();
 }
 
-// After the walk: the nonzero counters of each dense slice, listed as
-// (device id, count) pairs in the query's touched list, so that top-k streams
-// the pairs instead of gathering from the slice.  One read of each slice:
-// a workgroup takes a segment of kSeg counters, each thread loads 16 of them
-// per round with 16 B loads, and one block scan plus one reservation on the
-// query's count place the round's pairs.  A list that would overflow its
-// capacity marks the query for full-slice scans (touched_count > capacity).
+// Reserves n list places for each thread of the workgroup, in thread order,
+// with one atomic on the query's count; returns the thread's first place.
+// Every thread of the workgroup calls it.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t *count, uint32_t *s_wave, uint32_t *s_base)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = n;  // inclusive prefix over the wavefront
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        before += w < wave ? s_wave[w] : 0u;
+        total += s_wave[w];
+    }
+    if (threadIdx.x == 0) *s_base = total ? atomicAdd(count, total) : 0u;
+    __syncthreads();
+    const uint32_t first = *s_base + before + incl - n;
+    __syncthreads();  // s_wave and s_base are reused by the next call
+    return first;
+}
+
+// After the walk: the visited vertices of each query, listed as (device id,
+// count) pairs in the query's touched list, so that top-k streams the pairs
+// instead of whole slices.  One read of each slice or table: the tasks are
+// (query, segment) with the segments of a dense slice kSeg counters long and
+// those of a hash table 1/segs of it (a multiple of 16 slots); each thread
+// takes 16 counters or 8 slots per round with 16 B loads, and one block scan
+// plus one reservation on the query's count place the round's pairs.  A list
+// that would overflow marks the query for full-slice scans
+// (touched_count > capacity).
 __global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
 {
     __shared__ uint32_t s_wave[4];
@@ -442,58 +474,79 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
     const uint32_t pitch = dense_pitch(p.V);
     const uint32_t segs = (pitch + kSeg - 1) / kSeg;
     const uint64_t tasks = (uint64_t)nq * segs;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
         const uint32_t q = (uint32_t)(t / segs);
+        const uint32_t seg = (uint32_t)(t % segs);
         const BatchQuery bq = p.queries[q];
-        if (bq.kind != SLICE_DENSE) continue;  // block-uniform
-        const uint32_t lo = (uint32_t)(t % segs) * kSeg;
-        const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // a multiple of 64
-        const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + bq.slice_off);
-        for (uint32_t base = lo; base < hi; base += 256 * 16) {
-            const uint32_t my0 = base + threadIdx.x * 16;  // my0 < hi implies my0 + 16 <= hi
-            uint32_t c[16];
-            if (my0 < hi) {
+        uint2 *list = p.touched + bq.touched_off;
+        if (bq.kind == SLICE_DENSE) {  // block-uniform
+            const uint32_t lo = seg * kSeg;
+            const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // a multiple of 64
+            const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + bq.slice_off);
+            for (uint32_t base = lo; base < hi; base += 256 * 16) {
+                const uint32_t my0 = base + threadIdx.x * 16;  // my0 < hi implies my0 + 16 <= hi
+                uint32_t c[16];
+                if (my0 < hi) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint4 x = slice4[(my0 >> 2) + j];
-                    c[4 * j] = x.x;
-                    c[4 * j + 1] = x.y;
-                    c[4 * j + 2] = x.z;
-                    c[4 * j + 3] = x.w;
+                    for (int j = 0; j < 4; ++j) {
+                        const uint4 x = slice4[(my0 >> 2) + j];
+                        c[4 * j] = x.x;
+                        c[4 * j + 1] = x.y;
+                        c[4 * j + 2] = x.z;
+                        c[4 * j + 3] = x.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) c[j] = 0;
                 }
-            } else {
+                uint32_t n = 0;
 #pragma unroll
-                for (int j = 0; j < 16; ++j) c[j] = 0;
-            }
-            uint32_t n = 0;
+                for (int j = 0; j < 16; ++j) n += c[j] != 0;
+                uint32_t idx = block_reserve(n, &p.touched_count[q], s_wave, &s_base);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) n += c[j] != 0;
-            uint32_t incl = n;  // inclusive prefix over the wavefront
-#pragma unroll
-            for (uint32_t o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(incl, o);
-                if (lane >= o) incl += y;
-            }
-            if (lane == 63) s_wave[wave] = incl;
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < 4; ++w) {
-                before += w < wave ? s_wave[w] : 0u;
-                total += s_wave[w];
-            }
-            if (threadIdx.x == 0) s_base = total ? atomicAdd(&p.touched_count[q], total) : 0u;
-            __syncthreads();
-            uint32_t idx = s_base + before + incl - n;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                if (c[j]) {
-                    if (idx < bq.touched_cap) p.touched[bq.touched_off + idx] = uint2{my0 + (uint32_t)j, c[j]};
-                    ++idx;
+                for (int j = 0; j < 16; ++j) {
+                    if (c[j]) {
+                        if (idx < bq.touched_cap) list[idx] = uint2{my0 + (uint32_t)j, c[j]};
+                        ++idx;
+                    }
                 }
             }
-            __syncthreads();  // s_wave and s_base are reused by the next round
+        } else {
+            // tables hold a power of two >= 2 slots and start at an even
+            // slot, so 16 B loads of slot pairs are aligned
+            const uint64_t cap = (uint64_t)bq.mask + 1;
+            const uint64_t span = ((cap + segs - 1) / segs + 15) & ~15ull;
+            const uint64_t lo = (uint64_t)seg * span;
+            if (lo >= cap) continue;  // block-uniform
+            const uint64_t hi = lo + span < cap ? lo + span : cap;
+            const uint2 *tab = p.hash + bq.slice_off;
+            for (uint64_t base = lo; base < hi; base += 256 * 8) {
+                const uint64_t my0 = base + threadIdx.x * 8;
+                uint2 e[8];
+                if (my0 + 8 <= hi) {
+                    const uint4 *t4 = reinterpret_cast<const uint4 *>(tab + my0);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint4 x = t4[j];
+                        e[2 * j] = uint2{x.x, x.y};
+                        e[2 * j + 1] = uint2{x.z, x.w};
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) e[j] = my0 + j < hi ? tab[my0 + j] : uint2{kEmpty, 0u};
+                }
+                uint32_t n = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) n += e[j].x != kEmpty && e[j].y != 0;
+                uint32_t idx = block_reserve(n, &p.touched_count[q], s_wave, &s_base);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (e[j].x != kEmpty && e[j].y != 0) {
+                        if (idx < bq.touched_cap) list[idx] = e[j];
+                        ++idx;
+                    }
+                }
+            }
         }
     }
 }
@@ -503,7 +556,7 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
 struct Entries {
     const BatchLaunch *p;
     BatchQuery Q;
-    uint32_t mode;  // 0 dense list, 1 dense full slice, 2 hash
+    uint32_t mode;  // 0 pair list, 1 dense full slice, 2 hash full table
     uint32_t n;
     uint32_t V;
     __device__ __forceinline__ void get(uint32_t i, uint32_t &v, uint32_t &c) const
@@ -568,8 +621,14 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
                 E.n = p.V;
             }
         } else {
-            E.mode = 2;
-            E.n = E.Q.mask + 1;
+            const uint32_t t = p.touched_count[q];
+            if (t <= E.Q.touched_cap) {
+                E.mode = 0;
+                E.n = t;
+            } else {
+                E.mode = 2;
+                E.n = E.Q.mask + 1;
+            }
         }
         // max count, and sum c and sum c ln c for the entropy of the visit
         // distribution; c ln c is summed in 2^-20 fixed point so the result
@@ -579,8 +638,13 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             s_u64[0] = 0;
             s_u64[1] = 0;
         }
+        for (uint32_t b = threadIdx.x; b < 32; b += blockDim.x) hist[b] = 0;
         __syncthreads();
-        uint32_t local_max = 0;
+        // the same pass counts entries per power-of-two count bucket, so the
+        // narrowing below starts inside the bucket holding the k-th count;
+        // the two lowest buckets (counts 1 and 2-3, most entries) are counted
+        // in registers first
+        uint32_t local_max = 0, n_b0 = 0, n_b1 = 0;
         unsigned long long local_sum = 0, local_clogc = 0;
         for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
             uint32_t v, c;
@@ -588,10 +652,15 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             local_max = c > local_max ? c : local_max;
             local_sum += c;
             if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
+            if (c == 1) ++n_b0;
+            else if (c < 4) { if (c) ++n_b1; }
+            else atomicAdd(&hist[31 - __clz(c)], 1u);
         }
         atomicMax(&s_u32[0], local_max);
         atomicAdd(&s_u64[0], local_sum);
         atomicAdd(&s_u64[1], local_clogc);
+        if (n_b0) atomicAdd(&hist[0], n_b0);
+        if (n_b1) atomicAdd(&hist[1], n_b1);
         __syncthreads();
         const uint32_t maxc = s_u32[0];
         if (threadIdx.x == 0) {
@@ -602,12 +671,38 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             __syncthreads();
             continue;
         }
-        // narrow [lo, hi] until the candidates fit the tile or one value remains
-        uint64_t lo = 1, hi = maxc;
-        uint32_t above = 0;  // entries with count > hi
-        uint32_t inbin = 0;
+        // the power-of-two bucket holding the k-th count (bucket 0 when fewer
+        // than k entries exist); every bucket is counted exactly
+        if (threadIdx.x == 0) {
+            uint32_t cum = 0, b = 0;
+            for (int bb = 31; bb >= 0; --bb) {
+                if (cum + hist[bb] >= k || bb == 0) {
+                    b = (uint32_t)bb;
+                    break;
+                }
+                cum += hist[bb];
+            }
+            s_u32[1] = b;
+            s_u32[2] = cum;      // entries in higher buckets
+            s_u32[3] = hist[b];  // entries in bucket b
+        }
+        __syncthreads();
+        uint64_t lo = 1ull << s_u32[1], hi = (2ull << s_u32[1]) - 1;
+        if (hi > maxc) hi = maxc;
+        uint32_t above = s_u32[2];  // entries with count > hi
+        uint32_t inbin = s_u32[3];
         bool overflow_ties = false;
-        while (maxc > 0) {
+        __syncthreads();
+        if (maxc == 0) {  // nothing visited: the candidate pass below takes nothing
+            lo = 1;
+            hi = 0;
+            above = 0;
+            inbin = 0;
+        }
+        // narrow [lo, hi] until the candidates (every count >= lo) fit the
+        // tile or one value remains
+        const bool fits = (uint64_t)above + inbin <= tile;
+        while (!fits && lo <= hi) {
             const uint64_t range = hi - lo + 1;
             const uint64_t width = (range + kBins - 1) / kBins;
             for (uint32_t b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;

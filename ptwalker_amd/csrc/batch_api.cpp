@@ -22,13 +22,6 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-uint64_t next_pow2(uint64_t x)
-{
-    uint64_t p = 1;
-    while (p < x) p <<= 1;
-    return p;
-}
-
 template <typename T>
 void hfree(T *&p)
 {
@@ -84,26 +77,23 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         B.thr = continue_threshold(qq.tp);
         B.walkers = qq.walkers;
         B.stream = qq.stream;
-        const uint64_t upper = (uint64_t)(2.0 * (double)qq.walkers * per_walker) + 64;
-        if (qq.walkers > thr_dense) {
+        const SliceSizes z = batch_slice_sizes(V, qq.walkers, qq.tp, thr_dense);
+        if (z.dense) {
             B.kind = SLICE_DENSE;
             B.slice_off = P.dense;
-            P.dense += dense_pitch(V);
-            B.touched_cap = (uint32_t)std::min<uint64_t>(V, upper);
-            B.touched_off = P.touched;
-            P.touched += B.touched_cap;
+            P.dense += z.slots;
             B.mask = 0;
             ++P.n_dense;
         } else {
-            const uint64_t cap = std::min<uint64_t>(next_pow2(2 * upper), next_pow2(2ull * V));
             B.kind = SLICE_HASH;
             B.slice_off = P.hash;
-            P.hash += cap;
-            B.mask = (uint32_t)(cap - 1);
-            B.touched_cap = 0;
-            B.touched_off = 0;
+            P.hash += z.slots;
+            B.mask = (uint32_t)(z.slots - 1);
             ++P.n_hash;
         }
+        B.touched_cap = z.touched_cap;
+        B.touched_off = P.touched;
+        P.touched += z.touched_cap;
     }
     uint64_t dense_walkers = 0;
     for (const BatchQuery &B : P.q)
@@ -406,11 +396,12 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     for (const auto &q : b.h_queries) {
         if (q.kind == ptw::SLICE_DENSE) {
             ++out->dense_queries;
-            bytes += 4ull * ptw::dense_pitch(ctx->dev.V) + 8ull * q.touched_cap;
+            bytes += 4ull * ptw::dense_pitch(ctx->dev.V);
         } else {
             ++out->hash_queries;
             bytes += 8ull * ((uint64_t)q.mask + 1);
         }
+        bytes += 8ull * q.touched_cap;
     }
     out->slice_bytes = bytes;
     out->total_ms = ptw::now_ms() - t0;

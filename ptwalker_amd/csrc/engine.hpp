@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <functional>
 #include <string>
 #include <vector>
@@ -134,6 +135,37 @@ enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1 };
 // so every slice starts on a 256 B boundary and can be read with 16 B loads.
 __host__ __device__ inline uint32_t dense_pitch(uint32_t V) { return (V + 63u) & ~63u; }
 
+// The sizes one batched query is planned with.  Visits are bounded by twice
+// walkers * (1 + 1/tp) plus 64 for the table ("upper").  A query above the
+// dense threshold gets a dense slice; one below it an open-addressing table
+// of the power of two >= 2 * upper slots (at most that of 2V).  Either one
+// gets a list of (vertex, count) pairs for the compaction after the walk,
+// sized for 1.25x the expected visits (distinct vertices never exceed
+// visits); a query whose list would overflow is read from its slice instead.
+struct SliceSizes {
+    bool dense;
+    uint64_t slots;        // dense: u32 counters (the pitch); hash: uint2 slots
+    uint32_t touched_cap;  // (vertex, count) pairs
+    uint64_t bytes() const { return (dense ? 4 * slots : 8 * slots) + 8ull * touched_cap; }
+};
+inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uint64_t dense_threshold)
+{
+    const double per_walker = 1.0 + 1.0 / tp;
+    const uint64_t upper = (uint64_t)(2.0 * (double)walkers * per_walker) + 64;
+    const uint64_t expect = (uint64_t)(1.25 * (double)walkers * per_walker) + 64;
+    SliceSizes z;
+    z.dense = walkers > dense_threshold;
+    if (z.dense) {
+        z.slots = dense_pitch(V);
+    } else {
+        uint64_t cap = 1;
+        while (cap < 2 * upper && cap < 2ull * V) cap <<= 1;
+        z.slots = cap;
+    }
+    z.touched_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(V, expect), z.dense ? V : z.slots);
+    return z;
+}
+
 // One query of a batch, as the device sees it.
 struct BatchQuery {
     uint32_t source_dev;
@@ -142,8 +174,8 @@ struct BatchQuery {
     uint64_t slice_off;    // dense: u32 element offset; hash: uint2 slot offset
     uint32_t kind;         // SliceKind
     uint32_t mask;         // hash: capacity - 1
-    uint64_t touched_off;  // dense: u32 offset of the first-touch list
-    uint32_t touched_cap;  // dense: list capacity
+    uint64_t touched_off;  // first pair of the (vertex, count) list
+    uint32_t touched_cap;  // list capacity in pairs
     uint32_t stream;       // 4th Philox counter word
 };
 

@@ -34,16 +34,11 @@ struct Item {
     uint64_t walkers;
 };
 
-// Slice bytes of one query under the batch planner's rule (dense above the
-// default threshold, hash below).
+// Slice bytes of one query under the batch planner's rule at the default
+// dense threshold.
 uint64_t slice_bytes(uint32_t V, uint64_t walkers, double tp)
 {
-    const double per_walker = 1.0 + 1.0 / tp;
-    const uint64_t upper = (uint64_t)(2.0 * (double)walkers * per_walker) + 64;
-    if (walkers > 100000) return 4ull * V + 4ull * std::min<uint64_t>(V, upper);
-    uint64_t cap = 1;
-    while (cap < 2 * upper && cap < 2ull * V) cap <<= 1;
-    return 8 * cap;
+    return ptw::batch_slice_sizes(V, walkers, tp, 100000).bytes();
 }
 
 void graph_features(const ptw::HostCsr &g, uint32_t s, float *f)
