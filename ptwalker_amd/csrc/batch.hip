@@ -123,6 +123,12 @@ __global__ void batch_begin_kernel(BatchLaunch p, uint32_t max_queries)
         p.touched_count[i] = 0;
         p.overflow[i] = 0;
         p.steps[i] = 0;
+        TopkStats *st = p.tk_stats + i;
+        for (int b = 0; b < 32; ++b) st->hist[b] = 0;
+        st->maxc = 0;
+        st->ncand = 0;
+        st->sum = 0;
+        st->clogc = 0;
     }
 }
 
@@ -576,6 +582,212 @@ struct Entries {
     }
 };
 
+// A query's entries after the compaction: its pair list, or the whole slice
+// or table when the list overflowed.
+__device__ __forceinline__ Entries entries_of(const BatchLaunch &p, uint32_t q)
+{
+    Entries E;
+    E.p = &p;
+    E.Q = p.queries[q];
+    E.V = p.V;
+    const uint32_t t = p.touched_count[q];
+    if (t <= E.Q.touched_cap) {
+        E.mode = 0;
+        E.n = t;
+    } else if (E.Q.kind == SLICE_DENSE) {
+        E.mode = 1;
+        E.n = p.V;
+    } else {
+        E.mode = 2;
+        E.n = E.Q.mask + 1;
+    }
+    return E;
+}
+
+// The power-of-two count bucket holding the k-th largest count (bucket 0
+// when fewer than k entries exist), the entries in higher buckets and in it.
+struct Bucket {
+    uint32_t b, above, inbin;
+};
+__device__ Bucket kth_bucket(const TopkStats *st, uint32_t k)
+{
+    Bucket r{0, 0, 0};
+    uint32_t cum = 0;
+    for (int bb = 31; bb >= 0; --bb) {
+        const uint32_t h = st->hist[bb];
+        if (cum + h >= k || bb == 0) {
+            r.b = (uint32_t)bb;
+            r.inbin = h;
+            break;
+        }
+        cum += h;
+    }
+    r.above = cum;
+    return r;
+}
+
+constexpr uint32_t kTopkSeg = 16384;   // entries per (query, segment) task
+constexpr uint32_t kTopkBlock = 512;  // workgroup size of the task kernels
+
+__device__ __forceinline__ uint32_t topk_segments(const Entries &E)
+{
+    return (uint32_t)(((uint64_t)E.n + kTopkSeg - 1) / kTopkSeg);
+}
+
+// Calls fn(q, E, s) for this workgroup's share of the (query, segment)
+// tasks: segments of kTopkSeg entries numbered across the queries in order,
+// task t going to workgroup t mod gridDim.x, so a large query is spread over
+// many workgroups.  Every thread calls it; fn is called workgroup-uniformly.
+template <typename F>
+__device__ void for_topk_segments(const BatchLaunch &p, F &&fn)
+{
+    __shared__ uint32_t s_wave[kTopkBlock / 64];
+    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_q[kTopkBlock], s_first[kTopkBlock];
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    const uint32_t G = gridDim.x, me = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    uint64_t carry = 0;  // tasks of the queries before this round's
+    for (uint32_t base = 0; base < nq; base += blockDim.x) {
+        const uint32_t q = base + threadIdx.x;
+        const uint32_t segs = q < nq ? topk_segments(entries_of(p, q)) : 0u;
+        uint32_t incl = segs;
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) s_wave[wave] = incl;
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            before += w < wave ? s_wave[w] : 0u;
+            total += s_wave[w];
+        }
+        const uint64_t t0 = carry + before + incl - segs;  // the query's first task
+        const uint32_t r = (uint32_t)(t0 % G);
+        const uint32_t first = me >= r ? me - r : me + G - r;  // first s with (t0 + s) mod G == me
+        if (first < segs) {
+            const uint32_t i = atomicAdd(&s_n, 1u);
+            s_q[i] = q;
+            s_first[i] = first;
+        }
+        __syncthreads();
+        const uint32_t n = s_n;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t qq = s_q[i];
+            const Entries E = entries_of(p, qq);
+            const uint32_t segs_q = topk_segments(E);
+            for (uint32_t sg = s_first[i]; sg < segs_q; sg += G) fn(qq, E, sg);
+        }
+        carry += total;
+        __syncthreads();  // the round's shared state is reused by the next
+    }
+}
+
+// Count statistics of every query (TopkStats, zeroed by batch_begin).
+__global__ __launch_bounds__(kTopkBlock) void batch_topk_stats_kernel(BatchLaunch p)
+{
+    __shared__ uint32_t s_hist[32];
+    __shared__ uint32_t s_max;
+    __shared__ unsigned long long s_sum, s_clogc;
+    const uint32_t lane = threadIdx.x & 63;
+    for_topk_segments(p, [&](uint32_t q, const Entries &E, uint32_t sg) {
+        if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            s_max = 0;
+            s_sum = 0;
+            s_clogc = 0;
+        }
+        __syncthreads();
+        const uint64_t lo = (uint64_t)sg * kTopkSeg;
+        const uint64_t hi = lo + kTopkSeg < E.n ? lo + kTopkSeg : (uint64_t)E.n;
+        // counts of 1 and of 2-3 (most entries) are counted in registers;
+        // c ln c is summed in 2^-20 fixed point so the sum does not depend on
+        // the order of the additions
+        uint32_t local_max = 0, n_b0 = 0, n_b1 = 0;
+        unsigned long long local_sum = 0, local_clogc = 0;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            uint32_t v, c;
+            E.get((uint32_t)i, v, c);
+            local_max = c > local_max ? c : local_max;
+            local_sum += c;
+            if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
+            if (c == 1) {
+                ++n_b0;
+            } else if (c >= 4) {
+                atomicAdd(&s_hist[31 - __clz(c)], 1u);
+            } else if (c) {
+                ++n_b1;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            local_max = max(local_max, (uint32_t)__shfl_xor(local_max, o));
+            local_sum += __shfl_xor(local_sum, o);
+            local_clogc += __shfl_xor(local_clogc, o);
+            n_b0 += __shfl_xor(n_b0, o);
+            n_b1 += __shfl_xor(n_b1, o);
+        }
+        if (lane == 0) {
+            if (local_max) atomicMax(&s_max, local_max);
+            if (local_sum) atomicAdd(&s_sum, local_sum);
+            if (local_clogc) atomicAdd(&s_clogc, local_clogc);
+            if (n_b0) atomicAdd(&s_hist[0], n_b0);
+            if (n_b1) atomicAdd(&s_hist[1], n_b1);
+        }
+        __syncthreads();
+        TopkStats *st = p.tk_stats + q;
+        if (threadIdx.x < 32) {
+            if (s_hist[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], s_hist[threadIdx.x]);
+        } else if (threadIdx.x == 32) {
+            if (s_max) atomicMax(&st->maxc, s_max);
+            if (s_sum) atomicAdd(&st->sum, s_sum);
+            if (s_clogc) atomicAdd(&st->clogc, s_clogc);
+        }
+        __syncthreads();
+    });
+}
+
+// The candidates of every query whose entries in and above the k-th count's
+// bucket fit its candidate slots: every entry with count >= 2^b, written in
+// any order to the query's part of the candidate pool.
+__global__ __launch_bounds__(kTopkBlock) void batch_topk_gather_kernel(BatchLaunch p)
+{
+    __shared__ uint32_t s_lo, s_ok;
+    const uint32_t k = (uint32_t)p.hdr[2];
+    const uint32_t lane = threadIdx.x & 63;
+    for_topk_segments(p, [&](uint32_t q, const Entries &E, uint32_t sg) {
+        TopkStats *st = p.tk_stats + q;
+        if (threadIdx.x == 0) {
+            const Bucket bk = kth_bucket(st, k);
+            s_ok = k > 0 && st->maxc > 0 && (uint64_t)bk.above + bk.inbin <= E.Q.cand_cap;
+            s_lo = 1u << bk.b;
+        }
+        __syncthreads();
+        if (s_ok) {  // workgroup-uniform
+            const uint32_t lo_c = s_lo;
+            const uint64_t lo = (uint64_t)sg * kTopkSeg;
+            const uint64_t hi = lo + kTopkSeg < E.n ? lo + kTopkSeg : (uint64_t)E.n;
+            for (uint64_t b0 = lo; b0 < hi; b0 += blockDim.x) {
+                const uint64_t i = b0 + threadIdx.x;
+                uint32_t v = 0, c = 0;
+                if (i < hi) E.get((uint32_t)i, v, c);
+                const bool take = c >= lo_c && c != 0;
+                const uint64_t m = __ballot(take);
+                if (m) {  // wave-uniform
+                    uint32_t slot = 0;
+                    if (lane == 0) slot = atomicAdd(&st->ncand, (uint32_t)__popcll(m));
+                    slot = (uint32_t)wave_broadcast_first(slot) + lane_rank(m);
+                    if (take && slot < E.Q.cand_cap)
+                        p.tk_cand[E.Q.cand_off + slot] = ((unsigned long long)c << 32) | (uint32_t)~v;
+                }
+            }
+        }
+        __syncthreads();  // s_lo and s_ok are reused by the next task
+    });
+}
+
 __device__ void block_bitonic_desc(unsigned long long *a, uint32_t n)
 {
     for (uint32_t size = 2; size <= n; size <<= 1) {
@@ -596,228 +808,170 @@ __device__ void block_bitonic_desc(unsigned long long *a, uint32_t n)
     }
 }
 
+// One workgroup per query: the entropy from the statistics, then the top-k.
+// Gathered candidates are loaded into the LDS tile; otherwise the count
+// range of the k-th count is narrowed with LDS histograms over the query's
+// entries, starting from its bucket, before the candidates are collected.
 __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const uint32_t *inv, uint32_t *out_v,
                                                          unsigned long long *out_c, uint32_t tile)
 {
     __shared__ uint32_t hist[kBins];
     __shared__ uint32_t s_u32[8];
-    __shared__ unsigned long long s_u64[4];
     extern __shared__ unsigned long long cand[];
     const uint32_t nq = (uint32_t)p.hdr[1];
     const uint32_t k = (uint32_t)p.hdr[2];
 
     for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-        Entries E;
-        E.p = &p;
-        E.Q = p.queries[q];
-        E.V = p.V;
-        if (E.Q.kind == SLICE_DENSE) {
-            const uint32_t t = p.touched_count[q];
-            if (t <= E.Q.touched_cap) {
-                E.mode = 0;
-                E.n = t;
-            } else {
-                E.mode = 1;
-                E.n = p.V;
-            }
-        } else {
-            const uint32_t t = p.touched_count[q];
-            if (t <= E.Q.touched_cap) {
-                E.mode = 0;
-                E.n = t;
-            } else {
-                E.mode = 2;
-                E.n = E.Q.mask + 1;
-            }
-        }
-        // max count, and sum c and sum c ln c for the entropy of the visit
-        // distribution; c ln c is summed in 2^-20 fixed point so the result
-        // does not depend on the (nondeterministic) order of the entries
+        const Entries E = entries_of(p, q);
+        const TopkStats *st = p.tk_stats + q;
         if (threadIdx.x == 0) {
-            s_u32[0] = 0;
-            s_u64[0] = 0;
-            s_u64[1] = 0;
+            const double N = (double)st->sum;
+            p.entropy[q] = N > 0 ? log(N) - ((double)st->clogc / 1048576.0) / N : 0.0;
+            const Bucket bk = kth_bucket(st, k);
+            s_u32[0] = st->maxc;
+            s_u32[1] = bk.b;
+            s_u32[2] = bk.above;
+            s_u32[3] = bk.inbin;
+            s_u32[4] = st->ncand;
         }
-        for (uint32_t b = threadIdx.x; b < 32; b += blockDim.x) hist[b] = 0;
-        __syncthreads();
-        // the same pass counts entries per power-of-two count bucket, so the
-        // narrowing below starts inside the bucket holding the k-th count;
-        // the two lowest buckets (counts 1 and 2-3, most entries) are counted
-        // in registers first
-        uint32_t local_max = 0, n_b0 = 0, n_b1 = 0;
-        unsigned long long local_sum = 0, local_clogc = 0;
-        for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
-            uint32_t v, c;
-            E.get(i, v, c);
-            local_max = c > local_max ? c : local_max;
-            local_sum += c;
-            if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
-            if (c == 1) ++n_b0;
-            else if (c < 4) { if (c) ++n_b1; }
-            else atomicAdd(&hist[31 - __clz(c)], 1u);
-        }
-        atomicMax(&s_u32[0], local_max);
-        atomicAdd(&s_u64[0], local_sum);
-        atomicAdd(&s_u64[1], local_clogc);
-        if (n_b0) atomicAdd(&hist[0], n_b0);
-        if (n_b1) atomicAdd(&hist[1], n_b1);
         __syncthreads();
         const uint32_t maxc = s_u32[0];
-        if (threadIdx.x == 0) {
-            const double N = (double)s_u64[0];
-            p.entropy[q] = N > 0 ? log(N) - ((double)s_u64[1] / 1048576.0) / N : 0.0;
-        }
-        if (k == 0) {  // entropy only
-            __syncthreads();
-            continue;
-        }
-        // the power-of-two bucket holding the k-th count (bucket 0 when fewer
-        // than k entries exist); every bucket is counted exactly
-        if (threadIdx.x == 0) {
-            uint32_t cum = 0, b = 0;
-            for (int bb = 31; bb >= 0; --bb) {
-                if (cum + hist[bb] >= k || bb == 0) {
-                    b = (uint32_t)bb;
-                    break;
-                }
-                cum += hist[bb];
-            }
-            s_u32[1] = b;
-            s_u32[2] = cum;      // entries in higher buckets
-            s_u32[3] = hist[b];  // entries in bucket b
-        }
-        __syncthreads();
-        uint64_t lo = 1ull << s_u32[1], hi = (2ull << s_u32[1]) - 1;
-        if (hi > maxc) hi = maxc;
+        const uint32_t b0 = s_u32[1];
         uint32_t above = s_u32[2];  // entries with count > hi
         uint32_t inbin = s_u32[3];
-        bool overflow_ties = false;
+        const uint32_t gathered = s_u32[4];
         __syncthreads();
-        if (maxc == 0) {  // nothing visited: the candidate pass below takes nothing
-            lo = 1;
-            hi = 0;
-            above = 0;
-            inbin = 0;
-        }
-        // narrow [lo, hi] until the candidates (every count >= lo) fit the
-        // tile or one value remains
-        const bool fits = (uint64_t)above + inbin <= tile;
-        while (!fits && lo <= hi) {
-            const uint64_t range = hi - lo + 1;
-            const uint64_t width = (range + kBins - 1) / kBins;
-            for (uint32_t b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
-            __syncthreads();
-            // The lowest bin is not counted: it holds most entries (counts of
-            // one or two visits) and would serialize the LDS atomics, yet it
-            // matters only when fewer than k entries lie above it -- then the
-            // next pass narrows into it.
-            for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
-                uint32_t v, c;
-                E.get(i, v, c);
-                if (c >= lo && c <= hi) {
-                    const uint32_t bin = (uint32_t)((c - lo) / width);
-                    if (bin) atomicAdd(&hist[bin], 1u);
-                }
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t cum = above, b = 0;
-                const uint32_t nb = (uint32_t)((range + width - 1) / width);
-                for (int bb = (int)nb - 1; bb >= 0; --bb) {
-                    if (cum + hist[bb] >= k || bb == 0) {
-                        b = (uint32_t)bb;
-                        break;
-                    }
-                    cum += hist[bb];
-                }
-                s_u32[1] = b;
-                s_u32[2] = cum;        // entries above bin b
-                s_u32[3] = hist[b];    // entries in bin b (bin 0: not counted)
-            }
-            __syncthreads();
-            const uint32_t b = s_u32[1];
-            const uint64_t nlo = lo + (uint64_t)b * width;
-            uint64_t nhi = nlo + width - 1;
-            if (nhi > hi) nhi = hi;
-            above = s_u32[2];
-            inbin = s_u32[3];
-            lo = nlo;
-            hi = nhi;
-            __syncthreads();
-            if (b != 0 && (uint64_t)above + inbin <= tile) break;
-            if (lo == hi) {
-                // one value left whose count is not known to fit: the exact
-                // id select below takes the k - above smallest ids among it
-                overflow_ties = true;
-                break;
-            }
-        }
-        // collect candidates: everything above hi, and the bin [lo, hi]
-        uint32_t tie_limit = 0xFFFFFFFFu;  // ties at count lo == hi keep ids <= tie_limit
-        if (overflow_ties) {
-            // the narrowing can end on an uncounted value: count its ties, and
-            // take them all when they fit beside the entries above
-            if (threadIdx.x == 0) s_u32[7] = 0;
-            __syncthreads();
-            uint32_t ties = 0;
-            for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
-                uint32_t v, c;
-                E.get(i, v, c);
-                ties += c == (uint32_t)lo;
-            }
-            if (ties) atomicAdd(&s_u32[7], ties);
-            __syncthreads();
-            if ((uint64_t)above + s_u32[7] <= tile) overflow_ties = false;
-            __syncthreads();
-        }
-        if (overflow_ties) {
-            // radix select of the (k - above) smallest device ids among count == lo
-            const uint32_t need = k - above;
-            uint32_t prefix = 0, pmask = 0;
-            const int shifts[3] = {21, 10, 0};
-            const uint32_t widths[3] = {11, 11, 10};
-            uint32_t rank_left = need;
-            for (int pass = 0; pass < 3; ++pass) {
-                for (uint32_t bb = threadIdx.x; bb < kBins; bb += blockDim.x) hist[bb] = 0;
+        if (k == 0) continue;  // entropy only
+        const bool pooled = maxc == 0 || (uint64_t)above + inbin <= E.Q.cand_cap;
+        if (pooled) {
+            uint32_t nc = maxc ? gathered : 0u;
+            if (nc > tile) nc = tile;  // cannot happen: the gather fits the slots
+            for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) cand[i] = p.tk_cand[E.Q.cand_off + i];
+            if (threadIdx.x == 0) s_u32[6] = nc;
+        } else {
+            uint64_t lo = 1ull << b0, hi = (2ull << b0) - 1;
+            if (hi > maxc) hi = maxc;
+            bool overflow_ties = false;
+            // narrow [lo, hi] until the candidates (every count >= lo) fit the
+            // tile or one value remains
+            const bool fits = (uint64_t)above + inbin <= tile;
+            while (!fits && lo <= hi) {
+                const uint64_t range = hi - lo + 1;
+                const uint64_t width = (range + kBins - 1) / kBins;
+                for (uint32_t b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
                 __syncthreads();
-                const uint32_t dmask = (1u << widths[pass]) - 1;
+                // The lowest bin is not counted: it holds most entries (counts of
+                // one or two visits) and would serialize the LDS atomics, yet it
+                // matters only when fewer than k entries lie above it -- then the
+                // next pass narrows into it.
                 for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
                     uint32_t v, c;
                     E.get(i, v, c);
-                    if (c == (uint32_t)lo && (v & pmask) == prefix) atomicAdd(&hist[(v >> shifts[pass]) & dmask], 1u);
+                    if (c >= lo && c <= hi) {
+                        const uint32_t bin = (uint32_t)((c - lo) / width);
+                        if (bin) atomicAdd(&hist[bin], 1u);
+                    }
                 }
                 __syncthreads();
                 if (threadIdx.x == 0) {
-                    uint32_t cum = 0, d = 0;
-                    for (uint32_t dd = 0; dd <= dmask; ++dd) {
-                        if (cum + hist[dd] >= rank_left) {
-                            d = dd;
+                    uint32_t cum = above, b = 0;
+                    const uint32_t nb = (uint32_t)((range + width - 1) / width);
+                    for (int bb = (int)nb - 1; bb >= 0; --bb) {
+                        if (cum + hist[bb] >= k || bb == 0) {
+                            b = (uint32_t)bb;
                             break;
                         }
-                        cum += hist[dd];
+                        cum += hist[bb];
                     }
-                    s_u32[4] = d;
-                    s_u32[5] = cum;
+                    s_u32[1] = b;
+                    s_u32[2] = cum;        // entries above bin b
+                    s_u32[3] = hist[b];    // entries in bin b (bin 0: not counted)
                 }
                 __syncthreads();
-                prefix |= s_u32[4] << shifts[pass];
-                pmask |= dmask << shifts[pass];
-                rank_left -= s_u32[5];
+                const uint32_t b = s_u32[1];
+                const uint64_t nlo = lo + (uint64_t)b * width;
+                uint64_t nhi = nlo + width - 1;
+                if (nhi > hi) nhi = hi;
+                above = s_u32[2];
+                inbin = s_u32[3];
+                lo = nlo;
+                hi = nhi;
+                __syncthreads();
+                if (b != 0 && (uint64_t)above + inbin <= tile) break;
+                if (lo == hi) {
+                    // one value left whose count is not known to fit: the exact
+                    // id select below takes the k - above smallest ids among it
+                    overflow_ties = true;
+                    break;
+                }
+            }
+            // collect candidates: everything above hi, and the bin [lo, hi]
+            uint32_t tie_limit = 0xFFFFFFFFu;  // ties at count lo == hi keep ids <= tie_limit
+            if (overflow_ties) {
+                // the narrowing can end on an uncounted value: count its ties, and
+                // take them all when they fit beside the entries above
+                if (threadIdx.x == 0) s_u32[7] = 0;
+                __syncthreads();
+                uint32_t ties = 0;
+                for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                    uint32_t v, c;
+                    E.get(i, v, c);
+                    ties += c == (uint32_t)lo;
+                }
+                if (ties) atomicAdd(&s_u32[7], ties);
+                __syncthreads();
+                if ((uint64_t)above + s_u32[7] <= tile) overflow_ties = false;
                 __syncthreads();
             }
-            tie_limit = prefix;
-        }
-        if (threadIdx.x == 0) s_u32[6] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
-            uint32_t v, c;
-            E.get(i, v, c);
-            if (c == 0) continue;
-            bool take = c > hi;
-            if (!take && c >= lo && c <= hi) take = !overflow_ties || v <= tie_limit;
-            if (take) {
-                const uint32_t slot = atomicAdd(&s_u32[6], 1u);
-                if (slot < tile) cand[slot] = ((unsigned long long)c << 32) | (uint32_t)~v;
+            if (overflow_ties) {
+                // radix select of the (k - above) smallest device ids among count == lo
+                const uint32_t need = k - above;
+                uint32_t prefix = 0, pmask = 0;
+                const int shifts[3] = {21, 10, 0};
+                const uint32_t widths[3] = {11, 11, 10};
+                uint32_t rank_left = need;
+                for (int pass = 0; pass < 3; ++pass) {
+                    for (uint32_t bb = threadIdx.x; bb < kBins; bb += blockDim.x) hist[bb] = 0;
+                    __syncthreads();
+                    const uint32_t dmask = (1u << widths[pass]) - 1;
+                    for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                        uint32_t v, c;
+                        E.get(i, v, c);
+                        if (c == (uint32_t)lo && (v & pmask) == prefix) atomicAdd(&hist[(v >> shifts[pass]) & dmask], 1u);
+                    }
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        uint32_t cum = 0, d = 0;
+                        for (uint32_t dd = 0; dd <= dmask; ++dd) {
+                            if (cum + hist[dd] >= rank_left) {
+                                d = dd;
+                                break;
+                            }
+                            cum += hist[dd];
+                        }
+                        s_u32[4] = d;
+                        s_u32[5] = cum;
+                    }
+                    __syncthreads();
+                    prefix |= s_u32[4] << shifts[pass];
+                    pmask |= dmask << shifts[pass];
+                    rank_left -= s_u32[5];
+                    __syncthreads();
+                }
+                tie_limit = prefix;
+            }
+            if (threadIdx.x == 0) s_u32[6] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < E.n; i += blockDim.x) {
+                uint32_t v, c;
+                E.get(i, v, c);
+                if (c == 0) continue;
+                bool take = c > hi;
+                if (!take && c >= lo && c <= hi) take = !overflow_ties || v <= tie_limit;
+                if (take) {
+                    const uint32_t slot = atomicAdd(&s_u32[6], 1u);
+                    if (slot < tile) cand[slot] = ((unsigned long long)c << 32) | (uint32_t)~v;
+                }
             }
         }
         __syncthreads();
@@ -950,6 +1104,12 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int l
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v, unsigned long long *out_c,
                              int tile, int num_cus, hipStream_t s)
 {
+    hipLaunchKernelGGL(batch_topk_stats_kernel, dim3(num_cus * 4), dim3(kTopkBlock), 0, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_topk_gather_kernel, dim3(num_cus * 4), dim3(kTopkBlock), 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_topk_kernel, dim3(num_cus * 2), dim3(1024), (size_t)tile * 8, s, p, inv, out_v, out_c,
                        (uint32_t)tile);
     return hipGetLastError();

@@ -38,7 +38,7 @@ void drop_exec(BatchState &b)
 struct Plan {
     std::vector<BatchQuery> q;
     std::vector<BatchChunk> chunks;  // workgroup chunks of dense queries, then wavefront chunks of hash queries
-    uint64_t dense = 0, hash = 0, touched = 0;  // elements needed
+    uint64_t dense = 0, hash = 0, touched = 0, cand = 0;  // elements needed
     uint32_t n_dense = 0, n_hash = 0;
     uint64_t dense_chunks = 0;
 };
@@ -94,6 +94,13 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         B.touched_cap = z.touched_cap;
         B.touched_off = P.touched;
         P.touched += z.touched_cap;
+        // top-k candidates: no more than the query's distinct vertices while
+        // its pair list holds them all, and never more than the LDS tile
+        B.cand_cap = d.topk ? std::min<uint32_t>((uint32_t)ctx->shape.topk_tile, std::max<uint32_t>(z.touched_cap, 1))
+                            : 0;
+        B.cand_off = P.cand;
+        P.cand += B.cand_cap;
+        B.pad = 0;
     }
     uint64_t dense_walkers = 0;
     for (const BatchQuery &B : P.q)
@@ -119,7 +126,8 @@ uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint
     if (P.dense > b.dense_cap) need += 4 * P.dense;
     if (P.hash > b.hash_cap) need += 8 * P.hash;
     if (P.touched > b.touched_cap) need += 8 * P.touched;
-    if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + 16);
+    if (P.cand > b.cand_cap) need += 8 * P.cand;
+    if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + sizeof(TopkStats) + 24);
     if (P.chunks.size() > b.chunk_cap) need += P.chunks.size() * sizeof(BatchChunk);
     if ((uint64_t)nq * k > b.topk_cap) need += (uint64_t)nq * k * 12;
     return need;
@@ -145,6 +153,8 @@ BatchLaunch launch_args(ptw_ctx *ctx)
     p.steps = b.steps;
     p.entropy = b.entropy;
     p.queue = b.queue;
+    p.tk_stats = b.tk_stats;
+    p.tk_cand = b.tk_cand;
     return p;
 }
 
@@ -188,13 +198,22 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
         hfree(b.overflow);
         hfree(b.steps);
         hfree(b.entropy);
+        hfree(b.tk_stats);
         const uint32_t cap = std::max<uint32_t>(nq, b.q_cap + b.q_cap / 2);
         HIP_TRY(ctx, hipMalloc((void **)&b.queries, (size_t)cap * sizeof(BatchQuery)));
         HIP_TRY(ctx, hipMalloc((void **)&b.touched_count, (size_t)cap * 4));
         HIP_TRY(ctx, hipMalloc((void **)&b.overflow, (size_t)cap * 4));
         HIP_TRY(ctx, hipMalloc((void **)&b.steps, (size_t)cap * 8));
         HIP_TRY(ctx, hipMalloc((void **)&b.entropy, (size_t)cap * 8));
+        HIP_TRY(ctx, hipMalloc((void **)&b.tk_stats, (size_t)cap * sizeof(TopkStats)));
         b.q_cap = cap;
+        moved = true;
+    }
+    if (P.cand > b.cand_cap || !b.tk_cand) {
+        hfree(b.tk_cand);
+        const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(P.cand, 1), b.cand_cap + b.cand_cap / 2);
+        HIP_TRY(ctx, hipMalloc((void **)&b.tk_cand, cap * 8));
+        b.cand_cap = cap;
         moved = true;
     }
     if (P.chunks.size() > b.chunk_cap) {
@@ -254,6 +273,8 @@ void free_batch(ptw_ctx *ctx)
     hfree(b.overflow);
     hfree(b.steps);
     hfree(b.entropy);
+    hfree(b.tk_stats);
+    hfree(b.tk_cand);
     hfree(b.chunks);
     hfree(b.topk_v);
     hfree(b.topk_c);

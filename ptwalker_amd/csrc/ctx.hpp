@@ -26,7 +26,10 @@ struct BatchState {
     uint32_t *touched_count = nullptr, *overflow = nullptr;
     unsigned long long *steps = nullptr;
     double *entropy = nullptr;
+    TopkStats *tk_stats = nullptr;
     uint32_t q_cap = 0;
+    unsigned long long *tk_cand = nullptr;
+    uint64_t cand_cap = 0;
     BatchChunk *chunks = nullptr;
     uint64_t chunk_cap = 0;
     uint32_t *topk_v = nullptr;

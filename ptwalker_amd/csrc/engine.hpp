@@ -177,6 +177,22 @@ struct BatchQuery {
     uint64_t touched_off;  // first pair of the (vertex, count) list
     uint32_t touched_cap;  // list capacity in pairs
     uint32_t stream;       // 4th Philox counter word
+    uint64_t cand_off;     // first top-k candidate slot in the candidate pool
+    uint32_t cand_cap;     // candidate slots (0 without top-k)
+    uint32_t pad;
+};
+
+// Per-query statistics of the visit counts, summed by many workgroups with
+// atomics (integers only, so the result does not depend on their order):
+// entries per power-of-two count bucket (bucket b holds counts in
+// [2^b, 2^(b+1))), the largest count, sum c and sum c ln c in 2^-20 fixed
+// point; ncand counts the top-k candidates gathered so far.
+struct TopkStats {
+    uint32_t hist[32];
+    uint32_t maxc;
+    uint32_t ncand;
+    unsigned long long sum;
+    unsigned long long clogc;
 };
 
 struct BatchChunk {
@@ -205,6 +221,8 @@ struct BatchLaunch {
     unsigned long long *steps;  // per query
     double *entropy;            // per query, written by the top-k kernel
     unsigned long long *queue;  // [0]: next wavefront chunk, [1]: next workgroup chunk
+    TopkStats *tk_stats;        // per query
+    unsigned long long *tk_cand;  // candidate pool: (count << 32 | ~device id)
 };
 
 // Zeroes the per-query counters and the chunk queue of the batch in hdr.
@@ -214,8 +232,11 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 // of dense slices.
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
                              hipStream_t s);
-// Per-query top-k (count desc, device id asc) into out_v (graph ids) / out_c,
-// one workgroup per query, `tile` candidate slots of LDS per workgroup.
+// Per-query entropy and top-k (count desc, device id asc) into out_v (graph
+// ids) / out_c: count statistics and the candidate gather run over segments
+// of every query's entries on the whole grid; one workgroup per query sorts
+// the candidates in a `tile`-slot LDS tile (or narrows the count range over
+// the query's entries when they do not fit).
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v,
                              unsigned long long *out_c, int tile, int num_cus, hipStream_t s);
 // Zero what a batch left in its slices (dense: the touched entries or the whole

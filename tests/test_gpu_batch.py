@@ -142,6 +142,32 @@ def test_topk_tie_overflow_path(ctx):
         ctx.set_tuning(topk_tile=t.topk_tile)
 
 
+@pytest.mark.parametrize("dense_threshold", [1, 0])
+def test_pair_list_overflow_reads_slices(ctx, g12, dense_threshold):
+    # two or three walkers at tp = 0.01 take ~100 steps each, and the long
+    # walks visit more distinct vertices than the pair list holds (1.25x the
+    # expected visits + 64): top-k and the entropy of those queries read the
+    # whole dense slice (threshold 1: every query dense) or hash table
+    # (default threshold: every query hashed)
+    gi, rp, col = g12
+    ctx.from_csr(rp, col)
+    n = 128
+    rng = np.random.default_rng(4)
+    src = rng.integers(0, gi.num_vertices, n)
+    W = rng.integers(2, 4, n)
+    tp = np.full(n, 0.01)
+    r = ctx.batch_ppr(src, W, tp, topk=20, seed=5, dense_threshold=dense_threshold)
+    assert (r.dense_queries == n) == (dense_threshold == 1)
+    overflowed = 0
+    for i in range(n):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), 0.01, i, seed=5)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+        v, c = oracle.topk(ref["counts"], rp, 20)
+        assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], c), i
+        overflowed += np.count_nonzero(ref["counts"]) > int(1.25 * W[i] * 101) + 64
+    assert overflowed > 0
+
+
 def test_batch_small_graph_dangling_and_edges(ctx):
     rp, col = random_graph(30, 0.15, seed=3, dangling=3)
     ctx.from_csr(rp, col)
@@ -172,12 +198,16 @@ def test_batch_invalid(ctx):
 
 def test_batch_scale20_properties(ctx):
     gi = ctx.rmat(scale=20, edge_factor=16, seed=1)
+    rp, _ = ctx.export_csr()
     src, W, tp = make_queries(gi, 512, seed=11, wmin=1000, wmax=1000000)
     r = ctx.batch_ppr(src, W, tp, topk=100, seed=1)
-    for i in (0, 100, 511):
+    # the largest queries' pair lists span many top-k segments (16384 entries)
+    big = int(np.argmax(W))
+    for i in (0, 100, 511, big):
         c = ctx.batch_counts(i)
         assert int(c.sum()) == int(W[i]) + int(r.steps[i])
         assert c[src[i]] >= W[i]
-        order = np.argsort(-c.astype(np.float64), kind="stable")[:100]
-        assert np.array_equal(np.sort(r.topk_count[i])[::-1], np.sort(c[order])[::-1])
+        v, cc = oracle.topk(c, rp, 100)
+        assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], cc), i
+    assert np.count_nonzero(ctx.batch_counts(big)) > 4 * 16384
     assert abs(r.total_steps - float(np.sum(W / tp))) < 6 * np.sqrt(np.sum(W * (1 - tp) / tp ** 2))
