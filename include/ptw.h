@@ -386,6 +386,8 @@ typedef struct ptw_tuning {
                               (lds_hubs + 2 cache_slots) * 4 B <= 159 KiB */
     int32_t batch_chunk;   /* walker ids per queue entry in batched launches (>= 64) */
     int32_t topk_tile;     /* LDS candidate slots of the top-k kernel (power of two, 256..4096) */
+    int32_t dense_chunk;   /* walker ids per workgroup chunk of a dense batched query, a multiple
+                              of 1024 up to 2^20; 0 = from the batch's dense walkers */
 } ptw_tuning;
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
 int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);

@@ -135,6 +135,14 @@ class Query(ctypes.Structure):
     ]
 
 
+# the same layout as a numpy record, so query tables are filled without a
+# Python loop
+QUERY_DTYPE = np.dtype({"names": [f for f, _ in Query._fields_],
+                        "formats": [np.uint32, np.uint32, np.uint64, np.float64],
+                        "offsets": [getattr(Query, f).offset for f, _ in Query._fields_],
+                        "itemsize": ctypes.sizeof(Query)})
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("queries", ctypes.c_void_p),
@@ -175,6 +183,7 @@ class Tuning(ctypes.Structure):
         ("cache_slots", ctypes.c_int32),
         ("batch_chunk", ctypes.c_int32),
         ("topk_tile", ctypes.c_int32),
+        ("dense_chunk", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -418,12 +427,16 @@ class Context:
     @staticmethod
     def _batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold):
         n = len(sources)
-        qs = (Query * max(n, 1))()
-        streams = range(n) if streams is None else streams
-        for i, (s, w, tp, st) in enumerate(zip(sources, walkers, tps, streams)):
-            qs[i].source, qs[i].walkers, qs[i].tp, qs[i].stream = int(s), int(w), float(tp), int(st)
+        qs = np.zeros(max(n, 1), dtype=QUERY_DTYPE)
+        if n:
+            if not (len(walkers) == len(tps) == n and (streams is None or len(streams) == n)):
+                raise ValueError("sources, walkers, tps and streams must have one entry per query")
+            qs["source"][:n] = np.asarray(sources, dtype=np.int64)
+            qs["walkers"][:n] = np.asarray(walkers, dtype=np.int64)
+            qs["tp"][:n] = np.asarray(tps, dtype=np.float64)
+            qs["stream"][:n] = np.arange(n) if streams is None else np.asarray(streams, dtype=np.int64)
         d = BatchDesc()
-        d.queries = ctypes.cast(qs, ctypes.c_void_p)
+        d.queries = qs.ctypes.data
         d.num_queries = n
         d.topk = topk
         d.seed = seed

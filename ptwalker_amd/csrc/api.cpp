@@ -580,6 +580,7 @@ int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)
     t->cache_slots = ctx->shape.cache_slots;
     t->batch_chunk = ctx->shape.batch_chunk;
     t->topk_tile = ctx->shape.topk_tile;
+    t->dense_chunk = ctx->shape.dense_chunk;
     return PTW_OK;
 }
 
@@ -600,7 +601,10 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
     if (t->batch_chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "batch_chunk must be >= 64");
     if (t->topk_tile < 256 || t->topk_tile > 4096 || (t->topk_tile & (t->topk_tile - 1)))
         return fail(ctx, PTW_ERR_INVALID_ARG, "topk_tile must be a power of two in [256, 4096]");
+    if (t->dense_chunk < 0 || t->dense_chunk > (1 << 20) || t->dense_chunk % 1024)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "dense_chunk must be 0 or a multiple of 1024 up to 2^20");
     ptw::drop_batch_graph(ctx);
+    ctx->shape.dense_chunk = t->dense_chunk;
     ctx->shape.batch_chunk = t->batch_chunk;
     ctx->shape.topk_tile = t->topk_tile;
     ctx->shape.block = t->block;

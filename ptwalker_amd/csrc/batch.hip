@@ -45,6 +45,9 @@ constexpr uint32_t kWaveHubs = 1024;  // per-wavefront direct hub counters
 constexpr int kCacheProbes = 8;
 constexpr uint32_t kSeg = 65536;  // dense-slice compaction segment
 
+// 16 B vector for streaming stores that bypass the caches' reuse
+typedef uint32_t vec4u __attribute__((ext_vector_type(4)));
+
 struct LaneQuery {
     uint32_t q = kEmpty;
     uint32_t src = 0, thr = 0, kind = 0, mask = 0, stream = 0;
@@ -123,13 +126,13 @@ __global__ void batch_begin_kernel(BatchLaunch p, uint32_t max_queries)
         p.touched_count[i] = 0;
         p.overflow[i] = 0;
         p.steps[i] = 0;
-        TopkStats *st = p.tk_stats + i;
-        for (int b = 0; b < 32; ++b) st->hist[b] = 0;
-        st->maxc = 0;
-        st->ncand = 0;
-        st->sum = 0;
-        st->clogc = 0;
     }
+    // the top-k statistics, as one flat array of 16 B words
+    const uint32_t q_used = nq < max_queries ? nq : max_queries;
+    const uint64_t words = (uint64_t)q_used * (sizeof(TopkStats) / 16);
+    uint4 *st4 = reinterpret_cast<uint4 *>(p.tk_stats);
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        st4[w] = uint4{0u, 0u, 0u, 0u};
 }
 
 __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
@@ -436,86 +439,108 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     flush();
 }
 
-// Reserves n list places for each thread of the workgroup, in thread order,
-// with one atomic on the query's count; returns the thread's first place.
-// Every thread of the workgroup calls it.
-__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t *count, uint32_t *s_wave, uint32_t *s_base)
+// Lane order within a wavefront: LDS writes of the wavefront are complete
+// and visible to its other lanes after this point.
+__device__ __forceinline__ void wave_lds_sync()
 {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kCompactBlock = 256;
+constexpr uint32_t kCompactRound = 1024;  // pair places a wavefront round can fill
+
+// One round of a wavefront's compaction: each lane brings up to 16 pairs
+// (valid[j]); they are ranked in lane order, staged in the wavefront's LDS
+// row and written to the query's list from one reservation, as consecutive
+// 8 B stores.  Every lane of the wavefront calls it.
+__device__ __forceinline__ void compact_round(const uint2 (&e)[16], const bool (&valid)[16], uint2 *stage,
+                                              uint32_t *count, uint2 *list, uint32_t cap)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) n += valid[j];
     uint32_t incl = n;  // inclusive prefix over the wavefront
 #pragma unroll
     for (uint32_t o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(incl, o);
         if (lane >= o) incl += y;
     }
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) return;  // wave-uniform
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(count, total);
+    base = __shfl(base, 63);
+    uint32_t at = incl - n;
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
-        before += w < wave ? s_wave[w] : 0u;
-        total += s_wave[w];
-    }
-    if (threadIdx.x == 0) *s_base = total ? atomicAdd(count, total) : 0u;
-    __syncthreads();
-    const uint32_t first = *s_base + before + incl - n;
-    __syncthreads();  // s_wave and s_base are reused by the next call
-    return first;
+    for (int j = 0; j < 16; ++j)
+        if (valid[j]) stage[at++] = e[j];
+    wave_lds_sync();
+    for (uint32_t i = lane; i < total; i += 64)
+        if (base + i < cap) list[base + i] = stage[i];
+    wave_lds_sync();  // the row is reused by the next round
 }
 
 // After the walk: the visited vertices of each query, listed as (device id,
 // count) pairs in the query's touched list, so that top-k streams the pairs
 // instead of whole slices.  One read of each slice or table: the tasks are
 // (query, segment) with the segments of a dense slice kSeg counters long and
-// those of a hash table 1/segs of it (a multiple of 16 slots); each thread
-// takes 16 counters or 8 slots per round with 16 B loads, and one block scan
-// plus one reservation on the query's count place the round's pairs.  A list
+// those of a hash table 1/segs of it (a multiple of 16 slots).  Wavefronts
+// work independently: each takes 1024 counters (16 per lane) or 1024 slots
+// (16 per lane) per round with 16 B loads, and writes the round's pairs
+// coalesced through LDS with one reservation on the query's count.  A list
 // that would overflow marks the query for full-slice scans
 // (touched_count > capacity).
-__global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
+__global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunch p)
 {
-    __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_base;
+    __shared__ uint2 stage[kCompactBlock / 64][kCompactRound];
     const uint32_t nq = (uint32_t)p.hdr[1];
     const uint32_t pitch = dense_pitch(p.V);
     const uint32_t segs = (pitch + kSeg - 1) / kSeg;
     const uint64_t tasks = (uint64_t)nq * segs;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    uint2 *row = stage[wave];
     for (uint64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
         const uint32_t q = (uint32_t)(t / segs);
         const uint32_t seg = (uint32_t)(t % segs);
         const BatchQuery bq = p.queries[q];
         uint2 *list = p.touched + bq.touched_off;
-        if (bq.kind == SLICE_DENSE) {  // block-uniform
+        uint2 e[16];
+        bool valid[16];
+        if (bq.kind == SLICE_DENSE) {
             const uint32_t lo = seg * kSeg;
             const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // a multiple of 64
             const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + bq.slice_off);
-            for (uint32_t base = lo; base < hi; base += 256 * 16) {
-                const uint32_t my0 = base + threadIdx.x * 16;  // my0 < hi implies my0 + 16 <= hi
-                uint32_t c[16];
-                if (my0 < hi) {
+            const uint32_t step = waves * 1024;
+            // the next round's 64 B per lane is loaded before this round is
+            // placed, so two rounds of loads are in flight per wavefront
+            uint4 nx[4];
+            uint32_t base = lo + wave * 1024;
+            {
+                const uint32_t my0 = base + lane * 16;  // my0 < hi implies my0 + 16 <= hi
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const uint4 x = slice4[(my0 >> 2) + j];
-                        c[4 * j] = x.x;
-                        c[4 * j + 1] = x.y;
-                        c[4 * j + 2] = x.z;
-                        c[4 * j + 3] = x.w;
-                    }
-                } else {
+                for (int j = 0; j < 4; ++j) nx[j] = my0 < hi ? slice4[(my0 >> 2) + j] : uint4{0u, 0u, 0u, 0u};
+            }
+            for (; base < hi; base += step) {  // wave-uniform
+                const uint32_t my0 = base + lane * 16;
+                uint4 cur[4];
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) c[j] = 0;
+                for (int j = 0; j < 4; ++j) cur[j] = nx[j];
+                const uint32_t nmy0 = my0 + step;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) nx[j] = nmy0 < hi ? slice4[(nmy0 >> 2) + j] : uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    e[4 * j] = uint2{my0 + 4 * j, cur[j].x};
+                    e[4 * j + 1] = uint2{my0 + 4 * j + 1, cur[j].y};
+                    e[4 * j + 2] = uint2{my0 + 4 * j + 2, cur[j].z};
+                    e[4 * j + 3] = uint2{my0 + 4 * j + 3, cur[j].w};
                 }
-                uint32_t n = 0;
 #pragma unroll
-                for (int j = 0; j < 16; ++j) n += c[j] != 0;
-                uint32_t idx = block_reserve(n, &p.touched_count[q], s_wave, &s_base);
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    if (c[j]) {
-                        if (idx < bq.touched_cap) list[idx] = uint2{my0 + (uint32_t)j, c[j]};
-                        ++idx;
-                    }
-                }
+                for (int j = 0; j < 16; ++j) valid[j] = e[j].y != 0;
+                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap);
             }
         } else {
             // tables hold a power of two >= 2 slots and start at an even
@@ -526,32 +551,23 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(BatchLaunch p)
             if (lo >= cap) continue;  // block-uniform
             const uint64_t hi = lo + span < cap ? lo + span : cap;
             const uint2 *tab = p.hash + bq.slice_off;
-            for (uint64_t base = lo; base < hi; base += 256 * 8) {
-                const uint64_t my0 = base + threadIdx.x * 8;
-                uint2 e[8];
-                if (my0 + 8 <= hi) {
+            for (uint64_t base = lo + wave * 1024; base < hi; base += waves * 1024) {  // wave-uniform
+                const uint64_t my0 = base + lane * 16;
+                if (my0 + 16 <= hi) {
                     const uint4 *t4 = reinterpret_cast<const uint4 *>(tab + my0);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < 8; ++j) {
                         const uint4 x = t4[j];
                         e[2 * j] = uint2{x.x, x.y};
                         e[2 * j + 1] = uint2{x.z, x.w};
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) e[j] = my0 + j < hi ? tab[my0 + j] : uint2{kEmpty, 0u};
+                    for (int j = 0; j < 16; ++j) e[j] = my0 + j < hi ? tab[my0 + j] : uint2{kEmpty, 0u};
                 }
-                uint32_t n = 0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) n += e[j].x != kEmpty && e[j].y != 0;
-                uint32_t idx = block_reserve(n, &p.touched_count[q], s_wave, &s_base);
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (e[j].x != kEmpty && e[j].y != 0) {
-                        if (idx < bq.touched_cap) list[idx] = e[j];
-                        ++idx;
-                    }
-                }
+                for (int j = 0; j < 16; ++j) valid[j] = e[j].x != kEmpty && e[j].y != 0;
+                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap);
             }
         }
     }
@@ -604,29 +620,54 @@ __device__ __forceinline__ Entries entries_of(const BatchLaunch &p, uint32_t q)
     return E;
 }
 
-// The power-of-two count bucket holding the k-th largest count (bucket 0
-// when fewer than k entries exist), the entries in higher buckets and in it.
+// The count bin holding the k-th largest count (bin 1, the count 1, when
+// fewer than k entries exist), the entries in higher bins and in it.  Every
+// lane of the wavefront calls it: lane l reads bins 4l..4l+3.
 struct Bucket {
     uint32_t b, above, inbin;
 };
 __device__ Bucket kth_bucket(const TopkStats *st, uint32_t k)
 {
-    Bucket r{0, 0, 0};
-    uint32_t cum = 0;
-    for (int bb = 31; bb >= 0; --bb) {
-        const uint32_t h = st->hist[bb];
-        if (cum + h >= k || bb == 0) {
-            r.b = (uint32_t)bb;
-            r.inbin = h;
-            break;
-        }
-        cum += h;
+    static_assert(kCountBins == 256, "four bins per lane");
+    const uint32_t lane = threadIdx.x & 63;
+    const uint4 h4 = reinterpret_cast<const uint4 *>(st->hist)[lane];
+    const uint32_t h[4] = {h4.x, h4.y, h4.z, h4.w};
+    const uint32_t tot = h[0] + h[1] + h[2] + h[3];
+    uint32_t suf = tot;  // entries in this lane's bins and all higher ones
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_down(suf, o);
+        if (lane + o < 64) suf += y;
     }
-    r.above = cum;
+    const uint32_t higher = suf - tot;
+    const uint64_t m = __ballot(higher < k && suf >= k);
+    Bucket r;
+    if (m) {  // wave-uniform
+        uint32_t cum = higher, b = 0, inbin = 0;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+            if (cum + h[j] >= k) {
+                b = 4 * lane + (uint32_t)j;
+                inbin = h[j];
+                break;
+            }
+            cum += h[j];
+        }
+        const int L = __ffsll((unsigned long long)m) - 1;
+        r.b = __shfl(b, L);
+        r.above = __shfl(cum, L);
+        r.inbin = __shfl(inbin, L);
+    } else {
+        const uint32_t total = __shfl(suf, 0);
+        const uint32_t ones = __shfl(h[1], 0);
+        r.b = 1;
+        r.inbin = ones;
+        r.above = total - ones;
+    }
     return r;
 }
 
-constexpr uint32_t kTopkSeg = 16384;   // entries per (query, segment) task
+constexpr uint32_t kTopkSeg = 32768;   // entries per (query, segment) task
 constexpr uint32_t kTopkBlock = 512;  // workgroup size of the task kernels
 
 __device__ __forceinline__ uint32_t topk_segments(const Entries &E)
@@ -689,12 +730,12 @@ __device__ void for_topk_segments(const BatchLaunch &p, F &&fn)
 // Count statistics of every query (TopkStats, zeroed by batch_begin).
 __global__ __launch_bounds__(kTopkBlock) void batch_topk_stats_kernel(BatchLaunch p)
 {
-    __shared__ uint32_t s_hist[32];
+    __shared__ uint32_t s_hist[kCountBins];
     __shared__ uint32_t s_max;
     __shared__ unsigned long long s_sum, s_clogc;
     const uint32_t lane = threadIdx.x & 63;
     for_topk_segments(p, [&](uint32_t q, const Entries &E, uint32_t sg) {
-        if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
+        for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
         if (threadIdx.x == 0) {
             s_max = 0;
             s_sum = 0;
@@ -703,10 +744,10 @@ __global__ __launch_bounds__(kTopkBlock) void batch_topk_stats_kernel(BatchLaunc
         __syncthreads();
         const uint64_t lo = (uint64_t)sg * kTopkSeg;
         const uint64_t hi = lo + kTopkSeg < E.n ? lo + kTopkSeg : (uint64_t)E.n;
-        // counts of 1 and of 2-3 (most entries) are counted in registers;
-        // c ln c is summed in 2^-20 fixed point so the sum does not depend on
-        // the order of the additions
-        uint32_t local_max = 0, n_b0 = 0, n_b1 = 0;
+        // counts of 1, 2 and 3 (most entries) are counted in registers; c ln c
+        // is summed in 2^-20 fixed point so the sum does not depend on the
+        // order of the additions
+        uint32_t local_max = 0, n1 = 0, n2 = 0, n3 = 0;
         unsigned long long local_sum = 0, local_clogc = 0;
         for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
             uint32_t v, c;
@@ -714,33 +755,35 @@ __global__ __launch_bounds__(kTopkBlock) void batch_topk_stats_kernel(BatchLaunc
             local_max = c > local_max ? c : local_max;
             local_sum += c;
             if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
-            if (c == 1) {
-                ++n_b0;
-            } else if (c >= 4) {
-                atomicAdd(&s_hist[31 - __clz(c)], 1u);
-            } else if (c) {
-                ++n_b1;
+            if (c >= 4) {
+                atomicAdd(&s_hist[count_bin(c)], 1u);
+            } else {
+                n1 += c == 1;
+                n2 += c == 2;
+                n3 += c == 3;
             }
         }
         for (int o = 32; o > 0; o >>= 1) {
             local_max = max(local_max, (uint32_t)__shfl_xor(local_max, o));
             local_sum += __shfl_xor(local_sum, o);
             local_clogc += __shfl_xor(local_clogc, o);
-            n_b0 += __shfl_xor(n_b0, o);
-            n_b1 += __shfl_xor(n_b1, o);
+            n1 += __shfl_xor(n1, o);
+            n2 += __shfl_xor(n2, o);
+            n3 += __shfl_xor(n3, o);
         }
         if (lane == 0) {
             if (local_max) atomicMax(&s_max, local_max);
             if (local_sum) atomicAdd(&s_sum, local_sum);
             if (local_clogc) atomicAdd(&s_clogc, local_clogc);
-            if (n_b0) atomicAdd(&s_hist[0], n_b0);
-            if (n_b1) atomicAdd(&s_hist[1], n_b1);
+            if (n1) atomicAdd(&s_hist[1], n1);
+            if (n2) atomicAdd(&s_hist[2], n2);
+            if (n3) atomicAdd(&s_hist[3], n3);
         }
         __syncthreads();
         TopkStats *st = p.tk_stats + q;
-        if (threadIdx.x < 32) {
-            if (s_hist[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], s_hist[threadIdx.x]);
-        } else if (threadIdx.x == 32) {
+        for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x)
+            if (s_hist[b]) atomicAdd(&st->hist[b], s_hist[b]);
+        if (threadIdx.x == 0) {
             if (s_max) atomicMax(&st->maxc, s_max);
             if (s_sum) atomicAdd(&st->sum, s_sum);
             if (s_clogc) atomicAdd(&st->clogc, s_clogc);
@@ -759,10 +802,12 @@ __global__ __launch_bounds__(kTopkBlock) void batch_topk_gather_kernel(BatchLaun
     const uint32_t lane = threadIdx.x & 63;
     for_topk_segments(p, [&](uint32_t q, const Entries &E, uint32_t sg) {
         TopkStats *st = p.tk_stats + q;
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {
             const Bucket bk = kth_bucket(st, k);
-            s_ok = k > 0 && st->maxc > 0 && (uint64_t)bk.above + bk.inbin <= E.Q.cand_cap;
-            s_lo = 1u << bk.b;
+            if (threadIdx.x == 0) {
+                s_ok = k > 0 && st->maxc > 0 && (uint64_t)bk.above + bk.inbin <= E.Q.cand_cap;
+                s_lo = (uint32_t)count_bin_lo(bk.b);
+            }
         }
         __syncthreads();
         if (s_ok) {  // workgroup-uniform
@@ -824,15 +869,17 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
     for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
         const Entries E = entries_of(p, q);
         const TopkStats *st = p.tk_stats + q;
-        if (threadIdx.x == 0) {
-            const double N = (double)st->sum;
-            p.entropy[q] = N > 0 ? log(N) - ((double)st->clogc / 1048576.0) / N : 0.0;
+        if (threadIdx.x < 64) {
             const Bucket bk = kth_bucket(st, k);
-            s_u32[0] = st->maxc;
-            s_u32[1] = bk.b;
-            s_u32[2] = bk.above;
-            s_u32[3] = bk.inbin;
-            s_u32[4] = st->ncand;
+            if (threadIdx.x == 0) {
+                const double N = (double)st->sum;
+                p.entropy[q] = N > 0 ? log(N) - ((double)st->clogc / 1048576.0) / N : 0.0;
+                s_u32[0] = st->maxc;
+                s_u32[1] = bk.b;
+                s_u32[2] = bk.above;
+                s_u32[3] = bk.inbin;
+                s_u32[4] = st->ncand;
+            }
         }
         __syncthreads();
         const uint32_t maxc = s_u32[0];
@@ -849,7 +896,7 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
             for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) cand[i] = p.tk_cand[E.Q.cand_off + i];
             if (threadIdx.x == 0) s_u32[6] = nc;
         } else {
-            uint64_t lo = 1ull << b0, hi = (2ull << b0) - 1;
+            uint64_t lo = count_bin_lo(b0), hi = count_bin_hi(b0);
             if (hi > maxc) hi = maxc;
             bool overflow_ties = false;
             // narrow [lo, hi] until the candidates (every count >= lo) fit the
@@ -1000,24 +1047,49 @@ __global__ __launch_bounds__(1024) void batch_topk_kernel(BatchLaunch p, const u
 
 // ---- cleanup and export --------------------------------------------------------
 
-__global__ __launch_bounds__(1024) void batch_reset_kernel(BatchLaunch p)
+// Zeroes what a batch left in its slices, over (query, segment) tasks that
+// spread every slice across the grid (the compaction's segments): a dense
+// slice with few touched entries has them cleared one by one from its pair
+// list (a share of the list per segment); otherwise, and for hash tables,
+// each task streams zeros over its segment with 16 B stores.
+__global__ __launch_bounds__(256) void batch_reset_kernel(BatchLaunch p)
 {
     const uint32_t nq = (uint32_t)p.hdr[1];
-    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    const uint32_t pitch = dense_pitch(p.V);
+    const uint32_t segs = (pitch + kSeg - 1) / kSeg;
+    const uint64_t tasks = (uint64_t)nq * segs;
+    for (uint64_t t = blockIdx.x; t < tasks; t += gridDim.x) {
+        const uint32_t q = (uint32_t)(t / segs);
+        const uint32_t seg = (uint32_t)(t % segs);
         const BatchQuery Q = p.queries[q];
         if (Q.kind == SLICE_DENSE) {
-            // few touched entries: clear them one by one; otherwise stream
-            // zeros over the whole slice (a scattered 4 B store moves a line)
-            const uint32_t t = p.touched_count[q];
-            const uint32_t pitch = dense_pitch(p.V);
-            if (t <= Q.touched_cap && t < pitch / 16) {
-                for (uint32_t i = threadIdx.x; i < t; i += blockDim.x) p.dense[Q.slice_off + p.touched[Q.touched_off + i].x] = 0;
+            const uint32_t n = p.touched_count[q];
+            if (n <= Q.touched_cap && n < pitch / 16) {
+                // a scattered 4 B store moves a whole line: only for sparse slices
+                const uint32_t lo = (uint32_t)((uint64_t)n * seg / segs);
+                const uint32_t hi = (uint32_t)((uint64_t)n * (seg + 1) / segs);
+                for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+                    p.dense[Q.slice_off + p.touched[Q.touched_off + i].x] = 0;
             } else {
-                uint4 *d4 = reinterpret_cast<uint4 *>(p.dense + Q.slice_off);
-                for (uint32_t i = threadIdx.x; i < pitch / 4; i += blockDim.x) d4[i] = uint4{0u, 0u, 0u, 0u};
+                const uint32_t lo = seg * kSeg;
+                const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // multiples of 64
+                vec4u *d4 = reinterpret_cast<vec4u *>(p.dense + Q.slice_off);
+                const vec4u z = {0u, 0u, 0u, 0u};
+                for (uint32_t i = (lo >> 2) + threadIdx.x; i < (hi >> 2); i += blockDim.x)
+                    __builtin_nontemporal_store(z, &d4[i]);
             }
         } else {
-            for (uint32_t i = threadIdx.x; i <= Q.mask; i += blockDim.x) p.hash[Q.slice_off + i] = uint2{kEmpty, 0u};
+            // tables hold a power of two >= 2 slots from an even slot: 16 B
+            // stores of slot pairs are aligned
+            const uint64_t cap = (uint64_t)Q.mask + 1;
+            const uint64_t span = ((cap + segs - 1) / segs + 15) & ~15ull;
+            const uint64_t lo = (uint64_t)seg * span;
+            if (lo >= cap) continue;
+            const uint64_t hi = lo + span < cap ? lo + span : cap;
+            vec4u *h4 = reinterpret_cast<vec4u *>(p.hash + Q.slice_off);
+            const vec4u e = {kEmpty, 0u, kEmpty, 0u};
+            for (uint64_t i = (lo >> 1) + threadIdx.x; i < (hi >> 1); i += blockDim.x)
+                __builtin_nontemporal_store(e, &h4[i]);
         }
     }
 }
@@ -1097,7 +1169,7 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int l
     hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * num_cus), dim3(block), lds, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(kCompactBlock), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1117,7 +1189,7 @@ hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t
 
 hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s)
 {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(num_cus * 2), dim3(1024), 0, s, p);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(num_cus * 8), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

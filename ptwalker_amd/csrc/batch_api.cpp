@@ -43,15 +43,16 @@ struct Plan {
     uint64_t dense_chunks = 0;
 };
 
-// Walker ids per workgroup chunk of a dense query: about eight chunks per
-// resident workgroup over the batch's dense walkers, so the last chunks
-// balance the tail, and at least 32768 (32 per lane of a 1024-thread
-// workgroup) so that the barrier at a change of query stays rare.
-uint64_t dense_chunk_size(uint64_t dense_walkers, int num_cus)
+// Walker ids per workgroup chunk of a dense query: about 1/48 of the
+// query's walkers, so that the resident workgroups (one per CU) share a
+// handful of queries at any time and those queries' slices stay in the
+// Infinity Cache; at least 16384 (16 walkers per lane of a 1024-thread
+// workgroup, so the drain at the end of a chunk stays short) and at most
+// 49152 (a large query gains nothing from longer chunks).
+uint64_t dense_chunk_size(uint64_t walkers)
 {
-    const uint64_t target = dense_walkers / ((uint64_t)std::max(num_cus, 1) * 8);
-    uint64_t c = (target + 4095) / 4096 * 4096;
-    return std::min<uint64_t>(std::max<uint64_t>(c, 32768), 1u << 20);
+    const uint64_t c = (walkers / 48 + 4095) / 4096 * 4096;
+    return std::min<uint64_t>(std::max<uint64_t>(c, 16384), 49152);
 }
 
 int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
@@ -102,12 +103,10 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         P.cand += B.cand_cap;
         B.pad = 0;
     }
-    uint64_t dense_walkers = 0;
-    for (const BatchQuery &B : P.q)
-        if (B.kind == SLICE_DENSE) dense_walkers += B.walkers;
-    const uint64_t dchunk = dense_chunk_size(dense_walkers, ctx->num_cus);
     for (uint32_t i = 0; i < d.num_queries; ++i) {
         if (P.q[i].kind != SLICE_DENSE) continue;
+        const uint64_t dchunk =
+            ctx->shape.dense_chunk ? (uint64_t)ctx->shape.dense_chunk : dense_chunk_size(P.q[i].walkers);
         for (uint64_t s = 0; s < P.q[i].walkers; s += dchunk)
             P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(dchunk, P.q[i].walkers - s), s});
     }

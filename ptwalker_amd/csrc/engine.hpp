@@ -115,6 +115,7 @@ struct LaunchShape {
     int cache_slots = 1024;  // LDS visit-cache slots per workgroup (power of two, 0 = off)
     int batch_chunk = 16384;  // walker ids per chunk of a batched launch
     int topk_tile = 2048;    // top-k LDS candidate slots
+    int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)
 };
 
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
@@ -182,18 +183,41 @@ struct BatchQuery {
     uint32_t pad;
 };
 
+// Count bins of the top-k statistics: counts below 16 have a bin each; from
+// 16 on, every power of two is cut into 8 bins by the 3 bits after the
+// leading one (bin width at most 1/8 of the bin's lowest count).  240 bins
+// cover all u32 counts; bin 0 (count 0) stays empty.
+constexpr uint32_t kCountBins = 256;
+__host__ __device__ inline uint32_t count_bin(uint32_t c)
+{
+    if (c < 16) return c;
+    const uint32_t e = 31 - __builtin_clz(c);
+    return 8 * (e - 2) + ((c >> (e - 3)) & 7);
+}
+__host__ __device__ inline uint64_t count_bin_lo(uint32_t b)
+{
+    if (b < 16) return b;
+    return (uint64_t)(8 + b % 8) << (b / 8 - 1);
+}
+__host__ __device__ inline uint64_t count_bin_hi(uint32_t b)
+{
+    if (b < 16) return b;
+    return ((uint64_t)(9 + b % 8) << (b / 8 - 1)) - 1;
+}
+
 // Per-query statistics of the visit counts, summed by many workgroups with
 // atomics (integers only, so the result does not depend on their order):
-// entries per power-of-two count bucket (bucket b holds counts in
-// [2^b, 2^(b+1))), the largest count, sum c and sum c ln c in 2^-20 fixed
-// point; ncand counts the top-k candidates gathered so far.
+// entries per count bin, the largest count, sum c and sum c ln c in 2^-20
+// fixed point; ncand counts the top-k candidates gathered so far.
 struct TopkStats {
-    uint32_t hist[32];
+    uint32_t hist[kCountBins];
     uint32_t maxc;
     uint32_t ncand;
     unsigned long long sum;
     unsigned long long clogc;
+    uint32_t pad[2];  // 16 B multiple: hist rows stay 16 B aligned
 };
+static_assert(sizeof(TopkStats) % 16 == 0, "TopkStats rows must stay 16 B aligned");
 
 struct BatchChunk {
     uint32_t q;

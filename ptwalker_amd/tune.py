@@ -14,7 +14,9 @@ one warm-up run.
 Knobs:
   walks   chunk (walker ids per wavefront refill), block, lds_hubs (direct LDS
           counters), cache_slots (LDS visit cache)
-  batches batch_chunk (walker ids per queue entry), block, dense_threshold
+  batches batch_chunk (walker ids per queue entry of a hash query),
+          dense_chunk (walker ids per workgroup chunk of a dense query; 0 =
+          the planner's per-query rule), block, lds_hubs, dense_threshold
           (walkers above which a query gets a dense counter slice rather than
           a hash slice), topk_tile (LDS candidate slots of the top-k kernel)
 
@@ -36,14 +38,16 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ptwalker_amd", 
 WALK_SPACE: Dict[str, List[int]] = {
     "chunk": [1024, 2048, 4096, 8192, 16384],
     "block": [256, 512, 1024],
-    "lds_hubs": [0, 2048, 4096, 8192, 16384],
-    "cache_slots": [0, 1024, 2048, 4096, 8192],
+    "lds_hubs": [8192, 16384, 24576, 36864],
+    "cache_slots": [0, 512, 1024, 2048],
 }
 
 BATCH_SPACE: Dict[str, List[int]] = {
     "batch_chunk": [2048, 4096, 8192, 16384, 32768, 65536],
+    "dense_chunk": [0, 16384, 32768, 49152],
     "block": [256, 512, 1024],
-    "dense_threshold": [25_000, 50_000, 100_000, 200_000, 400_000],
+    "lds_hubs": [16384, 24576, 36864],
+    "dense_threshold": [12_500, 25_000, 50_000, 100_000, 200_000],
     "topk_tile": [256, 512, 1024, 2048, 4096],
 }
 
