@@ -21,12 +21,15 @@
 // After the walk a compaction pass lists each query's visited vertices as
 // (vertex, count) pairs, so top-k and the cleanup read only those.
 //
-// Top-k: one workgroup per query.  The pass that sums the entropy terms also
-// counts entries per power-of-two count bucket; the bucket holding the k-th
-// count starts a narrowing of the value range with LDS histograms until the
-// candidates fit the tile; they are gathered into LDS and bitonic-sorted by
-// (count desc, device id asc); ties that do not fit the tile are cut by an
-// exact radix select on the device id.
+// The compaction also sums each query's count statistics: entries per count
+// bin (8 bins per power of two), the largest count and the entropy terms.
+// Top-k: the bin holding the k-th count is read from them; when that bin and
+// the ones above fit the candidate slots, every workgroup gathers its share
+// of the candidates (counts >= the bin's lowest) and one workgroup per query
+// bitonic-sorts them in LDS by (count desc, device id asc).  Otherwise that
+// workgroup narrows the count range with LDS histograms over the query's
+// entries until the candidates fit; ties that do not fit the tile are cut by
+// an exact radix select on the device id.
 #include "../../include/ptw.h"
 #include "engine.hpp"
 #include "philox.hpp"
@@ -61,13 +64,20 @@ __device__ __forceinline__ void slice_add(const BatchLaunch &p, uint32_t q, uint
         atomicAdd(&p.dense[off + v], n);
         return;
     }
+    // a slot {key, count} is one little-endian u64 (count << 32 | key): an
+    // existing key takes one 64-bit add on the count half, an empty slot one
+    // compare-and-swap that installs the key with its first count
+    constexpr unsigned long long kEmptySlot = kEmpty;  // {kEmpty, 0}
     uint32_t h = (v * 0x9E3779B1u) & mask;
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        uint2 *slot = &p.hash[off + h];
-        uint32_t k = slot->x;
-        if (k == kEmpty) k = atomicCAS(&slot->x, kEmpty, v);
-        if (k == kEmpty || k == v) {
-            atomicAdd(&slot->y, n);
+        unsigned long long *slot = reinterpret_cast<unsigned long long *>(&p.hash[off + h]);
+        unsigned long long cur = *slot;
+        if (cur == kEmptySlot) {
+            cur = atomicCAS(slot, kEmptySlot, ((unsigned long long)n << 32) | v);
+            if (cur == kEmptySlot) return;
+        }
+        if ((uint32_t)cur == v) {
+            atomicAdd(slot, (unsigned long long)n << 32);
             return;
         }
         h = (h + 1) & mask;
@@ -448,15 +458,81 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A lane's share of a query's count statistics (TopkStats): counts of 1, 2
+// and 3 (most entries) in registers, larger counts in the workgroup's LDS
+// bins; c ln c in 2^-20 fixed point, so the sums do not depend on the order
+// of the additions.
+struct LaneStats {
+    uint32_t maxc = 0, n1 = 0, n2 = 0, n3 = 0;
+    unsigned long long sum = 0, clogc = 0;
+    __device__ __forceinline__ void add(uint32_t c, uint32_t *s_hist)
+    {
+        maxc = c > maxc ? c : maxc;
+        sum += c;
+        if (c > 1) clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
+        if (c >= 4) {
+            atomicAdd(&s_hist[count_bin(c)], 1u);
+        } else {
+            n1 += c == 1;
+            n2 += c == 2;
+            n3 += c == 3;
+        }
+    }
+};
+
+// Adds the workgroup's statistics of one task to the query's TopkStats and
+// clears the LDS bins.  Workgroup-uniform: every thread calls it.
+__device__ void flush_stats(LaneStats &ls, uint32_t *s_hist, uint32_t *s_max, unsigned long long *s_sums,
+                            TopkStats *st)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (int o = 32; o > 0; o >>= 1) {
+        ls.maxc = max(ls.maxc, (uint32_t)__shfl_xor(ls.maxc, o));
+        ls.sum += __shfl_xor(ls.sum, o);
+        ls.clogc += __shfl_xor(ls.clogc, o);
+        ls.n1 += __shfl_xor(ls.n1, o);
+        ls.n2 += __shfl_xor(ls.n2, o);
+        ls.n3 += __shfl_xor(ls.n3, o);
+    }
+    if (lane == 0) {
+        if (ls.maxc) atomicMax(s_max, ls.maxc);
+        if (ls.sum) atomicAdd(&s_sums[0], ls.sum);
+        if (ls.clogc) atomicAdd(&s_sums[1], ls.clogc);
+        if (ls.n1) atomicAdd(&s_hist[1], ls.n1);
+        if (ls.n2) atomicAdd(&s_hist[2], ls.n2);
+        if (ls.n3) atomicAdd(&s_hist[3], ls.n3);
+    }
+    ls = LaneStats{};
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x) {
+        const uint32_t h = s_hist[b];
+        if (h) {
+            atomicAdd(&st->hist[b], h);
+            s_hist[b] = 0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (*s_max) atomicMax(&st->maxc, *s_max);
+        if (s_sums[0]) atomicAdd(&st->sum, s_sums[0]);
+        if (s_sums[1]) atomicAdd(&st->clogc, s_sums[1]);
+        *s_max = 0;
+        s_sums[0] = 0;
+        s_sums[1] = 0;
+    }
+    __syncthreads();
+}
+
 constexpr uint32_t kCompactBlock = 256;
 constexpr uint32_t kCompactRound = 1024;  // pair places a wavefront round can fill
 
 // One round of a wavefront's compaction: each lane brings up to 16 pairs
 // (valid[j]); they are ranked in lane order, staged in the wavefront's LDS
 // row and written to the query's list from one reservation, as consecutive
-// 8 B stores.  Every lane of the wavefront calls it.
+// 8 B stores, and their counts are added to the lanes' statistics on the
+// way out.  Every lane of the wavefront calls it.
 __device__ __forceinline__ void compact_round(const uint2 (&e)[16], const bool (&valid)[16], uint2 *stage,
-                                              uint32_t *count, uint2 *list, uint32_t cap)
+                                              uint32_t *count, uint2 *list, uint32_t cap, LaneStats &ls,
+                                              uint32_t *s_hist)
 {
     const uint32_t lane = threadIdx.x & 63;
     uint32_t n = 0;
@@ -478,24 +554,40 @@ __device__ __forceinline__ void compact_round(const uint2 (&e)[16], const bool (
     for (int j = 0; j < 16; ++j)
         if (valid[j]) stage[at++] = e[j];
     wave_lds_sync();
-    for (uint32_t i = lane; i < total; i += 64)
-        if (base + i < cap) list[base + i] = stage[i];
+    for (uint32_t i = lane; i < total; i += 64) {
+        const uint2 x = stage[i];
+        ls.add(x.y, s_hist);
+        if (base + i < cap) list[base + i] = x;
+    }
     wave_lds_sync();  // the row is reused by the next round
 }
 
 // After the walk: the visited vertices of each query, listed as (device id,
 // count) pairs in the query's touched list, so that top-k streams the pairs
-// instead of whole slices.  One read of each slice or table: the tasks are
-// (query, segment) with the segments of a dense slice kSeg counters long and
-// those of a hash table 1/segs of it (a multiple of 16 slots).  Wavefronts
-// work independently: each takes 1024 counters (16 per lane) or 1024 slots
-// (16 per lane) per round with 16 B loads, and writes the round's pairs
-// coalesced through LDS with one reservation on the query's count.  A list
-// that would overflow marks the query for full-slice scans
-// (touched_count > capacity).
+// instead of whole slices, and the query's count statistics for top-k and
+// the entropy.  One read of each slice or table: the tasks are (query,
+// segment) with the segments of a dense slice kSeg counters long and those
+// of a hash table 1/segs of it (a multiple of 16 slots).  Wavefronts work
+// independently within a task: each takes 1024 counters (16 per lane) or
+// 1024 slots (16 per lane) per round with 16 B loads, and writes the round's
+// pairs coalesced through LDS with one reservation on the query's count; the
+// statistics are added to the query's once per task.  A list that would
+// overflow marks the query for full-slice scans (touched_count > capacity);
+// the statistics cover every entry either way.
 __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunch p)
 {
     __shared__ uint2 stage[kCompactBlock / 64][kCompactRound];
+    __shared__ uint32_t s_hist[kCountBins];
+    __shared__ uint32_t s_max;
+    __shared__ unsigned long long s_sums[2];
+    for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
+    if (threadIdx.x == 0) {
+        s_max = 0;
+        s_sums[0] = 0;
+        s_sums[1] = 0;
+    }
+    __syncthreads();
+    LaneStats ls;
     const uint32_t nq = (uint32_t)p.hdr[1];
     const uint32_t pitch = dense_pitch(p.V);
     const uint32_t segs = (pitch + kSeg - 1) / kSeg;
@@ -540,7 +632,7 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
                 }
 #pragma unroll
                 for (int j = 0; j < 16; ++j) valid[j] = e[j].y != 0;
-                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap);
+                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap, ls, s_hist);
             }
         } else {
             // tables hold a power of two >= 2 slots and start at an even
@@ -567,9 +659,10 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
                 }
 #pragma unroll
                 for (int j = 0; j < 16; ++j) valid[j] = e[j].x != kEmpty && e[j].y != 0;
-                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap);
+                compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap, ls, s_hist);
             }
         }
+        flush_stats(ls, s_hist, &s_max, s_sums, p.tk_stats + q);
     }
 }
 
@@ -725,71 +818,6 @@ __device__ void for_topk_segments(const BatchLaunch &p, F &&fn)
         carry += total;
         __syncthreads();  // the round's shared state is reused by the next
     }
-}
-
-// Count statistics of every query (TopkStats, zeroed by batch_begin).
-__global__ __launch_bounds__(kTopkBlock) void batch_topk_stats_kernel(BatchLaunch p)
-{
-    __shared__ uint32_t s_hist[kCountBins];
-    __shared__ uint32_t s_max;
-    __shared__ unsigned long long s_sum, s_clogc;
-    const uint32_t lane = threadIdx.x & 63;
-    for_topk_segments(p, [&](uint32_t q, const Entries &E, uint32_t sg) {
-        for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
-        if (threadIdx.x == 0) {
-            s_max = 0;
-            s_sum = 0;
-            s_clogc = 0;
-        }
-        __syncthreads();
-        const uint64_t lo = (uint64_t)sg * kTopkSeg;
-        const uint64_t hi = lo + kTopkSeg < E.n ? lo + kTopkSeg : (uint64_t)E.n;
-        // counts of 1, 2 and 3 (most entries) are counted in registers; c ln c
-        // is summed in 2^-20 fixed point so the sum does not depend on the
-        // order of the additions
-        uint32_t local_max = 0, n1 = 0, n2 = 0, n3 = 0;
-        unsigned long long local_sum = 0, local_clogc = 0;
-        for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-            uint32_t v, c;
-            E.get((uint32_t)i, v, c);
-            local_max = c > local_max ? c : local_max;
-            local_sum += c;
-            if (c > 1) local_clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
-            if (c >= 4) {
-                atomicAdd(&s_hist[count_bin(c)], 1u);
-            } else {
-                n1 += c == 1;
-                n2 += c == 2;
-                n3 += c == 3;
-            }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            local_max = max(local_max, (uint32_t)__shfl_xor(local_max, o));
-            local_sum += __shfl_xor(local_sum, o);
-            local_clogc += __shfl_xor(local_clogc, o);
-            n1 += __shfl_xor(n1, o);
-            n2 += __shfl_xor(n2, o);
-            n3 += __shfl_xor(n3, o);
-        }
-        if (lane == 0) {
-            if (local_max) atomicMax(&s_max, local_max);
-            if (local_sum) atomicAdd(&s_sum, local_sum);
-            if (local_clogc) atomicAdd(&s_clogc, local_clogc);
-            if (n1) atomicAdd(&s_hist[1], n1);
-            if (n2) atomicAdd(&s_hist[2], n2);
-            if (n3) atomicAdd(&s_hist[3], n3);
-        }
-        __syncthreads();
-        TopkStats *st = p.tk_stats + q;
-        for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x)
-            if (s_hist[b]) atomicAdd(&st->hist[b], s_hist[b]);
-        if (threadIdx.x == 0) {
-            if (s_max) atomicMax(&st->maxc, s_max);
-            if (s_sum) atomicAdd(&st->sum, s_sum);
-            if (s_clogc) atomicAdd(&st->clogc, s_clogc);
-        }
-        __syncthreads();
-    });
 }
 
 // The candidates of every query whose entries in and above the k-th count's
@@ -1176,11 +1204,8 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int l
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v, unsigned long long *out_c,
                              int tile, int num_cus, hipStream_t s)
 {
-    hipLaunchKernelGGL(batch_topk_stats_kernel, dim3(num_cus * 4), dim3(kTopkBlock), 0, s, p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_topk_gather_kernel, dim3(num_cus * 4), dim3(kTopkBlock), 0, s, p);
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_topk_kernel, dim3(num_cus * 2), dim3(1024), (size_t)tile * 8, s, p, inv, out_v, out_c,
                        (uint32_t)tile);

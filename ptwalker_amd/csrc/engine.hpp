@@ -253,14 +253,15 @@ struct BatchLaunch {
 hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStream_t s);
 // Dense queries (workgroup chunks, LDS hub counters + cache of `lds_hubs` /
 // `cache_slots`), then hash queries (wavefront chunks), then the compaction
-// of dense slices.
+// of every query's slice into (vertex, count) pairs and count statistics.
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
                              hipStream_t s);
 // Per-query entropy and top-k (count desc, device id asc) into out_v (graph
-// ids) / out_c: count statistics and the candidate gather run over segments
-// of every query's entries on the whole grid; one workgroup per query sorts
-// the candidates in a `tile`-slot LDS tile (or narrows the count range over
-// the query's entries when they do not fit).
+// ids) / out_c, from the count statistics the compaction summed: the
+// candidate gather runs over segments of every query's entries on the whole
+// grid; one workgroup per query sorts the candidates in a `tile`-slot LDS
+// tile (or narrows the count range over the query's entries when they do
+// not fit).
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v,
                              unsigned long long *out_c, int tile, int num_cus, hipStream_t s);
 // Zero what a batch left in its slices (dense: the touched entries or the whole
