@@ -168,6 +168,30 @@ def test_pair_list_overflow_reads_slices(ctx, g12, dense_threshold):
     assert overflowed > 0
 
 
+def test_long_walks_beyond_the_planned_visits(ctx):
+    # one walker per query at tp = 0.001 takes ~1000 steps; a hash query is
+    # planned for 2 x 1001 + 64 visits, and the longest walks here make more
+    # than twice that many (plus 256): the table, sized at twice the planned
+    # visits, still holds them and the counts stay exact.
+    gi = ctx.rmat(scale=14, edge_factor=8, seed=4)
+    rp, col = ctx.export_csr()
+    n = 128
+    rng = np.random.default_rng(8)
+    src = rng.integers(0, gi.num_vertices, n)
+    W = np.ones(n, dtype=np.int64)
+    tp = np.full(n, 0.001)
+    r = ctx.batch_ppr(src, W, tp, topk=10, seed=5)
+    assert r.hash_queries == n
+    planned = 2 * 1001 + 64 + 256
+    past = 0
+    for i in range(n):
+        ref = replay_query(rp, col, int(src[i]), 1, 0.001, i, seed=5)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+        assert r.steps[i] == ref["steps"]
+        past += ref["steps"] + 1 > 2 * planned
+    assert past >= 2
+
+
 def test_batch_small_graph_dangling_and_edges(ctx):
     rp, col = random_graph(30, 0.15, seed=3, dangling=3)
     ctx.from_csr(rp, col)
