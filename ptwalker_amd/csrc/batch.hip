@@ -51,19 +51,41 @@ constexpr uint32_t kSeg = 65536;  // dense-slice compaction segment
 // 16 B vector for streaming stores that bypass the caches' reuse
 typedef uint32_t vec4u __attribute__((ext_vector_type(4)));
 
+// Where a query's counts go: its dense slice, or its hub slice and table.
+struct SliceRef {
+    uint32_t kind = 0, mask = 0, hub_len = 0;
+    uint64_t off = 0, hub_off = 0;
+};
+__device__ __forceinline__ SliceRef slice_ref(const BatchQuery &bq)
+{
+    SliceRef r;
+    r.kind = bq.kind;
+    r.mask = bq.mask;
+    r.hub_len = bq.hub_len;
+    r.off = bq.slice_off;
+    r.hub_off = bq.hub_off;
+    return r;
+}
+
 struct LaneQuery {
     uint32_t q = kEmpty;
-    uint32_t src = 0, thr = 0, kind = 0, mask = 0, stream = 0;
-    uint64_t off = 0;
+    uint32_t src = 0, thr = 0, stream = 0;
+    SliceRef r;
 };
 
-__device__ __forceinline__ void slice_add(const BatchLaunch &p, uint32_t q, uint32_t kind, uint64_t off,
-                                          uint32_t mask, uint32_t v, uint32_t n)
+__device__ __forceinline__ void slice_add(const BatchLaunch &p, uint32_t q, const SliceRef &r, uint32_t v,
+                                          uint32_t n)
 {
-    if (kind == SLICE_DENSE) {
-        atomicAdd(&p.dense[off + v], n);
+    if (r.kind == SLICE_DENSE) {
+        atomicAdd(&p.dense[r.off + v], n);
         return;
     }
+    if (v < r.hub_len) {
+        atomicAdd(&p.dense[r.hub_off + v], n);
+        return;
+    }
+    const uint64_t off = r.off;
+    const uint32_t mask = r.mask;
     // a slot {key, count} is one little-endian u64 (count << 32 | key): an
     // existing key takes one 64-bit add on the count half, an empty slot one
     // compare-and-swap that installs the key with its first count
@@ -116,8 +138,7 @@ struct WaveCache {
             const unsigned long long k = keys[s];
             if (k != kEmpty64) {
                 const uint32_t q = (uint32_t)(k >> 32);
-                const BatchQuery bq = p.queries[q];
-                slice_add(p, q, bq.kind, bq.slice_off, bq.mask, (uint32_t)k, cnt[s]);
+                slice_add(p, q, slice_ref(p.queries[q]), (uint32_t)k, cnt[s]);
                 keys[s] = kEmpty64;
                 cnt[s] = 0;
             }
@@ -176,17 +197,17 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     // hub counters belong to cq, the query of the wavefront's current chunk
     auto flush_hubs = [&]() {
         if (cq == kEmpty) return;
-        const BatchQuery bq = p.queries[cq];
+        const SliceRef r = slice_ref(p.queries[cq]);
         for (uint32_t s = lane; s < kWaveHubs; s += 64) {
             if (hub[s]) {
-                slice_add(p, cq, bq.kind, bq.slice_off, bq.mask, s, hub[s]);
+                slice_add(p, cq, r, s, hub[s]);
                 hub[s] = 0;
             }
         }
     };
     auto flush = [&]() {
         if (Q.q == kEmpty) return;
-        if (my_src) slice_add(p, Q.q, Q.kind, Q.off, Q.mask, Q.src, my_src);
+        if (my_src) slice_add(p, Q.q, Q.r, Q.src, my_src);
         if (my_steps) atomicAdd(&p.steps[Q.q], (unsigned long long)my_steps);
         my_src = 0;
         my_steps = 0;
@@ -223,10 +244,8 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                     Q.q = cq;
                     Q.src = bq.source_dev;
                     Q.thr = bq.thr;
-                    Q.kind = bq.kind;
-                    Q.mask = bq.mask;
                     Q.stream = bq.stream;
-                    Q.off = bq.slice_off;
+                    Q.r = slice_ref(bq);
                 }
                 w = c_next + rank;
                 v = Q.src;
@@ -260,8 +279,10 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                         ++my_src;
                     else if (v < kWaveHubs && Q.q == cq)
                         atomicAdd(&hub[v], 1u);
+                    else if (v < Q.r.hub_len)
+                        atomicAdd(&p.dense[Q.r.hub_off + v], 1u);  // no reply to wait for
                     else if (!C.add(Q.q, v))
-                        slice_add(p, Q.q, Q.kind, Q.off, Q.mask, v, 1u);
+                        slice_add(p, Q.q, Q.r, v, 1u);
                 }
             }
             if (stop) {
@@ -601,10 +622,11 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
         uint2 *list = p.touched + bq.touched_off;
         uint2 e[16];
         bool valid[16];
-        if (bq.kind == SLICE_DENSE) {
-            const uint32_t lo = seg * kSeg;
-            const uint32_t hi = lo + kSeg < pitch ? lo + kSeg : pitch;  // a multiple of 64
-            const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + bq.slice_off);
+        // counters [lo, hi) of a dense range (a dense slice, or a hash
+        // query's hub slice) starting at `base` in the dense pool; lo and hi
+        // are multiples of 64
+        auto compact_dense = [&](uint64_t base_off, uint32_t lo, uint32_t hi) {
+            const uint4 *slice4 = reinterpret_cast<const uint4 *>(p.dense + base_off);
             const uint32_t step = waves * 1024;
             // the next round's 64 B per lane is loaded before this round is
             // placed, so two rounds of loads are in flight per wavefront
@@ -634,13 +656,20 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
                 for (int j = 0; j < 16; ++j) valid[j] = e[j].y != 0;
                 compact_round(e, valid, row, &p.touched_count[q], list, bq.touched_cap, ls, s_hist);
             }
+        };
+        if (bq.kind == SLICE_DENSE) {
+            const uint32_t lo = seg * kSeg;
+            compact_dense(bq.slice_off, lo, lo + kSeg < pitch ? lo + kSeg : pitch);
         } else {
+            if ((uint64_t)seg * kSeg < bq.hub_len) {
+                const uint32_t lo = seg * kSeg;
+                compact_dense(bq.hub_off, lo, lo + kSeg < bq.hub_len ? lo + kSeg : bq.hub_len);
+            }
             // tables hold a power of two >= 2 slots and start at an even
             // slot, so 16 B loads of slot pairs are aligned
             const uint64_t cap = (uint64_t)bq.mask + 1;
             const uint64_t span = ((cap + segs - 1) / segs + 15) & ~15ull;
             const uint64_t lo = (uint64_t)seg * span;
-            if (lo >= cap) continue;  // block-uniform
             const uint64_t hi = lo + span < cap ? lo + span : cap;
             const uint2 *tab = p.hash + bq.slice_off;
             for (uint64_t base = lo + wave * 1024; base < hi; base += waves * 1024) {  // wave-uniform
@@ -671,7 +700,7 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
 struct Entries {
     const BatchLaunch *p;
     BatchQuery Q;
-    uint32_t mode;  // 0 pair list, 1 dense full slice, 2 hash full table
+    uint32_t mode;  // 0 pair list, 1 dense full slice, 2 hash hub slice then full table
     uint32_t n;
     uint32_t V;
     __device__ __forceinline__ void get(uint32_t i, uint32_t &v, uint32_t &c) const
@@ -683,8 +712,11 @@ struct Entries {
         } else if (mode == 1) {
             v = i;
             c = p->dense[Q.slice_off + i];
+        } else if (i < Q.hub_len) {
+            v = i;
+            c = p->dense[Q.hub_off + i];
         } else {
-            const uint2 s = p->hash[Q.slice_off + i];
+            const uint2 s = p->hash[Q.slice_off + (i - Q.hub_len)];
             v = s.x;
             c = s.x == kEmpty ? 0u : s.y;
         }
@@ -708,7 +740,7 @@ __device__ __forceinline__ Entries entries_of(const BatchLaunch &p, uint32_t q)
         E.n = p.V;
     } else {
         E.mode = 2;
-        E.n = E.Q.mask + 1;
+        E.n = E.Q.hub_len + E.Q.mask + 1;
     }
     return E;
 }
@@ -1107,6 +1139,14 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(BatchLaunch p)
                     __builtin_nontemporal_store(z, &d4[i]);
             }
         } else {
+            if ((uint64_t)seg * kSeg < Q.hub_len) {  // the hub slice, streamed
+                const uint32_t lo = seg * kSeg;
+                const uint32_t hi = lo + kSeg < Q.hub_len ? lo + kSeg : Q.hub_len;  // multiples of 64
+                vec4u *d4 = reinterpret_cast<vec4u *>(p.dense + Q.hub_off);
+                const vec4u z = {0u, 0u, 0u, 0u};
+                for (uint32_t i = (lo >> 2) + threadIdx.x; i < (hi >> 2); i += blockDim.x)
+                    __builtin_nontemporal_store(z, &d4[i]);
+            }
             // tables hold a power of two >= 2 slots from an even slot: 16 B
             // stores of slot pairs are aligned
             const uint64_t cap = (uint64_t)Q.mask + 1;
@@ -1132,6 +1172,8 @@ __global__ void batch_gather_kernel(BatchLaunch p, uint64_t n, const uint32_t *q
         uint64_t c = 0;
         if (Q.kind == SLICE_DENSE) {
             c = p.dense[Q.slice_off + v];
+        } else if (v < Q.hub_len) {
+            c = p.dense[Q.hub_off + v];
         } else {
             uint32_t h = (v * 0x9E3779B1u) & Q.mask;
             for (uint32_t probe = 0; probe <= Q.mask; ++probe) {
@@ -1163,6 +1205,10 @@ __global__ void batch_export_kernel(BatchLaunch p, uint32_t q, const uint32_t *p
         for (uint64_t i = t0; i <= Q.mask; i += stride) {
             const uint2 s = p.hash[Q.slice_off + i];
             if (s.x != kEmpty) out[inv ? inv[s.x] : s.x] = s.y;
+        }
+        for (uint64_t i = t0; i < Q.hub_len && i < p.V; i += stride) {
+            const uint32_t c = p.dense[Q.hub_off + i];
+            if (c) out[inv ? inv[i] : i] = c;
         }
     }
 }

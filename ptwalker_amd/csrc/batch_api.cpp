@@ -92,6 +92,10 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
             B.mask = (uint32_t)(z.slots - 1);
             ++P.n_hash;
         }
+        // a multiple of 64 counters, like every dense slice: 256 B aligned
+        B.hub_len = z.hub_len;
+        B.hub_off = P.dense;
+        P.dense += z.hub_len;
         B.touched_cap = z.touched_cap;
         B.touched_off = P.touched;
         P.touched += z.touched_cap;
@@ -101,7 +105,6 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
                             : 0;
         B.cand_off = P.cand;
         P.cand += B.cand_cap;
-        B.pad = 0;
     }
     for (uint32_t i = 0; i < d.num_queries; ++i) {
         if (P.q[i].kind != SLICE_DENSE) continue;
@@ -419,7 +422,7 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
             bytes += 4ull * ptw::dense_pitch(ctx->dev.V);
         } else {
             ++out->hash_queries;
-            bytes += 8ull * ((uint64_t)q.mask + 1);
+            bytes += 8ull * ((uint64_t)q.mask + 1) + 4ull * q.hub_len;
         }
         bytes += 8ull * q.touched_cap;
     }

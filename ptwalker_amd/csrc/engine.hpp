@@ -136,10 +136,16 @@ enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1 };
 // so every slice starts on a 256 B boundary and can be read with 16 B loads.
 __host__ __device__ inline uint32_t dense_pitch(uint32_t V) { return (V + 63u) & ~63u; }
 
+// Device ids below this (the highest-degree vertices after the relabel) of a
+// hash query are counted in a dense "hub slice" of the dense pool (1 MiB),
+// so most of its visits are plain atomic adds; only the rest probe its table.
+constexpr uint32_t kHashHubLen = 262144;
+
 // The sizes one batched query is planned with.  Visits are bounded by twice
 // walkers * (1 + 1/tp) plus 64 for the table ("upper").  A query above the
-// dense threshold gets a dense slice; one below it an open-addressing table
-// of the power of two >= 2 * upper slots (at most that of 2V).  Either one
+// dense threshold gets a dense slice; one below it a hub slice of the first
+// kHashHubLen device ids and an open-addressing table of the power of two
+// >= 2 * upper slots (at most that of 2V) for the rest.  Either one
 // gets a list of (vertex, count) pairs for the compaction after the walk,
 // sized for 1.25x the expected visits (distinct vertices never exceed
 // visits); a query whose list would overflow is read from its slice instead.
@@ -147,7 +153,8 @@ struct SliceSizes {
     bool dense;
     uint64_t slots;        // dense: u32 counters (the pitch); hash: uint2 slots
     uint32_t touched_cap;  // (vertex, count) pairs
-    uint64_t bytes() const { return (dense ? 4 * slots : 8 * slots) + 8ull * touched_cap; }
+    uint32_t hub_len;      // hash: u32 counters of the hub slice
+    uint64_t bytes() const { return (dense ? 4 * slots : 8 * slots + 4ull * hub_len) + 8ull * touched_cap; }
 };
 inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uint64_t dense_threshold)
 {
@@ -163,7 +170,8 @@ inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uin
         while (cap < 2 * upper && cap < 2ull * V) cap <<= 1;
         z.slots = cap;
     }
-    z.touched_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(V, expect), z.dense ? V : z.slots);
+    z.hub_len = z.dense ? 0 : std::min<uint32_t>(dense_pitch(V), kHashHubLen);
+    z.touched_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(V, expect), z.dense ? V : z.slots + z.hub_len);
     return z;
 }
 
@@ -180,8 +188,10 @@ struct BatchQuery {
     uint32_t stream;       // 4th Philox counter word
     uint64_t cand_off;     // first top-k candidate slot in the candidate pool
     uint32_t cand_cap;     // candidate slots (0 without top-k)
-    uint32_t pad;
+    uint32_t hub_len;      // hash: device ids below this count in a dense hub slice
+    uint64_t hub_off;      // hash: u32 offset of the hub slice in the dense pool
 };
+
 
 // Count bins of the top-k statistics: counts below 16 have a bin each; from
 // 16 on, every power of two is cut into 8 bins by the 3 bits after the

@@ -220,6 +220,37 @@ def test_batch_invalid(ctx):
     assert ctx.batch_plan_bytes([1, 2], [10, 10 ** 6], [0.2, 0.2]) >= 0
 
 
+def test_hash_queries_hub_slice_and_table_scale20(ctx):
+    # above 262144 vertices a hash query counts its top-degree device ids in a
+    # dense hub slice and the rest in its table: both parts bit-exact against
+    # the oracle, with sources of every degree and a mix of dense queries
+    gi = ctx.rmat(scale=20, edge_factor=16, seed=3)
+    assert gi.num_vertices > 262144
+    rp, col = ctx.export_csr()
+    rng = np.random.default_rng(12)
+    deg = np.diff(rp.astype(np.int64))
+    order = np.argsort(-deg, kind="stable")
+    src = np.concatenate([order[:4], order[1000:1004], order[-4:], rng.integers(0, gi.num_vertices, 12)])
+    n = len(src)
+    W = np.exp(rng.uniform(np.log(200), np.log(20000), n)).astype(np.int64)
+    W[:2] = 150000  # dense
+    tp = rng.uniform(0.15, 0.3, n)
+    r = ctx.batch_ppr(src, W, tp, topk=50, seed=7)
+    assert r.dense_queries == 2 and r.hash_queries == n - 2
+    table_hits = 0
+    for i in range(n):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=7)
+        got = ctx.batch_counts(i)
+        assert np.array_equal(got, ref["counts"]), i
+        v, c = oracle.topk(ref["counts"], rp, 50)
+        assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], c), i
+        # graph ids of degree rank >= 262144 land in the table
+        table_hits += int(ref["counts"][order[262144:]].sum())
+    assert table_hits > 0
+    g = ctx.batch_gather(np.arange(n), src)
+    assert all(int(g[i]) == int(ctx.batch_counts(i)[src[i]]) for i in range(n))
+
+
 def test_batch_scale20_properties(ctx):
     gi = ctx.rmat(scale=20, edge_factor=16, seed=1)
     rp, _ = ctx.export_csr()
