@@ -267,11 +267,41 @@ def bench_deepwalk(args):
                      "achieved": 24.0 * steps / (wt_ms * 1e-3) / 1e9},
         "random_gather": {"gathers_per_s": gps, "frac_unweighted": 2.0 * steps / (uw_ms * 1e-3) / gps,
                           "frac_weighted": 3.0 * steps / (wt_ms * 1e-3) / gps},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_deepwalk(ctx, L, args),
     }
     print(json.dumps(line), flush=True)
     paths.free()
     ctx.close()
+
+
+def cpu_cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline_deepwalk(ctx, L, args):
+    """The reference process (oracle: xorshift*, OpenMP over walkers) walking
+    unweighted L-vertex walks from uniform starts on the same graph, on a
+    bounded walker sample.  It stores no paths, so it bounds the CPU rate
+    from above."""
+    import oracle
+
+    rp, col = ctx.export_csr()
+    cores = cpu_cores()
+    probe = 20_000
+    t = time.perf_counter()
+    r = oracle.ref_walk(rp, col, oracle.APP_DEEPWALK, probe, walk_len=L, seed=1, threads=cores)
+    rate = r["steps"] / max(time.perf_counter() - t, 1e-9)
+    n = int(min(2e8, max(probe, rate * args.cpu_seconds / max(L - 1, 1))))
+    t = time.perf_counter()
+    r = oracle.ref_walk(rp, col, oracle.APP_DEEPWALK, n, walk_len=L, seed=1, threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": r["steps"] / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} unweighted walks of {L} vertices from uniform starts ({r['steps']} steps, {dt:.1f} s), "
+                      f"xorshift* reference CPU path without path stores, {cores} threads"}
 
 
 def batch_queries(gi, rp, n, seed=1):
@@ -356,15 +386,39 @@ def bench_batch(args):
                     "kernel_ms": untuned["graph"][1], "shape": ctx_default_shape},
         # algorithmic bytes of the batched walk: 12 B gather + 4 B u32 slice add
         # per step, over the whole captured batch (walks, compaction, top-k, reset)
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_batch(ctx, src, W, tp, args),
         "roofline": {"bound": "hbm", "kernel": "whole batch (batch_dense_kernel, batch_walk_kernel, compaction, "
                                               "top-k, reset)",
                      "achieved": 16.0 * r.total_steps / (kms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": 16.0 * r.total_steps / (kms * 1e-3) / 1e9 / 8000.0, "traffic": None,
                      "bytes_per_step": 16},
-        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def cpu_baseline_batch(ctx, src, W, tp, args):
+    """The reference process runs one PPR query at a time (oracle: xorshift*,
+    OpenMP over the query's walkers): the batch's queries in order until the
+    sample reaches about cpu_seconds."""
+    import oracle
+
+    rp, col = ctx.export_csr()
+    cores = cpu_cores()
+    steps = done = 0
+    t = time.perf_counter()
+    for i in range(len(src)):
+        r = oracle.ref_walk(rp, col, oracle.APP_PPR, int(W[i]), tp=float(tp[i]), source=int(src[i]),
+                            start_uniform=False, seed=1, threads=cores)
+        steps += r["steps"]
+        done += 1
+        if time.perf_counter() - t > args.cpu_seconds:
+            break
+    dt = time.perf_counter() - t
+    return {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+            "queries_per_s": done / dt,
+            "sample": f"the first {done} of the batch's queries one after another ({steps} steps, {dt:.1f} s), "
+                      f"xorshift* reference-semantics CPU path, {cores} threads"}
 
 
 def serve_requests(rp, n, rate, seed=1, eps_choices=(1e-3, 3e-4)):
