@@ -47,6 +47,15 @@ namespace {
 constexpr uint64_t kTile = 65536;  // bytes per newline-count tile
 constexpr int kTileThreads = 256;  // each thread scans kTile / kTileThreads = 256 bytes
 
+// High bit of each byte of w that is a newline (exact, carry-free: see
+// newlines_in).
+__device__ __forceinline__ uint32_t newline_mask(uint32_t w)
+{
+    const uint32_t y = w ^ 0x0A0A0A0Au;
+    const uint32_t t = (y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return ~(t | y | 0x7F7F7F7Fu);
+}
+
 __device__ __forceinline__ uint32_t newlines_in(uint4 x)
 {
     uint32_t n = 0;
@@ -88,11 +97,17 @@ __global__ __launch_bounds__(kTileThreads) void newline_positions_kernel(const u
                                                                         const uint64_t *tile_base, uint64_t *pos)
 {
     __shared__ uint32_t s_count[kTileThreads];
+    constexpr uint32_t kWords = kTile / 16 / kTileThreads;  // 16 B words per thread
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint64_t first = t * kTile + (uint64_t)threadIdx.x * (kTile / kTileThreads);
         const uint4 *p = data + first / 16;
+        uint4 w[kWords];  // the thread's bytes stay in registers for the second pass
         uint32_t n = 0;
-        for (uint32_t k = 0; k < kTile / 16 / kTileThreads; ++k) n += newlines_in(p[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < kWords; ++k) {
+            w[k] = p[k];
+            n += newlines_in(w[k]);
+        }
         s_count[threadIdx.x] = n;
         __syncthreads();
         for (uint32_t off = 1; off < kTileThreads; off <<= 1) {  // inclusive scan
@@ -102,11 +117,17 @@ __global__ __launch_bounds__(kTileThreads) void newline_positions_kernel(const u
             __syncthreads();
         }
         uint64_t out = tile_base[t] + s_count[threadIdx.x] - n;
-        const unsigned char *b = reinterpret_cast<const unsigned char *>(p);
-        for (uint32_t k = 0; k < kTile / kTileThreads && n; ++k) {
-            if (b[k] == '\n') {
-                pos[out++] = first + k;
-                --n;
+#pragma unroll
+        for (uint32_t k = 0; k < kWords; ++k) {
+            const uint32_t c[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                uint32_t m = newline_mask(c[j]);
+                while (m) {  // byte b of the word is a newline when bit 8b + 7 is set
+                    const uint32_t b = (uint32_t)(__ffs(m) - 1) >> 3;
+                    pos[out++] = first + 16 * k + 4 * j + b;
+                    m &= m - 1;
+                }
             }
         }
         __syncthreads();
@@ -206,20 +227,31 @@ __global__ void scatter_ids_kernel(const uint32_t *head, const uint32_t *run_exc
         mapped[svals[k]] = new_id[run_excl[k] + head[k] - 1];
 }
 
-__global__ void edges_kernel(const uint32_t *mapped, uint64_t m, int undirected, uint32_t *esrc, uint32_t *edst,
-                             uint32_t *deg)
+__global__ void edges_kernel(const uint32_t *mapped, uint64_t m, int undirected, uint32_t *esrc, uint32_t *edst)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         const uint32_t s = mapped[2 * i], d = mapped[2 * i + 1];
         esrc[i] = s;
         edst[i] = d;
-        atomicAdd(&deg[s], 1u);
         if (undirected) {
             esrc[m + i] = d;
             edst[m + i] = s;
-            atomicAdd(&deg[d], 1u);
         }
+    }
+}
+
+// Row offsets from the sources sorted ascending: row[v] is the first index
+// whose source is >= v, row[V] = E.  Index i writes row[v] for the vertices
+// v in (ssrc[i - 1], ssrc[i]] (from 0 at i = 0, up to V at i = E), so every
+// vertex is written once, those without out-edges included.
+__global__ void row_offsets_kernel(const uint32_t *ssrc, uint64_t E, uint32_t V, uint32_t *row)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= E; i += stride) {
+        const uint64_t lo = i == 0 ? 0 : (uint64_t)ssrc[i - 1] + 1;
+        const uint64_t hi = i == E ? V : ssrc[i];
+        for (uint64_t v = lo; v <= hi; ++v) row[v] = (uint32_t)i;
     }
 }
 
@@ -478,19 +510,18 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     ING_TRY(hipGetLastError());
     // ---- 4. CSR ----
     const uint64_t E = total;
-    uint32_t *esrc = nullptr, *edst = nullptr, *ssrc = nullptr, *col = nullptr, *deg = nullptr, *row = nullptr;
+    uint32_t *esrc = nullptr, *edst = nullptr, *ssrc = nullptr, *col = nullptr, *row = nullptr;
     ING_TRY(B.get(&esrc, E));
     ING_TRY(B.get(&edst, E));
     ING_TRY(B.get(&ssrc, E));
     ING_TRY(B.get(&col, E));
-    ING_TRY(B.get(&deg, (uint64_t)V + 1));
     ING_TRY(B.get(&row, (uint64_t)V + 1));
-    ING_TRY(hipMemsetAsync(deg, 0, ((uint64_t)V + 1) * 4, s));
     hipLaunchKernelGGL(edges_kernel, dim3(grid_for(m)), dim3(256), 0, s, mapped, m, as_undirected ? 1 : 0, esrc,
-                       edst, deg);
+                       edst);
     ING_TRY(hipGetLastError());
     if ((rc = sort_pairs(B, esrc, ssrc, edst, col, E, bits_for(V - 1), s, err))) return rc;
-    if ((rc = exclusive_sum(B, deg, row, (uint64_t)V + 1, s, err))) return rc;
+    hipLaunchKernelGGL(row_offsets_kernel, dim3(grid_for(E + 1)), dim3(256), 0, s, ssrc, E, V, row);
+    ING_TRY(hipGetLastError());
     g.V = V;
     g.row_ptr.resize((size_t)V + 1);
     g.col.resize(E);
