@@ -20,7 +20,7 @@ struct BatchState {
     uint64_t dense_cap = 0;  // u32 counters
     uint2 *hash = nullptr;
     uint64_t hash_cap = 0;  // slots
-    uint2 *touched = nullptr;  // (device id, count) of each dense slice's nonzero entries
+    uint2 *touched = nullptr;  // (device id, count) of each query's visited vertices
     uint64_t touched_cap = 0;
     BatchQuery *queries = nullptr;
     uint32_t *touched_count = nullptr, *overflow = nullptr;

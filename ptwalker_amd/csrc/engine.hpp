@@ -248,8 +248,8 @@ struct BatchLaunch {
     // come first in the chunk table; the wavefront chunks of hash queries follow)
     const unsigned long long *hdr;
     uint32_t *dense;         // dense slice pool (u32 counters, device ids)
-    uint2 *hash;             // hash slice pool {key, count}
-    uint2 *touched;          // per dense query: (device id, count) of its nonzero entries
+    uint2 *hash;             // hash table pool {key, count}
+    uint2 *touched;          // per query: (device id, count) of its nonzero entries
     uint32_t *touched_count; // per query
     uint32_t *overflow;      // per query: hash insertions that found no slot
     unsigned long long *steps;  // per query
@@ -274,8 +274,9 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int l
 // not fit).
 hipError_t launch_batch_topk(const BatchLaunch &p, const uint32_t *inv, uint32_t *out_v,
                              unsigned long long *out_c, int tile, int num_cus, hipStream_t s);
-// Zero what a batch left in its slices (dense: the touched entries or the whole
-// slice on list overflow; hash: every slot), so the pools stay clean.
+// Zero what a batch left in its slices, so the pools stay clean: a sparse
+// dense slice entry by entry from its pair list, otherwise (and hash queries'
+// hub slices and tables) by streaming zeros over every segment.
 hipError_t launch_batch_reset(const BatchLaunch &p, int num_cus, hipStream_t s);
 // out[i] = count of (graph id) vertices[i] in query qidx[i].
 hipError_t launch_batch_gather(const BatchLaunch &p, uint64_t n, const uint32_t *qidx, const uint32_t *vertices,
