@@ -263,10 +263,11 @@ def bench_deepwalk(args):
                      "achieved": 16.0 * steps / (uw_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": 16.0 * steps / (uw_ms * 1e-3) / 1e9 / 8000.0, "traffic": None, "bytes_per_step": 16,
                      "kernel_ms": uw_ms, "steps_per_launch": steps},
-        "weighted": {"kernel_ms": wt_ms, "bytes_per_step": 24,
-                     "achieved": 24.0 * steps / (wt_ms * 1e-3) / 1e9},
+        # weighted: row_ptr (8 B) + one 16 B weighted-edge record + 4 B path store
+        "weighted": {"kernel_ms": wt_ms, "bytes_per_step": 28,
+                     "achieved": 28.0 * steps / (wt_ms * 1e-3) / 1e9},
         "random_gather": {"gathers_per_s": gps, "frac_unweighted": 2.0 * steps / (uw_ms * 1e-3) / gps,
-                          "frac_weighted": 3.0 * steps / (wt_ms * 1e-3) / gps},
+                          "frac_weighted": 2.0 * steps / (wt_ms * 1e-3) / gps},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_deepwalk(ctx, L, args),
     }
     print(json.dumps(line), flush=True)

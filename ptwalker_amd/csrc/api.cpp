@@ -29,7 +29,7 @@ void free_graph(ptw_ctx *ctx)
     if (d.col) (void)hipFree(d.col);
     if (d.perm) (void)hipFree(d.perm);
     if (d.inv) (void)hipFree(d.inv);
-    if (d.alias) (void)hipFree(d.alias);
+    if (d.wedge) (void)hipFree(d.wedge);
     if (d.t_col) (void)hipFree(d.t_col);
     if (d.scol) (void)hipFree(d.scol);
     if (d.out_deg) (void)hipFree(d.out_deg);
@@ -52,7 +52,9 @@ void free_graph(ptw_ctx *ctx)
     ctx->alias_idx.clear();
 }
 
-// Builds alias tables from ctx->weights and uploads them in device order.
+// Builds alias tables from ctx->weights and uploads them in device order as
+// weighted-edge records {neighbour, threshold, alias slot's neighbour, 0}
+// (device ids), so a weighted pick is one 16 B read.
 int install_weights(ptw_ctx *ctx)
 {
     int rc = ptw::build_alias_tables(ctx->host, ctx->weights, ctx->alias_thr, ctx->alias_idx, ctx->err);
@@ -60,25 +62,25 @@ int install_weights(ptw_ctx *ctx)
     if (ctx->device < 0) return PTW_OK;
     auto &d = ctx->dev;
     const HostCsr &h = ctx->host;
-    std::vector<uint2> tab(h.E());
-    if (d.relabelled) {
-        ptw::parallel_for(h.V, [&](uint64_t lo, uint64_t hi) {
-            for (uint64_t dv = lo; dv < hi; ++dv) {
-                const uint32_t u = ctx->h_inv[dv];
-                const uint32_t src = h.row_ptr[u], n = h.row_ptr[u + 1] - src, dst = ctx->h_drow[dv];
-                for (uint32_t k = 0; k < n; ++k) tab[dst + k] = uint2{ctx->alias_thr[src + k], ctx->alias_idx[src + k]};
-            }
-        });
-    } else {
-        ptw::parallel_for(h.E(), [&](uint64_t lo, uint64_t hi) {
-            for (uint64_t e = lo; e < hi; ++e) tab[e] = uint2{ctx->alias_thr[e], ctx->alias_idx[e]};
-        });
-    }
+    std::vector<uint4> tab(h.E());
+    const bool rel = d.relabelled;
+    ptw::parallel_for(h.V, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t dv = lo; dv < hi; ++dv) {
+            const uint32_t u = rel ? ctx->h_inv[dv] : (uint32_t)dv;
+            const uint32_t src = h.row_ptr[u], n = h.row_ptr[u + 1] - src, dst = rel ? ctx->h_drow[dv] : src;
+            auto nbr = [&](uint32_t k) { return rel ? ctx->h_perm[h.col[src + k]] : h.col[src + k]; };
+            for (uint32_t k = 0; k < n; ++k)
+                tab[dst + k] = uint4{nbr(k), ctx->alias_thr[src + k], nbr(ctx->alias_idx[src + k]), 0u};
+        }
+    });
     ptw::drop_batch_graph(ctx);
-    if (d.alias) (void)hipFree(d.alias);
-    d.alias = nullptr;
-    if ((rc = upload(ctx, &d.alias, tab.data(), tab.size()))) return rc;
-    d.bytes += 8ull * h.E();
+    if (d.wedge) {
+        (void)hipFree(d.wedge);
+        d.bytes -= 16ull * h.E();
+    }
+    d.wedge = nullptr;
+    if ((rc = upload(ctx, &d.wedge, tab.data(), tab.size()))) return rc;
+    d.bytes += 16ull * h.E();
     return PTW_OK;
 }
 
@@ -441,7 +443,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     if (path_app && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
     if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
-    if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
+    if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
     const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
     if (on_dev && (((uintptr_t)out->paths & 15) || ((uintptr_t)out->ends & 3) || ((uintptr_t)out->counts & 7)))
         return fail(ctx, PTW_ERR_INVALID_ARG, "device outputs must be aligned: paths 16 B, ends 4 B, counts 8 B");
@@ -498,7 +500,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     p.col = ctx->dev.col;
     p.perm = ctx->dev.perm;
     p.inv = ctx->dev.inv;
-    p.alias = weighted ? ctx->dev.alias : nullptr;
+    p.wedge = weighted ? ctx->dev.wedge : nullptr;
     p.V = V;
     p.hubs = ctx->dev.hubs;
     p.app = d.app;

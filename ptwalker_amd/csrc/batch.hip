@@ -183,7 +183,7 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     const uint64_t first_chunk = p.hdr[6];  // the workgroup chunks of dense queries come first
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
-    const uint2 *alias = p.hdr[5] ? p.alias : nullptr;
+    const uint4 *wedge = p.hdr[5] ? p.wedge : nullptr;
     uint64_t c_next = 0, c_end = 0;  // wave-uniform current chunk
     uint32_t cq = kEmpty;
     bool exhausted = false;
@@ -268,12 +268,13 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                 if (deg == 0) {
                     stop = true;  // batches end walkers at vertices without out-edges
                 } else {
-                    uint32_t j = mulhi64(r.x, r.y, deg);
-                    if (alias) {
-                        const uint2 a = alias[beg + j];
-                        if (r.w >= a.x) j = a.y;
+                    const uint32_t j = mulhi64(r.x, r.y, deg);
+                    if (wedge) {
+                        const uint4 e = wedge[beg + j];
+                        v = r.w >= e.y ? e.z : e.x;
+                    } else {
+                        v = p.col[beg + j];
                     }
-                    v = p.col[beg + j];
                     ++step;
                     if (v == Q.src)
                         ++my_src;
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     const uint64_t n_dense = p.hdr[6];
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
-    const uint2 *alias = p.hdr[5] ? p.alias : nullptr;
+    const uint4 *wedge = p.hdr[5] ? p.wedge : nullptr;
     uint32_t cq = kEmpty;  // workgroup-uniform: the query whose counts the LDS holds
     uint32_t src = 0, thr = 0, stream = 0;
     uint64_t off = 0;
@@ -429,12 +430,13 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                     if (deg == 0) {
                         stop = true;  // batches end walkers at vertices without out-edges
                     } else {
-                        uint32_t j = mulhi64(r.x, r.y, deg);
-                        if (alias) {
-                            const uint2 a = alias[beg + j];
-                            if (r.w >= a.x) j = a.y;
+                        const uint32_t j = mulhi64(r.x, r.y, deg);
+                        if (wedge) {
+                            const uint4 e = wedge[beg + j];
+                            v = r.w >= e.y ? e.z : e.x;
+                        } else {
+                            v = p.col[beg + j];
                         }
-                        v = p.col[beg + j];
                         ++step;
                         if (v == src) {
                             ++my_src;

@@ -142,7 +142,7 @@ BatchLaunch launch_args(ptw_ctx *ctx)
     BatchLaunch p{};
     p.row_ptr = ctx->dev.row_ptr;
     p.col = ctx->dev.col;
-    p.alias = ctx->dev.alias;
+    p.wedge = ctx->dev.wedge;
     p.V = ctx->dev.V;
     p.queries = b.queries;
     p.chunks = b.chunks;
@@ -318,7 +318,7 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
         return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
     const bool weighted = (d.flags & PTW_BATCH_WEIGHTED) != 0;
-    if (weighted && !ctx->dev.alias) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted batch on a graph without weights");
+    if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted batch on a graph without weights");
     const double t0 = ptw::now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     ptw::BatchState &b = ctx->batch;

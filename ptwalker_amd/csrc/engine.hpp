@@ -29,7 +29,10 @@ struct DeviceGraph {
     uint32_t *col = nullptr;      // E, device ids, per-vertex order kept
     uint32_t *perm = nullptr;     // graph id -> device id (nullptr: identity)
     uint32_t *inv = nullptr;      // device id -> graph id (nullptr: identity)
-    uint2 *alias = nullptr;       // E, {threshold, local alias index}, device order
+    // E weighted-edge records, device order: {neighbour, alias threshold,
+    // the alias slot's neighbour, 0}.  A weighted pick of slot j reads one
+    // 16 B record instead of the alias entry and then the neighbour.
+    uint4 *wedge = nullptr;
     // transpose (in-edges, cut into pieces) and out-degrees for the FP64
     // solver, built on first use
     uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
@@ -82,7 +85,7 @@ struct WalkLaunch {
     const uint32_t *col;
     const uint32_t *perm;  // nullptr: identity
     const uint32_t *inv;   // nullptr: identity
-    const uint2 *alias;    // nullptr: uniform neighbour choice
+    const uint4 *wedge;    // weighted-edge records; nullptr: uniform neighbour choice
     uint32_t V;
     uint32_t hubs;         // LDS-privatized counters for device ids < hubs
     // process
@@ -238,7 +241,7 @@ struct BatchChunk {
 struct BatchLaunch {
     const uint32_t *row_ptr;
     const uint32_t *col;
-    const uint2 *alias;  // used when hdr[5] != 0
+    const uint4 *wedge;  // weighted-edge records, used when hdr[5] != 0
     uint32_t V;
     const BatchQuery *queries;
     const BatchChunk *chunks;

@@ -301,12 +301,13 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     if (KIND == KIND_N2V && step > 0) {
                         nx = n2v_pick(p, wlo, whi, step, r, beg, deg, prev);
                     } else {
-                        uint32_t j = mulhi64(r.x, r.y, deg);
-                        if (p.alias) {  // weighted: alias coin from the 4th word
-                            const uint2 a = p.alias[beg + j];
-                            if (r.w >= a.x) j = a.y;
+                        const uint32_t j = mulhi64(r.x, r.y, deg);
+                        if (p.wedge) {  // weighted: alias coin from the 4th word
+                            const uint4 e = p.wedge[beg + j];
+                            nx = r.w >= e.y ? e.z : e.x;
+                        } else {
+                            nx = p.col[beg + j];
                         }
-                        nx = p.col[beg + j];
                     }
                     if (KIND == KIND_N2V) prev = v;
                     v = nx;
