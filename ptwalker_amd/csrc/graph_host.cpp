@@ -542,21 +542,35 @@ void relabel_by_degree(const HostCsr &g, std::vector<uint32_t> &perm, std::vecto
                        std::vector<uint32_t> &d_row_ptr, std::vector<uint32_t> &d_col)
 {
     const uint32_t V = g.V;
-    inv.resize(V);
-    std::iota(inv.begin(), inv.end(), 0u);
     const uint32_t *rp = g.row_ptr.data();
-    std::stable_sort(inv.begin(), inv.end(), [rp](uint32_t x, uint32_t y) {
-        return (rp[x + 1] - rp[x]) > (rp[y + 1] - rp[y]);
-    });
+    // Device order: degree descending, ties by graph id ascending -- a
+    // counting sort on the degree (stable: vertices are placed in id order).
+    uint32_t maxdeg = 0;
+    for (uint32_t u = 0; u < V; ++u) maxdeg = std::max(maxdeg, rp[u + 1] - rp[u]);
+    std::vector<uint64_t> cnt((size_t)maxdeg + 1, 0), first((size_t)maxdeg + 1, 0);
+    for (uint32_t u = 0; u < V; ++u) ++cnt[rp[u + 1] - rp[u]];
+    uint64_t pos = 0;
+    for (uint64_t d = (uint64_t)maxdeg + 1; d-- > 0;) {  // positions of degree d: [first[d], first[d] + cnt[d])
+        first[d] = pos;
+        pos += cnt[d];
+    }
+    inv.resize(V);
+    {
+        std::vector<uint64_t> next(first);
+        for (uint32_t u = 0; u < V; ++u) inv[next[rp[u + 1] - rp[u]]++] = u;
+    }
     perm.resize(V);
     parallel_for(V, [&](uint64_t lo, uint64_t hi) {
         for (uint64_t d = lo; d < hi; ++d) perm[inv[d]] = (uint32_t)d;
     });
+    // row offsets in device order, block by block: no vertex is looked up
     d_row_ptr.assign((size_t)V + 1, 0);
     uint64_t acc = 0;
-    for (uint32_t d = 0; d < V; ++d) {
-        d_row_ptr[d] = (uint32_t)acc;
-        acc += rp[inv[d] + 1] - rp[inv[d]];
+    for (uint64_t d = (uint64_t)maxdeg + 1; d-- > 0;) {
+        for (uint64_t k = first[d]; k < first[d] + cnt[d]; ++k) {
+            d_row_ptr[k] = (uint32_t)acc;
+            acc += d;
+        }
     }
     d_row_ptr[V] = (uint32_t)acc;
     d_col.resize(acc);
