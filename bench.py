@@ -361,6 +361,7 @@ def bench_batch(args):
         tuned_cfg = {"choice": tr.best, "tuning_s": round(time.time() - t0, 1), "candidates": len(tr.trials)}
         res = measure(dense_threshold=tr.best.get("dense_threshold", 0))
     el, kms, r = res["graph"]
+    gps, _ = ctx.bench_gather(4 * gi.num_edges, 1 << 30, seed=1)
     line = {
         "metric": "walk-steps/s",
         "value": r.total_steps / el,
@@ -381,6 +382,9 @@ def bench_batch(args):
                    "slice_bytes": r.slice_bytes, "graph_build_s": round(build_s, 1)},
         "queries_per_s": args.queries / el,
         "kernel_ms": kms,
+        "random_gather": {"gathers_per_s": gps, "table_bytes": 4 * gi.num_edges,
+                          "frac": 2.0 * r.total_steps / el / gps,
+                          "note": "(batch steps/s x 2 random gathers per step) / measured random u32 gathers/s"},
         "eager": {"ms_per_batch": 1e3 * res["eager"][0], "kernel_ms": res["eager"][1]},
         "autotune": tuned_cfg,
         "untuned": {"steps_per_s": untuned["graph"][2].total_steps / untuned["graph"][0],
