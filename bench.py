@@ -361,6 +361,7 @@ def bench_batch(args):
         tuned_cfg = {"choice": tr.best, "tuning_s": round(time.time() - t0, 1), "candidates": len(tr.trials)}
         res = measure(dense_threshold=tr.best.get("dense_threshold", 0))
     el, kms, r = res["graph"]
+    audit = audit_batch(ctx, src, W, tp, r)
     gps, _ = ctx.bench_gather(4 * gi.num_edges, 1 << 30, seed=1)
     line = {
         "metric": "walk-steps/s",
@@ -386,6 +387,7 @@ def bench_batch(args):
                           "frac": 2.0 * r.total_steps / el / gps,
                           "note": "(batch steps/s x 2 random gathers per step) / measured random u32 gathers/s"},
         "eager": {"ms_per_batch": 1e3 * res["eager"][0], "kernel_ms": res["eager"][1]},
+        "fp64_audit": audit,
         "autotune": tuned_cfg,
         "untuned": {"steps_per_s": untuned["graph"][2].total_steps / untuned["graph"][0],
                     "kernel_ms": untuned["graph"][1], "shape": ctx_default_shape},
@@ -400,6 +402,30 @@ def bench_batch(args):
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def audit_batch(ctx, src, W, tp, r, n=64, k=100):
+    """Scores n queries spread over the last batch against the FP64 solver:
+    the top-k L1 of the visit distribution, read from the batch's own slices at
+    the exact top-k vertices, and how many of the batch's top-k vertices are in
+    the exact top-k."""
+    import numpy as np
+
+    idx = np.linspace(0, len(src) - 1, min(n, len(src))).astype(np.int64)
+    t0 = time.perf_counter()
+    _, tv, tval, iters = ctx.ppr_exact(src[idx], tp[idx], tol=1e-7, full=False, k=k)
+    solve_s = time.perf_counter() - t0
+    cnt = ctx.batch_gather(np.repeat(idx.astype(np.uint32), k), tv.reshape(-1)).reshape(len(idx), k)
+    visits = W[idx].astype(np.float64) + r.steps[idx].astype(np.float64)
+    est = cnt.astype(np.float64) / visits[:, None]
+    l1 = np.abs(est - tval / (1.0 + 1.0 / tp[idx])[:, None]).sum(axis=1)
+    overlap = np.array([len(np.intersect1d(r.topk_vertex[q], tv[j])) / k for j, q in enumerate(idx)])
+    return {"queries": int(len(idx)), "k": k, "solver_iterations": int(iters), "solver_s": round(solve_s, 2),
+            "walkers_min": int(W[idx].min()), "walkers_max": int(W[idx].max()),
+            "topk_l1_median": float(np.median(l1)), "topk_l1_max": float(l1.max()),
+            "topk_l1_median_above_1e5_walkers": float(np.median(l1[W[idx] > 100000])) if (W[idx] > 100000).any()
+            else None,
+            "topk_overlap_mean": round(float(overlap.mean()), 4)}
 
 
 def cpu_baseline_batch(ctx, src, W, tp, args):
