@@ -381,9 +381,9 @@ typedef struct ptw_tuning {
     int32_t block;         /* threads per workgroup: multiple of 64, 64..1024 */
     int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query */
     int32_t chunk;         /* walker ids a wavefront takes from the queue at once (>= 64) */
-    int32_t lds_hubs;      /* LDS counters for the top-degree vertices, 0..40704 */
+    int32_t lds_hubs;      /* u16 LDS counters for the top-degree vertices, 0..81408 */
     int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 16384; 0 = off);
-                              (lds_hubs + 2 cache_slots) * 4 B <= 159 KiB */
+                              2 B * lds_hubs + 8 B * cache_slots <= 159 KiB */
     int32_t batch_chunk;   /* walker ids per queue entry in batched launches (>= 64) */
     int32_t topk_tile;     /* LDS candidate slots of the top-k kernel (power of two, 256..4096) */
     int32_t dense_chunk;   /* walker ids per workgroup chunk of a dense batched query, a multiple

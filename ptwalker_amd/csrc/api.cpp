@@ -594,12 +594,13 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
     if (t->blocks_per_cu < 0 || t->blocks_per_cu > 32)
         return fail(ctx, PTW_ERR_INVALID_ARG, "blocks_per_cu must be in [0, 32]");
     if (t->chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "chunk must be >= 64");
-    if (t->lds_hubs < 0 || t->lds_hubs > 40704) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 40704]");
+    if (t->lds_hubs < 0 || t->lds_hubs > 81408) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 81408]");
     if (t->cache_slots < 0 || t->cache_slots > 16384 || (t->cache_slots & (t->cache_slots - 1)))
         return fail(ctx, PTW_ERR_INVALID_ARG, "cache_slots must be 0 or a power of two <= 16384");
-    // hub counters and cache share one workgroup's LDS (160 KiB per CU, 1 KiB kept for the rest)
-    if (((int64_t)t->lds_hubs + 2 * (int64_t)t->cache_slots) * 4 > 159 * 1024)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs + 2 * cache_slots must fit in 159 KiB of u32 counters");
+    // u16 hub counters and the u32 cache share one workgroup's LDS (160 KiB per
+    // CU, 1 KiB kept for the rest)
+    if ((((int64_t)t->lds_hubs + 1) / 2 + 2 * (int64_t)t->cache_slots) * 4 > 159 * 1024)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "2 B per lds_hubs + 8 B per cache slot must fit in 159 KiB");
     if (t->batch_chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "batch_chunk must be >= 64");
     if (t->topk_tile < 256 || t->topk_tile > 4096 || (t->topk_tile & (t->topk_tile - 1)))
         return fail(ctx, PTW_ERR_INVALID_ARG, "topk_tile must be a power of two in [256, 4096]");

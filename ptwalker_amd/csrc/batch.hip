@@ -306,19 +306,22 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
 // query's dense slice.  The LDS counts are flushed to the slice at a barrier
 // when the workgroup's next chunk belongs to another query, and at the end.
 // Every wavefront reaches each barrier: a chunk runs dry and every walk ends.
-// LDS counts are u32 and cannot wrap: none exceeds its query's visit total,
-// which the host checks is below 2^32.
+// The direct counters are u16, two per LDS word (vertex x in half x & 1 of
+// word x >> 1); the increment that takes one to 2^15 moves 2^15 to the slice.
+// The cache counts are u32 and cannot wrap: none exceeds its query's visit
+// total, which the host checks is below 2^32.
 __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32_t hubs, uint32_t slots)
 {
     extern __shared__ uint32_t dense_lds[];
+    const uint32_t hub_words = (hubs + 1) >> 1;
     uint32_t *const direct = dense_lds;
-    uint32_t *const keys = dense_lds + hubs;
+    uint32_t *const keys = dense_lds + hub_words;
     uint32_t *const vals = keys + slots;
     const uint32_t mask = slots ? slots - 1 : 0u;
     __shared__ uint32_t s_q, s_len, s_off, s_done;
     __shared__ unsigned long long s_start;
 
-    for (uint32_t i = threadIdx.x; i < hubs; i += blockDim.x) direct[i] = 0;
+    for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) direct[i] = 0;
     for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) {
         keys[i] = kEmpty;
         vals[i] = 0;
@@ -348,10 +351,11 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
         }
         my_src = 0;
         my_steps = 0;
-        for (uint32_t i = threadIdx.x; i < hubs; i += blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) {
             const uint32_t c = direct[i];
             if (c) {
-                atomicAdd(&p.dense[off + i], c);
+                if (c & 0xFFFFu) atomicAdd(&p.dense[off + 2 * i], c & 0xFFFFu);
+                if (c >> 16) atomicAdd(&p.dense[off + 2 * i + 1], c >> 16);
                 direct[i] = 0;
             }
         }
@@ -441,7 +445,13 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                         if (v == src) {
                             ++my_src;
                         } else if (v < hubs) {
-                            atomicAdd(&direct[v], 1u);
+                            // the u16 spill rule of walk.hip's lds_add16, into the u32 slice
+                            const uint32_t sh = (v & 1u) << 4;
+                            const uint32_t old = atomicAdd(&direct[v >> 1], 1u << sh);
+                            if (((old >> sh) & 0xFFFFu) == 0x7FFFu) {
+                                atomicSub(&direct[v >> 1], 0x8000u << sh);
+                                atomicAdd(&p.dense[off + v], 0x8000u);
+                            }
                         } else {
                             bool done = false;
                             if (slots) {
@@ -1229,7 +1239,7 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
                              hipStream_t s)
 {
-    const size_t dlds = ((size_t)lds_hubs + 2 * (size_t)cache_slots) * 4;
+    const size_t dlds = (((size_t)lds_hubs + 1) / 2 + 2 * (size_t)cache_slots) * 4;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_dense_kernel, block, dlds) != hipSuccess ||
         per_cu < 1)

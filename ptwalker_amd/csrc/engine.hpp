@@ -114,8 +114,8 @@ struct LaunchShape {
     int block = 1024;        // threads per workgroup
     int blocks_per_cu = 0;   // 0: fill the device from the occupancy query
     int chunk = 1024;        // walker ids a wave takes from the queue at once
-    int lds_hubs = 36864;    // u32 LDS counters for device ids below this (count apps)
-    int cache_slots = 1024;  // LDS visit-cache slots per workgroup (power of two, 0 = off)
+    int lds_hubs = 77824;    // u16 LDS counters for device ids below this (count apps)
+    int cache_slots = 512;   // LDS visit-cache slots per workgroup (power of two, 0 = off)
     int batch_chunk = 16384;  // walker ids per chunk of a batched launch
     int topk_tile = 2048;    // top-k LDS candidate slots
     int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)

@@ -15,8 +15,9 @@
 // on lane, slot, wave or launch order.
 //
 // Count apps privatize the `lds_hubs` highest-degree vertices (device ids
-// below `hubs` after the degree-descending relabel) in LDS u64 counters,
-// flushed once per workgroup; other visits are device-scope u64 atomics.
+// below `hubs` after the degree-descending relabel) in LDS u16 counters, two
+// per LDS word, flushed once per workgroup; other visits go to an LDS visit
+// cache or to device-scope u64 atomics.
 #include <algorithm>
 
 #include "../../include/ptw.h"
@@ -111,16 +112,18 @@ __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v)
 }
 
 // Workgroup-private visit counters in LDS.
-//   direct: u32 counters for device ids [0, hubs) (the top-degree vertices)
+//   direct: u16 counters for device ids [0, hubs) (the top-degree vertices),
+//           vertex x in half (x & 1) of word x >> 1: twice the hubs of u32
+//           counters in the same LDS
 //   cache:  open-addressing table of `slots` (power of two) {u32 key, u32 count}
 //           that admits any vertex first come, first served -- the vertices a
 //           workgroup visits first and most (a source's neighbourhood, the
 //           hubs of a walk) end up resident; a vertex that finds no slot in
 //           kProbes probes goes to a device-scope u64 atomic.
-// Counts are u32; the increment that takes a counter to 2^31 moves 2^31 to
-// the device counter, so no LDS counter can wrap.
+// The increment that takes a direct counter to 2^15 moves 2^15 to the device
+// counter (cache counts: 2^31), so no LDS counter reaches its width.
 struct LdsCounters {
-    uint32_t *direct;
+    uint32_t *direct;  // (hubs + 1) / 2 words
     uint32_t *keys;
     uint32_t *vals;
     uint32_t hubs;
@@ -139,10 +142,25 @@ __device__ __forceinline__ void lds_add1(uint32_t *c, unsigned long long *dev_co
     }
 }
 
+// One visit to the u16 counter in bits [shift, shift + 16) of an LDS word.
+// Exactly one increment reads 0x7FFF; it takes 2^15 back out at once.  In
+// between, the other lanes of the workgroup can add at most about one visit
+// each (every further visit of a lane waits for its next step's global
+// loads), so the half stays below 2^15 + 1024 and never carries into the
+// other half.
+__device__ __forceinline__ void lds_add16(uint32_t *word, uint32_t shift, unsigned long long *dev_counter)
+{
+    const uint32_t old = atomicAdd(word, 1u << shift);
+    if (((old >> shift) & 0xFFFFu) == 0x7FFFu) {
+        atomicSub(word, 0x8000u << shift);
+        atomicAdd(dev_counter, 0x8000ull);
+    }
+}
+
 __device__ __forceinline__ void count_in_lds(const LdsCounters &L, uint32_t x, unsigned long long *counts)
 {
     if (x < L.hubs) {
-        lds_add1(&L.direct[x], &counts[x]);
+        lds_add16(&L.direct[x >> 1], (x & 1u) << 4, &counts[x]);
         return;
     }
     if (L.mask != 0xFFFFFFFFu) {
@@ -166,10 +184,11 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                                                    uint32_t cache_slots)
 {
     extern __shared__ uint32_t lds_words[];
-    LdsCounters L{lds_words, lds_words + lds_hubs, lds_words + lds_hubs + cache_slots, lds_hubs,
+    const uint32_t hub_words = (lds_hubs + 1) >> 1;
+    LdsCounters L{lds_words, lds_words + hub_words, lds_words + hub_words + cache_slots, lds_hubs,
                   cache_slots ? cache_slots - 1 : 0xFFFFFFFFu};
     if (KIND == KIND_COUNT) {
-        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) L.direct[i] = 0;
+        for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) L.direct[i] = 0;
         for (uint32_t i = threadIdx.x; i < cache_slots; i += blockDim.x) {
             L.keys[i] = kEmptyKey;
             L.vals[i] = 0;
@@ -346,9 +365,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     }
     if (KIND == KIND_COUNT) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < lds_hubs; i += blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) {
             const uint32_t c = L.direct[i];
-            if (c) atomicAdd(&p.counts_dev[i], (unsigned long long)c);
+            if (c & 0xFFFFu) atomicAdd(&p.counts_dev[2 * i], (unsigned long long)(c & 0xFFFFu));
+            if (c >> 16) atomicAdd(&p.counts_dev[2 * i + 1], (unsigned long long)(c >> 16));
         }
         for (uint32_t i = threadIdx.x; i < cache_slots; i += blockDim.x) {
             const uint32_t k = L.keys[i], c = L.vals[i];
@@ -446,7 +466,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
         hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
         slots = (uint32_t)shape.cache_slots;
     }
-    size_t lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
+    size_t lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
@@ -465,7 +485,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
             per_cu = 1;
             hubs = hubs < 8192u ? hubs : 8192u;
             slots = p.start_mode == PTW_START_SOURCE ? (slots < 1024u ? slots : 1024u) : 0u;
-            lds = ((size_t)hubs + 2ull * slots) * sizeof(uint32_t);
+            lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
         }
     }
     const int blocks = per_cu * num_cus;

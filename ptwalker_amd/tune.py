@@ -38,7 +38,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ptwalker_amd", 
 WALK_SPACE: Dict[str, List[int]] = {
     "chunk": [1024, 2048, 4096, 8192, 16384],
     "block": [256, 512, 1024],
-    "lds_hubs": [8192, 16384, 24576, 36864],
+    "lds_hubs": [16384, 36864, 55296, 77824],
     "cache_slots": [0, 512, 1024, 2048],
 }
 
@@ -46,7 +46,7 @@ BATCH_SPACE: Dict[str, List[int]] = {
     "batch_chunk": [2048, 4096, 8192, 16384, 32768, 65536],
     "dense_chunk": [0, 16384, 32768, 49152],
     "block": [256, 512, 1024],
-    "lds_hubs": [16384, 24576, 36864],
+    "lds_hubs": [36864, 55296, 77824],
     "dense_threshold": [12_500, 25_000, 50_000, 100_000, 200_000],
     "topk_tile": [256, 512, 1024, 2048, 4096],
 }

@@ -121,6 +121,24 @@ def test_batch_weighted(ctx):
         assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
 
 
+def test_dense_hub_counter_spill(ctx):
+    # star: hub 0 joined to 64 leaves; from a leaf every other visit is the hub,
+    # so a 49152-walker chunk of one query takes about 1.2e5 hub visits and its
+    # u16 LDS counter spills to the query's slice several times per chunk
+    n = 64
+    rp = np.zeros(n + 2, dtype=np.uint32)
+    rp[1] = n
+    rp[2:] = n + np.arange(1, n + 1)
+    col = np.concatenate([np.arange(1, n + 1), np.zeros(n)]).astype(np.uint32)
+    ctx.from_csr(rp, col)
+    qs = ((1, 6_000_000, 0.2), (2, 3_000_000, 0.25))
+    r = ctx.batch_ppr([q[0] for q in qs], [q[1] for q in qs], [q[2] for q in qs], topk=10, seed=3)
+    assert r.dense_queries == 2
+    for i, (s, W, tp) in enumerate(qs):
+        ref = replay_query(rp, col, s, W, tp, i, seed=3)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+
+
 def test_topk_tie_overflow_path(ctx):
     # star: hub 0 joined to 5000 leaves; tp = 1 makes each walker visit the hub then one
     # leaf, so leaf counts are small and tied far beyond the 256-slot tile

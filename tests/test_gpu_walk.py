@@ -145,7 +145,7 @@ def test_lds_and_global_counters_scale16(ctx):
     check(ctx, rp, col, ptw.RWR(0.2, int(gi.max_degree_vertex)), 1_000_000, seed=1)
 
 
-@pytest.mark.parametrize("hubs,slots", [(0, 0), (256, 0), (0, 4096), (256, 1024), (8192, 4096)])
+@pytest.mark.parametrize("hubs,slots", [(0, 0), (256, 0), (0, 4096), (255, 1024), (8192, 4096)])
 def test_counter_paths_explicit_shapes(ctx, hubs, slots):
     """Every counting path, pinned by an explicit launch shape (blocks_per_cu set,
     so the small-walk rule does not apply): direct LDS hub counters, the LDS
@@ -160,6 +160,26 @@ def test_counter_paths_explicit_shapes(ctx, hubs, slots):
         check(ctx, rp, col, ptw.PageRank(0.15), 300_000, seed=5)
     finally:
         ctx.set_tuning(**default)
+
+
+def star(n):
+    """Vertex 0 joined to leaves 1..n, both directions."""
+    rp = np.zeros(n + 2, dtype=np.uint32)
+    rp[1] = n
+    rp[2:] = n + np.arange(1, n + 1)
+    col = np.concatenate([np.arange(1, n + 1), np.zeros(n)]).astype(np.uint32)
+    return rp, col
+
+
+def test_hub_counter_spill(ctx):
+    """The centre of a star takes every other visit.  From a leaf, 8e6 walkers at
+    tp 0.2 make about 2e7 centre visits, far above 2^15 per workgroup, so every
+    workgroup's u16 LDS counter of the centre spills to the device counter."""
+    rp, col = star(64)
+    ctx.from_csr(rp, col)
+    assert ctx.info().hub_vertices == 65
+    check(ctx, rp, col, ptw.PPR(0.2, 1), 8_000_000, seed=3)
+    check(ctx, rp, col, ptw.PageRank(0.2), 4_000_000, seed=4)
 
 
 def test_device_outputs_and_accumulate(ctx):
