@@ -807,17 +807,23 @@ def bench_node2vec(args):
         except AttributeError:
             cores = os.cpu_count() or 1
         cores = max(1, min(cores, 16))
+        # every replay first sorts the rows and checks the graph (the device
+        # does this once, outside its timing): time it alone and leave it out
+        t = time.perf_counter()
+        oracle.node2vec_replay(rp, col, 1, L, 0.5, 2.0, seed=1, threads=cores)
+        setup = time.perf_counter() - t
         n = 20000
         t = time.perf_counter()
         r = oracle.node2vec_replay(rp, col, n, L, 0.5, 2.0, seed=1, threads=cores)
-        dt = time.perf_counter() - t
-        n = int(min(W, max(n, n * args.cpu_seconds / max(dt, 1e-3))))
+        dt = max(time.perf_counter() - t - setup, 1e-3)
+        n = int(min(W, max(n, n * args.cpu_seconds / dt)))
         t = time.perf_counter()
         r = oracle.node2vec_replay(rp, col, n, L, 0.5, 2.0, seed=1, threads=cores)
-        dt = time.perf_counter() - t
+        dt = max(time.perf_counter() - t - setup, 1e-3)
         cpu = {"value": r["steps"] / dt, "unit": "steps/s", "cores": cores, "kind": "port",
-               "sample": f"{n} node2vec walks (p 0.5, q 2) of {L} vertices, {r['steps']} steps, {dt:.1f} s, "
-                         f"the oracle's replay of the same decisions, {cores} threads"}
+               "sample": f"{n} node2vec walks (p 0.5, q 2) of {L} vertices, {r['steps']} steps, {dt:.1f} s "
+                         f"after {setup:.1f} s of row sorting left out, the oracle's replay of the same "
+                         f"decisions, {cores} threads"}
     line = {
         "metric": "walk-steps/s",
         "value": main["steps_per_s"],

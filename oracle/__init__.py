@@ -58,7 +58,8 @@ class WalkDesc(ctypes.Structure):
 
 
 class WalkStats(ctypes.Structure):
-    _fields_ = [("steps", ctypes.c_uint64), ("visits", ctypes.c_uint64), ("dangling", ctypes.c_uint64)]
+    _fields_ = [("steps", ctypes.c_uint64), ("visits", ctypes.c_uint64), ("dangling", ctypes.c_uint64),
+                ("proposals", ctypes.c_uint64), ("exact_draws", ctypes.c_uint64)]
 
 
 _lib = None
@@ -138,7 +139,9 @@ def node2vec_replay(row_ptr, col, walkers: int, walk_len: int, p: float, q: floa
                     source: int = 0, start_mode: int = START_UNIFORM, stream: int = 0, wid_base: int = 0,
                     threads: int = 0):
     """Philox replay of the engine's node2vec walks (oracle.c orc_node2vec_replay).
-    Returns dict(paths [walkers, walk_len], steps, dangling)."""
+    Returns dict(paths [walkers, walk_len], steps, dangling, proposals,
+    exact_draws): the rejection sampler's proposals and the steps its exact
+    draw decided."""
     rp, cl = _u32(row_ptr), _u32(col)
     V = len(rp) - 1
     d = WalkDesc(APP_NODE2VEC, SEM_REFERENCE, start_mode, source, 0.5, walk_len, seed, stream, wid_base, walkers)
@@ -149,7 +152,8 @@ def node2vec_replay(row_ptr, col, walkers: int, walk_len: int, p: float, q: floa
                                    threads)
     if rc != 0:
         raise ValueError("invalid oracle node2vec arguments")
-    return dict(paths=paths[: walkers * walk_len].reshape(walkers, walk_len), steps=st.steps, dangling=st.dangling)
+    return dict(paths=paths[: walkers * walk_len].reshape(walkers, walk_len), steps=st.steps, dangling=st.dangling,
+                proposals=st.proposals, exact_draws=st.exact_draws)
 
 
 def node2vec_law(row_ptr, col, prev: int, cur: int, p: float, q: float):

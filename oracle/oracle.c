@@ -207,16 +207,59 @@ int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
     uint32_t *scol = (uint32_t *)malloc((E ? E : 1) * sizeof(uint32_t));
     if (!scol) return -1;
     memcpy(scol, col, E * sizeof(uint32_t));
-    for (uint32_t v = 0; v < V; ++v) qsort(scol + row_ptr[v], row_ptr[v + 1] - row_ptr[v], 4, cmp_u32);
+    uint32_t maxdeg = 0;
+    /* simple (no row repeats a neighbour) and symmetric (u in x's row for every
+     * edge u -> x): then the previous vertex is in the current row exactly once */
+    int64_t bad = 0;
+#ifdef _OPENMP
+    const int nt0 = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt0) reduction(max : maxdeg)
+#endif
+    for (int64_t v = 0; v < (int64_t)V; ++v) {
+        qsort(scol + row_ptr[v], row_ptr[v + 1] - row_ptr[v], 4, cmp_u32);
+        if (row_ptr[v + 1] - row_ptr[v] > maxdeg) maxdeg = row_ptr[v + 1] - row_ptr[v];
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt0) reduction(+ : bad)
+#endif
+    for (int64_t u = 0; u < (int64_t)V; ++u)
+        for (uint32_t k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
+            const uint32_t x = scol[k];
+            if ((k > row_ptr[u] && x == scol[k - 1]) ||
+                !n2v_has(scol + row_ptr[x], row_ptr[x + 1] - row_ptr[x], (uint32_t)u)) {
+                ++bad;
+                break;
+            }
+        }
+    const int simple_sym = bad == 0;
+    /* Weights floor(2^32 w / M).  Unfolded: M = max(1/p, 1, 1/q), one proposal
+     * slot per neighbour, each weight an acceptance threshold <= 2^32.
+     * Folded (1/p the largest weight on a simple symmetric graph): M =
+     * max(1, 1/q); prev's own slot accepts always and its excess weight over
+     * 2^32 is spread over `fold` extra slots that accept prev below w_fold
+     * (outlier folding, Yang et al., KnightKing, SOSP 2019). */
     const double wr = 1.0 / p, wo = 1.0 / q;
-    double M = wr > 1.0 ? wr : 1.0;
-    if (wo > M) M = wo;
-    const uint64_t w_ret = n2v_thr(wr, M), w_in = n2v_thr(1.0, M), w_out = n2v_thr(wo, M);
-    uint64_t tot_steps = 0, tot_dangling = 0;
+    const double Mf = wo > 1.0 ? wo : 1.0;
+    const uint64_t t_ret = (uint64_t)floor(wr / Mf * 4294967296.0);
+    const int folded = simple_sym && wr > Mf && t_ret > 4294967296ull && maxdeg < (1u << 31);
+    const double M = folded ? Mf : (wr > Mf ? wr : Mf);
+    const uint64_t w_in = n2v_thr(1.0, M), w_out = n2v_thr(wo, M);
+    uint64_t w_ret, w_fold = 0;
+    uint32_t fold = 0;
+    if (folded) {
+        const uint64_t excess = t_ret - 4294967296ull;
+        fold = (uint32_t)((excess + 4294967295ull) >> 32);
+        w_fold = excess / fold;
+        w_ret = 4294967296ull + (uint64_t)fold * w_fold;
+    } else {
+        w_ret = n2v_thr(wr, M);
+    }
+    uint64_t tot_steps = 0, tot_dangling = 0, tot_prop = 0, tot_exact = 0;
     const int64_t W = (int64_t)d->walkers;
 #ifdef _OPENMP
     int nt = threads > 0 ? threads : omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+ : tot_steps, tot_dangling)
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) \
+    reduction(+ : tot_steps, tot_dangling, tot_prop, tot_exact)
 #else
     (void)threads;
 #endif
@@ -247,10 +290,18 @@ int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
                 int found = 0;
                 for (uint32_t a = 0; a < 64 && !found; ++a) {
                     if (a) draw(d->seed, d->stream, wid, i + (a << 20), r);
-                    const uint32_t y = col[beg + orc_mulhi64(r[0], r[1], deg)];
-                    if ((uint64_t)r[2] < n2v_w(&c, y)) { x = y; found = 1; }
+                    ++tot_prop;
+                    const uint32_t s = orc_mulhi64(r[0], r[1], deg + fold);
+                    if (s >= deg) {  /* an extra slot of the return edge */
+                        if ((uint64_t)r[2] < w_fold) { x = prev; found = 1; }
+                        continue;
+                    }
+                    const uint32_t y = col[beg + s];
+                    const uint64_t wy = (fold && y == prev) ? 4294967296ull : n2v_w(&c, y);
+                    if ((uint64_t)r[2] < wy) { x = y; found = 1; }
                 }
                 if (!found) {
+                    ++tot_exact;
                     uint64_t total = 0;
                     for (uint32_t k = 0; k < deg; ++k) total += n2v_w(&c, col[beg + k]);
                     draw(d->seed, d->stream, wid, i + (64u << 20), r);
@@ -275,6 +326,8 @@ int orc_node2vec_replay(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
     if (st) {
         st->steps = tot_steps;
         st->dangling = tot_dangling;
+        st->proposals = tot_prop;
+        st->exact_draws = tot_exact;
     }
     return 0;
 }

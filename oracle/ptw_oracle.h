@@ -84,6 +84,8 @@ typedef struct {
     uint64_t steps;     /* sum over walkers of transitions taken */
     uint64_t visits;    /* sum of count increments (count apps) */
     uint64_t dangling;  /* walkers stopped at a degree-0 vertex with budget left */
+    uint64_t proposals;   /* node2vec: proposals drawn by the rejection sampler */
+    uint64_t exact_draws; /* node2vec: steps decided by the exact draw */
 } orc_walk_stats;
 
 /* Philox replay.  counts: u64[V] accumulated (+=), may be NULL;

@@ -4,6 +4,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -139,6 +140,24 @@ int ensure_sorted_adjacency(ptw_ctx *ctx)
             std::sort(scol.begin() + out, scol.begin() + out + (e - b));
         }
     });
+    // simple and symmetric: no sorted row repeats a neighbour, and every
+    // edge u -> x finds u in x's sorted row
+    const uint32_t *ro = rel ? ctx->h_drow.data() : h.row_ptr.data();
+    std::atomic<bool> ok{true};
+    ptw::parallel_for(V, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t u = lo; u < hi && ok.load(std::memory_order_relaxed); ++u) {
+            const uint32_t *a = scol.data() + ro[u];
+            const uint32_t n = ro[u + 1] - ro[u];
+            for (uint32_t k = 0; k < n; ++k) {
+                const uint32_t *b = scol.data() + ro[a[k]];
+                if ((k && a[k] == a[k - 1]) || !std::binary_search(b, b + (ro[a[k] + 1] - ro[a[k]]), (uint32_t)u)) {
+                    ok.store(false, std::memory_order_relaxed);
+                    break;
+                }
+            }
+        }
+    });
+    d.simple_symmetric = ok.load();
     return upload(ctx, &d.scol, scol.data(), scol.size());
 }
 
@@ -526,15 +545,34 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     if (d.app == PTW_APP_NODE2VEC) {
         if (int rc = ensure_sorted_adjacency(ctx)) return rc;
         p.scol = ctx->dev.scol;
-        // integer acceptance thresholds: floor(2^32 w / max(1/p, 1, 1/q)), in [2^-32.., 2^32]
-        const double wr = 1.0 / d.p, wo = 1.0 / d.q, M = std::max(std::max(wr, 1.0), wo);
+        // Integer weights floor(2^32 w / M).  Unfolded, M = max(1/p, 1, 1/q)
+        // and every weight is an acceptance threshold in [0, 2^32].  Folded
+        // (1/p above both other weights on a simple symmetric graph), M =
+        // max(1, 1/q) and the return weight's excess over 2^32 is split over
+        // ceil(excess / 2^32) extra proposal slots.
+        const double wr = 1.0 / d.p, wo = 1.0 / d.q;
+        const double Mf = std::max(1.0, wo);
+        const uint64_t t_ret = (uint64_t)std::floor(wr / Mf * 4294967296.0);  // <= 2^48
+        const bool fold = ctx->dev.simple_symmetric && wr > Mf && t_ret > 4294967296ull &&
+                          ctx->max_degree < (1u << 31);
+        const double M = fold ? Mf : std::max(wr, Mf);
         auto thr = [M](double w) {
             const double t = std::floor(w / M * 4294967296.0);
             return t >= 4294967296.0 ? 4294967296ull : (uint64_t)t;
         };
-        p.w_ret = thr(wr);
         p.w_in = thr(1.0);
         p.w_out = thr(wo);
+        if (fold) {
+            const uint64_t excess = t_ret - 4294967296ull;
+            const uint64_t slots = (excess + 4294967295ull) >> 32;  // in [1, 2^16]
+            p.n2v_fold = (uint32_t)slots;
+            p.w_fold = excess / slots;  // <= 2^32
+            p.w_ret = 4294967296ull + slots * p.w_fold;
+        } else {
+            p.n2v_fold = 0;
+            p.w_fold = 0;
+            p.w_ret = thr(wr);
+        }
     }
 
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));

@@ -37,8 +37,11 @@ struct DeviceGraph {
     // solver, built on first use
     uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
     // each vertex's neighbours sorted by device id (node2vec membership
-    // tests), built on first use
+    // tests), built on first use, and whether the graph is simple and
+    // symmetric: no row repeats a neighbour and every edge has its reverse,
+    // so a walker's previous vertex is in its current row exactly once
     uint32_t *scol = nullptr;
+    bool simple_symmetric = false;
     uint64_t pieces = 0;
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
     bool relabelled = false;
@@ -98,10 +101,14 @@ struct WalkLaunch {
     uint32_t seed_lo, seed_hi, stream;
     uint64_t wid_base;
     uint64_t walkers;
-    // node2vec: sorted adjacency and the acceptance thresholds (in [0, 2^32])
-    // of a neighbour that is the previous vertex, a neighbour of it, or neither
+    // node2vec: sorted adjacency and the integer weights of a neighbour that
+    // is the previous vertex, a neighbour of it, or neither.  w_in and w_out
+    // are acceptance thresholds in [0, 2^32].  Unfolded, w_ret is one too.
+    // Folded (n2v_fold > 0 extra proposal slots, each accepting the previous
+    // vertex below w_fold), w_ret = 2^32 + n2v_fold * w_fold.
     const uint32_t *scol;
-    uint64_t w_ret, w_in, w_out;
+    uint64_t w_ret, w_in, w_out, w_fold;
+    uint32_t n2v_fold;
     // outputs (device)
     unsigned long long *counts_dev;  // device-id counters (count apps)
     uint32_t *ends;                  // graph ids

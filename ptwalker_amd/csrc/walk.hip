@@ -57,12 +57,21 @@ __device__ __forceinline__ uint64_t n2v_weight(uint32_t x, uint32_t prev, const 
     return sorted_has(pn, pdeg, x) ? w_in : w_out;
 }
 
+__device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
+                              uint32_t deg, uint32_t prev, const uint32_t *pn, uint32_t pdeg);
+
 // The next vertex from the row [beg, beg + deg) of the current vertex.
-// Attempt a proposes a uniform neighbour from Philox(wid, step + a * 2^20)
-// and accepts it iff r.z < its weight (weights are scaled so the largest is
-// 2^32); attempt 0's block is `r`, the step's own.  After kN2vAttempts
-// rejections one exact draw from the same integer weights follows, so the
-// law is exactly w(x) / sum w whichever attempt decides.
+// Attempt a draws Philox(wid, step + a * 2^20) (attempt 0's block is `r`, the
+// step's own), proposes a uniform slot and accepts it iff r.z < its weight.
+// Unfolded, the slots are the deg neighbours and the weights are scaled so
+// the largest is 2^32.  Folded (the return weight 1/p is the largest and the
+// graph is simple and symmetric, so prev sits in the row exactly once), the
+// envelope is the larger of the other two weights; the return edge keeps
+// 2^32 in its own slot and its excess rides on n2v_fold extra slots, each
+// accepting prev below w_fold (KnightKing's outlier folding: Yang et al.,
+// SOSP 2019).  A p < 1 walk then needs about half the proposals.  After
+// kN2vAttempts rejections one exact draw from the same integer weights
+// follows, so the law is exactly w(x) / sum w whichever attempt decides.
 __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, u32x4 r,
                              uint32_t beg, uint32_t deg, uint32_t prev)
 {
@@ -72,8 +81,25 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
     // The same decisions with fewer searches: a coin below every weight
     // accepts whatever x is, a coin at or above both non-return weights
     // rejects any x but prev; only coins in between need x's class.
-    const uint64_t w_min = p.w_ret < p.w_in ? (p.w_ret < p.w_out ? p.w_ret : p.w_out) : (p.w_in < p.w_out ? p.w_in : p.w_out);
     const uint64_t w_io = p.w_in > p.w_out ? p.w_in : p.w_out;
+    if (p.n2v_fold) {
+        const uint64_t w_min = p.w_in < p.w_out ? p.w_in : p.w_out;
+        for (uint32_t a = 0; a < kN2vAttempts; ++a) {
+            if (a) r = philox4x32_10(wlo, whi, step + (a << 20), p.stream, p.seed_lo, p.seed_hi);
+            const uint32_t s = mulhi64(r.x, r.y, deg + p.n2v_fold);
+            const uint64_t z = r.z;
+            if (s >= deg) {  // an extra slot of the return edge
+                if (z < p.w_fold) return prev;
+                continue;
+            }
+            const uint32_t x = p.col[beg + s];
+            if (x == prev || z < w_min) return x;  // prev's own slot weighs 2^32
+            if (z >= w_io) continue;
+            if (z < (sorted_has(pn, pdeg, x) ? p.w_in : p.w_out)) return x;
+        }
+        return n2v_exact(p, wlo, whi, step, beg, deg, prev, pn, pdeg);
+    }
+    const uint64_t w_min = p.w_ret < p.w_in ? (p.w_ret < p.w_out ? p.w_ret : p.w_out) : (p.w_in < p.w_out ? p.w_in : p.w_out);
     for (uint32_t a = 0; a < kN2vAttempts; ++a) {
         if (a) r = philox4x32_10(wlo, whi, step + (a << 20), p.stream, p.seed_lo, p.seed_hi);
         const uint32_t x = p.col[beg + mulhi64(r.x, r.y, deg)];
@@ -86,9 +112,17 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
         if (z >= w_io) continue;
         if (z < (sorted_has(pn, pdeg, x) ? p.w_in : p.w_out)) return x;
     }
-    uint64_t total = 0;  // < deg * 2^32 < 2^64
+    return n2v_exact(p, wlo, whi, step, beg, deg, prev, pn, pdeg);
+}
+
+// The exact draw after kN2vAttempts rejections: a pick from the whole row by
+// cumulative integer weight (the return edge at its full w_ret).
+__device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
+                              uint32_t deg, uint32_t prev, const uint32_t *pn, uint32_t pdeg)
+{
+    uint64_t total = 0;  // < deg * 2^32 + w_ret < 2^64
     for (uint32_t k = 0; k < deg; ++k) total += n2v_weight(p.col[beg + k], prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);
-    r = philox4x32_10(wlo, whi, step + (kN2vAttempts << 20), p.stream, p.seed_lo, p.seed_hi);
+    const u32x4 r = philox4x32_10(wlo, whi, step + (kN2vAttempts << 20), p.stream, p.seed_lo, p.seed_hi);
     const uint64_t pick = __umul64hi(((uint64_t)r.y << 32) | r.x, total);  // floor(u * total / 2^64)
     uint64_t cum = 0;
     for (uint32_t k = 0; k < deg; ++k) {

@@ -1,7 +1,10 @@
 """Node2vec on the device: paths bit-exact against the oracle's replay for
 several (p, q), start rules, seeds, streams and walker-id offsets, with and
-without the degree relabel, on a graph where the exact fallback decides most
-steps, and equal to DeepWalk when p = q = 1; the CLI binary end to end."""
+without the degree relabel, with the return edge folded (p < 1 on simple
+symmetric graphs) and unfolded (graphs with a one-way edge or a repeated
+neighbour), on a graph where the exact fallback decides a fifth of the steps
+at its centre, and equal to DeepWalk when p = q = 1; the CLI binary end to
+end."""
 import os
 import subprocess
 
@@ -10,7 +13,7 @@ import pytest
 
 import oracle
 import ptwalker_amd as ptw
-from test_node2vec_oracle import star_ring
+from test_node2vec_oracle import star_ring, with_chord
 from walkstats import random_graph
 
 pytestmark = pytest.mark.gpu
@@ -37,7 +40,8 @@ def rmat12(ctx):
     return gi, rp, col
 
 
-@pytest.mark.parametrize("p,q", [(1.0, 1.0), (0.5, 2.0), (2.0, 0.5), (1.0 / 64, 64.0), (64.0, 1.0 / 64)])
+@pytest.mark.parametrize("p,q", [(1.0, 1.0), (0.5, 2.0), (2.0, 0.5), (1.0 / 64, 64.0), (64.0, 1.0 / 64),
+                                 (0.3, 1.0), (1.0 / 3000, 0.5)])
 def test_parity_rmat12(ctx, rmat12, p, q):
     gi, rp, col = rmat12
     check(ctx, rp, col, 20000, 20, p, q, seed=3)
@@ -58,10 +62,27 @@ def test_equals_deepwalk_when_p_q_one(ctx, rmat12):
 
 
 def test_fallback_heavy_and_no_relabel(ctx):
-    rp, col = star_ring(40)
-    for flags in (0, ptw.GRAPH_NO_RELABEL):
-        ctx.from_csr(rp, col, flags=flags)
-        check(ctx, rp, col, 20000, 30, 1.0 / 64, 64.0, seed=3)
+    # the symmetric star ring folds the return edge; one directed chord keeps
+    # it unfolded, and then the exact fallback decides a fifth of the centre's
+    # steps
+    for rp, col in (star_ring(40), with_chord(*star_ring(40), 5, 25)):
+        for flags in (0, ptw.GRAPH_NO_RELABEL):
+            ctx.from_csr(rp, col, flags=flags)
+            check(ctx, rp, col, 20000, 30, 1.0 / 64, 64.0, seed=3)
+
+
+def test_repeated_neighbour_is_unfolded(ctx):
+    # a row that lists a neighbour twice holds prev twice on the way back:
+    # the graph is not simple, so the return edge is not folded
+    rp, col = random_graph(40, 0.15, seed=8)
+    rows = [list(col[rp[v]:rp[v + 1]]) for v in range(len(rp) - 1)]
+    rows[3].append(rows[3][0])
+    rp2 = np.zeros(len(rp), dtype=np.uint32)
+    rp2[1:] = np.cumsum([len(r) for r in rows])
+    col2 = np.array([y for r in rows for y in r], dtype=np.uint32)
+    for g in ((rp, col), (rp2, col2)):
+        ctx.from_csr(*g)
+        check(ctx, *g, 8000, 20, 0.25, 2.0, seed=5)
 
 
 def test_dangling_vertices(ctx):

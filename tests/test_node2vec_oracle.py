@@ -88,19 +88,54 @@ def test_transition_law_random_graph(p, q):
     check_law(rp, col, r["paths"], p, q)
 
 
+def with_chord(rp, col, u, x):
+    """The graph plus one directed edge u -> x (no reverse): not symmetric."""
+    rows = [list(col[rp[v]:rp[v + 1]]) for v in range(len(rp) - 1)]
+    rows[u].append(x)
+    rp2 = np.zeros(len(rp), dtype=np.uint32)
+    rp2[1:] = np.cumsum([len(r) for r in rows])
+    return rp2, np.array([y for r in rows for y in r], dtype=np.uint32)
+
+
 def test_transition_law_where_the_fallback_decides():
-    # p = 1/64, q = 64: from a leaf into the centre the acceptance per proposal
-    # is (64 + 2 + 37/64) / (64 * 40) ~ 2.6 %, so ~19 % of those steps end in
-    # the exact draw after 64 rejections
+    # p = 1/64, q = 64 on a graph that is not symmetric (one directed chord),
+    # so the return edge is not folded: from a leaf into the centre the
+    # acceptance per proposal is (64 + 2 + 37/64) / (64 * 40) ~ 2.6 %, and
+    # ~19 % of those steps end in the exact draw after 64 rejections
+    p, q = 1.0 / 64, 64.0
+    rp, col = with_chord(*star_ring(40), 5, 25)
+    r = oracle.node2vec_replay(rp, col, 20000, 30, p, q, seed=3)
+    check_law(rp, col, r["paths"], p, q, min_count=200)
+    t = transitions(r["paths"])
+    centre_steps = sum(sum(d.values()) for (prev, cur), d in t.items() if cur == 0)
+    assert 0.15 < r["exact_draws"] / centre_steps < 0.22
+    # the centre sends walkers back to where they came from most of the time
+    back = sum(d.get(prev, 0) for (prev, cur), d in t.items() if cur == 0)
+    assert back / centre_steps > 0.9
+
+
+def test_transition_law_with_the_return_edge_folded():
+    # the same p, q on the symmetric star ring: 1/p is the largest weight, so
+    # prev keeps weight 1 in its own slot and its other 63 ride on 63 extra
+    # slots; from a leaf into the centre a proposal is accepted with
+    # probability (64 + 2 + 37/64) / (40 + 63) ~ 65 %, and the exact draw
+    # (after 64 rejections) practically never decides
     p, q = 1.0 / 64, 64.0
     rp, col = star_ring(40)
     r = oracle.node2vec_replay(rp, col, 20000, 30, p, q, seed=3)
     check_law(rp, col, r["paths"], p, q, min_count=200)
-    t = transitions(r["paths"])
-    # the centre sends walkers back to where they came from most of the time
-    back = sum(d.get(prev, 0) for (prev, cur), d in t.items() if cur == 0)
-    total = sum(sum(d.values()) for (prev, cur), d in t.items() if cur == 0)
-    assert back / total > 0.9
+    assert r["exact_draws"] == 0
+    unfolded = oracle.node2vec_replay(*with_chord(rp, col, 5, 25), 20000, 30, p, q, seed=3)
+    assert r["proposals"] < unfolded["proposals"] / 5
+
+
+@pytest.mark.parametrize("p,q", [(0.5, 2.0), (0.3, 1.0), (1.0 / 3000, 0.5)])
+def test_folded_law_random_graph(p, q):
+    # 1/p above max(1, 1/q): folded with 1, 1 (0.3: 2^32 * 2.33 excess over 3
+    # slots) and 3000 / 2 extra slots
+    rp, col = random_graph(10, 0.4, seed=11)
+    r = oracle.node2vec_replay(rp, col, 40000, 20, p, q, seed=2)
+    check_law(rp, col, r["paths"], p, q)
 
 
 def test_threads_do_not_change_paths():
