@@ -32,7 +32,7 @@ void free_graph(ptw_ctx *ctx)
     if (d.inv) (void)hipFree(d.inv);
     if (d.wedge) (void)hipFree(d.wedge);
     if (d.t_col) (void)hipFree(d.t_col);
-    if (d.scol) (void)hipFree(d.scol);
+    if (d.eset) (void)hipFree(d.eset);
     if (d.out_deg) (void)hipFree(d.out_deg);
     if (d.piece_begin) (void)hipFree(d.piece_begin);
     if (d.vertex_piece) (void)hipFree(d.vertex_piece);
@@ -123,10 +123,13 @@ int warm_up(ptw_ctx *ctx)
 
 // Each vertex's neighbours sorted by device id, for the node2vec membership
 // test (is x a neighbour of the previous vertex); built once per graph.
-int ensure_sorted_adjacency(ptw_ctx *ctx)
+// The node2vec tables, built on first use: whether the graph is simple and
+// symmetric (checked on the host over rows sorted by device id) and the
+// device edge set that answers "is x a neighbour of prev".
+int ensure_node2vec_tables(ptw_ctx *ctx)
 {
     ptw::DeviceGraph &d = ctx->dev;
-    if (d.scol) return PTW_OK;
+    if (d.eset) return PTW_OK;
     const HostCsr &h = ctx->host;
     const uint32_t V = d.V;
     std::vector<uint32_t> scol(h.E());
@@ -158,7 +161,25 @@ int ensure_sorted_adjacency(ptw_ctx *ctx)
         }
     });
     d.simple_symmetric = ok.load();
-    return upload(ctx, &d.scol, scol.data(), scol.size());
+    std::vector<uint32_t>().swap(scol);
+    // open addressing at most half full: the smallest power of two >= 2E slots
+    const uint64_t E = h.E();
+    uint64_t slots = 2;
+    while (slots < 2 * E) slots <<= 1;
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } t;
+    HIP_TRY(ctx, hipMalloc(&t.p, slots * sizeof(unsigned long long)));
+    unsigned long long *table = (unsigned long long *)t.p;
+    HIP_TRY(ctx, ptw::launch_fill_u64(table, slots, ptw::kEdgeSetEmpty, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_edge_set_build(d.row_ptr, d.col, V, E, table, slots - 1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    t.p = nullptr;
+    d.eset = table;
+    d.eset_mask = slots - 1;
+    d.bytes += slots * sizeof(unsigned long long);
+    return PTW_OK;
 }
 
 int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
@@ -543,8 +564,9 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     p.paths = paths_dev;
     p.scratch = ctx->scratch;
     if (d.app == PTW_APP_NODE2VEC) {
-        if (int rc = ensure_sorted_adjacency(ctx)) return rc;
-        p.scol = ctx->dev.scol;
+        if (int rc = ensure_node2vec_tables(ctx)) return rc;
+        p.eset = ctx->dev.eset;
+        p.eset_mask = ctx->dev.eset_mask;
         // Integer weights floor(2^32 w / M).  Unfolded, M = max(1/p, 1, 1/q)
         // and every weight is an acceptance threshold in [0, 2^32].  Folded
         // (1/p above both other weights on a simple symmetric graph), M =

@@ -36,11 +36,13 @@ struct DeviceGraph {
     // transpose (in-edges, cut into pieces) and out-degrees for the FP64
     // solver, built on first use
     uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
-    // each vertex's neighbours sorted by device id (node2vec membership
-    // tests), built on first use, and whether the graph is simple and
-    // symmetric: no row repeats a neighbour and every edge has its reverse,
-    // so a walker's previous vertex is in its current row exactly once
-    uint32_t *scol = nullptr;
+    // node2vec, built on first use: the set of edges (u << 32 | x, device
+    // ids) in an open-addressing table of eset_mask + 1 >= 2E slots, and
+    // whether the graph is simple and symmetric (no row repeats a neighbour,
+    // every edge has its reverse), so a walker's previous vertex is in its
+    // current row exactly once
+    unsigned long long *eset = nullptr;
+    uint64_t eset_mask = 0;
     bool simple_symmetric = false;
     uint64_t pieces = 0;
     uint32_t hubs = 0;            // device ids [0, hubs) are the top-degree vertices
@@ -106,7 +108,8 @@ struct WalkLaunch {
     // are acceptance thresholds in [0, 2^32].  Unfolded, w_ret is one too.
     // Folded (n2v_fold > 0 extra proposal slots, each accepting the previous
     // vertex below w_fold), w_ret = 2^32 + n2v_fold * w_fold.
-    const uint32_t *scol;
+    const unsigned long long *eset;  // edge set: is x a neighbour of prev
+    uint64_t eset_mask;
     uint64_t w_ret, w_in, w_out, w_fold;
     uint32_t n2v_fold;
     // outputs (device)
@@ -129,6 +132,11 @@ struct LaunchShape {
 };
 
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
+// Inserts every CSR edge into the edge set (table of mask + 1 slots, filled
+// with kEdgeSetEmpty beforehand, at least twice the edges).
+constexpr unsigned long long kEdgeSetEmpty = ~0ull;
+hipError_t launch_edge_set_build(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
+                                 unsigned long long *table, uint64_t mask, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s);
 

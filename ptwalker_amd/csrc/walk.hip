@@ -35,30 +35,42 @@ enum Kind { KIND_COUNT = 0, KIND_END = 1, KIND_PATH = 2, KIND_N2V = 3 };
 
 constexpr uint32_t kN2vAttempts = 64;  // proposals before the exact draw
 
-__device__ __forceinline__ bool sorted_has(const uint32_t *a, uint32_t n, uint32_t x)
+// Slot of an edge key u << 32 | x: MurmurHash3's 64-bit finalizer (fmix64,
+// A. Appleby), which spreads the clustered keys of one row over the table.
+__host__ __device__ __forceinline__ uint64_t edge_hash(uint64_t k)
 {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo < n && a[lo] == x;
+    k ^= k >> 33;
+    k *= 0xFF51AFD7ED558CCDull;
+    k ^= k >> 33;
+    k *= 0xC4CEB9FE1A85EC53ull;
+    k ^= k >> 33;
+    return k;
 }
 
-// Integer weight of moving to x after coming from prev, whose neighbours are
-// pn[0, pdeg) sorted: the return, in and out thresholds.
-__device__ __forceinline__ uint64_t n2v_weight(uint32_t x, uint32_t prev, const uint32_t *pn, uint32_t pdeg,
-                                               uint64_t w_ret, uint64_t w_in, uint64_t w_out)
+// Is x a neighbour of u: linear probing from the key's slot until the key or
+// an empty slot (the table is at most half full, so one exists).  About one
+// random line per test, where a binary search in u's sorted row takes
+// log2(deg u) dependent reads.
+__device__ __forceinline__ bool edge_has(const unsigned long long *T, uint64_t mask, uint32_t u, uint32_t x)
 {
-    if (x == prev) return w_ret;
-    return sorted_has(pn, pdeg, x) ? w_in : w_out;
+    const unsigned long long key = ((unsigned long long)u << 32) | x;
+    for (uint64_t h = edge_hash(key) & mask;; h = (h + 1) & mask) {
+        const unsigned long long k = T[h];
+        if (k == key) return true;
+        if (k == kEdgeSetEmpty) return false;
+    }
+}
+
+// Integer weight of moving to x after coming from prev: the return, in and
+// out weights.
+__device__ __forceinline__ uint64_t n2v_weight(const WalkLaunch &p, uint32_t x, uint32_t prev)
+{
+    if (x == prev) return p.w_ret;
+    return edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out;
 }
 
 __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
-                              uint32_t deg, uint32_t prev, const uint32_t *pn, uint32_t pdeg);
+                              uint32_t deg, uint32_t prev);
 
 // The next vertex from the row [beg, beg + deg) of the current vertex.
 // Attempt a draws Philox(wid, step + a * 2^20) (attempt 0's block is `r`, the
@@ -75,9 +87,6 @@ __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, u
 __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, u32x4 r,
                              uint32_t beg, uint32_t deg, uint32_t prev)
 {
-    const uint32_t pb = p.row_ptr[prev];
-    const uint32_t pdeg = p.row_ptr[prev + 1] - pb;
-    const uint32_t *pn = p.scol + pb;
     // The same decisions with fewer searches: a coin below every weight
     // accepts whatever x is, a coin at or above both non-return weights
     // rejects any x but prev; only coins in between need x's class.
@@ -95,9 +104,9 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
             const uint32_t x = p.col[beg + s];
             if (x == prev || z < w_min) return x;  // prev's own slot weighs 2^32
             if (z >= w_io) continue;
-            if (z < (sorted_has(pn, pdeg, x) ? p.w_in : p.w_out)) return x;
+            if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return x;
         }
-        return n2v_exact(p, wlo, whi, step, beg, deg, prev, pn, pdeg);
+        return n2v_exact(p, wlo, whi, step, beg, deg, prev);
     }
     const uint64_t w_min = p.w_ret < p.w_in ? (p.w_ret < p.w_out ? p.w_ret : p.w_out) : (p.w_in < p.w_out ? p.w_in : p.w_out);
     for (uint32_t a = 0; a < kN2vAttempts; ++a) {
@@ -110,27 +119,51 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
             continue;
         }
         if (z >= w_io) continue;
-        if (z < (sorted_has(pn, pdeg, x) ? p.w_in : p.w_out)) return x;
+        if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return x;
     }
-    return n2v_exact(p, wlo, whi, step, beg, deg, prev, pn, pdeg);
+    return n2v_exact(p, wlo, whi, step, beg, deg, prev);
 }
 
 // The exact draw after kN2vAttempts rejections: a pick from the whole row by
 // cumulative integer weight (the return edge at its full w_ret).
 __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
-                              uint32_t deg, uint32_t prev, const uint32_t *pn, uint32_t pdeg)
+                              uint32_t deg, uint32_t prev)
 {
     uint64_t total = 0;  // < deg * 2^32 + w_ret < 2^64
-    for (uint32_t k = 0; k < deg; ++k) total += n2v_weight(p.col[beg + k], prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);
+    for (uint32_t k = 0; k < deg; ++k) total += n2v_weight(p, p.col[beg + k], prev);
     const u32x4 r = philox4x32_10(wlo, whi, step + (kN2vAttempts << 20), p.stream, p.seed_lo, p.seed_hi);
     const uint64_t pick = __umul64hi(((uint64_t)r.y << 32) | r.x, total);  // floor(u * total / 2^64)
     uint64_t cum = 0;
     for (uint32_t k = 0; k < deg; ++k) {
         const uint32_t x = p.col[beg + k];
-        cum += n2v_weight(x, prev, pn, pdeg, p.w_ret, p.w_in, p.w_out);
+        cum += n2v_weight(p, x, prev);
         if (pick < cum) return x;
     }
     return p.col[beg + deg - 1];  // not reached: pick < total
+}
+
+// One thread per edge: its source is the last row starting at or before it
+// (row_ptr[lo] <= e < row_ptr[lo + 1]), then a compare-and-swap insert
+// (a repeated edge finds its own key and stops).
+__global__ void edge_set_build_kernel(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
+                                      unsigned long long *T, uint64_t mask)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        uint32_t lo = 0, hi = V;  // row_ptr[lo] <= e < row_ptr[hi] = E
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (row_ptr[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const unsigned long long key = ((unsigned long long)lo << 32) | col[e];
+        for (uint64_t h = edge_hash(key) & mask;; h = (h + 1) & mask) {
+            const unsigned long long old = atomicCAS(&T[h], kEdgeSetEmpty, key);
+            if (old == kEdgeSetEmpty || old == key) break;
+        }
+    }
 }
 
 __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
@@ -476,6 +509,17 @@ hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_u64_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_set_build(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
+                                 unsigned long long *table, uint64_t mask, hipStream_t s)
+{
+    if (E == 0) return hipSuccess;
+    uint64_t blocks = (E + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(edge_set_build_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, row_ptr, col, V, E, table,
+                       mask);
     return hipGetLastError();
 }
 
