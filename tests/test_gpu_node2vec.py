@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 import ptwalker_amd as ptw
-from test_node2vec_oracle import star_ring, with_chord
+from test_node2vec_oracle import star_ring, with_chord, with_self_loop
 from walkstats import random_graph
 
 pytestmark = pytest.mark.gpu
@@ -69,6 +69,12 @@ def test_fallback_heavy_and_no_relabel(ctx):
         for flags in (0, ptw.GRAPH_NO_RELABEL):
             ctx.from_csr(rp, col, flags=flags)
             check(ctx, rp, col, 20000, 30, 1.0 / 64, 64.0, seed=3)
+
+
+def test_self_loop_folded(ctx):
+    rp, col = with_self_loop(*star_ring(12), 0)
+    ctx.from_csr(rp, col)
+    check(ctx, rp, col, 20000, 20, 0.25, 2.0, seed=6)
 
 
 def test_repeated_neighbour_is_unfolded(ctx):

@@ -129,6 +129,25 @@ def test_transition_law_with_the_return_edge_folded():
     assert r["proposals"] < unfolded["proposals"] / 5
 
 
+def with_self_loop(rp, col, u):
+    """The graph plus the edge u -> u: still simple and symmetric."""
+    rows = [list(col[rp[v]:rp[v + 1]]) for v in range(len(rp) - 1)]
+    rows[u].insert(1, u)
+    rp2 = np.zeros(len(rp), dtype=np.uint32)
+    rp2[1:] = np.cumsum([len(r) for r in rows])
+    return rp2, np.array([y for r in rows for y in r], dtype=np.uint32)
+
+
+def test_folded_law_with_a_self_loop():
+    # a walker that takes the centre's loop has prev == cur; the loop is in
+    # the centre's row once, so the return edge is still folded
+    p, q = 0.25, 2.0
+    rp, col = with_self_loop(*star_ring(12), 0)
+    r = oracle.node2vec_replay(rp, col, 40000, 20, p, q, seed=6)
+    check_law(rp, col, r["paths"], p, q)
+    assert any((row[i] == 0 and row[i + 1] == 0) for row in r["paths"][:2000] for i in range(19))
+
+
 @pytest.mark.parametrize("p,q", [(0.5, 2.0), (0.3, 1.0), (1.0 / 3000, 0.5)])
 def test_folded_law_random_graph(p, q):
     # 1/p above max(1, 1/q): folded with 1, 1 (0.3: 2^32 * 2.33 excess over 3
