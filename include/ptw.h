@@ -58,10 +58,16 @@ extern "C" {
  * reference declares the app (src/core/node2vec.cpp, flags -p -q in
  * include/option.hpp:447-482) but its sampler is not wired in
  * (src/core/solver.hpp:155-159).  Decisions: up to 64 proposals, attempt a
- * drawing Philox(seed; wid, step + a * 2^20, stream), j = mulhi64(r.x, r.y,
- * deg), accepted iff r.z < floor(2^32 w(x) / max(1/p, 1, 1/q)); then one
- * exact draw from those integer weights (64-bit r.y:r.x, attempt 64).  With
- * p = q = 1 the paths are DEEPWALK's. */
+ * drawing Philox(seed; wid, step + a * 2^20, stream), slot j = mulhi64(r.x,
+ * r.y, deg + k), accepted iff r.z < the slot's weight; then one exact draw
+ * from the integer weights (64-bit r.y:r.x, attempt 64).  Weights are
+ * floor(2^32 w(x) / M), M = max(1/p, 1, 1/q), and k = 0.  On a graph where
+ * no row repeats a neighbour and every edge has its reverse, with 1/p above
+ * max(1, 1/q): M = max(1, 1/q), prev's own slot accepts always, and its
+ * excess e = floor(2^32 / (p M)) - 2^32 rides on k = ceil(e / 2^32) extra
+ * slots accepting prev below floor(e / k).  With p = q = 1 the paths are
+ * DEEPWALK's.  The first node2vec walk on a graph builds an edge set of
+ * 8 B x (the smallest power of two >= 2E) on the device. */
 #define PTW_APP_NODE2VEC 4
 
 /* step-budget semantics for the probabilistic-termination apps */
