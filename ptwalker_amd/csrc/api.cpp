@@ -121,8 +121,6 @@ int warm_up(ptw_ctx *ctx)
     return PTW_OK;
 }
 
-// Each vertex's neighbours sorted by device id, for the node2vec membership
-// test (is x a neighbour of the previous vertex); built once per graph.
 // The node2vec tables, built on first use: whether the graph is simple and
 // symmetric (checked on the host over rows sorted by device id) and the
 // device edge set that answers "is x a neighbour of prev".
