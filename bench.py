@@ -865,17 +865,21 @@ def cpu_baseline(ctx, hub, args):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    probe = 200_000
-    t = time.perf_counter()
-    r = oracle.ref_walk(rp, col, oracle.APP_PPR, probe, tp=args.tp, source=hub, start_uniform=False, seed=1,
-                        threads=cores)
-    dt = time.perf_counter() - t
-    rate = r["steps"] / max(dt, 1e-9)
-    n = int(min(5e8, max(probe, rate * args.cpu_seconds * args.tp)))
-    t = time.perf_counter()
-    r = oracle.ref_walk(rp, col, oracle.APP_PPR, n, tp=args.tp, source=hub, start_uniform=False, seed=1,
-                        threads=cores)
-    dt = time.perf_counter() - t
+    def run(n):
+        t = time.perf_counter()
+        r = oracle.ref_walk(rp, col, oracle.APP_PPR, n, tp=args.tp, source=hub, start_uniform=False, seed=1,
+                            threads=cores)
+        return r, time.perf_counter() - t
+
+    # size the sample from a probe long enough to time (thread start-up
+    # dominates a short one), and once more if it still falls well short
+    probe = 2_000_000
+    r, dt = run(probe)
+    n = int(min(5e8, max(probe, r["steps"] / max(dt, 1e-9) * args.cpu_seconds * args.tp)))
+    r, dt = run(n)
+    if dt < 0.5 * args.cpu_seconds and n < 5e8:
+        n = int(min(5e8, n * args.cpu_seconds / max(dt, 1e-3)))
+        r, dt = run(n)
     return {
         "value": r["steps"] / dt,
         "unit": "steps/s",
