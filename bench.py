@@ -292,14 +292,19 @@ def cpu_baseline_deepwalk(ctx, L, args):
 
     rp, col = ctx.export_csr()
     cores = cpu_cores()
-    probe = 20_000
-    t = time.perf_counter()
-    r = oracle.ref_walk(rp, col, oracle.APP_DEEPWALK, probe, walk_len=L, seed=1, threads=cores)
-    rate = r["steps"] / max(time.perf_counter() - t, 1e-9)
-    n = int(min(2e8, max(probe, rate * args.cpu_seconds / max(L - 1, 1))))
-    t = time.perf_counter()
-    r = oracle.ref_walk(rp, col, oracle.APP_DEEPWALK, n, walk_len=L, seed=1, threads=cores)
-    dt = time.perf_counter() - t
+    def run(n):
+        t = time.perf_counter()
+        r = oracle.ref_walk(rp, col, oracle.APP_DEEPWALK, n, walk_len=L, seed=1, threads=cores)
+        return r, time.perf_counter() - t
+
+    # a probe long enough to time, and one resize if the sample falls short
+    probe = 200_000
+    r, dt = run(probe)
+    n = int(min(2e8, max(probe, r["steps"] / max(dt, 1e-9) * args.cpu_seconds / max(L - 1, 1))))
+    r, dt = run(n)
+    if dt < 0.5 * args.cpu_seconds and n < 2e8:
+        n = int(min(2e8, n * args.cpu_seconds / max(dt, 1e-3)))
+        r, dt = run(n)
     return {"value": r["steps"] / dt, "unit": "steps/s", "cores": cores, "kind": "port",
             "sample": f"{n} unweighted walks of {L} vertices from uniform starts ({r['steps']} steps, {dt:.1f} s), "
                       f"xorshift* reference CPU path without path stores, {cores} threads"}
