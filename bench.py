@@ -63,8 +63,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(walkers shard by id range with one counter all-reduce; see the design notes)", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine runs on a single device "
+              "(walkers would shard by id range with one counter all-reduce; not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "deepwalk":
         return bench_deepwalk(args)
