@@ -182,6 +182,17 @@ def test_hub_counter_spill(ctx):
     check(ctx, rp, col, ptw.PageRank(0.2), 4_000_000, seed=4)
 
 
+def test_high_degree_source_shape(ctx):
+    """A source of degree 3000 (above 4 x 512 cache slots) walks with the
+    visit cache's LDS given to hub counters; 3e6 walkers keep the launch off
+    the small-walk shape.  A leaf source keeps the cache.  Counts equal the
+    replay either way."""
+    rp, col = star(3000)
+    ctx.from_csr(rp, col)
+    check(ctx, rp, col, ptw.PPR(0.2, 0), 3_000_000, seed=7)
+    check(ctx, rp, col, ptw.PPR(0.2, 17), 3_000_000, seed=8)
+
+
 def test_device_outputs_and_accumulate(ctx):
     gi = ctx.rmat(scale=12, edge_factor=8, seed=3)
     rp, col = ctx.export_csr()
