@@ -598,13 +598,16 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    // A source with more neighbours than four times the visit cache spreads
-    // its first-step visits too thin for the cache (about two per workgroup
-    // per neighbour at 1e8 walkers from a 1.6e5-degree hub): on an automatic
-    // shape the cache's LDS goes to hub counters instead (4 per slot).
+    // The visit cache pays only for the hot neighbourhood of a single source.
+    // Uniform starts spread their visits, and a source with more neighbours
+    // than four times the cache spreads its first-step visits too thin (about
+    // two per workgroup per neighbour at 1e8 walkers from a 1.6e5-degree
+    // hub): on an automatic shape the cache's LDS then goes to hub counters
+    // (4 per slot).
     ptw::LaunchShape shape = ctx->shape;
-    if (count_app && d.start_mode == PTW_START_SOURCE && shape.blocks_per_cu == 0 && shape.cache_slots > 0 &&
-        (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] > 4ull * shape.cache_slots) {
+    if (count_app && shape.blocks_per_cu == 0 && shape.cache_slots > 0 &&
+        (d.start_mode != PTW_START_SOURCE ||
+         (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] > 4ull * shape.cache_slots)) {
         shape.lds_hubs = std::min(81408, shape.lds_hubs + 4 * shape.cache_slots);
         shape.cache_slots = 0;
     }
