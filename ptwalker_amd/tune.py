@@ -16,7 +16,7 @@ Knobs:
           counters), cache_slots (LDS visit cache)
   batches batch_chunk (walker ids per queue entry of a hash query),
           dense_chunk (walker ids per workgroup chunk of a dense query; 0 =
-          the planner's per-query rule), block, lds_hubs, dense_threshold
+          the planner's per-query rule), block, lds_hubs, cache_slots, dense_threshold
           (walkers above which a query gets a dense counter slice rather than
           a hash slice), topk_tile (LDS candidate slots of the top-k kernel)
 
@@ -46,7 +46,8 @@ BATCH_SPACE: Dict[str, List[int]] = {
     "batch_chunk": [2048, 4096, 8192, 16384, 32768, 65536],
     "dense_chunk": [0, 16384, 32768, 49152],
     "block": [256, 512, 1024],
-    "lds_hubs": [36864, 55296, 77824],
+    "lds_hubs": [36864, 55296, 77824, 79872],
+    "cache_slots": [0, 512],
     "dense_threshold": [12_500, 25_000, 50_000, 100_000, 200_000],
     "topk_tile": [256, 512, 1024, 2048, 4096],
 }
