@@ -103,6 +103,8 @@ def lib():
         L.orc_ref_step_histogram.restype = None
         L.orc_xorshift_gen.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _P]
         L.orc_xorshift_gen.restype = None
+        L.orc_xorshift_gen_float.argtypes = [ctypes.c_uint64, ctypes.c_float, ctypes.c_uint64, _P]
+        L.orc_xorshift_gen_float.restype = None
         L.orc_power_iteration.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, _P]
         L.orc_power_iteration.restype = ctypes.c_int
@@ -255,6 +257,12 @@ def ref_step_histogram(tp: float, n: int, seed: int = 1, hist_len: int = 512) ->
 def xorshift_gen(state: int, bound: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     lib().orc_xorshift_gen(state, bound, n, out.ctypes.data)
+    return out
+
+
+def xorshift_gen_float(state: int, upper: float, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.float32)
+    lib().orc_xorshift_gen_float(state, upper, n, out.ctypes.data)
     return out
 
 

@@ -469,6 +469,13 @@ void orc_xorshift_gen(uint64_t state, uint32_t bound, uint64_t n, uint32_t *out)
     for (uint64_t i = 0; i < n; ++i) out[i] = xs_gen(&g, bound);
 }
 
+void orc_xorshift_gen_float(uint64_t state, float upper, uint64_t n, float *out)
+{
+    xorshift_t g;
+    g.s = state;
+    for (uint64_t i = 0; i < n; ++i) out[i] = xs_gen_float(&g, upper);
+}
+
 int orc_ref_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
                  int app, uint32_t source, int start_uniform, double tp, uint32_t walk_len,
                  uint64_t walkers, uint64_t seed, int threads,

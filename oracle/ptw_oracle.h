@@ -145,6 +145,7 @@ void orc_ref_step_histogram(double tp, uint64_t seed, uint64_t n, uint64_t *hist
 /* xorshift* generator (include/random.hpp:161-176) exposed for tests:
  * writes n outputs of gen(bound) starting from `state`. */
 void orc_xorshift_gen(uint64_t state, uint32_t bound, uint64_t n, uint32_t *out);
+void orc_xorshift_gen_float(uint64_t state, float upper, uint64_t n, float *out);
 
 /* FP64 oracle.  mode 0: expected visits per walker (count apps), mode 1:
  * expected endpoint distribution (RWR).  start: source vertex, or

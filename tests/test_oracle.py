@@ -255,12 +255,106 @@ def test_reference_step_draw_distribution(tp):
     assert abs(mean - (1 / tp + 0.0015)) < 0.002  # quantization bias of the 16-bit draw
 
 
-def test_xorshift_zero_state_is_fixed_point_and_uniform():
-    assert np.all(oracle.xorshift_gen(0, 1000, 100) == 0)
-    x = oracle.xorshift_gen(88172645463325252, 10, 100000)
-    obs = np.bincount(x, minlength=10)
-    chi2 = np.sum((obs - 10000) ** 2 / 10000)
-    assert chi2 < 9 + 6 * np.sqrt(18)
+# ---- the reference's xorshift* generator (include/random.hpp:144-180) ------------------
+# Vigna's xorshift64* (ACM TOMS 42(4), 2016): shift triple (12, 25, 27) right/left/right,
+# output multiplier 2685821657736338717.  The reference returns the low 32 bits of
+# s * M taken from the state *before* the update, reduced by `% bound`.
+_XS_M = 2685821657736338717
+_MASK64 = (1 << 64) - 1
+# 2^64 - 1 = 3 * 5 * 17 * 257 * 641 * 65537 * 6700417 (Fermat-number factors)
+_PERIOD_PRIMES = (3, 5, 17, 257, 641, 65537, 6700417)
+
+
+def _gf2_apply(cols, x):
+    r = 0
+    j = 0
+    while x:
+        if x & 1:
+            r ^= cols[j]
+        x >>= 1
+        j += 1
+    return r
+
+
+def _gf2_mul(a, b):
+    return [_gf2_apply(a, c) for c in b]
+
+
+def _gf2_pow(a, e):
+    r = [1 << j for j in range(64)]
+    while e:
+        if e & 1:
+            r = _gf2_mul(r, a)
+        a = _gf2_mul(a, a)
+        e >>= 1
+    return r
+
+
+def _xorshift_matrix():
+    """The transition as a product of three GF(2) matrices (I + S), S a shift."""
+    ident = [1 << j for j in range(64)]
+    right = lambda k: [c ^ (c >> k) for c in ident]
+    left = lambda k: [c ^ ((c << k) & _MASK64) for c in ident]
+    return _gf2_mul(right(27), _gf2_mul(left(25), right(12)))
+
+
+def test_xorshift_constants():
+    assert _XS_M == 0x2545F4914F6CDD1D
+    prod = 1
+    for p in _PERIOD_PRIMES:
+        prod *= p
+    assert prod == _MASK64
+
+
+def test_xorshift_transition_has_full_period():
+    """(12, 25, 27) is a full-period triple: the transition's order is 2^64 - 1.
+    A mistyped triple (or shift direction) fails this for almost every choice."""
+    T = _xorshift_matrix()
+    ident = [1 << j for j in range(64)]
+    assert _gf2_pow(T, _MASK64) == ident
+    for p in _PERIOD_PRIMES:
+        assert _gf2_pow(T, _MASK64 // p) != ident, p
+
+
+def test_xorshift_outputs_come_from_the_pre_update_state():
+    """The low 32 bits of s * M determine the low 32 bits of s (M is odd), so each
+    output of gen(2^32 - 1) reveals the state it was computed from.  Output k must
+    come from T^k(s0): the first from s0 itself (output before update), the
+    following ones from the GF(2) matrix powers of the transition."""
+    T = _xorshift_matrix()
+    inv_m = pow(_XS_M & 0xFFFFFFFF, -1, 1 << 32)
+    for s0 in (88172645463325252, 0x0123456789ABCDEF, 1, 1 << 63):
+        n = 40
+        out = oracle.xorshift_gen(s0, 0xFFFFFFFF, n).astype(np.uint64)
+        s = s0
+        for k in range(n):
+            if (s * _XS_M) & 0xFFFFFFFF != 0xFFFFFFFF:  # the one value that `% bound` folds
+                assert (int(out[k]) * inv_m) & 0xFFFFFFFF == s & 0xFFFFFFFF, (s0, k)
+            s = _gf2_apply(T, s)
+        # the canonical xorshift64* emits after the update: its first output differs
+        assert (int(out[0]) * inv_m) & 0xFFFFFFFF != _gf2_apply(T, s0) & 0xFFFFFFFF
+
+
+def test_xorshift_bound_is_a_plain_remainder():
+    s0 = 0x9E3779B97F4A7C15
+    for bound in (7, 1000, 65536, 0x7FFFFFFF):
+        full = oracle.xorshift_gen(s0, 0xFFFFFFFF, 16).astype(np.uint64)
+        got = oracle.xorshift_gen(s0, bound, 16).astype(np.uint64)
+        assert np.array_equal(got, full % np.uint64(bound))
+
+
+def test_xorshift_gen_float_range_ends():
+    """gen_float is (low 16 bits of s * M) / 65535: numerator 0xFFFF gives exactly
+    1.0 (which the geometric table's total can never reach) and 0 gives 0.0."""
+    inv16 = pow(_XS_M & 0xFFFF, -1, 1 << 16)
+    T = _xorshift_matrix()
+    for numer, want in ((0xFFFF, 1.0), (0, 0.0), (0x7FFF, np.float32(0x7FFF) / np.float32(65535))):
+        s = (0xABCD << 48) | ((numer * inv16) & 0xFFFF)
+        f = oracle.xorshift_gen_float(s, 1.0, 2)
+        assert f[0] == np.float32(want), (numer, f[0])
+        n1 = (_gf2_apply(T, s) * _XS_M) & 0xFFFF  # the second draw comes from T(s)
+        assert f[1] == np.float32(n1) / np.float32(65535)
+    assert np.all(oracle.xorshift_gen(0, 1000, 100) == 0)  # 0 is the one fixed point
 
 
 def test_reference_path_agrees_with_fp64():
