@@ -108,7 +108,8 @@ typedef struct ptw_graph_info {
     uint64_t num_edges;
     uint32_t max_degree_vertex; /* graph id; lowest id among ties */
     uint32_t num_dangling;      /* vertices with out-degree 0 */
-    uint32_t hub_vertices;      /* device ids below this count in LDS hub counters */
+    uint32_t hub_vertices;      /* configured LDS hub counters (min(V, lds_hubs)); the count a
+                                   walk used is in ptw_walk_out.launch_lds_hubs */
     uint32_t relabelled;        /* 1 if the device holds a degree relabel */
     uint64_t device_bytes;      /* HBM held by the resident graph */
     uint32_t weighted;          /* 1 if edge weights and alias tables are resident */
@@ -145,6 +146,14 @@ typedef struct ptw_walk_out {
     uint64_t dangling; /* walkers stopped at a degree-0 vertex */
     double kernel_ms;  /* device time of the walk kernel (HIP events) */
     double total_ms;   /* wall time of the call */
+    /* the launch the walk ran with, after the automatic rules of
+     * ptw_tuning (blocks_per_cu = 0, cache_slots = PTW_CACHE_AUTO) */
+    uint32_t launch_block;       /* threads per workgroup */
+    uint32_t launch_blocks;      /* workgroups */
+    uint32_t launch_chunk;       /* walker ids per queue take */
+    uint32_t launch_lds_hubs;    /* u16 LDS hub counters (count apps) */
+    uint32_t launch_cache_slots; /* LDS visit-cache slots (count apps) */
+    uint32_t launch_reserved;
 } ptw_walk_out;
 
 /* ---- context ---------------------------------------------------------- */
@@ -383,13 +392,23 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
  * (include/option.hpp:243-300) and the sampler cost catalogue of
  * src/core/mini_bmk.hpp:29-118; here the knobs are GPU launch parameters,
  * chosen by the autotuner (ptwalker_amd.tune). */
+/* cache_slots value that lets the library choose per launch: a count walk
+ * from one source of out-degree <= 2048 keeps a 512-slot visit cache; any
+ * other count walk gives the cache's 4 KiB to 2048 more hub counters (up to
+ * 81408); batched launches keep 512 slots.  ptw_walk_out reports the choice. */
+#define PTW_CACHE_AUTO (-1)
+
 typedef struct ptw_tuning {
     int32_t block;         /* threads per workgroup: multiple of 64, 64..1024 */
-    int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query */
+    int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query, and a walk
+                              under 8 walkers per resident lane then runs one workgroup of
+                              at most 512 threads per CU with at most 8192 hub counters and
+                              at most 1024 cache slots (none for uniform starts) */
     int32_t chunk;         /* walker ids a wavefront takes from the queue at once (>= 64) */
     int32_t lds_hubs;      /* u16 LDS counters for the top-degree vertices, 0..81408 */
-    int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 16384; 0 = off);
-                              2 B * lds_hubs + 8 B * cache_slots <= 159 KiB */
+    int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 16384; 0 = off;
+                              PTW_CACHE_AUTO, the default);
+                              2 B * lds_hubs + 8 B * cache_slots <= 159 KiB (auto: 512 slots) */
     int32_t batch_chunk;   /* walker ids per queue entry in batched launches (>= 64) */
     int32_t topk_tile;     /* LDS candidate slots of the top-k kernel (power of two, 256..4096) */
     int32_t dense_chunk;   /* walker ids per workgroup chunk of a dense batched query, a multiple

@@ -123,7 +123,21 @@ class WalkOut(ctypes.Structure):
         ("dangling", ctypes.c_uint64),
         ("kernel_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("launch_block", ctypes.c_uint32),
+        ("launch_blocks", ctypes.c_uint32),
+        ("launch_chunk", ctypes.c_uint32),
+        ("launch_lds_hubs", ctypes.c_uint32),
+        ("launch_cache_slots", ctypes.c_uint32),
+        ("launch_reserved", ctypes.c_uint32),
     ]
+
+    def launch_shape(self) -> dict:
+        """The launch the walk ran with (after the automatic rules)."""
+        return {"block": self.launch_block, "blocks": self.launch_blocks, "chunk": self.launch_chunk,
+                "lds_hubs": self.launch_lds_hubs, "cache_slots": self.launch_cache_slots}
+
+
+CACHE_AUTO = -1
 
 
 class Query(ctypes.Structure):
@@ -415,10 +429,24 @@ class Context:
 
     # ---- walks -----------------------------------------------------------
     def walk(self, desc: WalkDesc, counts=None, ends=None, paths=None, device_ptrs: bool = False) -> WalkOut:
+        """ptw_walk.  Host outputs must be C-contiguous numpy arrays: u64 counts of at
+        least V entries, u32 ends of at least `walkers`, u32 paths of at least
+        walkers * walk_len (the library writes that many).  With device_ptrs the
+        arguments are device addresses (ints) and are passed through unchecked."""
         out = WalkOut()
         if device_ptrs:
             out.counts, out.ends, out.paths = counts, ends, paths
         else:
+            V = self.info().num_vertices
+            need = {"counts": (counts, np.uint64, V),
+                    "ends": (ends, np.uint32, desc.walkers),
+                    "paths": (paths, np.uint32, desc.walkers * max(desc.walk_len, 1))}
+            for name, (a, dt, n) in need.items():
+                if a is None:
+                    continue
+                if not isinstance(a, np.ndarray) or a.dtype != dt or not a.flags.c_contiguous or a.size < n:
+                    raise ValueError(f"{name} must be a C-contiguous {np.dtype(dt).name} array of at least {n} "
+                                     f"entries")
             out.counts, out.ends, out.paths = _ptr(counts), _ptr(ends), _ptr(paths)
         self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
         return out
@@ -787,6 +815,7 @@ class WalkResult:
     dangling: int
     kernel_ms: float
     total_ms: float
+    shape: Optional[dict] = None  # the launch the walk ran with (WalkOut.launch_shape)
 
 
 def walk(ctx: Context, walker_num: int, app, seed: int = 1, wid_base: int = 0, stream: int = 0,
@@ -806,4 +835,5 @@ def walk(ctx: Context, walker_num: int, app, seed: int = 1, wid_base: int = 0, s
         ends = ends[:walker_num]
     if paths is not None:
         paths = paths[: walker_num * d.walk_len].reshape(walker_num, d.walk_len)
-    return WalkResult(counts, ends, paths, out.steps, out.visits, out.dangling, out.kernel_ms, out.total_ms)
+    return WalkResult(counts, ends, paths, out.steps, out.visits, out.dangling, out.kernel_ms, out.total_ms,
+                      out.launch_shape())

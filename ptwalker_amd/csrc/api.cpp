@@ -488,6 +488,8 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
     out->steps = out->visits = out->dangling = 0;
     out->kernel_ms = 0.0;
+    out->launch_block = out->launch_blocks = out->launch_chunk = 0;
+    out->launch_lds_hubs = out->launch_cache_slots = out->launch_reserved = 0;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
 
@@ -598,20 +600,26 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    // The visit cache pays only for the hot neighbourhood of a single source.
-    // Uniform starts spread their visits, and a source with more neighbours
-    // than four times the cache spreads its first-step visits too thin (about
-    // two per workgroup per neighbour at 1e8 walkers from a 1.6e5-degree
-    // hub): on an automatic shape the cache's LDS then goes to hub counters
-    // (4 per slot).
+    // cache_slots = PTW_CACHE_AUTO: the visit cache pays only for the hot
+    // neighbourhood of a single source.  Uniform starts spread their visits,
+    // and a source with more neighbours than four times the cache spreads its
+    // first-step visits too thin (about two per workgroup per neighbour at
+    // 1e8 walkers from a 1.6e5-degree hub), so there the cache's LDS goes to
+    // hub counters (4 per slot).  An explicit cache_slots is used as given.
     ptw::LaunchShape shape = ctx->shape;
-    if (count_app && shape.blocks_per_cu == 0 && shape.cache_slots > 0 &&
-        (d.start_mode != PTW_START_SOURCE ||
-         (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] > 4ull * shape.cache_slots)) {
-        shape.lds_hubs = std::min(81408, shape.lds_hubs + 4 * shape.cache_slots);
-        shape.cache_slots = 0;
+    if (shape.cache_slots < 0) {
+        const bool keep_cache = d.start_mode == PTW_START_SOURCE &&
+                                (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] <=
+                                    4ull * ptw::kAutoCacheSlots;
+        if (count_app && !keep_cache) {
+            shape.lds_hubs = std::min(ptw::kMaxLdsHubs, shape.lds_hubs + 4 * ptw::kAutoCacheSlots);
+            shape.cache_slots = 0;
+        } else {
+            shape.cache_slots = ptw::kAutoCacheSlots;
+        }
     }
-    if (W) HIP_TRY(ctx, ptw::launch_walk(p, shape, ctx->num_cus, s));
+    ptw::EffectiveShape eff;
+    if (W) HIP_TRY(ctx, ptw::launch_walk(p, shape, ctx->num_cus, s, &eff));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     if (count_app) {
         const bool from_source = d.start_mode == PTW_START_SOURCE;
@@ -637,6 +645,12 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     out->dangling = sc[2];
     out->visits = count_app ? W + sc[1] : 0;
     out->total_ms = now_ms() - t0;
+    out->launch_block = (uint32_t)eff.block;
+    out->launch_blocks = (uint32_t)eff.blocks;
+    out->launch_chunk = eff.chunk;
+    out->launch_lds_hubs = eff.hubs;
+    out->launch_cache_slots = eff.slots;
+    out->launch_reserved = 0;
     if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
         return fail(ctx, PTW_ERR_DANGLING,
                     std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
@@ -665,12 +679,14 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
     if (t->blocks_per_cu < 0 || t->blocks_per_cu > 32)
         return fail(ctx, PTW_ERR_INVALID_ARG, "blocks_per_cu must be in [0, 32]");
     if (t->chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "chunk must be >= 64");
-    if (t->lds_hubs < 0 || t->lds_hubs > 81408) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 81408]");
-    if (t->cache_slots < 0 || t->cache_slots > 16384 || (t->cache_slots & (t->cache_slots - 1)))
-        return fail(ctx, PTW_ERR_INVALID_ARG, "cache_slots must be 0 or a power of two <= 16384");
+    if (t->lds_hubs < 0 || t->lds_hubs > ptw::kMaxLdsHubs) return fail(ctx, PTW_ERR_INVALID_ARG, "lds_hubs must be in [0, 81408]");
+    if (t->cache_slots != PTW_CACHE_AUTO &&
+        (t->cache_slots < 0 || t->cache_slots > 16384 || (t->cache_slots & (t->cache_slots - 1))))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "cache_slots must be PTW_CACHE_AUTO, 0 or a power of two <= 16384");
     // u16 hub counters and the u32 cache share one workgroup's LDS (160 KiB per
-    // CU, 1 KiB kept for the rest)
-    if ((((int64_t)t->lds_hubs + 1) / 2 + 2 * (int64_t)t->cache_slots) * 4 > 159 * 1024)
+    // CU, 1 KiB kept for the rest); the automatic setting may use 512 slots
+    const int64_t slots = t->cache_slots == PTW_CACHE_AUTO ? ptw::kAutoCacheSlots : t->cache_slots;
+    if ((((int64_t)t->lds_hubs + 1) / 2 + 2 * slots) * 4 > 159 * 1024)
         return fail(ctx, PTW_ERR_INVALID_ARG, "2 B per lds_hubs + 8 B per cache slot must fit in 159 KiB");
     if (t->batch_chunk < 64) return fail(ctx, PTW_ERR_INVALID_ARG, "batch_chunk must be >= 64");
     if (t->topk_tile < 256 || t->topk_tile > 4096 || (t->topk_tile & (t->topk_tile - 1)))

@@ -243,7 +243,8 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
 int enqueue_run(ptw_ctx *ctx, const BatchLaunch &p)
 {
     HIP_TRY(ctx, launch_batch_begin(p, ctx->batch.q_cap, ctx->stream));
-    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->shape.lds_hubs, ctx->shape.cache_slots,
+    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->shape.lds_hubs,
+                                   ctx->shape.cache_slots < 0 ? ptw::kAutoCacheSlots : ctx->shape.cache_slots,
                                    ctx->stream));
     HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, ctx->batch.topk_v, ctx->batch.topk_c, ctx->shape.topk_tile,
                                    ctx->num_cus, ctx->stream));

@@ -125,11 +125,22 @@ struct LaunchShape {
     int blocks_per_cu = 0;   // 0: fill the device from the occupancy query
     int chunk = 1024;        // walker ids a wave takes from the queue at once
     int lds_hubs = 77824;    // u16 LDS counters for device ids below this (count apps)
-    int cache_slots = 512;   // LDS visit-cache slots per workgroup (power of two, 0 = off)
+    int cache_slots = -1;    // LDS visit-cache slots per workgroup (power of two, 0 = off, -1 = automatic)
     int batch_chunk = 16384;  // walker ids per chunk of a batched launch
     int topk_tile = 2048;    // top-k LDS candidate slots
     int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)
 };
+
+// The launch a walk actually ran with, after the automatic rules.
+struct EffectiveShape {
+    int block = 0, blocks = 0;
+    uint32_t chunk = 0, hubs = 0, slots = 0;
+};
+
+// Visit-cache slots of the automatic setting (cache_slots = -1) where a cache
+// is kept, and the hub counters given the cache's LDS where it is not.
+constexpr int kAutoCacheSlots = 512;
+constexpr int kMaxLdsHubs = 81408;
 
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 // Inserts every CSR edge into the edge set (table of mask + 1 slots, filled
@@ -138,7 +149,10 @@ constexpr unsigned long long kEdgeSetEmpty = ~0ull;
 hipError_t launch_edge_set_build(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
                                  unsigned long long *table, uint64_t mask, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
-hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s);
+// `shape.cache_slots` must be resolved (>= 0) by the caller; `eff`, if given,
+// receives the launch after the small-walk rule of blocks_per_cu = 0.
+hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s,
+                       EffectiveShape *eff = nullptr);
 
 // Graph ingest on the device with the host loader's semantics (ingest.hip).
 int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, HostCsr &g,

@@ -532,7 +532,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s)
+hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s, EffectiveShape *eff)
 {
     const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
                      : p.app == PTW_APP_RWR                            ? KIND_END
@@ -542,7 +542,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     uint32_t hubs = 0, slots = 0;
     if (kind == KIND_COUNT) {
         hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
-        slots = (uint32_t)shape.cache_slots;
+        slots = shape.cache_slots > 0 ? (uint32_t)shape.cache_slots : 0u;
     }
     size_t lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
@@ -573,6 +573,13 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     uint64_t fair = p.walkers / (waves * 4);
     fair = (fair + 63) / 64 * 64;
     const uint32_t chunk = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(fair, (uint64_t)shape.chunk));
+    if (eff) {
+        eff->block = block;
+        eff->blocks = blocks;
+        eff->chunk = chunk;
+        eff->hubs = hubs;
+        eff->slots = slots;
+    }
     if (kind == KIND_COUNT)
         hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);
     else if (kind == KIND_END)

@@ -39,15 +39,16 @@ WALK_SPACE: Dict[str, List[int]] = {
     "chunk": [1024, 2048, 4096, 8192, 16384],
     "block": [256, 512, 1024],
     "lds_hubs": [16384, 36864, 55296, 77824],
-    "cache_slots": [0, 512, 1024, 2048],
+    "cache_slots": [-1, 0, 512, 1024, 2048],  # -1: the library's per-walk choice
 }
 
 BATCH_SPACE: Dict[str, List[int]] = {
     "batch_chunk": [2048, 4096, 8192, 16384, 32768, 65536],
     "dense_chunk": [0, 16384, 32768, 49152],
     "block": [256, 512, 1024],
-    "lds_hubs": [36864, 55296, 77824, 79872],
+    # the cache before the hub counts: 79872 hubs fit the LDS only without a cache
     "cache_slots": [0, 512],
+    "lds_hubs": [36864, 55296, 77824, 79872],
     "dense_threshold": [12_500, 25_000, 50_000, 100_000, 200_000],
     "topk_tile": [256, 512, 1024, 2048, 4096],
 }
@@ -59,6 +60,9 @@ class TuneResult:
     best_ms: float
     start_ms: float
     trials: List[dict] = field(default_factory=list)
+    # walks: the launch the best setting ran with, after the library's automatic
+    # rules (blocks_per_cu = 0, cache_slots = -1)
+    effective: Optional[Dict[str, int]] = None
 
     @property
     def speedup(self) -> float:
@@ -216,9 +220,10 @@ def tune_walk(ctx, app, walkers: int, *, seed: int = 1, repeats: int = 3, rounds
 
     res = coordinate_descent(measure, start, space, rounds=rounds)
     ctx.set_tuning(**res.best)
+    res.effective = walk(ctx, walkers, app, seed=seed).shape
     if cache_path:
         _store(cache_path, f"{graph_signature(ctx)}|walk-{type(app).__name__}-{walkers}",
-               {"best": res.best, "best_ms": res.best_ms, "start_ms": res.start_ms})
+               {"best": res.best, "best_ms": res.best_ms, "start_ms": res.start_ms, "effective": res.effective})
     return res
 
 

@@ -184,3 +184,25 @@ def test_ingest_flags_checked(host_ctx, tmp_path):
         host_ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_DEVICE)
     gi = host_ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_HOST)
     assert gi.num_vertices == 3 and gi.num_edges == 2
+
+
+def test_walk_host_outputs_checked_before_the_call(host_ctx):
+    """Context.walk rejects host outputs the library would overrun (wrong dtype,
+    too short, not contiguous) before it calls ptw_walk."""
+    gi = host_ctx.rmat(scale=8, edge_factor=4, seed=1)
+    V = gi.num_vertices
+    d = ptw.make_desc(ptw.PPR(0.2, 0), 100)
+    bad_counts = [np.zeros(V, dtype=np.uint32), np.zeros(V - 1, dtype=np.uint64),
+                  np.zeros(2 * V, dtype=np.uint64)[::2], [0] * V]
+    for counts in bad_counts:
+        with pytest.raises(ValueError):
+            host_ctx.walk(d, counts=counts)
+    d = ptw.make_desc(ptw.RWR(0.2, 0), 100)
+    with pytest.raises(ValueError):
+        host_ctx.walk(d, ends=np.zeros(99, dtype=np.uint32))
+    d = ptw.make_desc(ptw.DeepWalk(5), 100)
+    with pytest.raises(ValueError):
+        host_ctx.walk(d, paths=np.zeros(499, dtype=np.uint32))
+    # well-formed outputs reach the library, which has no device here
+    with pytest.raises(ptw.PtwError):
+        host_ctx.walk(d, paths=np.zeros(500, dtype=np.uint32))

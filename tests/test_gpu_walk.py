@@ -182,15 +182,33 @@ def test_hub_counter_spill(ctx):
     check(ctx, rp, col, ptw.PageRank(0.2), 4_000_000, seed=4)
 
 
-def test_high_degree_source_shape(ctx):
-    """A source of degree 3000 (above 4 x 512 cache slots) walks with the
-    visit cache's LDS given to hub counters; 3e6 walkers keep the launch off
-    the small-walk shape.  A leaf source keeps the cache.  Counts equal the
-    replay either way."""
-    rp, col = star(3000)
-    ctx.from_csr(rp, col)
-    check(ctx, rp, col, ptw.PPR(0.2, 0), 3_000_000, seed=7)
-    check(ctx, rp, col, ptw.PPR(0.2, 17), 3_000_000, seed=8)
+def test_automatic_shape_rmat17(ctx):
+    """The default (automatic) launch shape on a graph with more vertices than
+    any LDS hub range (R-MAT scale 17: 90,202 vertices), so device ids above the
+    hub counters reach the visit cache or device atomics.  A source of degree
+    above 2048 and uniform starts give the cache's LDS to hub counters (79,872,
+    no cache); a degree-1 source keeps 77,824 hubs and 512 cache slots.  3e6
+    walkers keep the launch off the small-walk shape.  Counts equal the replay."""
+    gi = ctx.rmat(scale=17, edge_factor=16, seed=1)
+    assert gi.num_vertices > 81408
+    rp, col = ctx.export_csr()
+    deg = np.diff(rp.astype(np.int64))
+    low = int(np.argmin(deg))
+    assert ctx.tuning().cache_slots == ptw.CACHE_AUTO and ctx.tuning().blocks_per_cu == 0
+    got = check(ctx, rp, col, ptw.PPR(0.2, int(gi.max_degree_vertex)), 3_000_000, seed=7)
+    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (79872, 0), got.shape
+    got = check(ctx, rp, col, ptw.PPR(0.2, low), 3_000_000, seed=8)
+    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (77824, 512), got.shape
+    got = check(ctx, rp, col, ptw.PageRank(0.15), 3_000_000, seed=9)
+    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (79872, 0), got.shape
+    # an explicit setting is used as given
+    default = ctx.tuning().as_dict()
+    try:
+        ctx.set_tuning(cache_slots=1024, lds_hubs=36864)
+        got = check(ctx, rp, col, ptw.PPR(0.2, int(gi.max_degree_vertex)), 3_000_000, seed=10)
+        assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (36864, 1024), got.shape
+    finally:
+        ctx.set_tuning(**default)
 
 
 def test_device_outputs_and_accumulate(ctx):
