@@ -88,10 +88,10 @@ def main():
     t0 = time.time()
     gi = ctx.rmat(scale=args.scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1)
     build_s = time.time() - t0
-    rng = np.random.default_rng(1)
-    hub = int(gi.max_degree_vertex)
-    rnd = int(rng.integers(0, gi.num_vertices))
-    sources = [hub, rnd]
+    from ptwalker_amd.workloads import walk_sources
+
+    sources = walk_sources(gi, seed=1)
+    hub = sources[0]
     V = gi.num_vertices
     counts = ctx.alloc(8 * V)
     ends = ctx.alloc(4 * W)
@@ -311,16 +311,9 @@ def cpu_baseline_deepwalk(ctx, L, args):
 
 
 def batch_queries(gi, rp, n, seed=1):
-    """Sources drawn with probability proportional to degree, walker budgets
-    log-uniform in [1e3, 1e6], termination probabilities uniform in [0.15, 0.3]."""
-    import numpy as np
+    from ptwalker_amd.workloads import batch_queries as bq
 
-    rng = np.random.default_rng(seed)
-    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
-    src = rng.choice(gi.num_vertices, size=n, p=deg / deg.sum())
-    W = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n)).astype(np.int64)
-    tp = rng.uniform(0.15, 0.3, n)
-    return src, W, tp
+    return bq(gi, rp, n, seed)
 
 
 def bench_batch(args):
@@ -458,17 +451,9 @@ def cpu_baseline_batch(ctx, src, W, tp, args):
 
 
 def serve_requests(rp, n, rate, seed=1, eps_choices=(1e-3, 3e-4)):
-    """Half degree-weighted, half uniform sources; eps uniform over `eps_choices`;
-    Poisson arrivals at `rate` queries/s."""
-    import numpy as np
+    from ptwalker_amd.workloads import serve_requests as sr
 
-    rng = np.random.default_rng(seed)
-    deg = np.diff(rp.astype(np.int64)).astype(np.float64)
-    src = np.concatenate([rng.choice(len(deg), n // 2, p=deg / deg.sum()), rng.integers(0, len(deg), n - n // 2)])
-    rng.shuffle(src)
-    eps = rng.choice(list(eps_choices), n)
-    arr = np.cumsum(rng.exponential(1.0 / rate, n))
-    return src, eps, arr
+    return sr(rp, n, rate, seed, eps_choices)
 
 
 def score_served(ctx, res, src, eps, tp, seed, sample, k=100):

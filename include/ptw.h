@@ -140,6 +140,9 @@ typedef struct ptw_walk_out {
     uint64_t *counts; /* PAGERANK/PPR: u64[V] visit counts, or NULL */
     uint32_t *ends;   /* RWR: u32[walkers] end vertex per walker, or NULL */
     uint32_t *paths;  /* DEEPWALK, NODE2VEC: u32[walkers * walk_len], or NULL */
+    uint32_t *starts; /* RWR, optional: u32[walkers] start vertex per walker, so (starts[w],
+                         ends[w]) is the reference's {start_v, end_v} pair (src/core/RWR.cpp:9-13,
+                         76-89, 99-105); NULL for the other apps */
     /* filled on return */
     uint64_t steps;    /* sum of transitions over walkers */
     uint64_t visits;   /* count increments (count apps) */
@@ -205,6 +208,14 @@ int ptw_graph_export_weights(ptw_ctx *ctx, float *weights, uint32_t *alias_thr, 
 
 int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info);
 
+/* The id each graph id had before compaction, names[V]: the file's own id
+ * for a loaded edge list (the inverse of the first-appearance map, which the
+ * reference builds in src/core/graph.hpp:293-319 and leaves unexported,
+ * id2name "TBD" at :663-665), the generator's id for R-MAT, the CSR index for
+ * a CSR handed in.  Maps every output (counts, ends, starts, paths, top-k)
+ * back to the input's ids. */
+int ptw_graph_export_names(ptw_ctx *ctx, uint32_t *names);
+
 /* Copy the resident graph's CSR in graph ids to host memory. */
 int ptw_graph_export_csr(ptw_ctx *ctx, uint32_t *row_ptr, uint32_t *col);
 
@@ -228,20 +239,29 @@ int ptw_graph_write(ptw_ctx *ctx, const char *path, int format);
 int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
 
 /* ---- batched queries -------------------------------------------------- */
-/* Many single-source PPR queries (visit counts, type 0) in one persistent
- * launch.  The reference runs one query per process (src/core/ppr.cpp:121-139,
- * one start vertex, src/core/ppr.cpp:54-57); batching is an addition.
- * Query i's walker w draws from Philox(seed; w, step, queries[i].stream), so
- * its counts equal ptw_walk(PPR, source, walkers, tp, stream) alone, in any
+/* Many single-source queries in one persistent launch: PPR queries count
+ * every visit (type 0, the start included), RWR queries count each walker's
+ * last vertex (type 1, src/core/RWR.cpp:99-105), so their counts are the
+ * endpoint histogram.  The reference runs one query per process
+ * (src/core/ppr.cpp:121-139, src/core/RWR.cpp:109-128, one start vertex,
+ * src/core/ppr.cpp:54-57); batching is an addition.  Query i's walker w
+ * draws from Philox(seed; w, step, queries[i].stream), so its counts equal
+ * those of ptw_walk(PPR or RWR, source, walkers, tp, stream) alone, in any
  * mix.  Per-query counters live in u32 slices: dense (V counters) for
  * walkers > dense_threshold, a hash table otherwise.  Walkers that reach a
  * vertex without out-edges stop there. */
+#define PTW_QUERY_VISITS 0    /* PPR: visit counts */
+#define PTW_QUERY_ENDPOINTS 1 /* RWR: end-vertex counts */
+
 typedef struct ptw_query {
     uint32_t source;  /* graph id */
     uint32_t stream;  /* 4th Philox counter word of this query */
-    uint64_t walkers; /* at most 2^32 / (1.5 (1 + 1/tp)); a query whose visit total
-                         reaches 2^32 (u32 counters may have wrapped) fails with PTW_ERR_OOM */
+    uint64_t walkers; /* at most 2^32 / (1.5 c), c = 1 + 1/tp (visits) or 1 (endpoints); a
+                         query whose count total reaches 2^32 (u32 counters may have
+                         wrapped) fails with PTW_ERR_OOM */
     double tp;
+    uint32_t kind;    /* PTW_QUERY_* */
+    uint32_t reserved;
 } ptw_query;
 
 #define PTW_BATCH_GRAPH 1u    /* run the device sequence as a captured hipGraph (replayed across batches) */
@@ -274,9 +294,21 @@ typedef struct ptw_batch_out {
 int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out);
 /* HBM the batch would need beyond what the context already holds (admission). */
 int ptw_batch_plan_bytes(ptw_ctx *ctx, const ptw_batch_desc *desc, uint64_t *bytes);
-/* Full visit-count vector (graph ids, host u64[V]) of query `index` of the
- * last batch. */
+/* Full count vector (graph ids, host u64[V]) of query `index` of the last
+ * batch: visits, or end vertices for an endpoint query. */
 int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts);
+/* Writes the top-k lists of a finished batch as text, one line per entry:
+ * "query source kind rank vertex count" (kind 0 visits, 1 endpoints; rank
+ * from 0; padding entries are left out), after a "#" header line.  Vertex
+ * ids are graph ids (PTW_IDS_GRAPH) or the input's own ids
+ * (PTW_IDS_ORIGINAL, see ptw_graph_export_names).  desc and out are the
+ * arguments of that batch's ptw_batch_ppr.  No counterpart in the reference,
+ * which prints one count per process (src/core/pagerank.cpp:110). */
+#define PTW_IDS_GRAPH 0
+#define PTW_IDS_ORIGINAL 1
+int ptw_batch_write_topk(ptw_ctx *ctx, const char *path, const ptw_batch_desc *desc, const ptw_batch_out *out,
+                         int id_space);
+
 /* Counts of n (query index, graph id) pairs of the last batch, on the device
  * in one pass: out[i] = count of vertices[i] in query query_index[i]. */
 int ptw_batch_gather(ptw_ctx *ctx, uint64_t n, const uint32_t *query_index, const uint32_t *vertices,

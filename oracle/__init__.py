@@ -358,10 +358,10 @@ def topk(counts, row_ptr, k):
     vertex id ascending; zero counts never listed.  Returns (vertices, counts)
     padded with (INVALID, 0) to length k."""
     counts = np.asarray(counts, dtype=np.uint64)
-    deg = np.diff(np.asarray(row_ptr, dtype=np.int64))
-    ids = np.arange(len(counts))
-    order = np.lexsort((ids, -deg, -counts.astype(np.float64)))
-    order = order[counts[order] > 0][:k]
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    ids = np.flatnonzero(counts)  # zero counts are never listed
+    deg = rp[ids + 1] - rp[ids]
+    order = ids[np.lexsort((ids, -deg, -counts[ids].astype(np.float64)))][:k]
     v = np.full(k, INVALID, dtype=np.uint32)
     c = np.zeros(k, dtype=np.uint64)
     v[: len(order)] = order

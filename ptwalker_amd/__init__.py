@@ -118,6 +118,7 @@ class WalkOut(ctypes.Structure):
         ("counts", ctypes.c_void_p),
         ("ends", ctypes.c_void_p),
         ("paths", ctypes.c_void_p),
+        ("starts", ctypes.c_void_p),
         ("steps", ctypes.c_uint64),
         ("visits", ctypes.c_uint64),
         ("dangling", ctypes.c_uint64),
@@ -146,13 +147,22 @@ class Query(ctypes.Structure):
         ("stream", ctypes.c_uint32),
         ("walkers", ctypes.c_uint64),
         ("tp", ctypes.c_double),
+        ("kind", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
+
+
+IDS_GRAPH = 0     # outputs in graph ids
+IDS_ORIGINAL = 1  # outputs in the input's own ids (ptw_graph_export_names)
+
+QUERY_VISITS = 0     # PPR: visit counts
+QUERY_ENDPOINTS = 1  # RWR: end-vertex counts
 
 
 # the same layout as a numpy record, so query tables are filled without a
 # Python loop
 QUERY_DTYPE = np.dtype({"names": [f for f, _ in Query._fields_],
-                        "formats": [np.uint32, np.uint32, np.uint64, np.float64],
+                        "formats": [np.uint32, np.uint32, np.uint64, np.float64, np.uint32, np.uint32],
                         "offsets": [getattr(Query, f).offset for f, _ in Query._fields_],
                         "itemsize": ctypes.sizeof(Query)})
 
@@ -285,12 +295,14 @@ _SIGNATURES = {
     "ptw_graph_export_weights": (ctypes.c_int, [_P, _P, _P, _P]),
     "ptw_graph_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
     "ptw_graph_export_csr": (ctypes.c_int, [_P, _P, _P]),
+    "ptw_graph_export_names": (ctypes.c_int, [_P, _P]),
     "ptw_graph_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "ptw_walk": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
     "ptw_batch_ppr": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchOut)]),
     "ptw_batch_plan_bytes": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_batch_counts": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "ptw_batch_gather": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, _P]),
+    "ptw_batch_write_topk": (ctypes.c_int, [_P, ctypes.c_char_p, _P, _P, ctypes.c_int]),
     "ptw_ppr_exact": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, _P,
                                      ctypes.c_uint32, _P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "ptw_mlp_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -409,6 +421,12 @@ class Context:
         self.check(self.lib.ptw_graph_export_csr(self.h, _ptr(row_ptr), _ptr(col)))
         return row_ptr, col[: gi.num_edges]
 
+    def export_names(self) -> np.ndarray:
+        """names[v] = the id graph id v had in the input (file id; R-MAT generator id)."""
+        names = np.empty(max(self.info().num_vertices, 1), dtype=np.uint32)
+        self.check(self.lib.ptw_graph_export_names(self.h, _ptr(names)))
+        return names[: self.info().num_vertices]
+
     def set_weights(self, w: np.ndarray):
         w = np.ascontiguousarray(w, dtype=np.float32)
         self.check(self.lib.ptw_graph_set_weights(self.h, _ptr(w), len(w)))
@@ -428,41 +446,45 @@ class Context:
         self.check(self.lib.ptw_graph_write(self.h, path.encode(), fmt))
 
     # ---- walks -----------------------------------------------------------
-    def walk(self, desc: WalkDesc, counts=None, ends=None, paths=None, device_ptrs: bool = False) -> WalkOut:
+    def walk(self, desc: WalkDesc, counts=None, ends=None, paths=None, device_ptrs: bool = False,
+             starts=None) -> WalkOut:
         """ptw_walk.  Host outputs must be C-contiguous numpy arrays: u64 counts of at
-        least V entries, u32 ends of at least `walkers`, u32 paths of at least
+        least V entries, u32 ends and starts of at least `walkers`, u32 paths of at least
         walkers * walk_len (the library writes that many).  With device_ptrs the
         arguments are device addresses (ints) and are passed through unchecked."""
         out = WalkOut()
         if device_ptrs:
-            out.counts, out.ends, out.paths = counts, ends, paths
+            out.counts, out.ends, out.paths, out.starts = counts, ends, paths, starts
         else:
             V = self.info().num_vertices
             need = {"counts": (counts, np.uint64, V),
                     "ends": (ends, np.uint32, desc.walkers),
-                    "paths": (paths, np.uint32, desc.walkers * max(desc.walk_len, 1))}
+                    "starts": (starts, np.uint32, desc.walkers),
+                    "paths": (paths, np.uint32, desc.walkers * desc.walk_len)}
             for name, (a, dt, n) in need.items():
                 if a is None:
                     continue
                 if not isinstance(a, np.ndarray) or a.dtype != dt or not a.flags.c_contiguous or a.size < n:
                     raise ValueError(f"{name} must be a C-contiguous {np.dtype(dt).name} array of at least {n} "
                                      f"entries")
-            out.counts, out.ends, out.paths = _ptr(counts), _ptr(ends), _ptr(paths)
+            out.counts, out.ends, out.paths, out.starts = _ptr(counts), _ptr(ends), _ptr(paths), _ptr(starts)
         self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
         return out
 
     # ---- batched queries -------------------------------------------------
     @staticmethod
-    def _batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold):
+    def _batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold, kinds=None):
         n = len(sources)
         qs = np.zeros(max(n, 1), dtype=QUERY_DTYPE)
         if n:
-            if not (len(walkers) == len(tps) == n and (streams is None or len(streams) == n)):
-                raise ValueError("sources, walkers, tps and streams must have one entry per query")
+            if not (len(walkers) == len(tps) == n and (streams is None or len(streams) == n)
+                    and (kinds is None or np.ndim(kinds) == 0 or len(kinds) == n)):
+                raise ValueError("sources, walkers, tps, streams and kinds must have one entry per query")
             qs["source"][:n] = np.asarray(sources, dtype=np.int64)
             qs["walkers"][:n] = np.asarray(walkers, dtype=np.int64)
             qs["tp"][:n] = np.asarray(tps, dtype=np.float64)
             qs["stream"][:n] = np.arange(n) if streams is None else np.asarray(streams, dtype=np.int64)
+            qs["kind"][:n] = QUERY_VISITS if kinds is None else np.asarray(kinds, dtype=np.int64)
         d = BatchDesc()
         d.queries = qs.ctypes.data
         d.num_queries = n
@@ -475,11 +497,13 @@ class Context:
 
     def batch_ppr(self, sources, walkers, tps, *, streams=None, topk: int = 100, seed: int = 1,
                   semantics: int = SEM_REFERENCE, graph: bool = False, weighted: bool = False,
-                  dense_threshold: int = 0) -> "BatchResult":
-        """Many single-source PPR queries in one launch (ptw_batch_ppr).  Query i uses Philox
-        stream `streams[i]` (default i)."""
+                  dense_threshold: int = 0, kinds=None) -> "BatchResult":
+        """Many single-source queries in one launch (ptw_batch_ppr).  Query i uses Philox
+        stream `streams[i]` (default i) and counts by `kinds[i]`: QUERY_VISITS (PPR, the
+        default) or QUERY_ENDPOINTS (RWR end vertices); `kinds` may be one value for all."""
         flags = (BATCH_GRAPH if graph else 0) | (BATCH_WEIGHTED if weighted else 0)
-        d, qs = self._batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold)
+        d, qs = self._batch_desc(sources, walkers, tps, streams, topk, seed, semantics, flags, dense_threshold,
+                                 kinds)
         n = len(sources)
         tv = np.zeros(max(n * topk, 1), dtype=np.uint32)
         tc = np.zeros(max(n * topk, 1), dtype=np.uint64)
@@ -492,6 +516,21 @@ class Context:
         return BatchResult(tv[: n * topk].reshape(n, topk), tc[: n * topk].reshape(n, topk), steps[:n],
                            out.total_steps, out.slice_bytes, out.dense_queries, out.hash_queries, out.kernel_ms,
                            out.total_ms, ent[:n])
+
+    def batch_write_topk(self, path: str, sources, walkers, tps, result: "BatchResult", *, streams=None,
+                         kinds=None, id_space: int = 1) -> None:
+        """Write a batch's top-k lists as text (ptw_batch_write_topk): one line per entry,
+        "query source kind rank vertex count", in the input's own ids (id_space 1,
+        IDS_ORIGINAL) or graph ids (0, IDS_GRAPH)."""
+        n, k = result.topk_vertex.shape
+        d, qs = self._batch_desc(sources, walkers, tps, streams, k, 1, SEM_REFERENCE, 0, 0, kinds)
+        tv = np.ascontiguousarray(result.topk_vertex, dtype=np.uint32)
+        tc = np.ascontiguousarray(result.topk_count, dtype=np.uint64)
+        out = BatchOut()
+        out.topk_vertex, out.topk_count = _ptr(tv), _ptr(tc)
+        self.check(self.lib.ptw_batch_write_topk(self.h, os.fsencode(path), ctypes.byref(d), ctypes.byref(out),
+                                                 id_space))
+        del qs
 
     def batch_gather(self, query_index, vertices) -> np.ndarray:
         """Counts of (query, graph id) pairs of the last batch."""
@@ -816,24 +855,30 @@ class WalkResult:
     kernel_ms: float
     total_ms: float
     shape: Optional[dict] = None  # the launch the walk ran with (WalkOut.launch_shape)
+    starts: Optional[np.ndarray] = None  # RWR with starts=True: start vertex per walker
 
 
 def walk(ctx: Context, walker_num: int, app, seed: int = 1, wid_base: int = 0, stream: int = 0,
-         dangling: int = DANGLING_ERROR) -> WalkResult:
-    """Run `walker_num` walkers of `app` on the context's graph (host outputs)."""
+         dangling: int = DANGLING_ERROR, starts: bool = False) -> WalkResult:
+    """Run `walker_num` walkers of `app` on the context's graph (host outputs).  With
+    `starts`, an RWR walk also returns each walker's start vertex."""
     gi = ctx.info()
     d = make_desc(app, walker_num, seed=seed, wid_base=wid_base, stream=stream, dangling=dangling)
-    counts = ends = paths = None
+    counts = ends = paths = st = None
     if d.app in (APP_PAGERANK, APP_PPR):
         counts = np.zeros(gi.num_vertices, dtype=np.uint64)
     elif d.app == APP_RWR:
         ends = np.zeros(max(walker_num, 1), dtype=np.uint32)
+        if starts:
+            st = np.zeros(max(walker_num, 1), dtype=np.uint32)
     else:
         paths = np.zeros(max(walker_num * d.walk_len, 1), dtype=np.uint32)
-    out = ctx.walk(d, counts=counts, ends=ends, paths=paths)
+    out = ctx.walk(d, counts=counts, ends=ends, paths=paths, starts=st)
+    if st is not None:
+        st = st[:walker_num]
     if ends is not None:
         ends = ends[:walker_num]
     if paths is not None:
         paths = paths[: walker_num * d.walk_len].reshape(walker_num, d.walk_len)
     return WalkResult(counts, ends, paths, out.steps, out.visits, out.dangling, out.kernel_ms, out.total_ms,
-                      out.launch_shape())
+                      out.launch_shape(), st)

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ptw.h"
@@ -58,6 +59,11 @@ struct Opts {
     int dangling = PTW_DANGLING_ERROR;
     bool undirected = false;
     std::string start = "";
+    // ids of --source, --queries and every output: the input's own (file) ids,
+    // or the compact graph ids the reference assigns on load
+    bool ids_original = true;
+    std::string queries;  // batch mode (PPR, RWR): one query per line
+    uint32_t topk = 100;
 };
 
 void usage(FILE *f, const char *prog, Parser parser)
@@ -94,6 +100,13 @@ void usage(FILE *f, const char *prog, Parser parser)
     fprintf(f, "      --semantics=[reference|textbook]  [optional] step budget 1+Geom(tp) or Geom(tp)\n");
     fprintf(f, "      --dangling=[error|stop]           [optional] walker at a vertex without out-edges\n");
     fprintf(f, "      --undirected                      [optional] add reversed edges on load\n");
+    fprintf(f, "      --out-ids=[original|graph]        [optional] ids of --source, --queries and outputs: the\n"
+               "                                        file's own (default) or the compact load order\n");
+    if (parser == Parser::RWPT || parser == Parser::WALK) {
+        fprintf(f, "      --queries=[path]                  [optional, PPR, RWR] batch mode: lines \"source [walkers\n"
+                   "                                        [tp]]\" (defaults -w, --sp); --out gets each query's top-k\n");
+        fprintf(f, "      --topk=[k]                        [optional] top-k per query in batch mode (default 100)\n");
+    }
 }
 
 [[noreturn]] void parse_error(const char *prog, Parser parser, const std::string &msg)
@@ -148,7 +161,8 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
     // short flags taking a value, long flags taking a value, long switches
     const char *short_val = parser == Parser::NODE2VEC ? "fgtsewlpq" : "fgtsewl";
     const std::vector<std::string> long_val = {"socket-mapping", "mem", "sp", "ws", "db", "seed", "source",
-                                               "device", "out", "semantics", "dangling", "start"};
+                                               "device", "out", "semantics", "dangling", "start", "out-ids",
+                                               "queries", "topk"};
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         std::string key, val;
@@ -226,6 +240,16 @@ Opts parse(const char *prog, Parser parser, int argc, char **argv)
         } else if (key == "start") {
             if (val != "uniform" && val != "roundrobin") bad();
             o.start = val;
+        } else if (key == "out-ids") {
+            if (val == "original") o.ids_original = true;
+            else if (val == "graph") o.ids_original = false;
+            else bad();
+        } else if (key == "queries") {
+            if (parser == Parser::NODE2VEC) parse_error(prog, parser, "Flag could not be matched: " + key);
+            o.queries = val;
+        } else if (key == "topk") {
+            if (!to_u64(val, u) || u == 0 || u > 1024) bad();
+            o.topk = (uint32_t)u;
         }
     }
     // validation in the order of the reference's parse() methods
@@ -271,9 +295,114 @@ const char *app_name(int app)
 
 }  // namespace
 
+namespace {
+
+// Graph id <-> the ids of the command line and the outputs.
+struct IdMap {
+    bool original = true;
+    std::vector<uint32_t> names;  // graph id -> file id
+    std::unordered_map<uint32_t, uint32_t> index;  // file id -> graph id, built on first lookup
+    uint32_t out(uint32_t v) const { return original && v != PTW_INVALID_VERTEX ? names[v] : v; }
+    // false if the id names no vertex of the graph
+    bool in(uint32_t id, uint32_t &v)
+    {
+        if (!original) {
+            v = id;
+            return id < names.size();
+        }
+        if (index.empty()) {
+            index.reserve(names.size());
+            for (uint32_t g = 0; g < names.size(); ++g) index.emplace(names[g], g);
+        }
+        auto it = index.find(id);
+        if (it == index.end()) return false;
+        v = it->second;
+        return true;
+    }
+};
+
+// Batch mode: every line of the query file is "source [walkers [tp]]" ('#'
+// starts a comment); all queries run in one batched launch and --out gets the
+// top-k of each (ptw_batch_write_topk).
+int run_batch(ptw_ctx *ctx, int app, const Opts &o, const ptw_graph_info &gi, IdMap &ids)
+{
+    FILE *qf = fopen(o.queries.c_str(), "r");
+    if (!qf) {
+        fprintf(stderr, "[error] cannot open %s\n", o.queries.c_str());
+        return 1;
+    }
+    std::vector<ptw_query> qs;
+    char line[512];
+    unsigned long lineno = 0;
+    const double tp_default = o.have_sp ? o.sp : (app == PTW_APP_RWR ? 0.5 : 0.15);
+    while (fgets(line, sizeof line, qf)) {
+        ++lineno;
+        char *c = line;
+        while (*c == ' ' || *c == '\t') ++c;
+        if (*c == '#' || *c == '\n' || *c == '\r' || *c == 0) continue;
+        unsigned long long src = 0, walkers = o.walkers;
+        double tp = tp_default;
+        const int got = sscanf(c, "%llu %llu %lf", &src, &walkers, &tp);
+        uint32_t v = 0;
+        if (got < 1 || src > 0xFFFFFFFFull || !ids.in((uint32_t)src, v)) {
+            fprintf(stderr, "[error] %s:%lu: no vertex %llu in the graph\n", o.queries.c_str(), lineno, src);
+            fclose(qf);
+            return 1;
+        }
+        ptw_query q;
+        memset(&q, 0, sizeof(q));
+        q.source = v;
+        q.stream = (uint32_t)qs.size();
+        q.walkers = walkers;
+        q.tp = tp;
+        q.kind = app == PTW_APP_RWR ? PTW_QUERY_ENDPOINTS : PTW_QUERY_VISITS;
+        qs.push_back(q);
+    }
+    fclose(qf);
+    ptw_batch_desc bd;
+    memset(&bd, 0, sizeof(bd));
+    bd.queries = qs.data();
+    bd.num_queries = (uint32_t)qs.size();
+    bd.topk = o.topk;
+    bd.seed = o.seed;
+    bd.semantics = o.semantics;
+    std::vector<uint32_t> tv((size_t)qs.size() * o.topk + 1);
+    std::vector<uint64_t> tc(tv.size()), steps(qs.size() + 1);
+    ptw_batch_out bo;
+    memset(&bo, 0, sizeof(bo));
+    bo.topk_vertex = tv.data();
+    bo.topk_count = tc.data();
+    bo.steps = steps.data();
+    printf("batch queries %zu topk %u\n", qs.size(), o.topk);
+    if (ptw_batch_ppr(ctx, &bd, &bo)) {
+        fprintf(stderr, "[error] batch: %s\n", ptw_last_error(ctx));
+        return 1;
+    }
+    printf("update %lf message %lf\n", bo.kernel_ms * 1e-3, 0.0);
+    printf("total time (including init start walker) %lf, finished step %lu\n", bo.total_ms * 1e-3,
+           (unsigned long)bo.total_steps);
+    if (!o.out.empty() &&
+        ptw_batch_write_topk(ctx, o.out.c_str(), &bd, &bo, ids.original ? PTW_IDS_ORIGINAL : PTW_IDS_GRAPH)) {
+        fprintf(stderr, "[error] writing %s: %s\n", o.out.c_str(), ptw_last_error(ctx));
+        return 1;
+    }
+    if (o.json) {
+        printf("{\"app\": \"%s\", \"queries\": %zu, \"vertices\": %u, \"edges\": %" PRIu64 ", \"steps\": %" PRIu64
+               ", \"seed\": %" PRIu64 ", \"kernel_ms\": %.3f, \"steps_per_s\": %.6g, \"queries_per_s\": %.6g}\n",
+               app_name(app), qs.size(), gi.num_vertices, gi.num_edges, bo.total_steps, o.seed, bo.kernel_ms,
+               bo.kernel_ms > 0 ? bo.total_steps / (bo.kernel_ms * 1e-3) : 0.0,
+               bo.kernel_ms > 0 ? qs.size() / (bo.kernel_ms * 1e-3) : 0.0);
+    }
+    return 0;
+}
+
+}  // namespace
+
 int run(int app, Parser parser, const char *prog, int argc, char **argv)
 {
     Opts o = parse(prog, parser, argc, argv);
+    if (!o.queries.empty() && app != PTW_APP_PPR && app != PTW_APP_RWR)
+        parse_error(prog, parser, "--queries is for the PPR and RWR binaries");
     ptw_ctx *ctx = nullptr;
     int rc = ptw_create(&ctx, o.device);
     if (rc) {
@@ -296,6 +425,16 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
     fprintf(stderr, "Vertices number: %u\n", gi.num_vertices);
     fprintf(stderr, "Edges number: %" PRIu64 "\n", gi.num_edges);
     printf("Partition time %lf s\n", now_s() - tl);
+    IdMap ids;
+    ids.original = o.ids_original;
+    ids.names.resize(gi.num_vertices);
+    if (ptw_graph_export_names(ctx, ids.names.data())) return die("graph names");
+
+    if (!o.queries.empty()) {
+        rc = run_batch(ctx, app, o, gi, ids);
+        ptw_destroy(ctx);
+        return rc;
+    }
 
     ptw_walk_desc d;
     memset(&d, 0, sizeof(d));
@@ -308,14 +447,20 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
     d.walk_len = (uint32_t)(o.walk_len > 0 ? o.walk_len : 1);
     d.start_mode = PTW_START_UNIFORM;
     if (o.start == "roundrobin") d.start_mode = PTW_START_ROUNDROBIN;
+    uint32_t source = 0;
+    if (o.have_source && !ids.in(o.source, source)) {
+        fprintf(stderr, "[error] --source %u is not a vertex of the graph\n", o.source);
+        ptw_destroy(ctx);
+        return 1;
+    }
     if (app == PTW_APP_PPR) {
         // the reference intends the highest-degree vertex (src/core/ppr.cpp:36-45)
         d.start_mode = PTW_START_SOURCE;
-        d.source = o.have_source ? o.source : gi.max_degree_vertex;
-        printf("Root vertex %u\n", d.source);
+        d.source = o.have_source ? source : gi.max_degree_vertex;
+        printf("Root vertex %u\n", ids.out(d.source));
     } else if (o.have_source) {
         d.start_mode = PTW_START_SOURCE;
-        d.source = o.source;
+        d.source = source;
     }
     if ((app == PTW_APP_DEEPWALK || app == PTW_APP_NODE2VEC) && o.walk_len < 1) check_failed("walk length >= 1");
     d.p = o.n2v_p;
@@ -323,7 +468,7 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
     fprintf(stderr, "walker size %lld\n", o.walker_size);
 
     std::vector<uint64_t> counts;
-    std::vector<uint32_t> ends, paths;
+    std::vector<uint32_t> ends, starts, paths;
     ptw_walk_out out;
     memset(&out, 0, sizeof(out));
     if (app == PTW_APP_PAGERANK || app == PTW_APP_PPR) {
@@ -332,6 +477,10 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
     } else if (app == PTW_APP_RWR) {
         ends.assign(d.walkers, 0);
         out.ends = ends.data();
+        if (!o.out.empty()) {  // the reference's {start_v, end_v} pairs (src/core/RWR.cpp:9-13)
+            starts.assign(d.walkers, 0);
+            out.starts = starts.data();
+        }
     } else {
         paths.assign(d.walkers * d.walk_len, 0);
         out.paths = paths.data();
@@ -357,14 +506,23 @@ int run(int app, Parser parser, const char *prog, int argc, char **argv)
             ptw_destroy(ctx);
             return 1;
         }
-        if (!counts.empty()) {
-            for (uint32_t v = 0; v < gi.num_vertices; ++v) fprintf(f, "%u %" PRIu64 "\n", v, counts[v]);
-        } else if (!ends.empty()) {
-            for (uint64_t w = 0; w < d.walkers; ++w) fprintf(f, "%" PRIu64 " %u\n", w, ends[w]);
-        } else {
-            fwrite(paths.data(), 4, paths.size(), f);
+        bool ok = true;
+        if (!counts.empty()) {  // "vertex count", vertices in load order
+            for (uint32_t v = 0; v < gi.num_vertices && ok; ++v)
+                ok = fprintf(f, "%u %" PRIu64 "\n", ids.out(v), counts[v]) > 0;
+        } else if (!ends.empty()) {  // "start end", one line per walker
+            for (uint64_t w = 0; w < d.walkers && ok; ++w)
+                ok = fprintf(f, "%u %u\n", ids.out(starts[w]), ids.out(ends[w])) > 0;
+        } else {  // u32 paths, walk_len per walker
+            if (ids.original)
+                for (auto &x : paths) x = ids.out(x);
+            ok = fwrite(paths.data(), 4, paths.size(), f) == paths.size();
         }
-        fclose(f);
+        if (fclose(f) != 0 || !ok) {
+            fprintf(stderr, "[error] short write to %s\n", o.out.c_str());
+            ptw_destroy(ctx);
+            return 1;
+        }
     }
     if (o.json) {
         printf("{\"app\": \"%s\", \"vertices\": %u, \"edges\": %" PRIu64 ", \"walkers\": %" PRIu64

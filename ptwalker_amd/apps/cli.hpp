@@ -9,7 +9,8 @@
 //   -p P  -q Q (required, node2vec)    Node2vecOptionHelper     :447-482
 //   -h / --help                        OptionParser             :21-59
 // plus opt-in flags of this engine: --seed, --source, --device, --out,
-// --json, --semantics, --dangling, --undirected.
+// --json, --semantics, --dangling, --undirected, --out-ids, and for PPR and
+// RWR the batch mode --queries / --topk.
 #pragma once
 
 namespace ptw_cli {

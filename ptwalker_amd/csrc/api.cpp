@@ -425,6 +425,19 @@ int ptw_graph_info_get(ptw_ctx *ctx, ptw_graph_info *info)
     return PTW_OK;
 }
 
+int ptw_graph_export_names(ptw_ctx *ctx, uint32_t *names)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (!names) return fail(ctx, PTW_ERR_INVALID_ARG, "null output buffer");
+    const HostCsr &h = ctx->host;
+    if (h.names.empty())
+        for (uint32_t v = 0; v < h.V; ++v) names[v] = v;
+    else
+        memcpy(names, h.names.data(), (size_t)h.V * 4);
+    return PTW_OK;
+}
+
 int ptw_graph_export_csr(ptw_ctx *ctx, uint32_t *row_ptr, uint32_t *col)
 {
     if (!ctx) return PTW_ERR_INVALID_ARG;
@@ -478,13 +491,17 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
         return fail(ctx, PTW_ERR_INVALID_ARG, "source vertex " + std::to_string(d.source) + " out of range");
     if (count_app && !out->counts) return fail(ctx, PTW_ERR_INVALID_ARG, "counts output required");
     if (d.app == PTW_APP_RWR && !out->ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
+    if (d.app != PTW_APP_RWR && out->starts)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "starts output is for RWR walks only");
     if (path_app && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
     if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
     if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
     const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
-    if (on_dev && (((uintptr_t)out->paths & 15) || ((uintptr_t)out->ends & 3) || ((uintptr_t)out->counts & 7)))
-        return fail(ctx, PTW_ERR_INVALID_ARG, "device outputs must be aligned: paths 16 B, ends 4 B, counts 8 B");
+    if (on_dev && (((uintptr_t)out->paths & 15) || ((uintptr_t)out->ends & 3) || ((uintptr_t)out->starts & 3) ||
+                   ((uintptr_t)out->counts & 7)))
+        return fail(ctx, PTW_ERR_INVALID_ARG,
+                    "device outputs must be aligned: paths 16 B, ends and starts 4 B, counts 8 B");
     const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
     out->steps = out->visits = out->dangling = 0;
     out->kernel_ms = 0.0;
@@ -494,19 +511,24 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     hipStream_t s = ctx->stream;
 
     // device-side output buffers
-    uint32_t *ends_dev = nullptr, *paths_dev = nullptr;
+    uint32_t *ends_dev = nullptr, *paths_dev = nullptr, *starts_dev = nullptr;
     uint64_t *counts_out_dev = nullptr;
     struct Tmp {
         void *p = nullptr;
         ~Tmp() { if (p) (void)hipFree(p); }
-    } t_ends, t_paths;
+    } t_ends, t_paths, t_starts;
     const uint64_t W = d.walkers;
     if (d.app == PTW_APP_RWR) {
         if (on_dev) {
             ends_dev = out->ends;
+            starts_dev = out->starts;
         } else if (W) {
             HIP_TRY(ctx, hipMalloc(&t_ends.p, W * 4));
             ends_dev = (uint32_t *)t_ends.p;
+            if (out->starts) {
+                HIP_TRY(ctx, hipMalloc(&t_starts.p, W * 4));
+                starts_dev = (uint32_t *)t_starts.p;
+            }
         }
     }
     if (path_app) {
@@ -561,6 +583,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     p.walkers = W;
     p.counts_dev = ctx->counts_dev;
     p.ends = ends_dev;
+    p.starts = starts_dev;
     p.paths = paths_dev;
     p.scratch = ctx->scratch;
     if (d.app == PTW_APP_NODE2VEC) {
@@ -632,6 +655,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
         if (count_app)
             HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned + 8, counts_out_dev, (size_t)V * 8, hipMemcpyDeviceToHost, s));
         if (ends_dev && W) HIP_TRY(ctx, hipMemcpyAsync(out->ends, ends_dev, W * 4, hipMemcpyDeviceToHost, s));
+        if (starts_dev && W) HIP_TRY(ctx, hipMemcpyAsync(out->starts, starts_dev, W * 4, hipMemcpyDeviceToHost, s));
         if (paths_dev && W)
             HIP_TRY(ctx, hipMemcpyAsync(out->paths, paths_dev, W * d.walk_len * 4, hipMemcpyDeviceToHost, s));
     }

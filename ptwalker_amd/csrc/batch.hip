@@ -1,4 +1,6 @@
-// Batched single-source walks (PPR visit counts) for gfx950.
+// Batched single-source walks for gfx950: PPR queries count every visit
+// (type 0, src/core/solver.hpp:242-262), RWR queries count each walker's
+// last vertex (type 1, src/core/solver.hpp:284-305, src/core/RWR.cpp:99-105).
 //
 // All queries of a batch run in two persistent launches.  The host cuts
 // every query into chunks of walker ids, those of dense queries first.
@@ -70,6 +72,7 @@ __device__ __forceinline__ SliceRef slice_ref(const BatchQuery &bq)
 struct LaneQuery {
     uint32_t q = kEmpty;
     uint32_t src = 0, thr = 0, stream = 0;
+    uint32_t ends = 0;  // count only the walker's last vertex
     SliceRef r;
 };
 
@@ -194,6 +197,20 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     uint32_t my_src = 0;
     uint64_t my_steps = 0;
 
+    // One count for the lane's query: its own source in a register, the
+    // wavefront's hub counters (only while they hold the lane's query), the
+    // query's hub slice, the LDS cache, then the query's slice.
+    auto count = [&](uint32_t x) {
+        if (x == Q.src)
+            ++my_src;
+        else if (x < kWaveHubs && Q.q == cq)
+            atomicAdd(&hub[x], 1u);
+        else if (x < Q.r.hub_len)
+            atomicAdd(&p.dense[Q.r.hub_off + x], 1u);  // no reply to wait for
+        else if (!C.add(Q.q, x))
+            slice_add(p, Q.q, Q.r, x, 1u);
+    };
+
     // hub counters belong to cq, the query of the wavefront's current chunk
     auto flush_hubs = [&]() {
         if (cq == kEmpty) return;
@@ -245,12 +262,13 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                     Q.src = bq.source_dev;
                     Q.thr = bq.thr;
                     Q.stream = bq.stream;
+                    Q.ends = bq.ends_only;
                     Q.r = slice_ref(bq);
                 }
                 w = c_next + rank;
                 v = Q.src;
                 step = 0;
-                ++my_src;  // the start visit
+                if (!Q.ends) ++my_src;  // the start visit
                 active = true;
             }
             c_next += take;
@@ -276,17 +294,11 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                         v = p.col[beg + j];
                     }
                     ++step;
-                    if (v == Q.src)
-                        ++my_src;
-                    else if (v < kWaveHubs && Q.q == cq)
-                        atomicAdd(&hub[v], 1u);
-                    else if (v < Q.r.hub_len)
-                        atomicAdd(&p.dense[Q.r.hub_off + v], 1u);  // no reply to wait for
-                    else if (!C.add(Q.q, v))
-                        slice_add(p, Q.q, Q.r, v, 1u);
+                    if (!Q.ends) count(v);
                 }
             }
             if (stop) {
+                if (Q.ends) count(v);
                 my_steps += step;
                 active = false;
             }
@@ -333,9 +345,41 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     const uint4 *wedge = p.hdr[5] ? p.wedge : nullptr;
     uint32_t cq = kEmpty;  // workgroup-uniform: the query whose counts the LDS holds
     uint32_t src = 0, thr = 0, stream = 0;
+    bool ends = false;  // workgroup-uniform: count only each walker's last vertex
     uint64_t off = 0;
     uint32_t my_src = 0;
     uint64_t my_steps = 0;
+
+    // One count of the workgroup's query: its source in a register, the LDS
+    // hub counters, the LDS cache, then the query's dense slice.
+    auto count = [&](uint32_t x) {
+        if (x == src) {
+            ++my_src;
+        } else if (x < hubs) {
+            // the u16 spill rule of walk.hip's lds_add16, into the u32 slice
+            const uint32_t sh = (x & 1u) << 4;
+            const uint32_t old = atomicAdd(&direct[x >> 1], 1u << sh);
+            if (((old >> sh) & 0xFFFFu) == 0x7FFFu) {
+                atomicSub(&direct[x >> 1], 0x8000u << sh);
+                atomicAdd(&p.dense[off + x], 0x8000u);
+            }
+        } else {
+            if (slots) {
+                uint32_t h = (x * 0x9E3779B1u) & mask;
+#pragma unroll 1
+                for (int probe = 0; probe < kCacheProbes; ++probe) {
+                    uint32_t k = keys[h];
+                    if (k == kEmpty) k = atomicCAS(&keys[h], kEmpty, x);
+                    if (k == kEmpty || k == x) {
+                        atomicAdd(&vals[h], 1u);
+                        return;
+                    }
+                    h = (h + 1) & mask;
+                }
+            }
+            atomicAdd(&p.dense[off + x], 1u);
+        }
+    };
 
     // Workgroup-uniform: every thread calls it, between barriers.
     auto flush = [&]() {
@@ -395,6 +439,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
             src = bq.source_dev;
             thr = bq.thr;
             stream = bq.stream;
+            ends = bq.ends_only != 0;
             off = bq.slice_off;
         }
         const uint64_t start = s_start;
@@ -418,7 +463,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                         w = start + base + rank;
                         v = src;
                         step = 0;
-                        ++my_src;  // the start visit
+                        if (!ends) ++my_src;  // the start visit
                         active = true;
                     }
                 }
@@ -442,37 +487,11 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                             v = p.col[beg + j];
                         }
                         ++step;
-                        if (v == src) {
-                            ++my_src;
-                        } else if (v < hubs) {
-                            // the u16 spill rule of walk.hip's lds_add16, into the u32 slice
-                            const uint32_t sh = (v & 1u) << 4;
-                            const uint32_t old = atomicAdd(&direct[v >> 1], 1u << sh);
-                            if (((old >> sh) & 0xFFFFu) == 0x7FFFu) {
-                                atomicSub(&direct[v >> 1], 0x8000u << sh);
-                                atomicAdd(&p.dense[off + v], 0x8000u);
-                            }
-                        } else {
-                            bool done = false;
-                            if (slots) {
-                                uint32_t h = (v * 0x9E3779B1u) & mask;
-#pragma unroll 1
-                                for (int probe = 0; probe < kCacheProbes; ++probe) {
-                                    uint32_t k = keys[h];
-                                    if (k == kEmpty) k = atomicCAS(&keys[h], kEmpty, v);
-                                    if (k == kEmpty || k == v) {
-                                        atomicAdd(&vals[h], 1u);
-                                        done = true;
-                                        break;
-                                    }
-                                    h = (h + 1) & mask;
-                                }
-                            }
-                            if (!done) atomicAdd(&p.dense[off + v], 1u);
-                        }
+                        if (!ends) count(v);
                     }
                 }
                 if (stop) {
+                    if (ends) count(v);
                     my_steps += step;
                     active = false;
                 }

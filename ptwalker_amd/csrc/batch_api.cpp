@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -67,8 +68,11 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         if (!(qq.tp > 0.0 && qq.tp <= 1.0))
             return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": tp must be in (0, 1]");
         if (qq.walkers == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": no walkers");
-        const double per_walker = 1.0 + 1.0 / qq.tp;  // expected visits per walker (upper: reference semantics)
-        // No u32 counter can pass the query's visit total; plan for 1.5x the
+        if (qq.kind != PTW_QUERY_VISITS && qq.kind != PTW_QUERY_ENDPOINTS)
+            return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(i) + ": unknown kind");
+        const bool ends = qq.kind == PTW_QUERY_ENDPOINTS;
+        const double per_walker = batch_counts_per_walker(qq.tp, ends);
+        // No u32 counter can pass the query's count total; plan for 1.5x the
         // expected total here and check the actual total after the walk.
         if ((double)qq.walkers * per_walker * 1.5 > 4294967296.0)
             return fail(ctx, PTW_ERR_INVALID_ARG,
@@ -78,7 +82,9 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         B.thr = continue_threshold(qq.tp);
         B.walkers = qq.walkers;
         B.stream = qq.stream;
-        const SliceSizes z = batch_slice_sizes(V, qq.walkers, qq.tp, thr_dense);
+        B.ends_only = ends ? 1u : 0u;
+        B.pad = 0;
+        const SliceSizes z = batch_slice_sizes(V, qq.walkers, qq.tp, thr_dense, ends);
         if (z.dense) {
             B.kind = SLICE_DENSE;
             B.slice_off = P.dense;
@@ -432,10 +438,49 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     for (uint32_t i = 0; i < d.num_queries; ++i) {
         if (over[i])
             return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
-        if (d.queries[i].walkers + steps[i] >= 4294967296ull)
+        const uint64_t total_counts =
+            d.queries[i].walkers + (d.queries[i].kind == PTW_QUERY_ENDPOINTS ? 0 : steps[i]);
+        if (total_counts >= 4294967296ull)
             return fail(ctx, PTW_ERR_OOM,
-                        "query " + std::to_string(i) + ": visit total reached 2^32, u32 counters may have wrapped");
+                        "query " + std::to_string(i) + ": count total reached 2^32, u32 counters may have wrapped");
     }
+    return PTW_OK;
+}
+
+int ptw_batch_write_topk(ptw_ctx *ctx, const char *path, const ptw_batch_desc *desc, const ptw_batch_out *out,
+                         int id_space)
+{
+    if (!ctx || !path || !desc || !out) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (id_space != PTW_IDS_GRAPH && id_space != PTW_IDS_ORIGINAL)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown id space");
+    const uint32_t n = desc->num_queries, k = desc->topk;
+    if ((n && !desc->queries) || (n && k && (!out->topk_vertex || !out->topk_count)))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "null query or top-k array");
+    const ptw::HostCsr &h = ctx->host;
+    auto id = [&](uint32_t v) { return id_space == PTW_IDS_ORIGINAL ? h.name(v) : v; };
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(ctx, PTW_ERR_IO, std::string("cannot open '") + path + "' for writing");
+    bool ok = fprintf(f, "# query source kind rank vertex count\n") > 0;
+    for (uint32_t q = 0; q < n && ok; ++q) {
+        const ptw_query &qq = desc->queries[q];
+        if (qq.source >= h.V) {
+            fclose(f);
+            return fail(ctx, PTW_ERR_INVALID_ARG, "query " + std::to_string(q) + ": source out of range");
+        }
+        for (uint32_t r = 0; r < k && ok; ++r) {
+            const uint32_t v = out->topk_vertex[(uint64_t)q * k + r];
+            if (v == PTW_INVALID_VERTEX) break;
+            if (v >= h.V) {
+                fclose(f);
+                return fail(ctx, PTW_ERR_INVALID_ARG, "top-k entry out of range");
+            }
+            ok = fprintf(f, "%u %u %u %u %u %llu\n", q, id(qq.source), qq.kind, r, id(v),
+                         (unsigned long long)out->topk_count[(uint64_t)q * k + r]) > 0;
+        }
+    }
+    if (fclose(f) != 0) ok = false;
+    if (!ok) return fail(ctx, PTW_ERR_IO, std::string("short write to '") + path + "'");
     return PTW_OK;
 }
 

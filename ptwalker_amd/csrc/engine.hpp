@@ -18,7 +18,11 @@ struct HostCsr {
     uint32_t V = 0;
     std::vector<uint32_t> row_ptr;  // V + 1
     std::vector<uint32_t> col;      // E
+    // the id each vertex had before compaction (the file's id for a loaded
+    // edge list, the generator's for R-MAT); empty: graph ids are those ids
+    std::vector<uint32_t> names;
     uint64_t E() const { return col.size(); }
+    uint32_t name(uint32_t v) const { return names.empty() ? v : names[v]; }
 };
 
 // Device copy of the graph under the degree-descending relabel.
@@ -115,6 +119,7 @@ struct WalkLaunch {
     // outputs (device)
     unsigned long long *counts_dev;  // device-id counters (count apps)
     uint32_t *ends;                  // graph ids
+    uint32_t *starts;                // graph ids (end apps; nullptr: not written)
     uint32_t *paths;                 // graph ids
     // scratch (device): [0] queue head, [1] steps, [2] dangling
     unsigned long long *scratch;
@@ -188,9 +193,14 @@ struct SliceSizes {
     uint32_t hub_len;      // hash: u32 counters of the hub slice
     uint64_t bytes() const { return (dense ? 4 * slots : 8 * slots + 4ull * hub_len) + 8ull * touched_cap; }
 };
-inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uint64_t dense_threshold)
+// Counts per walker a query makes: every visit (1 + L; at most 1 + 1/tp on
+// average, reference semantics) or only its last vertex (1).
+inline double batch_counts_per_walker(double tp, bool ends_only) { return ends_only ? 1.0 : 1.0 + 1.0 / tp; }
+
+inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uint64_t dense_threshold,
+                                    bool ends_only = false)
 {
-    const double per_walker = 1.0 + 1.0 / tp;
+    const double per_walker = batch_counts_per_walker(tp, ends_only);
     const uint64_t upper = (uint64_t)(2.0 * (double)walkers * per_walker) + 64;
     const uint64_t expect = (uint64_t)(1.25 * (double)walkers * per_walker) + 64;
     SliceSizes z;
@@ -222,6 +232,8 @@ struct BatchQuery {
     uint32_t cand_cap;     // candidate slots (0 without top-k)
     uint32_t hub_len;      // hash: device ids below this count in a dense hub slice
     uint64_t hub_off;      // hash: u32 offset of the hub slice in the dense pool
+    uint32_t ends_only;    // 1: count each walker's last vertex only (PTW_QUERY_ENDPOINTS)
+    uint32_t pad;
 };
 
 

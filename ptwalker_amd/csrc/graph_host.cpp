@@ -24,6 +24,7 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 #include "../../include/ptw.h"
 #include "engine.hpp"
@@ -252,17 +253,43 @@ int load_edge_list(const char *path, int format, bool as_undirected, HostCsr &g,
     // first-appearance compaction (src/core/graph.hpp:293-319)
     uint32_t max_name = 0;
     for (uint64_t e = 0; e < m; ++e) max_name = std::max(max_name, std::max(src[e], dst[e]));
-    std::vector<uint32_t> name2id(m ? (size_t)max_name + 1 : 0, PTW_INVALID_VERTEX);
+    std::vector<uint32_t> names;  // the inverse: file id of each compact id
     uint32_t V = 0;
-    for (uint64_t e = 0; e < m; ++e) {
-        uint32_t &a = name2id[src[e]];
-        if (a == PTW_INVALID_VERTEX) a = V++;
-        uint32_t &b = name2id[dst[e]];
-        if (b == PTW_INVALID_VERTEX) b = V++;
-        src[e] = a;
-        dst[e] = b;
+    // A table indexed by file id, as the reference keeps; a hash map when the
+    // ids are sparse (the table would be far larger than the edge list).
+    auto compact = [&](auto &name2id) {
+        for (uint64_t e = 0; e < m; ++e) {
+            uint32_t &a = name2id[src[e]];
+            if (a == PTW_INVALID_VERTEX) {
+                a = V++;
+                names.push_back(src[e]);
+            }
+            uint32_t &b = name2id[dst[e]];
+            if (b == PTW_INVALID_VERTEX) {
+                b = V++;
+                names.push_back(dst[e]);
+            }
+            src[e] = a;
+            dst[e] = b;
+        }
+    };
+    if ((uint64_t)max_name + 1 <= 16 * m + (1u << 20)) {
+        std::vector<uint32_t> name2id(m ? (size_t)max_name + 1 : 0, PTW_INVALID_VERTEX);
+        compact(name2id);
+    } else {
+        struct Slot {
+            uint32_t v = PTW_INVALID_VERTEX;
+        };
+        std::unordered_map<uint32_t, Slot> map;
+        map.reserve(2 * m);
+        struct Ref {
+            std::unordered_map<uint32_t, Slot> &m;
+            uint32_t &operator[](uint32_t k) { return m[k].v; }
+        } name2id{map};
+        compact(name2id);
     }
     csr_from_edges(V, src, dst, as_undirected, g);
+    g.names = std::move(names);
     return PTW_OK;
 }
 
@@ -436,6 +463,12 @@ int build_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double 
             newid[v] = V++;
         else
             newid[v] = PTW_INVALID_VERTEX;
+    }
+    g.names.clear();
+    if (V < N) {  // the generator's id of each kept vertex
+        g.names.resize(V);
+        for (uint64_t v = 0; v < N; ++v)
+            if (newid[v] != PTW_INVALID_VERTEX) g.names[newid[v]] = (uint32_t)v;
     }
     g.V = V;
     g.row_ptr.assign((size_t)V + 1, 0);

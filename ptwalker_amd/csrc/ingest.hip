@@ -197,9 +197,10 @@ __global__ void run_heads_kernel(const uint32_t *skeys, uint64_t n, uint32_t *he
 
 // run_of = exclusive scan of head: a head's run index; first_pos[run] is the
 // head's position (the smallest of its run: the sort is stable and positions
-// were ascending).
-__global__ void first_positions_kernel(const uint32_t *head, const uint32_t *run_excl, const uint32_t *svals,
-                                       uint64_t n, uint32_t *first_pos, uint32_t *run_idx)
+// were ascending) and run_key[run] the raw id the run holds.
+__global__ void first_positions_kernel(const uint32_t *head, const uint32_t *run_excl, const uint32_t *skeys,
+                                       const uint32_t *svals, uint64_t n, uint32_t *first_pos, uint32_t *run_idx,
+                                       uint32_t *run_key)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -207,15 +208,21 @@ __global__ void first_positions_kernel(const uint32_t *head, const uint32_t *run
             const uint32_t r = run_excl[k];
             first_pos[r] = svals[k];
             run_idx[r] = r;
+            run_key[r] = skeys[k];
         }
     }
 }
 
-__global__ void rank_runs_kernel(const uint32_t *order, uint32_t runs, uint32_t *new_id)
+// order[rank] is the run with the rank-th first appearance: that run's new id
+// is rank, and the raw id it holds is the new id's name.
+__global__ void rank_runs_kernel(const uint32_t *order, const uint32_t *run_key, uint32_t runs, uint32_t *new_id,
+                                 uint32_t *names)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += stride)
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += stride) {
         new_id[order[r]] = (uint32_t)r;
+        names[r] = run_key[order[r]];
+    }
 }
 
 // Every endpoint position gets its new id: the run of sorted element k is
@@ -509,16 +516,19 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     ING_TRY(hipStreamSynchronize(s));
     const uint32_t V = last_excl + last_head;
     uint32_t *first_pos = nullptr, *run_idx = nullptr, *sfirst = nullptr, *order = nullptr, *new_id = nullptr;
+    uint32_t *run_key = nullptr, *names = nullptr;
     ING_TRY(B.get(&first_pos, V));
     ING_TRY(B.get(&run_idx, V));
     ING_TRY(B.get(&sfirst, V));
     ING_TRY(B.get(&order, V));
     ING_TRY(B.get(&new_id, V));
-    hipLaunchKernelGGL(first_positions_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, svals, n,
-                       first_pos, run_idx);
+    ING_TRY(B.get(&run_key, V));
+    ING_TRY(B.get(&names, V));
+    hipLaunchKernelGGL(first_positions_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, skeys, svals, n,
+                       first_pos, run_idx, run_key);
     ING_TRY(hipGetLastError());
     if ((rc = sort_pairs(B, first_pos, sfirst, run_idx, order, V, bits_for(n - 1), s, err))) return rc;
-    hipLaunchKernelGGL(rank_runs_kernel, dim3(grid_for(V)), dim3(256), 0, s, order, V, new_id);
+    hipLaunchKernelGGL(rank_runs_kernel, dim3(grid_for(V)), dim3(256), 0, s, order, run_key, V, new_id, names);
     ING_TRY(hipGetLastError());
     uint32_t *mapped = skeys;  // reuse: position -> new id
     hipLaunchKernelGGL(scatter_ids_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, svals, new_id, n,
@@ -541,8 +551,10 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     g.V = V;
     g.row_ptr.resize((size_t)V + 1);
     g.col.resize(E);
+    g.names.resize(V);
     ING_TRY(hipMemcpyAsync(g.row_ptr.data(), row, ((uint64_t)V + 1) * 4, hipMemcpyDeviceToHost, s));
     ING_TRY(hipMemcpyAsync(g.col.data(), col, E * 4, hipMemcpyDeviceToHost, s));
+    ING_TRY(hipMemcpyAsync(g.names.data(), names, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
     ING_TRY(hipStreamSynchronize(s));
     return PTW_OK;
 }

@@ -173,7 +173,7 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
             const Item &it = items[i];
             const uint32_t id = reqs[it.req].id;
             const uint32_t stream = it.kind == MAIN ? id : it.kind == PILOT ? kPilotStream + id : kAuditStream + id;
-            q[i] = ptw_query{reqs[it.req].source, stream, it.walkers, C.tp};
+            q[i] = ptw_query{reqs[it.req].source, stream, it.walkers, C.tp, PTW_QUERY_VISITS, 0};
             results[it.req].launches++;
         }
         ptw_batch_desc d{};

@@ -342,6 +342,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                 uint32_t g;
                 if (p.start_mode == PTW_START_SOURCE) {
                     v = p.source_dev;
+                    if (KIND == KIND_END && p.starts) p.starts[w] = p.inv ? p.inv[v] : v;
                 } else {
                     if (p.start_mode == PTW_START_ROUNDROBIN) {
                         g = (uint32_t)(gw % p.V);
@@ -351,6 +352,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     }
                     v = p.perm ? p.perm[g] : g;
                     if (KIND == KIND_COUNT) count_visit(v);
+                    if (KIND == KIND_END && p.starts) p.starts[w] = g;
                 }
                 if (KIND == KIND_PATH || KIND == KIND_N2V) {
                     pos = w * p.walk_len;

@@ -30,3 +30,35 @@ def host_ctx():
     c = ptw.Context(-1)
     yield c
     c.close()
+
+
+def _graph_context(scale, weights):
+    import ptwalker_amd as ptw
+    from ptwalker_amd.workloads import RMAT_ABC
+
+    print(f"\n[fixture] R-MAT scale {scale} on its own context", flush=True)
+    c = ptw.Context(0)
+    a, b, cc = RMAT_ABC
+    gi = c.rmat(scale=scale, edge_factor=16, a=a, b=b, c=cc, seed=1)
+    if weights:
+        c.random_weights(seed=1)
+    rp, col = c.export_csr()
+    return c, gi, rp, col
+
+
+@pytest.fixture(scope="session")
+def ctx22():
+    """R-MAT scale 22 (the single-source and serving configurations), resident
+    on a context of its own for the whole GPU session."""
+    c, gi, rp, col = _graph_context(22, False)
+    yield c, gi, rp, col
+    c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx24():
+    """R-MAT scale 24 with U(0, 1] weights (the DeepWalk corpus and batched
+    configurations), resident on a context of its own."""
+    c, gi, rp, col = _graph_context(24, True)
+    yield c, gi, rp, col
+    c.close()

@@ -103,6 +103,73 @@ def test_loader_matches_reference_semantics(host_ctx, tmp_path, fmt, undirected)
     erp, ecol, name2id = oracle.load_edge_list(path, fmt, undirected)
     assert gi.num_vertices == len(name2id)
     assert np.array_equal(rp, erp) and np.array_equal(col, ecol)
+    assert np.array_equal(host_ctx.export_names(), _names_of(name2id))
+
+
+def _names_of(name2id):
+    names = np.zeros(len(name2id), dtype=np.uint32)
+    for name, v in name2id.items():
+        names[v] = name
+    return names
+
+
+def test_loader_sparse_ids_hash_path(host_ctx, tmp_path):
+    """Ids up to 2^32 - 2 over a few thousand edges: the loader compacts them
+    through a hash map (a table indexed by id would take 16 GiB) with the same
+    first-appearance ids and names."""
+    rng = np.random.default_rng(8)
+    pool = np.array([0xFFFFFFFE, 7, 1 << 31, 123456789, 4000000000, 5, 3999999999], dtype=np.uint64)
+    ids = np.concatenate([pool, rng.choice(np.arange(10, 1 << 32, 65537, dtype=np.uint64), 300, replace=False)])
+    e = ids[rng.integers(0, len(ids), size=(4000, 2))]
+    for fmt in ("text", "binary"):
+        path = str(tmp_path / f"s.{fmt}")
+        if fmt == "text":
+            with open(path, "w") as f:
+                f.write("".join(f"{a} {b}\n" for a, b in e))
+        else:
+            e.astype(np.uint32).tofile(path)
+        host_ctx.load(path, fmt=ptw.FORMAT_TEXT if fmt == "text" else ptw.FORMAT_BINARY)
+        rp, col = host_ctx.export_csr()
+        erp, ecol, name2id = oracle.load_edge_list(path, fmt)
+        assert np.array_equal(rp, erp) and np.array_equal(col, ecol)
+        assert np.array_equal(host_ctx.export_names(), _names_of(name2id))
+        assert host_ctx.export_names()[0] == e[0, 0]
+
+
+def test_rmat_names_are_generator_ids(host_ctx):
+    """R-MAT compacts isolated ids away; names are the generator's ids, in
+    order, and each kept vertex has the degree it has with isolated ids kept."""
+    host_ctx.rmat(scale=10, edge_factor=2, seed=3, flags=ptw.RMAT_KEEP_ISOLATED)
+    rp_all, _ = host_ctx.export_csr()
+    assert np.array_equal(host_ctx.export_names(), np.arange(1024))
+    host_ctx.rmat(scale=10, edge_factor=2, seed=3)
+    rp, _ = host_ctx.export_csr()
+    names = host_ctx.export_names()
+    assert len(names) < 1024 and np.all(np.diff(names.astype(np.int64)) > 0)
+    assert np.array_equal(np.diff(rp.astype(np.int64)), np.diff(rp_all.astype(np.int64))[names])
+
+
+def test_batch_topk_writer_format(host_ctx, tmp_path):
+    """ptw_batch_write_topk: one line per listed entry, padding left out, ids in
+    the input's own space or graph ids."""
+    path = str(tmp_path / "g.txt")
+    with open(path, "w") as f:
+        f.write("50 60\n60 70\n70 50\n90 50\n")
+    host_ctx.load(path)
+    names = host_ctx.export_names()
+    assert names.tolist() == [50, 60, 70, 90]
+    res = ptw.BatchResult(np.array([[1, 2, ptw.INVALID_VERTEX], [0, 3, 1]], dtype=np.uint32),
+                          np.array([[9, 4, 0], [7, 7, 1]], dtype=np.uint64), np.zeros(2, np.uint64), 0, 0, 0, 0,
+                          0.0, 0.0, np.zeros(2))
+    out = str(tmp_path / "t.txt")
+    host_ctx.batch_write_topk(out, [3, 1], [100, 200], [0.2, 0.3], res, kinds=[0, 1])
+    rows = [l.split() for l in open(out).read().splitlines()]
+    assert rows[0][0] == "#"
+    assert rows[1:] == [s.split() for s in ("0 90 0 0 60 9", "0 90 0 1 70 4", "1 60 1 0 50 7", "1 60 1 1 90 7",
+                                             "1 60 1 2 60 1")]
+    host_ctx.batch_write_topk(out, [3, 1], [100, 200], [0.2, 0.3], res, kinds=[0, 1], id_space=ptw.IDS_GRAPH)
+    rows = [l.split() for l in open(out).read().splitlines()]
+    assert rows[1] == "0 3 0 0 1 9".split()
 
 
 def test_large_text_parallel_parse(host_ctx, tmp_path):

@@ -8,6 +8,7 @@ import pytest
 
 import oracle
 import ptwalker_amd as ptw
+from ptwalker_amd.workloads import batch_queries
 from walkstats import random_graph
 
 pytestmark = pytest.mark.gpu
@@ -284,3 +285,100 @@ def test_batch_scale20_properties(ctx):
         assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], cc), i
     assert np.count_nonzero(ctx.batch_counts(big)) > 4 * 16384
     assert abs(r.total_steps - float(np.sum(W / tp))) < 6 * np.sqrt(np.sum(W * (1 - tp) / tp ** 2))
+
+
+def test_config_batch_scale24_4096_queries(ctx24):
+    """The batched configuration at its size: 4096 PPR queries on R-MAT scale 24
+    (sources by degree, walkers log-uniform in [1e3, 1e6], tp uniform in
+    [0.15, 0.3]), top-100, eager and as a captured graph.  32 queries -- the
+    8 largest (dense slices), the 8 smallest and 16 spread over the batch --
+    are replayed alone and must match bit for bit (full count vector, steps,
+    top-100); the batch's step total lies within 6 sigma of sum(W / tp)."""
+    c, gi, rp, col = ctx24
+    n, k = 4096, 100
+    src, W, tp = batch_queries(gi, rp, n, seed=1)
+    print(f"config 4: {n} queries on scale 24, {int(W.sum())} walkers", flush=True)
+    r = c.batch_ppr(src, W, tp, topk=k, seed=1)
+    assert r.dense_queries > 0 and r.hash_queries > 0
+    order = np.argsort(W, kind="stable")
+    pick = np.unique(np.concatenate([order[-8:], order[:8], np.linspace(0, n - 1, 16).astype(np.int64)]))
+    for i in pick:
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), int(i), seed=1)
+        assert np.array_equal(c.batch_counts(int(i)), ref["counts"]), i
+        assert r.steps[i] == ref["steps"], i
+        v, cnt = oracle.topk(ref["counts"], rp, k)
+        assert np.array_equal(r.topk_vertex[i], v), i
+        assert np.array_equal(r.topk_count[i], cnt), i
+    mean = float(np.sum(W / tp))
+    sd = float(np.sqrt(np.sum(W * (1 - tp) / tp ** 2)))
+    assert abs(r.total_steps - mean) < 6 * sd, (r.total_steps, mean, sd)
+    assert r.total_steps == int(r.steps.sum())
+    for _ in range(2):
+        g = c.batch_ppr(src, W, tp, topk=k, seed=1, graph=True)
+        assert np.array_equal(g.topk_vertex, r.topk_vertex) and np.array_equal(g.topk_count, r.topk_count)
+        assert np.array_equal(g.steps, r.steps)
+
+
+def replay_endpoints(rp, col, src, W, tp, stream, seed=1, semantics=oracle.SEM_REFERENCE):
+    ref = oracle.walk_replay(rp, col, oracle.APP_RWR, W, tp=tp, seed=seed, source=src, semantics=semantics,
+                             start_mode=oracle.START_SOURCE, stream=stream)
+    return np.bincount(ref["ends"], minlength=len(rp) - 1).astype(np.uint64), ref["steps"]
+
+
+@pytest.mark.parametrize("semantics", [oracle.SEM_REFERENCE, oracle.SEM_TEXTBOOK])
+def test_batch_rwr_endpoint_queries(ctx, g12, semantics):
+    """RWR queries in a batch count each walker's last vertex: a query's count
+    vector is the histogram of the oracle's RWR replay of that query alone
+    (its stream), dense and hash slices, mixed with PPR queries, alone and in
+    the mix, eager and as a replayed captured graph; top-k of the endpoint
+    counts is exact (src/core/RWR.cpp:99-105)."""
+    gi, rp, col = g12
+    src, W, tp = make_queries(gi, 24, seed=11, wmin=200, wmax=300000)
+    kinds = np.array([ptw.QUERY_ENDPOINTS if i % 3 else ptw.QUERY_VISITS for i in range(24)])
+    k = 30
+    r = ctx.batch_ppr(src, W, tp, topk=k, seed=6, semantics=semantics, kinds=kinds, dense_threshold=50000)
+    assert r.dense_queries > 0 and r.hash_queries > 0
+    counts = []
+    for i in range(24):
+        got = ctx.batch_counts(i)
+        counts.append(got)
+        if kinds[i] == ptw.QUERY_ENDPOINTS:
+            hist, steps = replay_endpoints(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=6,
+                                           semantics=semantics)
+            assert int(got.sum()) == W[i]
+        else:
+            ref = oracle.walk_replay(rp, col, oracle.APP_PPR, int(W[i]), tp=float(tp[i]), seed=6, source=int(src[i]),
+                                     start_mode=oracle.START_SOURCE, stream=i, semantics=semantics)
+            hist, steps = ref["counts"], ref["steps"]
+        assert np.array_equal(got, hist), i
+        assert r.steps[i] == steps, i
+        v, c = oracle.topk(hist, rp, k)
+        assert np.array_equal(r.topk_vertex[i], v), i
+        assert np.array_equal(r.topk_count[i], c), i
+    # alone equals in the mix
+    for i in (1, 2, 5):
+        alone = ctx.batch_ppr(src[i:i + 1], W[i:i + 1], tp[i:i + 1], streams=[i], topk=k, seed=6,
+                              semantics=semantics, kinds=kinds[i:i + 1], dense_threshold=50000)
+        assert np.array_equal(ctx.batch_counts(0), counts[i]), i
+        assert np.array_equal(alone.topk_vertex[0], r.topk_vertex[i])
+    # captured graph replays, an eager batch in between
+    for rep in range(3):
+        g = ctx.batch_ppr(src, W, tp, topk=k, seed=6, semantics=semantics, kinds=kinds, dense_threshold=50000,
+                          graph=True)
+        assert np.array_equal(g.topk_vertex, r.topk_vertex) and np.array_equal(g.topk_count, r.topk_count), rep
+        assert np.array_equal(g.steps, r.steps)
+        ctx.batch_ppr(src[:3], W[:3], tp[:3], topk=k, seed=7, kinds=kinds[:3])
+
+
+def test_batch_rwr_matches_single_walks(ctx, g12):
+    """An endpoint query's histogram equals the histogram of ptw_walk's RWR
+    endpoints with the same stream (the single-walk kernel)."""
+    gi, rp, col = g12
+    src = int(gi.max_degree_vertex)
+    r = ctx.batch_ppr([src, 3], [400000, 5000], [0.2, 0.3], streams=[9, 4], topk=10, seed=2,
+                      kinds=ptw.QUERY_ENDPOINTS)
+    for i, (s, w, t, st) in enumerate(((src, 400000, 0.2, 9), (3, 5000, 0.3, 4))):
+        one = ptw.walk(ctx, w, ptw.RWR(t, s), seed=2, stream=st, dangling=ptw.DANGLING_STOP)
+        hist = np.bincount(one.ends, minlength=gi.num_vertices).astype(np.uint64)
+        assert np.array_equal(ctx.batch_counts(i), hist), i
+        assert r.steps[i] == one.steps

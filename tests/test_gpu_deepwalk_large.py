@@ -1,7 +1,7 @@
-"""DeepWalk corpus at a large size (R-MAT scale 22, 10 walks per vertex x 80
-vertices, 7.7 GB of paths in HBM): sampled rows bit-exact against the
-oracle's Philox replay, unweighted and alias-weighted, and every sampled
-consecutive pair an edge."""
+"""The DeepWalk corpus configuration at its size (R-MAT scale 24, 10 walks per
+vertex x 80 vertices, 28.4 GB of paths in HBM): sampled rows bit-exact
+against the oracle's Philox replay, unweighted and alias-weighted, the step
+total exact, and every sampled consecutive pair an edge."""
 import numpy as np
 import pytest
 
@@ -12,17 +12,16 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def big(ctx):
-    gi = ctx.rmat(scale=22, edge_factor=16, seed=1)
-    ctx.random_weights(seed=1)
-    rp, col = ctx.export_csr()
-    w, _, _ = ctx.export_weights()
-    return gi, rp, col, oracle.build_alias(rp, w)
+def big(ctx24):
+    c, gi, rp, col = ctx24
+    w, _, _ = c.export_weights()
+    return c, gi, rp, col, oracle.build_alias(rp, w)
 
 
 @pytest.mark.parametrize("weighted", [False, True])
-def test_deepwalk_corpus_rows_replay(ctx, big, weighted):
-    gi, rp, col, alias = big
+def test_deepwalk_corpus_rows_replay(big, weighted):
+    ctx, gi, rp, col, alias = big
+    print(f"config 3: DeepWalk scale 24 (V {gi.num_vertices}), weighted={weighted}", flush=True)
     V, L = gi.num_vertices, 80
     W = 10 * V
     buf = ctx.alloc(4 * W * L)
@@ -31,7 +30,7 @@ def test_deepwalk_corpus_rows_replay(ctx, big, weighted):
     out = ctx.walk(d, paths=buf.ptr, device_ptrs=True)
     assert out.steps == W * (L - 1) and out.dangling == 0
     n = 3000
-    for base in (0, W // 2 + 7, W - n):
+    for base in (0, V - 5, W // 2 + 7, W - n):
         got = buf.download(np.uint32, n * L, offset=base * L).reshape(n, L)
         ref = oracle.walk_replay(rp, col, oracle.APP_DEEPWALK, n, walk_len=L, seed=1,
                                  start_mode=oracle.START_ROUNDROBIN, wid_base=base,

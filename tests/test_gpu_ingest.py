@@ -17,8 +17,10 @@ def both(ctx, host_ctx, path, fmt, undirected):
     ctx.load(path, fmt, undirected, flags=ptw.GRAPH_INGEST_DEVICE)
     rd, cd = ctx.export_csr()
     host_ctx.load(path, fmt, undirected, flags=ptw.GRAPH_INGEST_HOST)
+    nd = ctx.export_names()
     rh, ch = host_ctx.export_csr()
     assert np.array_equal(rd, rh) and np.array_equal(cd, ch)
+    assert np.array_equal(nd, host_ctx.export_names())  # the file id of each compact id
     return rd, cd
 
 

@@ -123,6 +123,9 @@ def test_cli_node2vec(ctx, tmp_path):
     assert "total time (including init start walker)" in r.stdout
     paths = np.fromfile(out, dtype=np.uint32).reshape(2000, 10)
     # the binary loads the file as undirected, as the reference's node2vec does
-    rp, col, _ = oracle.load_edge_list(str(g), "text", True)
+    rp, col, name2id = oracle.load_edge_list(str(g), "text", True)
+    names = np.zeros(len(name2id), dtype=np.uint32)
+    for name, v in name2id.items():
+        names[v] = name
     ref = oracle.node2vec_replay(rp, col, 2000, 10, 0.5, 2.0, seed=5)
-    assert np.array_equal(paths, ref["paths"])
+    assert np.array_equal(paths, names[ref["paths"]])  # in the file's own ids

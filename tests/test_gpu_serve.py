@@ -5,8 +5,12 @@ requests by the 10x walk."""
 import numpy as np
 import pytest
 
+import os
+
 import oracle
 import ptwalker_amd as ptw
+from ptwalker_amd.budget import BudgetPolicy
+from ptwalker_amd.workloads import serve_requests
 
 pytestmark = pytest.mark.gpu
 
@@ -138,3 +142,43 @@ def test_serve_rejects_bad_config(ctx, g12):
         ctx.serve(src, eps, arr, ids=[0, 1, 2, 1 << 30])
     with pytest.raises(ptw.PtwError):
         ctx.serve(src, eps, arr, w_max=10 ** 9)
+
+
+def test_config_serve_scale22_committed_policy(ctx22):
+    """The serving configuration at its size: R-MAT scale 22 resident, the
+    committed learned policy (pilot walk + MLP on the device), requests with
+    eps in {1e-3, 3e-4} arriving at 2000/s, 5 % audited.  Sampled answers equal
+    the replay of their own walk (budget, stream; the 10x audit stream when
+    topped up), and requests served alone give the same budget and answer."""
+    c, gi, rp, col = ctx22
+    pol = BudgetPolicy.load(os.path.join(os.path.dirname(ptw.__file__), "models", "budget_policy.json"))
+    tp = float(pol.meta["tp"])
+    net = pol.network(c)
+    n = 300
+    src, eps, arr = serve_requests(rp, n, 2000.0, seed=1)
+    print(f"config 5: serving {n} requests on scale 22 (V {gi.num_vertices})", flush=True)
+    kw = dict(mlp=net, tp=tp, seed=1, topk=100, budget_mode=ptw.BUDGET_LEARNED, margin=pol.margin,
+              fixed_c2=pol.fixed_c2, w_min=pol.w_min, w_max=pol.w_max, feat_lo=pol.feat_lo, feat_hi=pol.feat_hi,
+              audit_rate=0.05, max_batch=512)
+    mix = c.serve(src, eps, arr, **kw)
+    assert mix.stats["launches"] >= 2
+    print(f"config 5: {mix.stats['launches']} launches, {mix.stats['fallbacks']} fallbacks", flush=True)
+    assert mix.stats["fallbacks"] < n // 4
+    assert len(set(mix.budget.tolist())) > n // 2  # budgets differ by request
+    sample = np.unique(np.concatenate([np.linspace(0, n - 1, 10).astype(np.int64),
+                                       np.flatnonzero(mix.audited)[:2], np.flatnonzero(mix.topped_up)[:2]]))
+    for i in sample:
+        topped = bool(mix.topped_up[i])
+        w = int(mix.budget[i]) * (10 if topped else 1)
+        stream = ((1 << 31) + int(i)) if topped else int(i)
+        ref = oracle.walk_replay(rp, col, oracle.APP_PPR, w, tp=tp, seed=1, source=int(src[i]),
+                                 start_mode=oracle.START_SOURCE, stream=stream)
+        v, cnt = oracle.topk(ref["counts"], rp, 100)
+        assert np.array_equal(mix.topk_vertex[i], v), i
+        assert np.array_equal(mix.topk_count[i], cnt), i
+    for i in sample[:4]:
+        alone = c.serve(src[i:i + 1], eps[i:i + 1], [0.0], ids=[int(i)], **kw)
+        assert alone.budget[0] == mix.budget[i]
+        assert np.array_equal(alone.topk_vertex[0], mix.topk_vertex[i])
+        assert np.array_equal(alone.topk_count[0], mix.topk_count[i])
+    net.close()

@@ -8,6 +8,7 @@ import pytest
 
 import oracle
 import ptwalker_amd as ptw
+from ptwalker_amd.workloads import walk_sources
 from walkstats import random_graph
 
 pytestmark = pytest.mark.gpu
@@ -273,7 +274,8 @@ def test_cli_pagerank_end_to_end(ctx, tmp_path):
     ctx.rmat(scale=10, edge_factor=8, seed=4)
     g = str(tmp_path / "g.txt")
     ctx.write(g, ptw.FORMAT_TEXT)
-    rp, col, _ = oracle.load_edge_list(g)
+    rp, col, name2id = oracle.load_edge_list(g)
+    names = names_of(name2id)
     out = str(tmp_path / "counts.txt")
     exe = os.path.join(ptw.BIN_DIR, "pagerank")
     r = subprocess.run([exe, "-f", "text", "-g", g, "-w", "65536", "--sp", "0.15", "-s", "1", "-t", "20",
@@ -283,43 +285,159 @@ def test_cli_pagerank_end_to_end(ctx, tmp_path):
     assert lines[0].startswith("Partition time ")
     assert any(l.startswith("start to walk ") for l in lines)
     assert any(l.startswith("total time (including init start walker) ") for l in lines)
-    got = np.loadtxt(out, dtype=np.uint64)[:, 1]
+    got = np.loadtxt(out, dtype=np.uint64)
     ref = oracle.walk_replay(rp, col, oracle.APP_PAGERANK, 65536, tp=0.15, seed=1)
-    assert np.array_equal(got, ref["counts"])
+    assert np.array_equal(got[:, 0], names)  # the file's own ids, in load order
+    assert np.array_equal(got[:, 1], ref["counts"])
     assert lines[-1] == f"ppr value {int(ref['counts'][0]) & 0xFFFFFFFF}"
     steps = int(lines[-2].rsplit(" ", 1)[1])
     assert steps == ref["steps"]
+
+
+def names_of(name2id):
+    """The inverse of the loader's first-appearance map: file id of each graph id."""
+    names = np.zeros(len(name2id), dtype=np.uint32)
+    for name, v in name2id.items():
+        names[v] = name
+    return names
 
 
 def test_cli_ppr_rwr_deepwalk(ctx, tmp_path):
     ctx.rmat(scale=9, edge_factor=8, seed=6)
     g = str(tmp_path / "g.bin")
     ctx.write(g, ptw.FORMAT_BINARY)
-    rp, col, _ = oracle.load_edge_list(g, "binary")
+    rp, col, name2id = oracle.load_edge_list(g, "binary")
+    names = names_of(name2id)
     deg = np.diff(rp.astype(np.int64))
     hub = int(np.argmax(deg))
     b = ptw.BIN_DIR
     r = subprocess.run([f"{b}/PPR", "-f", "binary", "-g", g, "-w", "5000", "--sp", "0.2", "--out",
                         str(tmp_path / "p.txt")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert f"Root vertex {hub}" in r.stdout
+    assert f"Root vertex {names[hub]}" in r.stdout
     ref = oracle.walk_replay(rp, col, oracle.APP_PPR, 5000, tp=0.2, seed=1, source=hub, start_mode=oracle.START_SOURCE)
-    assert np.array_equal(np.loadtxt(tmp_path / "p.txt", dtype=np.uint64)[:, 1], ref["counts"])
+    got = np.loadtxt(tmp_path / "p.txt", dtype=np.uint64)
+    assert np.array_equal(got[:, 0], names) and np.array_equal(got[:, 1], ref["counts"])
     r = subprocess.run([f"{b}/RWR", "-f", "binary", "-g", g, "-w", "5000", "-l", "10", "--out",
                         str(tmp_path / "r.txt")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     ref = oracle.walk_replay(rp, col, oracle.APP_RWR, 5000, tp=0.5, seed=1)
-    assert np.array_equal(np.loadtxt(tmp_path / "r.txt", dtype=np.uint32)[:, 1], ref["ends"])
+    # a walker's start is the first vertex of a one-vertex DeepWalk path under the same start rule
+    starts = oracle.walk_replay(rp, col, oracle.APP_DEEPWALK, 5000, walk_len=1, seed=1)["paths"][:, 0]
+    pairs = np.loadtxt(tmp_path / "r.txt", dtype=np.uint32)
+    assert np.array_equal(pairs[:, 0], names[starts]) and np.array_equal(pairs[:, 1], names[ref["ends"]])
     r = subprocess.run([f"{b}/deepwalk", "-f", "binary", "-g", g, "-e", "2", "-l", "8", "--out",
                         str(tmp_path / "d.bin")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     V = len(rp) - 1
     ref = oracle.walk_replay(rp, col, oracle.APP_DEEPWALK, 2 * V, walk_len=8, seed=1)
     got = np.fromfile(tmp_path / "d.bin", dtype=np.uint32).reshape(2 * V, 8)
-    assert np.array_equal(got, ref["paths"])
+    assert np.array_equal(got, names[ref["paths"]])
     assert r.stdout.splitlines()[-1] == "ppr value 0"
+    # graph ids on request
+    r = subprocess.run([f"{b}/deepwalk", "-f", "binary", "-g", g, "-e", "2", "-l", "8", "--out-ids", "graph",
+                        "--out", str(tmp_path / "d.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(np.fromfile(tmp_path / "d.bin", dtype=np.uint32).reshape(2 * V, 8), ref["paths"])
+
+
+def sparse_id_file(path, n=400, m=5000, seed=3):
+    """A text edge list whose ids are sparse (up to 2^32 - 2) and appear out of order."""
+    rng = np.random.default_rng(seed)
+    ids = rng.choice(np.arange(1, 1 << 32, 9973, dtype=np.uint64), n, replace=False)
+    e = rng.integers(0, n, size=(m, 2))
+    e = e[e[:, 0] != e[:, 1]]
+    path.write_text("# sparse ids\n" + "\n".join(f"{ids[a]} {ids[b]}" for a, b in e) + "\n")
+    return ids
+
+
+def test_cli_original_ids_sparse_file(ctx, tmp_path):
+    """Every output of every binary in the file's own ids, on a file whose ids are
+    sparse and out of order; --source takes a file id too.  RWR writes the
+    reference's {start, end} pairs (src/core/RWR.cpp:9-13,99-105)."""
+    g = tmp_path / "g.txt"
+    sparse_id_file(g)
+    rp, col, name2id = oracle.load_edge_list(str(g), "text", True)
+    names = names_of(name2id)
+    assert not np.array_equal(names, np.arange(len(names)))
+    src = 17
+    b = ptw.BIN_DIR
+    common = ["-f", "text", "-g", str(g), "--undirected", "--seed", "3"]
+    r = subprocess.run([f"{b}/PPR", *common, "-w", "20000", "--sp", "0.2", "--source", str(names[src]),
+                        "--out", str(tmp_path / "p.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"Root vertex {names[src]}" in r.stdout
+    ref = oracle.walk_replay(rp, col, oracle.APP_PPR, 20000, tp=0.2, seed=3, source=src,
+                             start_mode=oracle.START_SOURCE)
+    got = np.loadtxt(tmp_path / "p.txt", dtype=np.uint64)
+    assert np.array_equal(got[:, 0], names) and np.array_equal(got[:, 1], ref["counts"])
+    r = subprocess.run([f"{b}/RWR", *common, "-w", "20000", "-l", "5", "--sp", "0.3", "--source", str(names[src]),
+                        "--out", str(tmp_path / "r.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = oracle.walk_replay(rp, col, oracle.APP_RWR, 20000, tp=0.3, seed=3, source=src,
+                             start_mode=oracle.START_SOURCE)
+    pairs = np.loadtxt(tmp_path / "r.txt", dtype=np.uint64)
+    assert np.all(pairs[:, 0] == names[src]) and np.array_equal(pairs[:, 1], names[ref["ends"]])
+    # an id that is not in the file is an error, not a silent vertex
+    r = subprocess.run([f"{b}/PPR", *common, "-w", "100", "--sp", "0.2", "--source", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "not a vertex" in r.stderr
+    ctx.load(str(g), ptw.FORMAT_TEXT, as_undirected=True)
+    assert np.array_equal(ctx.export_names(), names)
+
+
+def test_cli_batch_queries_topk(ctx, tmp_path):
+    """PPR and RWR batch mode: a query file in the file's own ids, one launch,
+    every query's top-k written in those ids, equal to the top-k of the
+    oracle's replay of that query alone (stream = its line)."""
+    g = tmp_path / "g.txt"
+    sparse_id_file(g, seed=5)
+    rp, col, name2id = oracle.load_edge_list(str(g), "text", True)
+    names = names_of(name2id)
+    qsrc = [0, 5, 11, 3]
+    spec = [(0, 30000, 0.2), (5, None, None), (11, 400, 0.3), (3, 150000, None)]
+    lines = ["# source walkers tp"] + [" ".join(str(x) for x in (names[s], w, t) if x is not None)
+                                        for s, w, t in spec]
+    (tmp_path / "q.txt").write_text("\n".join(lines) + "\n")
+    b = ptw.BIN_DIR
+    for exe, app in (("PPR", oracle.APP_PPR), ("RWR", oracle.APP_RWR)):
+        extra = ["-l", "5"] if exe == "RWR" else []
+        r = subprocess.run([f"{b}/{exe}", "-f", "text", "-g", str(g), "--undirected", "-w", "7000", "--sp", "0.25",
+                            *extra, "--queries", str(tmp_path / "q.txt"), "--topk", "12", "--seed", "4",
+                            "--out", str(tmp_path / "t.txt")], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        rows = np.loadtxt(tmp_path / "t.txt", dtype=np.uint64, comments="#")
+        for q, (s, w, t) in enumerate(spec):
+            w = 7000 if w is None else w
+            t = 0.25 if t is None else t
+            ref = oracle.walk_replay(rp, col, app, w, tp=t, seed=4, source=s, start_mode=oracle.START_SOURCE,
+                                     stream=q)
+            cnt = ref["counts"] if app == oracle.APP_PPR else np.bincount(ref["ends"], minlength=len(names))
+            v, c = oracle.topk(cnt, rp, 12)
+            mine = rows[rows[:, 0] == q]
+            keep = v != oracle.INVALID
+            assert np.all(mine[:, 1] == names[s]) and np.all(mine[:, 2] == (app == oracle.APP_RWR))
+            assert np.array_equal(mine[:, 3], np.arange(keep.sum()))
+            assert np.array_equal(mine[:, 4], names[v[keep]]) and np.array_equal(mine[:, 5], c[keep]), (exe, q)
 
 
 def test_gather_bench_runs(ctx):
     gps, ms = ctx.bench_gather(1 << 28, 1 << 26)
     assert gps > 1e9 and ms > 0
+
+
+@pytest.mark.parametrize("app_name", ["PPR", "RWR"])
+def test_config_scale22_1e8_walkers_full_replay(ctx22, app_name):
+    """The single-source configuration at its size: R-MAT scale 22, sources the
+    max-degree vertex and one uniform vertex, 1e8 walkers each, tp 0.2, Philox
+    seed 1 -- every one of the 5e8 steps per run replayed on the host threads,
+    counts (PPR) and endpoints (RWR) bit-exact (src/core/ppr.cpp:121-139,
+    src/core/RWR.cpp:109-128)."""
+    c, gi, rp, col = ctx22
+    for src in walk_sources(gi, seed=1):
+        app = ptw.PPR(0.2, src) if app_name == "PPR" else ptw.RWR(0.2, src)
+        print(f"config 2: {app_name} source {src} (degree {rp[src + 1] - rp[src]}), 1e8 walkers", flush=True)
+        got = check(c, rp, col, app, 100_000_000, seed=1)
+        assert abs(got.steps / 1e8 - 5.0) < 0.01  # E[L] = 1/tp
+        if app_name == "PPR":
+            assert int(got.counts.sum()) == 100_000_000 + got.steps
