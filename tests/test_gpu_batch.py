@@ -326,13 +326,14 @@ def replay_endpoints(rp, col, src, W, tp, stream, seed=1, semantics=oracle.SEM_R
 
 
 @pytest.mark.parametrize("semantics", [oracle.SEM_REFERENCE, oracle.SEM_TEXTBOOK])
-def test_batch_rwr_endpoint_queries(ctx, g12, semantics):
+def test_batch_rwr_endpoint_queries(ctx, semantics):
     """RWR queries in a batch count each walker's last vertex: a query's count
     vector is the histogram of the oracle's RWR replay of that query alone
     (its stream), dense and hash slices, mixed with PPR queries, alone and in
     the mix, eager and as a replayed captured graph; top-k of the endpoint
     counts is exact (src/core/RWR.cpp:99-105)."""
-    gi, rp, col = g12
+    gi = ctx.rmat(scale=12, edge_factor=16, seed=2)
+    rp, col = ctx.export_csr()
     src, W, tp = make_queries(gi, 24, seed=11, wmin=200, wmax=300000)
     kinds = np.array([ptw.QUERY_ENDPOINTS if i % 3 else ptw.QUERY_VISITS for i in range(24)])
     k = 30
@@ -370,10 +371,10 @@ def test_batch_rwr_endpoint_queries(ctx, g12, semantics):
         ctx.batch_ppr(src[:3], W[:3], tp[:3], topk=k, seed=7, kinds=kinds[:3])
 
 
-def test_batch_rwr_matches_single_walks(ctx, g12):
+def test_batch_rwr_matches_single_walks(ctx):
     """An endpoint query's histogram equals the histogram of ptw_walk's RWR
     endpoints with the same stream (the single-walk kernel)."""
-    gi, rp, col = g12
+    gi = ctx.rmat(scale=12, edge_factor=16, seed=2)
     src = int(gi.max_degree_vertex)
     r = ctx.batch_ppr([src, 3], [400000, 5000], [0.2, 0.3], streams=[9, 4], topk=10, seed=2,
                       kinds=ptw.QUERY_ENDPOINTS)
