@@ -238,6 +238,21 @@ int ptw_graph_write(ptw_ctx *ctx, const char *path, int format);
  * dangling stop).  Synchronous: returns when outputs are complete. */
 int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
 
+/* ptw_walk without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same process and
+ * results as ptw_walk; the outputs must be device buffers (PTW_FLAG_OUT_DEVICE).
+ * The walk's scratch -- the queue and total words and, for PAGERANK/PPR, the
+ * V device-id visit counters -- lives in a 32 MiB array in the library's
+ * device code, which each GPU holds until the process ends: count walks take
+ * graphs of at most 4,194,296 vertices (larger ones fail with
+ * PTW_ERR_INVALID_ARG); NODE2VEC fails with PTW_ERR_UNSUPPORTED (its edge set
+ * is built on first use).  The call allocates nothing, host memory included
+ * (the totals come back through page-locked words made at graph install).
+ * Every call on one GPU shares that array: calls that overlap on one GPU,
+ * from any context or thread, race. */
+int ptw_walk_static(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out);
+
 /* ---- batched queries -------------------------------------------------- */
 /* Many single-source queries in one persistent launch: PPR queries count
  * every visit (type 0, the start included), RWR queries count each walker's

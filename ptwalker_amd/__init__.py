@@ -298,6 +298,7 @@ _SIGNATURES = {
     "ptw_graph_export_names": (ctypes.c_int, [_P, _P]),
     "ptw_graph_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "ptw_walk": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
+    "ptw_walk_static": (ctypes.c_int, [_P, ctypes.POINTER(WalkDesc), ctypes.POINTER(WalkOut)]),
     "ptw_batch_ppr": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchOut)]),
     "ptw_batch_plan_bytes": (ctypes.c_int, [_P, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_batch_counts": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
@@ -469,6 +470,14 @@ class Context:
                                      f"entries")
             out.counts, out.ends, out.paths, out.starts = _ptr(counts), _ptr(ends), _ptr(paths), _ptr(starts)
         self.check(self.lib.ptw_walk(self.h, ctypes.byref(desc), ctypes.byref(out)))
+        return out
+
+    def walk_static(self, desc: WalkDesc, counts=None, ends=None, paths=None, starts=None) -> WalkOut:
+        """ptw_walk_static: the walk without any allocation.  desc.flags must hold
+        FLAG_OUT_DEVICE and the outputs are device addresses (ints)."""
+        out = WalkOut()
+        out.counts, out.ends, out.paths, out.starts = counts, ends, paths, starts
+        self.check(self.lib.ptw_walk_static(self.h, ctypes.byref(desc), ctypes.byref(out)))
         return out
 
     # ---- batched queries -------------------------------------------------

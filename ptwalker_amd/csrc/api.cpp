@@ -239,6 +239,170 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+bool is_count_app(int app) { return app == PTW_APP_PAGERANK || app == PTW_APP_PPR; }
+bool is_path_app(int app) { return app == PTW_APP_DEEPWALK || app == PTW_APP_NODE2VEC; }
+
+// Descriptor and output checks shared by ptw_walk and ptw_walk_static.
+int check_walk(ptw_ctx *ctx, const ptw_walk_desc &d, const ptw_walk_out &out)
+{
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: walks need a HIP device");
+    const uint32_t V = ctx->dev.V;
+    if (d.app < PTW_APP_PAGERANK || d.app > PTW_APP_NODE2VEC) return fail(ctx, PTW_ERR_INVALID_ARG, "unknown app");
+    const bool path_app = is_path_app(d.app), count_app = is_count_app(d.app);
+    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
+    if (d.start_mode < PTW_START_UNIFORM || d.start_mode > PTW_START_ROUNDROBIN)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown start mode");
+    if (d.dangling_policy != PTW_DANGLING_ERROR && d.dangling_policy != PTW_DANGLING_STOP)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown dangling policy");
+    if (!path_app && !(d.tp > 0.0 && d.tp <= 1.0))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "termination probability must be in (0, 1]");
+    if (path_app && d.walk_len == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "walk length must be >= 1");
+    if (d.app == PTW_APP_NODE2VEC) {
+        // attempts are counted in bits 20.. of the step word
+        if (d.walk_len > (1u << 20)) return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec walk length must be <= 2^20");
+        const double lo = 1.0 / 65536.0, hi = 65536.0;
+        if (!(d.p >= lo && d.p <= hi) || !(d.q >= lo && d.q <= hi))
+            return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec p and q must be in [2^-16, 2^16]");
+        if (d.flags & PTW_FLAG_WEIGHTED)
+            return fail(ctx, PTW_ERR_UNSUPPORTED, "node2vec on weighted graphs is not supported");
+    }
+    if (d.start_mode == PTW_START_SOURCE && d.source >= V)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "source vertex " + std::to_string(d.source) + " out of range");
+    if (count_app && !out.counts) return fail(ctx, PTW_ERR_INVALID_ARG, "counts output required");
+    if (d.app == PTW_APP_RWR && !out.ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
+    if (d.app != PTW_APP_RWR && out.starts)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "starts output is for RWR walks only");
+    if (path_app && !out.paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
+    if (path_app && d.walkers > UINT64_MAX / 4 / d.walk_len)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "path buffer size overflows");
+    if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
+    if ((d.flags & PTW_FLAG_WEIGHTED) && !ctx->dev.wedge)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
+    if ((d.flags & PTW_FLAG_OUT_DEVICE) &&
+        (((uintptr_t)out.paths & 15) || ((uintptr_t)out.ends & 3) || ((uintptr_t)out.starts & 3) ||
+         ((uintptr_t)out.counts & 7)))
+        return fail(ctx, PTW_ERR_INVALID_ARG,
+                    "device outputs must be aligned: paths 16 B, ends and starts 4 B, counts 8 B");
+    return PTW_OK;
+}
+
+// Launch parameters from the descriptor: graph, process, node2vec weights.
+// Outputs, counters and scratch are the caller's to set.
+int make_walk_launch(ptw_ctx *ctx, const ptw_walk_desc &d, ptw::WalkLaunch &p)
+{
+    p = ptw::WalkLaunch{};
+    p.row_ptr = ctx->dev.row_ptr;
+    p.col = ctx->dev.col;
+    p.perm = ctx->dev.perm;
+    p.inv = ctx->dev.inv;
+    p.wedge = (d.flags & PTW_FLAG_WEIGHTED) ? ctx->dev.wedge : nullptr;
+    p.V = ctx->dev.V;
+    p.hubs = ctx->dev.hubs;
+    p.app = d.app;
+    p.semantics = d.semantics;
+    p.start_mode = d.start_mode;
+    p.source_dev = d.start_mode == PTW_START_SOURCE && ctx->dev.relabelled ? ctx->h_perm[d.source] : d.source;
+    p.walk_len = d.walk_len;
+    p.thr = ptw::continue_threshold(d.tp);
+    p.seed_lo = (uint32_t)d.seed;
+    p.seed_hi = (uint32_t)(d.seed >> 32);
+    p.stream = d.stream;
+    p.wid_base = d.wid_base;
+    p.walkers = d.walkers;
+    if (d.app == PTW_APP_NODE2VEC) {
+        if (int rc = ensure_node2vec_tables(ctx)) return rc;
+        p.eset = ctx->dev.eset;
+        p.eset_mask = ctx->dev.eset_mask;
+        // Integer weights floor(2^32 w / M).  Unfolded, M = max(1/p, 1, 1/q)
+        // and every weight is an acceptance threshold in [0, 2^32].  Folded
+        // (1/p above both other weights on a simple symmetric graph), M =
+        // max(1, 1/q) and the return weight's excess over 2^32 is split over
+        // ceil(excess / 2^32) extra proposal slots.
+        const double wr = 1.0 / d.p, wo = 1.0 / d.q;
+        const double Mf = std::max(1.0, wo);
+        const uint64_t t_ret = (uint64_t)std::floor(wr / Mf * 4294967296.0);  // <= 2^48
+        const bool fold = ctx->dev.simple_symmetric && wr > Mf && t_ret > 4294967296ull &&
+                          ctx->max_degree < (1u << 31);
+        const double M = fold ? Mf : std::max(wr, Mf);
+        auto thr = [M](double w) {
+            const double t = std::floor(w / M * 4294967296.0);
+            return t >= 4294967296.0 ? 4294967296ull : (uint64_t)t;
+        };
+        p.w_in = thr(1.0);
+        p.w_out = thr(wo);
+        if (fold) {
+            const uint64_t excess = t_ret - 4294967296ull;
+            const uint64_t slots = (excess + 4294967295ull) >> 32;  // in [1, 2^16]
+            p.n2v_fold = (uint32_t)slots;
+            p.w_fold = excess / slots;  // <= 2^32
+            p.w_ret = 4294967296ull + slots * p.w_fold;
+        } else {
+            p.n2v_fold = 0;
+            p.w_fold = 0;
+            p.w_ret = thr(wr);
+        }
+    }
+    return PTW_OK;
+}
+
+// The context's launch shape with cache_slots = PTW_CACHE_AUTO resolved for
+// this walk.  The visit cache pays only for the hot neighbourhood of a single
+// source.  Uniform starts spread their visits, and a source with more
+// neighbours than four times the cache spreads its first-step visits too thin
+// (about two per workgroup per neighbour at 1e8 walkers from a 1.6e5-degree
+// hub), so there the cache's LDS goes to hub counters (4 per slot).  An
+// explicit cache_slots is used as given.
+ptw::LaunchShape walk_shape(const ptw_ctx *ctx, const ptw_walk_desc &d)
+{
+    ptw::LaunchShape shape = ctx->shape;
+    if (shape.cache_slots < 0) {
+        const bool keep_cache = d.start_mode == PTW_START_SOURCE &&
+                                (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] <=
+                                    4ull * ptw::kAutoCacheSlots;
+        if (is_count_app(d.app) && !keep_cache) {
+            shape.lds_hubs = std::min(ptw::kMaxLdsHubs, shape.lds_hubs + 4 * ptw::kAutoCacheSlots);
+            shape.cache_slots = 0;
+        } else {
+            shape.cache_slots = ptw::kAutoCacheSlots;
+        }
+    }
+    return shape;
+}
+
+void clear_walk_out(ptw_walk_out *out)
+{
+    out->steps = out->visits = out->dangling = 0;
+    out->kernel_ms = out->total_ms = 0.0;
+    out->launch_block = out->launch_blocks = out->launch_chunk = 0;
+    out->launch_lds_hubs = out->launch_cache_slots = out->launch_reserved = 0;
+}
+
+// Totals from the scratch words (copied to host: [1] steps, [2] dangling),
+// the kernel time and the launch shape, then the dangling-walker verdict.
+int finish_walk_out(ptw_ctx *ctx, const ptw_walk_desc &d, const uint64_t *sc, const ptw::EffectiveShape &eff,
+                    double t0, ptw_walk_out *out)
+{
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out->kernel_ms = ms;
+    out->steps = sc[1];
+    out->dangling = sc[2];
+    out->visits = is_count_app(d.app) ? d.walkers + sc[1] : 0;
+    out->total_ms = now_ms() - t0;
+    out->launch_block = (uint32_t)eff.block;
+    out->launch_blocks = (uint32_t)eff.blocks;
+    out->launch_chunk = eff.chunk;
+    out->launch_lds_hubs = eff.hubs;
+    out->launch_cache_slots = eff.slots;
+    out->launch_reserved = 0;
+    if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
+        return fail(ctx, PTW_ERR_DANGLING,
+                    std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
+    return PTW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -278,6 +442,7 @@ int ptw_create(ptw_ctx **out, int device)
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc((void **)&ctx->scratch, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        ptw::static_scratch_address(device, &ctx->static_scratch) != hipSuccess ||
         // first launch loads the device code, so no walk pays for it
         ptw::launch_fill_u64(ctx->scratch, 8, 0ull, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -461,52 +626,14 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
 {
     if (!ctx) return PTW_ERR_INVALID_ARG;
     if (!desc || !out) return fail(ctx, PTW_ERR_INVALID_ARG, "null descriptor or output");
-    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
-    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: walks need a HIP device");
     const double t0 = now_ms();
     const ptw_walk_desc &d = *desc;
+    if (int rc = check_walk(ctx, d, *out)) return rc;
     const uint32_t V = ctx->dev.V;
-    if (d.app < PTW_APP_PAGERANK || d.app > PTW_APP_NODE2VEC) return fail(ctx, PTW_ERR_INVALID_ARG, "unknown app");
-    const bool path_app = d.app == PTW_APP_DEEPWALK || d.app == PTW_APP_NODE2VEC;
-    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
-    if (d.start_mode < PTW_START_UNIFORM || d.start_mode > PTW_START_ROUNDROBIN)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown start mode");
-    if (d.dangling_policy != PTW_DANGLING_ERROR && d.dangling_policy != PTW_DANGLING_STOP)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown dangling policy");
-    const bool count_app = d.app == PTW_APP_PAGERANK || d.app == PTW_APP_PPR;
-    if (!path_app && !(d.tp > 0.0 && d.tp <= 1.0))
-        return fail(ctx, PTW_ERR_INVALID_ARG, "termination probability must be in (0, 1]");
-    if (path_app && d.walk_len == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "walk length must be >= 1");
-    if (d.app == PTW_APP_NODE2VEC) {
-        // attempts are counted in bits 20.. of the step word
-        if (d.walk_len > (1u << 20)) return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec walk length must be <= 2^20");
-        const double lo = 1.0 / 65536.0, hi = 65536.0;
-        if (!(d.p >= lo && d.p <= hi) || !(d.q >= lo && d.q <= hi))
-            return fail(ctx, PTW_ERR_INVALID_ARG, "node2vec p and q must be in [2^-16, 2^16]");
-        if (d.flags & PTW_FLAG_WEIGHTED)
-            return fail(ctx, PTW_ERR_UNSUPPORTED, "node2vec on weighted graphs is not supported");
-    }
-    if (d.start_mode == PTW_START_SOURCE && d.source >= V)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "source vertex " + std::to_string(d.source) + " out of range");
-    if (count_app && !out->counts) return fail(ctx, PTW_ERR_INVALID_ARG, "counts output required");
-    if (d.app == PTW_APP_RWR && !out->ends) return fail(ctx, PTW_ERR_INVALID_ARG, "ends output required");
-    if (d.app != PTW_APP_RWR && out->starts)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "starts output is for RWR walks only");
-    if (path_app && !out->paths) return fail(ctx, PTW_ERR_INVALID_ARG, "paths output required");
-    if (d.wid_base + d.walkers < d.wid_base) return fail(ctx, PTW_ERR_INVALID_ARG, "walker id range overflows");
-    const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
-    if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted walk on a graph without weights");
+    const bool count_app = is_count_app(d.app), path_app = is_path_app(d.app);
     const bool on_dev = (d.flags & PTW_FLAG_OUT_DEVICE) != 0;
-    if (on_dev && (((uintptr_t)out->paths & 15) || ((uintptr_t)out->ends & 3) || ((uintptr_t)out->starts & 3) ||
-                   ((uintptr_t)out->counts & 7)))
-        return fail(ctx, PTW_ERR_INVALID_ARG,
-                    "device outputs must be aligned: paths 16 B, ends and starts 4 B, counts 8 B");
     const bool accumulate = (d.flags & PTW_FLAG_ACCUMULATE) != 0;
-    out->steps = out->visits = out->dangling = 0;
-    out->kernel_ms = 0.0;
-    out->launch_block = out->launch_blocks = out->launch_chunk = 0;
-    out->launch_lds_hubs = out->launch_cache_slots = out->launch_reserved = 0;
+    clear_walk_out(out);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
 
@@ -532,7 +659,6 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
         }
     }
     if (path_app) {
-        if (W > UINT64_MAX / 4 / d.walk_len) return fail(ctx, PTW_ERR_INVALID_ARG, "path buffer size overflows");
         if (on_dev) {
             paths_dev = out->paths;
         } else if (W) {
@@ -552,95 +678,18 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
         }
     }
 
-    uint32_t source_dev = 0;
-    if (d.start_mode == PTW_START_SOURCE) {
-        source_dev = ctx->dev.relabelled ? ctx->h_perm[d.source] : d.source;
-    }
-
-    ptw::WalkLaunch p{};
-    p.row_ptr = ctx->dev.row_ptr;
-    p.col = ctx->dev.col;
-    p.perm = ctx->dev.perm;
-    p.inv = ctx->dev.inv;
-    p.wedge = weighted ? ctx->dev.wedge : nullptr;
-    p.V = V;
-    p.hubs = ctx->dev.hubs;
-    p.app = d.app;
-    p.semantics = d.semantics;
-    p.start_mode = d.start_mode;
-    p.source_dev = source_dev;
-    p.walk_len = d.walk_len;
-    {
-        double c = std::floor((1.0 - d.tp) * 4294967296.0);
-        if (c < 0.0) c = 0.0;
-        if (c > 4294967295.0) c = 4294967295.0;
-        p.thr = (uint32_t)c;
-    }
-    p.seed_lo = (uint32_t)d.seed;
-    p.seed_hi = (uint32_t)(d.seed >> 32);
-    p.stream = d.stream;
-    p.wid_base = d.wid_base;
-    p.walkers = W;
+    ptw::WalkLaunch p;
+    if (int rc = make_walk_launch(ctx, d, p)) return rc;
     p.counts_dev = ctx->counts_dev;
     p.ends = ends_dev;
     p.starts = starts_dev;
     p.paths = paths_dev;
     p.scratch = ctx->scratch;
-    if (d.app == PTW_APP_NODE2VEC) {
-        if (int rc = ensure_node2vec_tables(ctx)) return rc;
-        p.eset = ctx->dev.eset;
-        p.eset_mask = ctx->dev.eset_mask;
-        // Integer weights floor(2^32 w / M).  Unfolded, M = max(1/p, 1, 1/q)
-        // and every weight is an acceptance threshold in [0, 2^32].  Folded
-        // (1/p above both other weights on a simple symmetric graph), M =
-        // max(1, 1/q) and the return weight's excess over 2^32 is split over
-        // ceil(excess / 2^32) extra proposal slots.
-        const double wr = 1.0 / d.p, wo = 1.0 / d.q;
-        const double Mf = std::max(1.0, wo);
-        const uint64_t t_ret = (uint64_t)std::floor(wr / Mf * 4294967296.0);  // <= 2^48
-        const bool fold = ctx->dev.simple_symmetric && wr > Mf && t_ret > 4294967296ull &&
-                          ctx->max_degree < (1u << 31);
-        const double M = fold ? Mf : std::max(wr, Mf);
-        auto thr = [M](double w) {
-            const double t = std::floor(w / M * 4294967296.0);
-            return t >= 4294967296.0 ? 4294967296ull : (uint64_t)t;
-        };
-        p.w_in = thr(1.0);
-        p.w_out = thr(wo);
-        if (fold) {
-            const uint64_t excess = t_ret - 4294967296ull;
-            const uint64_t slots = (excess + 4294967295ull) >> 32;  // in [1, 2^16]
-            p.n2v_fold = (uint32_t)slots;
-            p.w_fold = excess / slots;  // <= 2^32
-            p.w_ret = 4294967296ull + slots * p.w_fold;
-        } else {
-            p.n2v_fold = 0;
-            p.w_fold = 0;
-            p.w_ret = thr(wr);
-        }
-    }
 
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    // cache_slots = PTW_CACHE_AUTO: the visit cache pays only for the hot
-    // neighbourhood of a single source.  Uniform starts spread their visits,
-    // and a source with more neighbours than four times the cache spreads its
-    // first-step visits too thin (about two per workgroup per neighbour at
-    // 1e8 walkers from a 1.6e5-degree hub), so there the cache's LDS goes to
-    // hub counters (4 per slot).  An explicit cache_slots is used as given.
-    ptw::LaunchShape shape = ctx->shape;
-    if (shape.cache_slots < 0) {
-        const bool keep_cache = d.start_mode == PTW_START_SOURCE &&
-                                (uint64_t)ctx->host.row_ptr[d.source + 1] - ctx->host.row_ptr[d.source] <=
-                                    4ull * ptw::kAutoCacheSlots;
-        if (count_app && !keep_cache) {
-            shape.lds_hubs = std::min(ptw::kMaxLdsHubs, shape.lds_hubs + 4 * ptw::kAutoCacheSlots);
-            shape.cache_slots = 0;
-        } else {
-            shape.cache_slots = ptw::kAutoCacheSlots;
-        }
-    }
+    const ptw::LaunchShape shape = walk_shape(ctx, d);
     ptw::EffectiveShape eff;
     if (W) HIP_TRY(ctx, ptw::launch_walk(p, shape, ctx->num_cus, s, &eff));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
@@ -661,24 +710,62 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
     if (!on_dev && count_app) std::memcpy(out->counts, ctx->pinned + 8, (size_t)V * 8);
-    const uint64_t *sc = ctx->pinned;
-    float ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    out->kernel_ms = ms;
-    out->steps = sc[1];
-    out->dangling = sc[2];
-    out->visits = count_app ? W + sc[1] : 0;
-    out->total_ms = now_ms() - t0;
-    out->launch_block = (uint32_t)eff.block;
-    out->launch_blocks = (uint32_t)eff.blocks;
-    out->launch_chunk = eff.chunk;
-    out->launch_lds_hubs = eff.hubs;
-    out->launch_cache_slots = eff.slots;
-    out->launch_reserved = 0;
-    if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
-        return fail(ctx, PTW_ERR_DANGLING,
-                    std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
-    return PTW_OK;
+    return finish_walk_out(ctx, d, ctx->pinned, eff, t0, out);
+}
+
+int ptw_walk_static(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!desc || !out) return fail(ctx, PTW_ERR_INVALID_ARG, "null descriptor or output");
+    const double t0 = now_ms();
+    const ptw_walk_desc &d = *desc;
+    if (int rc = check_walk(ctx, d, *out)) return rc;
+    const uint32_t V = ctx->dev.V;
+    const bool count_app = is_count_app(d.app);
+    if (!(d.flags & PTW_FLAG_OUT_DEVICE))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "ptw_walk_static writes device outputs only: set PTW_FLAG_OUT_DEVICE");
+    if (d.app == PTW_APP_NODE2VEC)
+        return fail(ctx, PTW_ERR_UNSUPPORTED,
+                    "ptw_walk_static does not run node2vec (its edge set is allocated on first use): use ptw_walk");
+    if (count_app && V > ptw::kStaticMaxVertices)
+        return fail(ctx, PTW_ERR_INVALID_ARG,
+                    "ptw_walk_static counts at most " + std::to_string(ptw::kStaticMaxVertices) +
+                        " vertices in its static scratch; the graph has " + std::to_string(V) + ": use ptw_walk");
+    if (!ctx->static_scratch) return fail(ctx, PTW_ERR_HIP, "no static scratch address for this device");
+    clear_walk_out(out);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    // [0, 8): queue and totals; [8, 8 + V): device-id counters.  The array
+    // keeps the last call's contents, so both are cleared first.
+    unsigned long long *const words = ctx->static_scratch;
+    unsigned long long *const counters = words + 8;
+
+    ptw::WalkLaunch p;
+    if (int rc = make_walk_launch(ctx, d, p)) return rc;
+    p.counts_dev = counters;
+    p.ends = d.app == PTW_APP_RWR ? out->ends : nullptr;
+    p.starts = d.app == PTW_APP_RWR ? out->starts : nullptr;
+    p.paths = is_path_app(d.app) ? out->paths : nullptr;
+    p.scratch = words;
+
+    HIP_TRY(ctx, ptw::launch_fill_u64(words, 8, 0ull, s));
+    if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(counters, V, 0ull, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    const ptw::LaunchShape shape = walk_shape(ctx, d);
+    ptw::EffectiveShape eff;
+    if (d.walkers) HIP_TRY(ctx, ptw::launch_walk(p, shape, ctx->num_cus, s, &eff));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    if (count_app) {
+        const bool from_source = d.start_mode == PTW_START_SOURCE;
+        HIP_TRY(ctx, ptw::launch_counts_to_graph_ids(counters, ctx->dev.perm, V, out->counts,
+                                                     (d.flags & PTW_FLAG_ACCUMULATE) != 0,
+                                                     from_source ? d.source : PTW_INVALID_VERTEX,
+                                                     from_source ? d.walkers : 0, s));
+    }
+    // the totals travel through the page-locked words the graph install made
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, words, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return finish_walk_out(ctx, d, ctx->pinned, eff, t0, out);
 }
 
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)

@@ -71,6 +71,9 @@ struct ptw_ctx {
     uint64_t *counts_stage = nullptr;          // V graph-id counts staged for a host output
     uint64_t *pinned = nullptr;                // host, page-locked: [0, 8) scratch words, [8, 8 + V) counts
     unsigned long long *scratch = nullptr;     // 8 words
+    // this device's address of the no-allocation walk's static scratch
+    // (looked up at ptw_create with the device current)
+    unsigned long long *static_scratch = nullptr;
     ptw::LaunchShape shape;
     ptw::BatchState batch;
 };

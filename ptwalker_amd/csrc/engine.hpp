@@ -147,6 +147,15 @@ struct EffectiveShape {
 constexpr int kAutoCacheSlots = 512;
 constexpr int kMaxLdsHubs = 81408;
 
+// The no-allocation walk's scratch (walk.hip): 8 control words, then device-id
+// u64 visit counters for graphs of at most kStaticMaxVertices vertices.
+constexpr uint64_t kStaticScratchWords = 4ull << 20;  // 32 MiB
+constexpr uint64_t kStaticMaxVertices = kStaticScratchWords - 8;
+constexpr int kMaxStaticDevices = 64;
+// Device address of the static scratch on `device`, which must be the current
+// device; looked up once per device and remembered.
+hipError_t static_scratch_address(int device, unsigned long long **out);
+
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 // Inserts every CSR edge into the edge set (table of mask + 1 slots, filled
 // with kEdgeSetEmpty beforehand, at least twice the edges).

@@ -19,6 +19,7 @@
 // per LDS word, flushed once per workgroup; other visits go to an LDS visit
 // cache or to device-scope u64 atomics.
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/ptw.h"
 #include "engine.hpp"
@@ -26,6 +27,31 @@
 #include "wave.hpp"
 
 namespace ptw {
+
+// Scratch of ptw_walk_static, which allocates nothing: kStaticScratchWords
+// u64 words (32 MiB) on every GPU the library runs on, held until the process
+// ends.  [0, 8) are the queue and total words of a walk; the rest are the
+// device-id visit counters of count apps.  In host code this name is a
+// host-side stand-in, not a device address: host code uses only the address
+// static_scratch_address returns for the current device.
+static __device__ __align__(256) unsigned long long scratch[kStaticScratchWords];
+
+hipError_t static_scratch_address(int device, unsigned long long **out)
+{
+    static std::mutex mu;
+    static unsigned long long *addr[kMaxStaticDevices] = {};
+    if (device < 0 || device >= kMaxStaticDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        // each GPU holds its own copy of the array; the caller made `device` current
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<unsigned long long *>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
 
 namespace {
 
