@@ -38,13 +38,16 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
-    ap.add_argument("--workload", default="ppr-rwr", choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest", "node2vec"],
+    ap.add_argument("--workload", default="ppr-rwr",
+                    choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest", "node2vec", "gather"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
                          "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22; "
                          "cli: the reference's pagerank command on R-MAT scale 16, binary vs CPU path; "
                          "ingest: loading an R-MAT scale 22 edge list, device vs host parsing; "
-                         "node2vec: second-order walks on R-MAT scale 22")
+                         "node2vec: second-order walks on R-MAT scale 22; "
+                         "gather: random u32 gather rate against table size (the Infinity Cache knee), and "
+                         "the stationary-step rate on R-MAT scale 22")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="2000,20000", help="serve: Poisson arrival rates (queries/s)")
@@ -78,6 +81,8 @@ def main():
         return bench_ingest(args)
     if args.workload == "node2vec":
         return bench_node2vec(args)
+    if args.workload == "gather":
+        return bench_gather_sweep(args)
 
     import numpy as np
 
@@ -132,10 +137,17 @@ def main():
     rwr_ms = float(np.mean(rec["rwr_ms"]))
     rwr_steps = float(np.mean(rec["rwr_steps"]))
     rwr_gbs = (12.0 * rwr_steps + 4.0 * W) / (rwr_ms * 1e-3) / 1e9
-    traffic = None
+    traffic = traffic_source = None
+    req_per_step = lines_per_step = None
     if os.path.exists(args.traffic):
         with open(args.traffic) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        req_per_step = tj.get("memory_requests_per_step")
+        lines_per_step = tj.get("l2_miss_lines_per_step")
+        traffic_source = (f"not measured in this run: read from {os.path.relpath(args.traffic, ROOT)}, the "
+                          "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of this kernel on this workload scaled by the "
+                          "copy-kernel calibration (profiles/walk_count_rmat22_traffic.md)")
 
     # measured random-gather roofline at the CSR's col[] footprint, and the
     # stream-copy rate (also the known-size kernel for counter calibration)
@@ -144,6 +156,8 @@ def main():
     copy_gbs, copy_ms = ctx.bench_copy(1 << 30)
     ppr_sps = ppr_steps / (ppr_ms * 1e-3)
     gather_frac = 2.0 * ppr_sps / gps
+    # the step's own access pattern from degree-drawn vertices, steps independent
+    step_sps, step_ms = ctx.bench_step(200_000_000, seed=1)
 
     cpu = None
     if not args.no_cpu:
@@ -181,6 +195,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "bytes_per_step": 20,
             "kernel_ms": ppr_ms,
             "steps_per_launch": ppr_steps,
@@ -191,6 +206,28 @@ def main():
             "frac": gather_frac,
             "note": "(PPR steps/s x 2 random gathers per step) / measured random u32 gathers/s",
         },
+        # the bound the counters show: memory requests (128 B L2-miss lines and
+        # 32 B memory-side atomics), against the random-gather request rate
+        # measured in this run (one 128 B line per gather)
+        "request_roofline": None if req_per_step is None else {
+            "unit": "memory requests/s",
+            "requests_per_step": req_per_step,
+            "lines_per_step": lines_per_step,
+            "achieved": req_per_step * ppr_sps,
+            "peak": gps,
+            "frac": req_per_step * ppr_sps / gps,
+            "lines_frac": lines_per_step * ppr_sps / gps,
+            "source": f"per-step counts from {os.path.relpath(args.traffic, ROOT)} (rocprofv3 TCC counters, "
+                      "profiles/walk_kernels_rmat22_counters.md); rates from this run",
+        },
+        "stationary_step": {
+            "steps_per_s": step_sps,
+            "kernel_ms": step_ms,
+            "frac": ppr_sps / step_sps,
+            "rwr_frac": (rwr_steps / (rwr_ms * 1e-3)) / step_sps,
+            "note": "PPR (and RWR) kernel steps/s / the rate of independent steps (row_ptr pair + one random col "
+                    "entry) from 2e8 vertices drawn in proportion to degree, ptw_bench_step",
+        },
         "stream_copy": {"GBs": copy_gbs, "bytes": 2 * (1 << 30), "kernel_ms": copy_ms},
         "rwr": {"kernel_ms": rwr_ms, "achieved_GBs": rwr_gbs, "bytes_per_step": 12},
         "cpu_baseline": cpu,
@@ -198,6 +235,32 @@ def main():
     print(json.dumps(line), flush=True)
     counts.free()
     ends.free()
+    ctx.close()
+
+
+def bench_gather_sweep(args):
+    """Random u32 gathers (1e9, Philox indices, 8 in flight per lane) against the
+    table size, 64 MiB to 16 GiB: tables that fit the 256 MB Infinity Cache and
+    tables far beyond it show the cache's share of a random gather; then the
+    stationary-step rate on the R-MAT graph (ptw_bench_step)."""
+    import ptwalker_amd as ptw
+
+    ctx = ptw.Context(0)
+    rows = []
+    for mib in (64, 128, 256, 513, 1024, 2048, 4096, 16384):
+        gps, ms = ctx.bench_gather(mib << 20, 1 << 30, seed=1)
+        rows.append({"table_MiB": mib, "gathers_per_s": gps, "kernel_ms": ms})
+        print(f"# gather {mib} MiB: {gps:.3e}/s", file=sys.stderr, flush=True)
+    gi = ctx.rmat(scale=args.scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1)
+    sps, sms = ctx.bench_step(200_000_000, seed=1)
+    copy_gbs, _ = ctx.bench_copy(1 << 30)
+    line = {"metric": "random gathers/s", "value": rows[-1]["gathers_per_s"], "unit": "gathers/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": "random u32 gathers by table size; stationary steps on R-MAT scale "
+                                   f"{args.scale}", "vertices": gi.num_vertices, "edges": gi.num_edges},
+            "gather": rows, "stationary_step": {"steps_per_s": sps, "kernel_ms": sms},
+            "stream_copy_GBs": copy_gbs}
+    print(json.dumps(line), flush=True)
     ctx.close()
 
 

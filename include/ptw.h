@@ -485,6 +485,15 @@ int ptw_device_memory(ptw_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes)
  * kernel is measured against. */
 int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed,
                      double *gathers_per_s, double *kernel_ms);
+/* Walk-step yardstick on the resident graph: `steps` vertices drawn in
+ * proportion to out-degree (the source of a uniform random edge: on a
+ * symmetric graph, the stationary state of a walk), then one step from each
+ * -- row_ptr[v], row_ptr[v + 1] and one uniformly picked col entry -- with
+ * the steps independent of each other and their vertices read in order.
+ * Reports steps/s and the kernel time: the memory system's rate for a step's
+ * own accesses, without the dependence between a walker's steps.  No
+ * counterpart in the reference. */
+int ptw_bench_step(ptw_ctx *ctx, uint64_t steps, uint64_t seed, double *steps_per_s, double *kernel_ms);
 /* Streaming copy of `bytes` (multiple of 16) between two device buffers:
  * the stream roofline, and the known-size kernel that calibrates HBM byte
  * counters.  No counterpart in the reference. */

@@ -326,6 +326,7 @@ _SIGNATURES = {
     "ptw_buffer_download": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
     "ptw_buffer_zero": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "ptw_device_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "ptw_bench_step": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _P, _P]),
     "ptw_bench_gather": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "ptw_bench_copy": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
@@ -643,6 +644,13 @@ class Context:
         ms = ctypes.c_double()
         self.check(self.lib.ptw_bench_gather(self.h, table_bytes, gathers, seed, ctypes.byref(gps), ctypes.byref(ms)))
         return gps.value, ms.value
+
+    def bench_step(self, steps: int, seed: int = 1):
+        """Independent walk steps from degree-drawn vertices (ptw_bench_step): (steps/s, ms)."""
+        sps = ctypes.c_double()
+        ms = ctypes.c_double()
+        self.check(self.lib.ptw_bench_step(self.h, steps, seed, ctypes.byref(sps), ctypes.byref(ms)))
+        return sps.value, ms.value
 
 
 class BudgetMLP:

@@ -906,6 +906,39 @@ int ptw_bench_copy(ptw_ctx *ctx, uint64_t bytes, double *gb_per_s, double *kerne
     return PTW_OK;
 }
 
+int ptw_bench_step(ptw_ctx *ctx, uint64_t steps, uint64_t seed, double *steps_per_s, double *kernel_ms)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (steps == 0 || ctx->dev.E == 0) return fail(ctx, PTW_ERR_INVALID_ARG, "no steps or no edges");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } starts, sink;
+    HIP_TRY(ctx, hipMalloc(&starts.p, steps * 4));
+    HIP_TRY(ctx, hipMalloc(&sink.p, 8));
+    hipStream_t s = ctx->stream;
+    const ptw::DeviceGraph &g = ctx->dev;
+    HIP_TRY(ctx, ptw::launch_fill_u64((unsigned long long *)sink.p, 1, 0ull, s));
+    HIP_TRY(ctx, ptw::launch_stationary_starts(g.row_ptr, g.V, g.E, steps, (uint32_t)seed, (uint32_t)(seed >> 32),
+                                               (uint32_t *)starts.p, s));
+    // warm-up launch over a slice, then the timed one
+    HIP_TRY(ctx, ptw::launch_step_bench(g.row_ptr, g.col, (const uint32_t *)starts.p, steps / 8 + 1, (uint32_t)seed,
+                                        (uint32_t)(seed >> 32) ^ 1u, (unsigned long long *)sink.p, ctx->num_cus, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    HIP_TRY(ctx, ptw::launch_step_bench(g.row_ptr, g.col, (const uint32_t *)starts.p, steps, (uint32_t)seed,
+                                        (uint32_t)(seed >> 32), (unsigned long long *)sink.p, ctx->num_cus, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (steps_per_s) *steps_per_s = ms > 0 ? (double)steps / (ms * 1e-3) : 0.0;
+    return PTW_OK;
+}
+
 int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed, double *gathers_per_s,
                      double *kernel_ms)
 {

@@ -356,6 +356,12 @@ hipError_t launch_exact_export(uint32_t V, uint32_t B, const uint32_t *perm, con
                                hipStream_t s);
 
 hipError_t launch_copy_bench(const uint4 *src, uint4 *dst, uint64_t n16, int num_cus, hipStream_t s);
+// n device ids drawn in proportion to out-degree (a uniform edge's source).
+hipError_t launch_stationary_starts(const uint32_t *row_ptr, uint32_t V, uint64_t E, uint64_t n, uint32_t seed_lo,
+                                   uint32_t seed_hi, uint32_t *out, hipStream_t s);
+// One independent walk step from each of n start records.
+hipError_t launch_step_bench(const uint32_t *row_ptr, const uint32_t *col, const uint32_t *starts, uint64_t n,
+                             uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink, int num_cus, hipStream_t s);
 hipError_t launch_gather_bench(const uint32_t *table, uint64_t table_words, uint64_t gathers,
                                uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink,
                                int num_cus, hipStream_t s);

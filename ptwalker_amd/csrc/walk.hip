@@ -511,6 +511,60 @@ __global__ __launch_bounds__(256) void gather_bench_kernel(const uint32_t *table
     if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);  // keeps the loads live
 }
 
+// Vertices in proportion to degree: the source of a uniform random edge
+// (Philox(i, seed) picks the edge; a binary search over row_ptr finds its row).
+__global__ void stationary_starts_kernel(const uint32_t *row_ptr, uint32_t V, uint64_t E, uint64_t n, uint32_t k0,
+                                         uint32_t k1, uint32_t *out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32x4 r = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), 0, 0x53544154u, k0, k1);
+        const uint64_t e = __umul64hi(((uint64_t)r.y << 32) | r.x, E);
+        uint32_t lo = 0, hi = V;  // row_ptr[lo] <= e < row_ptr[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (row_ptr[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        out[i] = lo;
+    }
+}
+
+// One walk step from each record's vertex -- row_ptr[v], row_ptr[v + 1], then
+// one uniformly picked neighbour -- with the records independent and read in
+// order, four per lane in flight: the memory system's rate for a step's own
+// accesses, without the dependence between a walker's steps.
+__global__ __launch_bounds__(256) void step_bench_kernel(const uint32_t *row_ptr, const uint32_t *col,
+                                                        const uint32_t *starts, uint64_t n, uint32_t k0, uint32_t k1,
+                                                        unsigned long long *sink)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t acc = 0;
+    for (uint64_t base = tid; base < n; base += 4 * nthreads) {
+        uint32_t v[4], beg[4], deg[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t i = base + (uint64_t)j * nthreads;
+            v[j] = i < n ? starts[i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            beg[j] = row_ptr[v[j]];
+            deg[j] = row_ptr[v[j] + 1] - beg[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t i = base + (uint64_t)j * nthreads;
+            const u32x4 r = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), 1, 0x53544550u, k0, k1);
+            if (i < n && deg[j]) acc ^= col[beg[j] + mulhi64(r.x, r.y, deg[j])];
+        }
+    }
+    if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);  // keeps the loads live
+}
+
 // Plain streaming copy, 16 B per lane per iteration: the HBM stream roofline
 // and the byte-count calibration for the FETCH_SIZE / WRITE_SIZE counters.
 __global__ __launch_bounds__(256) void copy_bench_kernel(const uint4 *src, uint4 *dst, uint64_t n16)
@@ -628,6 +682,28 @@ hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, cons
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(counts_to_graph_ids_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, counts_dev,
                        perm, V, out, accumulate ? 1 : 0, extra_vertex, extra);
+    return hipGetLastError();
+}
+
+hipError_t launch_stationary_starts(const uint32_t *row_ptr, uint32_t V, uint64_t E, uint64_t n, uint32_t seed_lo,
+                                   uint32_t seed_hi, uint32_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(stationary_starts_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, row_ptr, V, E, n, seed_lo,
+                       seed_hi, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_bench(const uint32_t *row_ptr, const uint32_t *col, const uint32_t *starts, uint64_t n,
+                             uint32_t seed_lo, uint32_t seed_hi, unsigned long long *sink, int num_cus, hipStream_t s)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_bench_kernel, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    hipLaunchKernelGGL(step_bench_kernel, dim3(per_cu * num_cus), dim3(256), 0, s, row_ptr, col, starts, n, seed_lo,
+                       seed_hi, sink);
     return hipGetLastError();
 }
 
