@@ -428,7 +428,14 @@ typedef struct ptw_serve_stats {
 } ptw_serve_stats;
 
 /* Serves n requests in real time (arrival_s relative to the call); mlp may be
- * NULL in fixed mode.  topk_vertex / topk_count: host [n * topk]. */
+ * NULL in fixed mode.  topk_vertex / topk_count: host [n * topk].
+ * Two lanes, each with its own stream (default priority), counter pools and
+ * half of the CUs for its persistent kernels, carry one batch each; a lane
+ * that is idle takes the next batch at once, so a request that arrives while
+ * a long batch runs is walked beside it.  The lanes overlap only on separate
+ * hardware queues: HIP maps streams onto GPU_MAX_HW_QUEUES queues (default 4),
+ * so a process holding more than one context should raise it (each context
+ * holds up to three streams) before HIP starts. */
 int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, const ptw_request *reqs, uint32_t n,
                      ptw_request_result *results, uint32_t *topk_vertex, uint64_t *topk_count,
                      ptw_serve_stats *stats);

@@ -463,6 +463,7 @@ void ptw_destroy(ptw_ctx *ctx)
     }
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ptw::destroy_serve_lanes(ctx);
     free_graph(ctx);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -142,9 +142,8 @@ uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint
 }
 
 // Kernel arguments hold addresses only; per-batch values live in hdr.
-BatchLaunch launch_args(ptw_ctx *ctx)
+BatchLaunch launch_args(ptw_ctx *ctx, const BatchState &b)
 {
-    BatchState &b = ctx->batch;
     BatchLaunch p{};
     p.row_ptr = ctx->dev.row_ptr;
     p.col = ctx->dev.col;
@@ -167,9 +166,9 @@ BatchLaunch launch_args(ptw_ctx *ctx)
 }
 
 // Grows the pools to fit the plan; new slice memory is filled clean.
-int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
+int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k)
 {
-    BatchState &b = ctx->batch;
+    BatchState &b = L.b;
     bool moved = false;
     if (!b.hdr) {
         HIP_TRY(ctx, hipMalloc((void **)&b.hdr, 8 * sizeof(unsigned long long)));
@@ -180,7 +179,7 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
         hfree(b.dense);
         const uint64_t cap = std::max<uint64_t>(P.dense, b.dense_cap + b.dense_cap / 2);
         HIP_TRY(ctx, hipMalloc((void **)&b.dense, cap * 4));
-        HIP_TRY(ctx, launch_fill_u32(b.dense, cap, 0u, ctx->stream));
+        HIP_TRY(ctx, launch_fill_u32(b.dense, cap, 0u, L.s));
         b.dense_cap = cap;
         moved = true;
     }
@@ -189,7 +188,7 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
         const uint64_t cap = std::max<uint64_t>(P.hash, b.hash_cap + b.hash_cap / 2);
         HIP_TRY(ctx, hipMalloc((void **)&b.hash, cap * 8));
         // {key = empty, count = 0}: 0x00000000FFFFFFFF little-endian per slot
-        HIP_TRY(ctx, launch_fill_u64((unsigned long long *)b.hash, cap, 0x00000000FFFFFFFFull, ctx->stream));
+        HIP_TRY(ctx, launch_fill_u64((unsigned long long *)b.hash, cap, 0x00000000FFFFFFFFull, L.s));
         b.hash_cap = cap;
         moved = true;
     }
@@ -246,33 +245,27 @@ int ensure_pools(ptw_ctx *ctx, const Plan &P, uint32_t nq, uint32_t k)
 }
 
 // Counter setup + walk + top-k for the batch described by the device tables.
-int enqueue_run(ptw_ctx *ctx, const BatchLaunch &p)
+int enqueue_run(ptw_ctx *ctx, LaneRef L, const BatchLaunch &p)
 {
-    HIP_TRY(ctx, launch_batch_begin(p, ctx->batch.q_cap, ctx->stream));
-    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, ctx->shape.lds_hubs,
-                                   ctx->shape.cache_slots < 0 ? ptw::kAutoCacheSlots : ctx->shape.cache_slots,
-                                   ctx->stream));
-    HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, ctx->batch.topk_v, ctx->batch.topk_c, ctx->shape.topk_tile,
-                                   ctx->num_cus, ctx->stream));
+    HIP_TRY(ctx, launch_batch_begin(p, L.b.q_cap, L.s));
+    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, L.cus, ctx->shape.lds_hubs,
+                                   ctx->shape.cache_slots < 0 ? ptw::kAutoCacheSlots : ctx->shape.cache_slots, L.s));
+    HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, L.b.topk_v, L.b.topk_c, ctx->shape.topk_tile, L.cus, L.s));
     return PTW_OK;
 }
 
-int clean_previous(ptw_ctx *ctx)
+int clean_previous(ptw_ctx *ctx, LaneRef L)
 {
-    BatchState &b = ctx->batch;
+    BatchState &b = L.b;
     if (!b.dirty) return PTW_OK;
-    BatchLaunch p = launch_args(ctx);
-    HIP_TRY(ctx, launch_batch_reset(p, ctx->num_cus, ctx->stream));
+    BatchLaunch p = launch_args(ctx, b);
+    HIP_TRY(ctx, launch_batch_reset(p, L.cus, L.s));
     b.dirty = false;
     return PTW_OK;
 }
 
-}  // namespace
-
-void free_batch(ptw_ctx *ctx)
+void free_state(BatchState &b)
 {
-    BatchState &b = ctx->batch;
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_exec(b);
     hfree(b.dense);
     hfree(b.hash);
@@ -290,6 +283,166 @@ void free_batch(ptw_ctx *ctx)
     hfree(b.hdr);
     hfree(b.queue);
     b = BatchState{};
+}
+
+}  // namespace
+
+int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
+{
+    if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: batches need a HIP device");
+    if (d.num_queries && !d.queries) return fail(ctx, PTW_ERR_INVALID_ARG, "null query array");
+    if (d.topk > 1024 || 2 * d.topk > (uint32_t)ctx->shape.topk_tile)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "topk must be <= 1024 and <= topk_tile / 2");
+    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
+    const bool weighted = (d.flags & PTW_BATCH_WEIGHTED) != 0;
+    if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted batch on a graph without weights");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BatchState &b = L.b;
+    Plan P;
+    int rc = make_plan(ctx, d, P);
+    if (rc) return rc;
+    // admission: what the plan needs beyond the pools must fit in free HBM
+    const uint64_t extra = plan_bytes_beyond(b, P, d.num_queries, d.topk);
+    if (extra) {
+        size_t f = 0, t = 0;
+        HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+        if (extra > f)
+            return fail(ctx, PTW_ERR_ADMISSION,
+                        "batch needs " + std::to_string(extra) + " more bytes of HBM, " + std::to_string(f) + " free");
+    }
+    if ((rc = clean_previous(ctx, L))) return rc;
+    if ((rc = ensure_pools(ctx, L, P, d.num_queries, d.topk))) return rc;
+
+    b.h_queries = std::move(P.q);
+    b.h_chunks = std::move(P.chunks);
+    b.h_hdr[0] = b.h_chunks.size();
+    b.h_hdr[1] = d.num_queries;
+    b.h_hdr[2] = d.topk;
+    b.h_hdr[3] = d.seed;
+    b.h_hdr[4] = (unsigned long long)d.semantics;
+    b.h_hdr[5] = weighted ? 1 : 0;
+    b.h_hdr[6] = P.dense_chunks;
+    b.last_nq = d.num_queries;
+    b.weighted = weighted;
+    hipStream_t s = L.s;
+    if (d.num_queries) {
+        HIP_TRY(ctx, hipMemcpyAsync(b.queries, b.h_queries.data(), b.h_queries.size() * sizeof(BatchQuery),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(b.chunks, b.h_chunks.data(), b.h_chunks.size() * sizeof(BatchChunk),
+                                    hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(b.hdr, b.h_hdr, sizeof(b.h_hdr), hipMemcpyHostToDevice, s));
+    b.dirty = true;
+
+    const BatchLaunch p = launch_args(ctx, b);
+    HIP_TRY(ctx, hipEventRecord(L.ev0, s));
+    if (d.flags & PTW_BATCH_GRAPH) {
+        // The sequence reads sizes, seed and semantics from hdr, so one capture
+        // serves every batch until a pool or the graph moves.
+        if (!b.exec) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_run(ctx, L, p);
+            hipError_t ec = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(ctx, ec);
+            hipError_t ei = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_TRY(ctx, ei);
+        }
+        HIP_TRY(ctx, hipGraphLaunch(b.exec, s));
+    } else {
+        if ((rc = enqueue_run(ctx, L, p))) return rc;
+    }
+    HIP_TRY(ctx, hipEventRecord(L.ev1, s));
+    return PTW_OK;
+}
+
+int batch_enqueue_results(ptw_ctx *ctx, LaneRef L, uint32_t nq, uint32_t k, unsigned long long *steps,
+                          uint32_t *over, double *ent, uint32_t *tv, unsigned long long *tc)
+{
+    if (!nq) return PTW_OK;
+    const BatchState &b = L.b;
+    if (steps) HIP_TRY(ctx, hipMemcpyAsync(steps, b.steps, (size_t)nq * 8, hipMemcpyDeviceToHost, L.s));
+    if (over) HIP_TRY(ctx, hipMemcpyAsync(over, b.overflow, (size_t)nq * 4, hipMemcpyDeviceToHost, L.s));
+    if (ent) HIP_TRY(ctx, hipMemcpyAsync(ent, b.entropy, (size_t)nq * 8, hipMemcpyDeviceToHost, L.s));
+    if (k && tv) HIP_TRY(ctx, hipMemcpyAsync(tv, b.topk_v, (size_t)nq * k * 4, hipMemcpyDeviceToHost, L.s));
+    if (k && tc) HIP_TRY(ctx, hipMemcpyAsync(tc, b.topk_c, (size_t)nq * k * 8, hipMemcpyDeviceToHost, L.s));
+    return PTW_OK;
+}
+
+int batch_check(ptw_ctx *ctx, const ptw_batch_desc &d, const unsigned long long *steps, const uint32_t *over)
+{
+    for (uint32_t i = 0; i < d.num_queries; ++i) {
+        if (over[i]) return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
+        const uint64_t total_counts =
+            d.queries[i].walkers + (d.queries[i].kind == PTW_QUERY_ENDPOINTS ? 0 : steps[i]);
+        if (total_counts >= 4294967296ull)
+            return fail(ctx, PTW_ERR_OOM,
+                        "query " + std::to_string(i) + ": count total reached 2^32, u32 counters may have wrapped");
+    }
+    return PTW_OK;
+}
+
+int batch_gather_lane(ptw_ctx *ctx, LaneRef L, uint64_t n, const uint32_t *query_index, const uint32_t *vertices,
+                      uint64_t *out)
+{
+    BatchState &b = L.b;
+    if (!b.dirty) return fail(ctx, PTW_ERR_INVALID_ARG, "no batch results to gather from");
+    for (uint64_t i = 0; i < n; ++i)
+        if (query_index[i] >= b.last_nq || vertices[i] >= ctx->dev.V)
+            return fail(ctx, PTW_ERR_INVALID_ARG, "gather pair " + std::to_string(i) + " out of range");
+    if (n == 0) return PTW_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    struct Tmp {
+        void *p = nullptr;
+        ~Tmp() { if (p) (void)hipFree(p); }
+    } tq, tv, to;
+    HIP_TRY(ctx, hipMalloc(&tq.p, n * 4));
+    HIP_TRY(ctx, hipMalloc(&tv.p, n * 4));
+    HIP_TRY(ctx, hipMalloc(&to.p, n * 8));
+    hipStream_t s = L.s;
+    HIP_TRY(ctx, hipMemcpyAsync(tq.p, query_index, n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(tv.p, vertices, n * 4, hipMemcpyHostToDevice, s));
+    const BatchLaunch p = launch_args(ctx, b);
+    HIP_TRY(ctx, launch_batch_gather(p, n, (const uint32_t *)tq.p, (const uint32_t *)tv.p, ctx->dev.perm,
+                                     (uint64_t *)to.p, s));
+    HIP_TRY(ctx, hipMemcpyAsync(out, to.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return PTW_OK;
+}
+
+void free_batch(ptw_ctx *ctx)
+{
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_state(ctx->batch);
+    for (auto &l : ctx->serve) {
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
+        free_state(l.b);
+    }
+}
+
+void destroy_serve_lanes(ptw_ctx *ctx)
+{
+    for (auto &l : ctx->serve) {
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
+        free_state(l.b);
+        if (l.ev0) (void)hipEventDestroy(l.ev0);
+        if (l.ev1) (void)hipEventDestroy(l.ev1);
+        if (l.done) (void)hipEventDestroy(l.done);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+        if (l.h_steps) (void)hipHostFree(l.h_steps);
+        if (l.h_over) (void)hipHostFree(l.h_over);
+        if (l.h_ent) (void)hipHostFree(l.h_ent);
+        if (l.h_tv) (void)hipHostFree(l.h_tv);
+        if (l.h_tc) (void)hipHostFree(l.h_tc);
+        l = ServeLane{};
+    }
 }
 
 }  // namespace ptw
@@ -322,96 +475,22 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
         return fail(ctx, PTW_ERR_INVALID_ARG, "topk must be <= 1024 and <= topk_tile / 2");
     if (d.topk && (!out->topk_vertex || !out->topk_count))
         return fail(ctx, PTW_ERR_INVALID_ARG, "top-k output arrays required");
-    if (d.semantics != PTW_SEM_REFERENCE && d.semantics != PTW_SEM_TEXTBOOK)
-        return fail(ctx, PTW_ERR_INVALID_ARG, "unknown semantics");
-    const bool weighted = (d.flags & PTW_BATCH_WEIGHTED) != 0;
-    if (weighted && !ctx->dev.wedge) return fail(ctx, PTW_ERR_INVALID_ARG, "weighted batch on a graph without weights");
     const double t0 = ptw::now_ms();
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    ptw::BatchState &b = ctx->batch;
     out->total_steps = 0;
     out->slice_bytes = 0;
     out->dense_queries = out->hash_queries = 0;
     out->kernel_ms = 0.0;
-
-    ptw::Plan P;
-    int rc = ptw::make_plan(ctx, d, P);
+    const ptw::LaneRef L = ptw::main_lane(ctx);
+    int rc = ptw::batch_enqueue(ctx, L, d);
     if (rc) return rc;
-    // admission: what the plan needs beyond the pools must fit in free HBM
-    const uint64_t extra = ptw::plan_bytes_beyond(b, P, d.num_queries, d.topk);
-    if (extra) {
-        size_t f = 0, t = 0;
-        HIP_TRY(ctx, hipMemGetInfo(&f, &t));
-        if (extra > f)
-            return fail(ctx, PTW_ERR_ADMISSION,
-                        "batch needs " + std::to_string(extra) + " more bytes of HBM, " + std::to_string(f) + " free");
-    }
-    if ((rc = ptw::clean_previous(ctx))) return rc;
-    if ((rc = ptw::ensure_pools(ctx, P, d.num_queries, d.topk))) return rc;
-
-    b.h_queries = std::move(P.q);
-    b.h_chunks = std::move(P.chunks);
-    b.h_hdr[0] = b.h_chunks.size();
-    b.h_hdr[1] = d.num_queries;
-    b.h_hdr[2] = d.topk;
-    b.h_hdr[3] = d.seed;
-    b.h_hdr[4] = (unsigned long long)d.semantics;
-    b.h_hdr[5] = weighted ? 1 : 0;
-    b.h_hdr[6] = P.dense_chunks;
-    b.last_nq = d.num_queries;
-    b.weighted = weighted;
-    hipStream_t s = ctx->stream;
-    if (d.num_queries) {
-        HIP_TRY(ctx, hipMemcpyAsync(b.queries, b.h_queries.data(), b.h_queries.size() * sizeof(ptw::BatchQuery),
-                                    hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, hipMemcpyAsync(b.chunks, b.h_chunks.data(), b.h_chunks.size() * sizeof(ptw::BatchChunk),
-                                    hipMemcpyHostToDevice, s));
-    }
-    HIP_TRY(ctx, hipMemcpyAsync(b.hdr, b.h_hdr, sizeof(b.h_hdr), hipMemcpyHostToDevice, s));
-    b.dirty = true;
-
-    const ptw::BatchLaunch p = ptw::launch_args(ctx);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    if (d.flags & PTW_BATCH_GRAPH) {
-        // The sequence reads sizes, seed and semantics from hdr, so one capture
-        // serves every batch until a pool or the graph moves.
-        if (!b.exec) {
-            hipGraph_t g = nullptr;
-            HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            rc = ptw::enqueue_run(ctx, p);
-            hipError_t ec = hipStreamEndCapture(s, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            HIP_TRY(ctx, ec);
-            hipError_t ei = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HIP_TRY(ctx, ei);
-        }
-        HIP_TRY(ctx, hipGraphLaunch(b.exec, s));
-    } else {
-        if ((rc = ptw::enqueue_run(ctx, p))) return rc;
-    }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-
-    std::vector<unsigned long long> steps(d.num_queries);
+    std::vector<unsigned long long> steps(d.num_queries), tc((size_t)d.num_queries * d.topk);
     std::vector<uint32_t> over(d.num_queries);
-    std::vector<unsigned long long> tc;
-    if (d.num_queries) {
-        HIP_TRY(ctx, hipMemcpyAsync(steps.data(), b.steps, d.num_queries * 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(over.data(), b.overflow, d.num_queries * 4, hipMemcpyDeviceToHost, s));
-        if (out->entropy)
-            HIP_TRY(ctx, hipMemcpyAsync(out->entropy, b.entropy, d.num_queries * 8, hipMemcpyDeviceToHost, s));
-        if (d.topk) {
-            tc.resize((size_t)d.num_queries * d.topk);
-            HIP_TRY(ctx, hipMemcpyAsync(out->topk_vertex, b.topk_v, tc.size() * 4, hipMemcpyDeviceToHost, s));
-            HIP_TRY(ctx, hipMemcpyAsync(tc.data(), b.topk_c, tc.size() * 8, hipMemcpyDeviceToHost, s));
-        }
-    }
-    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if ((rc = ptw::batch_enqueue_results(ctx, L, d.num_queries, d.topk, steps.data(), over.data(), out->entropy,
+                                         d.topk ? out->topk_vertex : nullptr, tc.data())))
+        return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(L.s));
     float ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, L.ev0, L.ev1));
     out->kernel_ms = ms;
     for (size_t i = 0; i < tc.size(); ++i) out->topk_count[i] = tc[i];
     uint64_t total = 0;
@@ -420,10 +499,8 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
         if (out->steps) out->steps[i] = steps[i];
     }
     out->total_steps = total;
-    out->dense_queries = 0;
-    out->hash_queries = 0;
     uint64_t bytes = 0;
-    for (const auto &q : b.h_queries) {
+    for (const auto &q : L.b.h_queries) {
         if (q.kind == ptw::SLICE_DENSE) {
             ++out->dense_queries;
             bytes += 4ull * ptw::dense_pitch(ctx->dev.V);
@@ -435,16 +512,7 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     }
     out->slice_bytes = bytes;
     out->total_ms = ptw::now_ms() - t0;
-    for (uint32_t i = 0; i < d.num_queries; ++i) {
-        if (over[i])
-            return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
-        const uint64_t total_counts =
-            d.queries[i].walkers + (d.queries[i].kind == PTW_QUERY_ENDPOINTS ? 0 : steps[i]);
-        if (total_counts >= 4294967296ull)
-            return fail(ctx, PTW_ERR_OOM,
-                        "query " + std::to_string(i) + ": count total reached 2^32, u32 counters may have wrapped");
-    }
-    return PTW_OK;
+    return ptw::batch_check(ctx, d, steps.data(), over.data());
 }
 
 int ptw_batch_write_topk(ptw_ctx *ctx, const char *path, const ptw_batch_desc *desc, const ptw_batch_out *out,
@@ -489,29 +557,7 @@ int ptw_batch_gather(ptw_ctx *ctx, uint64_t n, const uint32_t *query_index, cons
 {
     if (!ctx || (n && (!query_index || !vertices || !out))) return fail(ctx, PTW_ERR_INVALID_ARG, "null argument");
     if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context");
-    ptw::BatchState &b = ctx->batch;
-    if (!b.dirty) return fail(ctx, PTW_ERR_INVALID_ARG, "no batch results to gather from");
-    for (uint64_t i = 0; i < n; ++i)
-        if (query_index[i] >= b.last_nq || vertices[i] >= ctx->dev.V)
-            return fail(ctx, PTW_ERR_INVALID_ARG, "gather pair " + std::to_string(i) + " out of range");
-    if (n == 0) return PTW_OK;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    struct Tmp {
-        void *p = nullptr;
-        ~Tmp() { if (p) (void)hipFree(p); }
-    } tq, tv, to;
-    HIP_TRY(ctx, hipMalloc(&tq.p, n * 4));
-    HIP_TRY(ctx, hipMalloc(&tv.p, n * 4));
-    HIP_TRY(ctx, hipMalloc(&to.p, n * 8));
-    hipStream_t s = ctx->stream;
-    HIP_TRY(ctx, hipMemcpyAsync(tq.p, query_index, n * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(tv.p, vertices, n * 4, hipMemcpyHostToDevice, s));
-    const ptw::BatchLaunch p = ptw::launch_args(ctx);
-    HIP_TRY(ctx, ptw::launch_batch_gather(p, n, (const uint32_t *)tq.p, (const uint32_t *)tv.p, ctx->dev.perm,
-                                          (uint64_t *)to.p, s));
-    HIP_TRY(ctx, hipMemcpyAsync(out, to.p, n * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(ctx, hipStreamSynchronize(s));
-    return PTW_OK;
+    return ptw::batch_gather_lane(ctx, ptw::main_lane(ctx), n, query_index, vertices, out);
 }
 
 int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts)
@@ -524,7 +570,7 @@ int ptw_batch_counts(ptw_ctx *ctx, uint32_t index, uint64_t *counts)
     const uint32_t V = ctx->dev.V;
     uint64_t *tmp = nullptr;
     HIP_TRY(ctx, hipMalloc((void **)&tmp, (size_t)V * 8));
-    const ptw::BatchLaunch p = ptw::launch_args(ctx);
+    const ptw::BatchLaunch p = ptw::launch_args(ctx, b);
     hipError_t e = ptw::launch_batch_export(p, index, ctx->dev.perm, ctx->dev.inv, tmp, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(counts, tmp, (size_t)V * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -47,6 +47,23 @@ struct BatchState {
     hipGraphExec_t exec = nullptr;
 };
 
+// A batch launch pipeline of the serving loop: its own pools, stream and
+// events, so two batches can be on the device at once, and page-locked
+// staging that the batch's results are copied into without a wait.
+struct ServeLane {
+    BatchState b;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t done = nullptr;  // after the result copies of the lane's batch
+    unsigned long long *h_steps = nullptr;
+    uint32_t *h_over = nullptr;
+    double *h_ent = nullptr;
+    uint32_t *h_tv = nullptr;
+    unsigned long long *h_tc = nullptr;
+    uint64_t h_q = 0, h_k = 0;  // staging capacity: queries, top-k entries
+};
+constexpr int kServeLanes = 2;
+
 }  // namespace ptw
 
 struct ptw_ctx {
@@ -76,6 +93,7 @@ struct ptw_ctx {
     unsigned long long *static_scratch = nullptr;
     ptw::LaunchShape shape;
     ptw::BatchState batch;
+    ptw::ServeLane serve[ptw::kServeLanes];  // created by the first ptw_serve_replay
 };
 
 namespace ptw {
@@ -102,8 +120,35 @@ int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
     return PTW_OK;
 }
 
-// Frees the batch pools (graph change or context teardown).
+// Frees the batch pools, the serving lanes' included (graph change or
+// context teardown).
 void free_batch(ptw_ctx *ctx);
+// Destroys the serving lanes' streams, events and staging (context teardown).
+void destroy_serve_lanes(ptw_ctx *ctx);
+
+// Where a batch runs: its pools and tables, the stream its work is queued
+// on, the events around its device work, and the CUs its persistent
+// kernels fill (all of them, or a share when lanes run side by side).
+struct LaneRef {
+    BatchState &b;
+    hipStream_t s;
+    hipEvent_t ev0, ev1;
+    int cus;
+};
+inline LaneRef main_lane(ptw_ctx *ctx) { return LaneRef{ctx->batch, ctx->stream, ctx->ev0, ctx->ev1, ctx->num_cus}; }
+
+// ptw_batch_ppr's checks, plan, admission and device work on a lane, queued
+// without a wait; the caller queues the result copies and waits.
+int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d);
+// Queues the copies of the lane's last batch results into host memory
+// (page-locked for an asynchronous copy); any pointer may be null.
+int batch_enqueue_results(ptw_ctx *ctx, LaneRef L, uint32_t nq, uint32_t k, unsigned long long *steps,
+                          uint32_t *over, double *ent, uint32_t *tv, unsigned long long *tc);
+// After the lane's batch completed: hash overflows and u32 count totals.
+int batch_check(ptw_ctx *ctx, const ptw_batch_desc &d, const unsigned long long *steps, const uint32_t *over);
+// Counts of (query, graph id) pairs of the lane's last batch (synchronous).
+int batch_gather_lane(ptw_ctx *ctx, LaneRef L, uint64_t n, const uint32_t *query_index, const uint32_t *vertices,
+                      uint64_t *out);
 
 // Drops the captured batch graph (its kernels' arguments or launch shape changed).
 inline void drop_batch_graph(ptw_ctx *ctx)

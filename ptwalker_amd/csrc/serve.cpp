@@ -1,8 +1,11 @@
 // Serving loop (ptw_serve_replay): continuous batching of PPR requests over
-// the resident graph.  Each launch carries the pilot walks of newly arrived
-// requests, the main walks (and sampled 10x audits) of requests whose budgets
-// are known, admitted in arrival order while their counter slices fit the
-// HBM allowance.  Requests that arrive while a launch runs join the next one.
+// the resident graph.  Two lanes (each with its own stream, counter pools and
+// half of the CUs for its persistent kernels) carry one batch each; a lane
+// that is idle takes the next batch at once, so requests that arrive while a
+// batch runs start on the device beside it.  A batch carries the main walks
+// (and sampled 10x audits) of requests whose budgets are known, admitted in
+// arrival order while their counter slices fit the lane's HBM allowance, and
+// the pilot walks of newly arrived requests.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +53,59 @@ void graph_features(const ptw::HostCsr &g, uint32_t s, float *f)
     f[0] = (float)std::log(std::max(d, 1.0));
     f[1] = (float)std::log(std::max(two_hop / std::max(d, 1.0), 1.0));
     f[2] = (float)std::log(std::max(two_hop, 1.0));
+}
+
+// The launch a lane carries: its items and the query table they became.
+struct InFlight {
+    bool busy = false;
+    std::vector<Item> items;
+    std::vector<ptw_query> q;
+    ptw_batch_desc d{};
+};
+
+// Streams (default priority), events and result staging of the serving lanes,
+// made on the first serving call and kept with the context.
+int ensure_lanes(ptw_ctx *ctx)
+{
+    for (auto &l : ctx->serve) {
+        if (l.stream) continue;
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+        HIP_TRY(ctx, hipEventCreate(&l.ev0));
+        HIP_TRY(ctx, hipEventCreate(&l.ev1));
+        HIP_TRY(ctx, hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    }
+    return PTW_OK;
+}
+
+// Page-locked staging for m queries with k top-k entries each.
+int grow_staging(ptw_ctx *ctx, ptw::ServeLane &l, uint64_t m, uint64_t k)
+{
+    m = std::max<uint64_t>(m, 1);
+    if (m > l.h_q) {
+        if (l.h_steps) HIP_TRY(ctx, hipHostFree(l.h_steps));
+        if (l.h_over) HIP_TRY(ctx, hipHostFree(l.h_over));
+        if (l.h_ent) HIP_TRY(ctx, hipHostFree(l.h_ent));
+        l.h_steps = nullptr;
+        l.h_over = nullptr;
+        l.h_ent = nullptr;
+        l.h_q = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&l.h_steps, m * 8, hipHostMallocDefault));
+        HIP_TRY(ctx, hipHostMalloc((void **)&l.h_over, m * 4, hipHostMallocDefault));
+        HIP_TRY(ctx, hipHostMalloc((void **)&l.h_ent, m * 8, hipHostMallocDefault));
+        l.h_q = m;
+    }
+    const uint64_t mk = std::max<uint64_t>(m * k, 1);
+    if (mk > l.h_k) {
+        if (l.h_tv) HIP_TRY(ctx, hipHostFree(l.h_tv));
+        if (l.h_tc) HIP_TRY(ctx, hipHostFree(l.h_tc));
+        l.h_tv = nullptr;
+        l.h_tc = nullptr;
+        l.h_k = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&l.h_tv, mk * 4, hipHostMallocDefault));
+        HIP_TRY(ctx, hipHostMalloc((void **)&l.h_tc, mk * 8, hipHostMallocDefault));
+        l.h_k = mk;
+    }
+    return PTW_OK;
 }
 
 bool audit_pick(uint64_t seed, uint32_t req, double rate)
@@ -107,6 +163,15 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
         return (uint64_t)std::min<double>(std::max<double>(w, (double)C.w_min), (double)C.w_max);
     };
 
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (int rc = ensure_lanes(ctx)) return rc;
+    // each lane's persistent kernels fill half the CUs, so a batch launched on
+    // an idle lane runs beside the other lane's batch instead of after it
+    const int lane_cus = std::max(1, ctx->num_cus / ptw::kServeLanes);
+    size_t f0 = 0, tot0 = 0;
+    HIP_TRY(ctx, hipMemGetInfo(&f0, &tot0));
+    const uint64_t allowance = (C.hbm_limit ? C.hbm_limit : (uint64_t)(0.85 * (double)f0)) / ptw::kServeLanes;
+
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
@@ -114,107 +179,93 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
     std::deque<uint32_t> pilot_q, ready_q;
     uint32_t next = 0, done = 0;
     ptw_serve_stats st{};
-    std::vector<ptw_query> q;
-    std::vector<Item> items;
-    std::vector<uint32_t> tv;
-    std::vector<uint64_t> tc, steps;
-    std::vector<double> ent;
+    InFlight fl[ptw::kServeLanes];
     std::vector<float> feats, pred;
     std::vector<uint32_t> gq, gv;
     std::vector<uint64_t> gout;
 
-    while (done < n) {
-        const double t = now_s();
-        while (next < n && reqs[order[next]].arrival_s <= t) {
-            const uint32_t r = order[next++];
-            if (C.budget_mode == PTW_BUDGET_LEARNED) {
-                pilot_q.push_back(r);
-            } else {
-                results[r].budget = fixed_budget(reqs[r].eps);
-                ready_q.push_back(r);
-            }
-        }
-        if (pilot_q.empty() && ready_q.empty()) {
-            if (next < n) {
-                const double wait = reqs[order[next]].arrival_s - now_s();
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
-            }
-            continue;
-        }
-        // ---- compose one launch ----
-        size_t f = 0, tot = 0;
-        HIP_TRY(ctx, hipMemGetInfo(&f, &tot));
-        const uint64_t allowance = C.hbm_limit ? C.hbm_limit : (uint64_t)(0.85 * (double)f);
+    // Composes one batch for an idle lane and queues it (no wait): the mains
+    // (and sampled audits) of ready requests while their slices fit the lane's
+    // HBM allowance, then the pilots of new arrivals.
+    auto launch = [&](int li, double t) -> int {
+        ptw::ServeLane &SL = ctx->serve[li];
+        InFlight &F = fl[li];
+        F.items.clear();
         uint64_t used = 0;
-        items.clear();
-        while (!ready_q.empty() && items.size() + 2 <= C.max_batch) {
+        while (!ready_q.empty() && F.items.size() + 2 <= C.max_batch) {
             const uint32_t r = ready_q.front();
             const bool aud = audit_pick(C.seed, reqs[r].id, C.audit_rate);
             const uint64_t b = results[r].budget;
             uint64_t need = slice_bytes(V, b, C.tp) + (aud ? slice_bytes(V, 10 * b, C.tp) : 0);
-            if (used + need > allowance && !items.empty()) {
+            if (used + need > allowance && !F.items.empty()) {
                 ++st.admission_waits;
                 break;
             }
             used += need;
             ready_q.pop_front();
-            items.push_back(Item{r, MAIN, b});
-            if (aud) items.push_back(Item{r, AUDIT, 10 * b});
+            F.items.push_back(Item{r, MAIN, b});
+            if (aud) F.items.push_back(Item{r, AUDIT, 10 * b});
             results[r].admit_s = t;
             results[r].audited = aud ? 1 : 0;
         }
-        while (!pilot_q.empty() && items.size() < C.max_batch) {
-            items.push_back(Item{pilot_q.front(), PILOT, kPilotWalkers});
+        while (!pilot_q.empty() && F.items.size() < C.max_batch) {
+            F.items.push_back(Item{pilot_q.front(), PILOT, kPilotWalkers});
             pilot_q.pop_front();
         }
-        // ---- launch ----
-        q.resize(items.size());
-        for (size_t i = 0; i < items.size(); ++i) {
-            const Item &it = items[i];
+        const size_t m = F.items.size();
+        F.q.resize(m);
+        for (size_t i = 0; i < m; ++i) {
+            const Item &it = F.items[i];
             const uint32_t id = reqs[it.req].id;
             const uint32_t stream = it.kind == MAIN ? id : it.kind == PILOT ? kPilotStream + id : kAuditStream + id;
-            q[i] = ptw_query{reqs[it.req].source, stream, it.walkers, C.tp, PTW_QUERY_VISITS, 0};
+            F.q[i] = ptw_query{reqs[it.req].source, stream, it.walkers, C.tp, PTW_QUERY_VISITS, 0};
             results[it.req].launches++;
         }
-        ptw_batch_desc d{};
-        d.queries = q.data();
-        d.num_queries = (uint32_t)q.size();
-        d.topk = K;
-        d.seed = C.seed;
-        d.semantics = PTW_SEM_REFERENCE;
-        tv.assign(q.size() * K, 0);
-        tc.assign(q.size() * K, 0);
-        steps.assign(q.size(), 0);
-        ent.assign(q.size(), 0.0);
-        ptw_batch_out o{};
-        o.topk_vertex = tv.data();
-        o.topk_count = tc.data();
-        o.steps = steps.data();
-        o.entropy = ent.data();
-        int rc = ptw_batch_ppr(ctx, &d, &o);
-        if (rc) return rc;
-        st.walk_steps += o.total_steps;
+        F.d = ptw_batch_desc{};
+        F.d.queries = F.q.data();
+        F.d.num_queries = (uint32_t)m;
+        F.d.topk = K;
+        F.d.seed = C.seed;
+        F.d.semantics = PTW_SEM_REFERENCE;
+        if (int rc = grow_staging(ctx, SL, m, K)) return rc;
+        const ptw::LaneRef L{SL.b, SL.stream, SL.ev0, SL.ev1, lane_cus};
+        if (int rc = ptw::batch_enqueue(ctx, L, F.d)) return rc;
+        if (int rc = ptw::batch_enqueue_results(ctx, L, (uint32_t)m, K, SL.h_steps, SL.h_over, SL.h_ent, SL.h_tv,
+                                                SL.h_tc))
+            return rc;
+        HIP_TRY(ctx, hipEventRecord(SL.done, SL.stream));
+        F.busy = true;
         st.launches++;
-        const double t_done = now_s();
-        // ---- pilots: features -> budgets ----
+        return PTW_OK;
+    };
+
+    // A lane's batch has completed: budgets from its pilots, answers from its
+    // mains, and each audit's verdict on its main.
+    auto complete = [&](int li, double t_done) -> int {
+        ptw::ServeLane &SL = ctx->serve[li];
+        InFlight &F = fl[li];
+        F.busy = false;
+        const size_t m = F.items.size();
+        if (int rc = ptw::batch_check(ctx, F.d, SL.h_steps, SL.h_over)) return rc;
+        for (size_t i = 0; i < m; ++i) st.walk_steps += SL.h_steps[i];
         feats.clear();
         std::vector<uint32_t> pilot_reqs;
-        for (size_t i = 0; i < items.size(); ++i) {
-            if (items[i].kind != PILOT) continue;
-            const uint32_t r = items[i].req;
+        for (size_t i = 0; i < m; ++i) {
+            if (F.items[i].kind != PILOT) continue;
+            const uint32_t r = F.items[i].req;
             float x[6];
             graph_features(ctx->host, reqs[r].source, x);
             double top10 = 0;
-            for (uint32_t j = 0; j < std::min<uint32_t>(10, K); ++j) top10 += (double)tc[i * K + j];
-            x[3] = (float)(top10 / ((double)kPilotWalkers + (double)steps[i]));
-            x[4] = (float)ent[i];
+            for (uint32_t j = 0; j < std::min<uint32_t>(10, K); ++j) top10 += (double)SL.h_tc[i * K + j];
+            x[3] = (float)(top10 / ((double)kPilotWalkers + (double)SL.h_steps[i]));
+            x[4] = (float)SL.h_ent[i];
             x[5] = std::log(reqs[r].eps);
             feats.insert(feats.end(), x, x + 6);
             pilot_reqs.push_back(r);
         }
         if (!pilot_reqs.empty()) {
             pred.assign(pilot_reqs.size(), 0.f);
-            if ((rc = ptw_mlp_forward(mlp, feats.data(), (uint32_t)pilot_reqs.size(), pred.data()))) return rc;
+            if (int rc = ptw_mlp_forward(mlp, feats.data(), (uint32_t)pilot_reqs.size(), pred.data())) return rc;
             for (size_t j = 0; j < pilot_reqs.size(); ++j) {
                 const uint32_t r = pilot_reqs[j];
                 bool ood = false;
@@ -232,21 +283,20 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
                 ready_q.push_back(r);
             }
         }
-        // ---- mains: answers; audits judge their main and top it up if short ----
         gq.clear();
         gv.clear();
         std::vector<std::pair<size_t, size_t>> pairs;  // (main item, audit item)
-        for (size_t i = 0; i < items.size(); ++i) {
-            if (items[i].kind != MAIN) continue;
-            const uint32_t r = items[i].req;
-            std::copy(tv.begin() + i * K, tv.begin() + (i + 1) * K, topk_vertex + (size_t)r * K);
-            std::copy(tc.begin() + i * K, tc.begin() + (i + 1) * K, topk_count + (size_t)r * K);
+        for (size_t i = 0; i < m; ++i) {
+            if (F.items[i].kind != MAIN) continue;
+            const uint32_t r = F.items[i].req;
+            std::copy(SL.h_tv + i * K, SL.h_tv + (i + 1) * K, topk_vertex + (size_t)r * K);
+            std::copy(SL.h_tc + i * K, SL.h_tc + (i + 1) * K, topk_count + (size_t)r * K);
             results[r].done_s = t_done;
             ++done;
-            if (results[r].audited && i + 1 < items.size() && items[i + 1].kind == AUDIT) {
+            if (results[r].audited && i + 1 < m && F.items[i + 1].kind == AUDIT) {
                 pairs.emplace_back(i, i + 1);
                 for (uint32_t j = 0; j < K; ++j) {
-                    const uint32_t v = tv[(i + 1) * K + j];
+                    const uint32_t v = SL.h_tv[(i + 1) * K + j];
                     gq.push_back((uint32_t)i);
                     gv.push_back(v == PTW_INVALID_VERTEX ? reqs[r].source : v);
                 }
@@ -254,25 +304,62 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
         }
         if (!pairs.empty()) {
             gout.assign(gq.size(), 0);
-            if ((rc = ptw_batch_gather(ctx, gq.size(), gq.data(), gv.data(), gout.data()))) return rc;
+            const ptw::LaneRef L{SL.b, SL.stream, SL.ev0, SL.ev1, lane_cus};
+            if (int rc = ptw::batch_gather_lane(ctx, L, gq.size(), gq.data(), gv.data(), gout.data())) return rc;
             for (size_t p = 0; p < pairs.size(); ++p) {
                 const size_t mi = pairs[p].first, ai = pairs[p].second;
-                const uint32_t r = items[mi].req;
-                const double nm = (double)items[mi].walkers + (double)steps[mi];
-                const double na = (double)items[ai].walkers + (double)steps[ai];
+                const uint32_t r = F.items[mi].req;
+                const double nm = (double)F.items[mi].walkers + (double)SL.h_steps[mi];
+                const double na = (double)F.items[ai].walkers + (double)SL.h_steps[ai];
                 double l1 = 0;
                 for (uint32_t j = 0; j < K; ++j) {
-                    if (tv[ai * K + j] == PTW_INVALID_VERTEX) continue;
-                    l1 += std::fabs((double)gout[p * K + j] / nm - (double)tc[ai * K + j] / na);
+                    if (SL.h_tv[ai * K + j] == PTW_INVALID_VERTEX) continue;
+                    l1 += std::fabs((double)gout[p * K + j] / nm - (double)SL.h_tc[ai * K + j] / na);
                 }
                 st.audited++;
                 if (l1 > (double)reqs[r].eps) {
                     results[r].topped_up = 1;
                     st.topped_up++;
-                    std::copy(tv.begin() + ai * K, tv.begin() + (ai + 1) * K, topk_vertex + (size_t)r * K);
-                    std::copy(tc.begin() + ai * K, tc.begin() + (ai + 1) * K, topk_count + (size_t)r * K);
+                    std::copy(SL.h_tv + ai * K, SL.h_tv + (ai + 1) * K, topk_vertex + (size_t)r * K);
+                    std::copy(SL.h_tc + ai * K, SL.h_tc + (ai + 1) * K, topk_count + (size_t)r * K);
                 }
             }
+        }
+        return PTW_OK;
+    };
+
+    while (done < n) {
+        bool progressed = false;
+        // completions first: they free lanes and make budgets known
+        for (int li = 0; li < ptw::kServeLanes; ++li) {
+            if (!fl[li].busy) continue;
+            const hipError_t e = hipEventQuery(ctx->serve[li].done);
+            if (e == hipErrorNotReady) continue;
+            HIP_TRY(ctx, e);
+            if (int rc = complete(li, now_s())) return rc;
+            progressed = true;
+        }
+        const double t = now_s();
+        while (next < n && reqs[order[next]].arrival_s <= t) {
+            const uint32_t r = order[next++];
+            if (C.budget_mode == PTW_BUDGET_LEARNED) {
+                pilot_q.push_back(r);
+            } else {
+                results[r].budget = fixed_budget(reqs[r].eps);
+                ready_q.push_back(r);
+            }
+            progressed = true;
+        }
+        for (int li = 0; li < ptw::kServeLanes && (!pilot_q.empty() || !ready_q.empty()); ++li) {
+            if (fl[li].busy) continue;
+            if (int rc = launch(li, t)) return rc;
+            progressed = true;
+        }
+        if (!progressed) {
+            // nothing to do until a lane completes or the next request arrives
+            double wait = 50e-6;
+            if (next < n) wait = std::min(wait, std::max(0.0, reqs[order[next]].arrival_s - now_s()));
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
         }
     }
     // ---- statistics ----

@@ -182,3 +182,32 @@ def test_config_serve_scale22_committed_policy(ctx22):
         assert np.array_equal(alone.topk_vertex[0], mix.topk_vertex[i])
         assert np.array_equal(alone.topk_count[0], mix.topk_count[i])
     net.close()
+
+
+def test_serve_overlap_short_request_finishes_first(ctx):
+    """Continuous batching with work overlapping on the device: a long main walk
+    (1e8 walkers) starts at t = 0; a short request arriving 4 ms later is
+    admitted on the idle lane while the long walk runs and finishes before it.
+    Both answers equal the same requests served alone, and the short one
+    equals the oracle's replay of its walk."""
+    gi = ctx.rmat(scale=18, edge_factor=16, seed=3)
+    rp, col = ctx.export_csr()
+    big, small = int(gi.max_degree_vertex), 5
+    src = np.array([big, small])
+    eps = np.array([1e-6, 10.0])
+    arr = np.array([0.0, 0.004])
+    kw = dict(tp=TP, seed=SEED, topk=K, budget_mode=ptw.BUDGET_FIXED, fixed_c2=1.0, w_min=20000,
+              w_max=100_000_000, max_batch=8)
+    res = ctx.serve(src, eps, arr, **kw)
+    assert res.budget.tolist() == [100_000_000, 20000]
+    print(f"overlap: long main {res.admit_s[0]:.4f}-{res.done_s[0]:.4f} s, short "
+          f"{res.admit_s[1]:.4f}-{res.done_s[1]:.4f} s", flush=True)
+    assert res.admit_s[1] < res.done_s[0]  # admitted while the long main ran
+    assert res.done_s[1] < res.done_s[0]   # and answered before it
+    assert res.stats["launches"] == 2
+    for i in (0, 1):
+        alone = ctx.serve(src[i:i + 1], eps[i:i + 1], [0.0], ids=[i], **kw)
+        assert np.array_equal(alone.topk_vertex[0], res.topk_vertex[i]), i
+        assert np.array_equal(alone.topk_count[0], res.topk_count[i]), i
+    v, c = expected_topk(rp, col, small, 20000, 1)
+    assert np.array_equal(res.topk_vertex[1], v) and np.array_equal(res.topk_count[1], c)
