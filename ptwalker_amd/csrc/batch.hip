@@ -1255,15 +1255,15 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
-                             hipStream_t s)
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int walk_cus, int lds_hubs,
+                             int cache_slots, hipStream_t s)
 {
     const size_t dlds = (((size_t)lds_hubs + 1) / 2 + 2 * (size_t)cache_slots) * 4;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_dense_kernel, block, dlds) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
-    hipLaunchKernelGGL(batch_dense_kernel, dim3(per_cu * num_cus), dim3(block), dlds, s, p, (uint32_t)lds_hubs,
+    hipLaunchKernelGGL(batch_dense_kernel, dim3(per_cu * walk_cus), dim3(block), dlds, s, p, (uint32_t)lds_hubs,
                        (uint32_t)cache_slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1271,7 +1271,7 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int l
     per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_walk_kernel, block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * num_cus), dim3(block), lds, s, p);
+    hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * walk_cus), dim3(block), lds, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(kCompactBlock), 0, s, p);

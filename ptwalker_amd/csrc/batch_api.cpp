@@ -248,9 +248,10 @@ int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k
 int enqueue_run(ptw_ctx *ctx, LaneRef L, const BatchLaunch &p)
 {
     HIP_TRY(ctx, launch_batch_begin(p, L.b.q_cap, L.s));
-    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, L.cus, ctx->shape.lds_hubs,
+    HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, L.cus, ctx->shape.lds_hubs,
                                    ctx->shape.cache_slots < 0 ? ptw::kAutoCacheSlots : ctx->shape.cache_slots, L.s));
-    HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, L.b.topk_v, L.b.topk_c, ctx->shape.topk_tile, L.cus, L.s));
+    HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, L.b.topk_v, L.b.topk_c, ctx->shape.topk_tile, ctx->num_cus,
+                                   L.s));
     return PTW_OK;
 }
 
@@ -259,7 +260,7 @@ int clean_previous(ptw_ctx *ctx, LaneRef L)
     BatchState &b = L.b;
     if (!b.dirty) return PTW_OK;
     BatchLaunch p = launch_args(ctx, b);
-    HIP_TRY(ctx, launch_batch_reset(p, L.cus, L.s));
+    HIP_TRY(ctx, launch_batch_reset(p, ctx->num_cus, L.s));
     b.dirty = false;
     return PTW_OK;
 }
@@ -282,6 +283,9 @@ void free_state(BatchState &b)
     hfree(b.topk_c);
     hfree(b.hdr);
     hfree(b.queue);
+    hfree(b.g_q);
+    hfree(b.g_v);
+    hfree(b.g_out);
     b = BatchState{};
 }
 
@@ -399,20 +403,26 @@ int batch_gather_lane(ptw_ctx *ctx, LaneRef L, uint64_t n, const uint32_t *query
             return fail(ctx, PTW_ERR_INVALID_ARG, "gather pair " + std::to_string(i) + " out of range");
     if (n == 0) return PTW_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    struct Tmp {
-        void *p = nullptr;
-        ~Tmp() { if (p) (void)hipFree(p); }
-    } tq, tv, to;
-    HIP_TRY(ctx, hipMalloc(&tq.p, n * 4));
-    HIP_TRY(ctx, hipMalloc(&tv.p, n * 4));
-    HIP_TRY(ctx, hipMalloc(&to.p, n * 8));
     hipStream_t s = L.s;
-    HIP_TRY(ctx, hipMemcpyAsync(tq.p, query_index, n * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, hipMemcpyAsync(tv.p, vertices, n * 4, hipMemcpyHostToDevice, s));
+    // grown and kept: a gather frees nothing, so it never waits for the work
+    // of other streams (a free does)
+    if (n > b.g_cap) {
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        hfree(b.g_q);
+        hfree(b.g_v);
+        hfree(b.g_out);
+        b.g_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(n, 4096);
+        HIP_TRY(ctx, hipMalloc((void **)&b.g_q, cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.g_v, cap * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&b.g_out, cap * 8));
+        b.g_cap = cap;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(b.g_q, query_index, n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(b.g_v, vertices, n * 4, hipMemcpyHostToDevice, s));
     const BatchLaunch p = launch_args(ctx, b);
-    HIP_TRY(ctx, launch_batch_gather(p, n, (const uint32_t *)tq.p, (const uint32_t *)tv.p, ctx->dev.perm,
-                                     (uint64_t *)to.p, s));
-    HIP_TRY(ctx, hipMemcpyAsync(out, to.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, launch_batch_gather(p, n, b.g_q, b.g_v, ctx->dev.perm, b.g_out, s));
+    HIP_TRY(ctx, hipMemcpyAsync(out, b.g_out, n * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     return PTW_OK;
 }

@@ -43,6 +43,10 @@ struct BatchState {
     std::vector<BatchQuery> h_queries;
     std::vector<BatchChunk> h_chunks;
     unsigned long long h_hdr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // (query, vertex) gathers of the last batch, grown and kept
+    uint32_t *g_q = nullptr, *g_v = nullptr;
+    uint64_t *g_out = nullptr;
+    uint64_t g_cap = 0;
     // captured launch sequence, valid while the pools and the graph keep their addresses
     hipGraphExec_t exec = nullptr;
 };
@@ -127,8 +131,9 @@ void free_batch(ptw_ctx *ctx);
 void destroy_serve_lanes(ptw_ctx *ctx);
 
 // Where a batch runs: its pools and tables, the stream its work is queued
-// on, the events around its device work, and the CUs its persistent
-// kernels fill (all of them, or a share when lanes run side by side).
+// on, the events around its device work, and the CUs its persistent walk
+// kernels fill (all of them, or a share when lanes run side by side; the
+// other kernels are ordinary grids sized for the whole device).
 struct LaneRef {
     BatchState &b;
     hipStream_t s;

@@ -317,8 +317,10 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 // Dense queries (workgroup chunks, LDS hub counters + cache of `lds_hubs` /
 // `cache_slots`), then hash queries (wavefront chunks), then the compaction
 // of every query's slice into (vertex, count) pairs and count statistics.
-hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int lds_hubs, int cache_slots,
-                             hipStream_t s);
+// The persistent walk kernels fill walk_cus CUs; the compaction grid is
+// sized for num_cus.
+hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int walk_cus, int lds_hubs,
+                             int cache_slots, hipStream_t s);
 // Per-query entropy and top-k (count desc, device id asc) into out_v (graph
 // ids) / out_c, from the count statistics the compaction summed: the
 // candidate gather runs over segments of every query's entries on the whole

@@ -29,6 +29,10 @@ struct ptw_mlp {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *norm = nullptr;
     float *partial = nullptr, *loss_partial = nullptr, *loss = nullptr;
     uint32_t max_rows = 0;  // capacity of partial / loss_partial (rows of a gradient call)
+    // inference inputs and outputs, grown and kept: a forward call frees
+    // nothing, so it never waits for other streams' work (a free does)
+    float *fx = nullptr, *fy = nullptr;
+    uint32_t f_cap = 0;
     float h_norm[14] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 0, 1};
     int step = 0;
 };
@@ -114,7 +118,7 @@ void ptw_mlp_destroy(ptw_mlp *M)
 {
     if (!M) return;
     if (M->ctx && M->ctx->stream) (void)hipStreamSynchronize(M->ctx->stream);
-    for (float *b : {M->params, M->m, M->v, M->grad, M->norm, M->partial, M->loss_partial, M->loss})
+    for (float *b : {M->params, M->m, M->v, M->grad, M->norm, M->partial, M->loss_partial, M->loss, M->fx, M->fy})
         if (b) (void)hipFree(b);
     delete M;
 }
@@ -159,12 +163,20 @@ int ptw_mlp_forward(ptw_mlp *M, const float *x, uint32_t n, float *y)
     if (n == 0) return PTW_OK;
     ptw_ctx *ctx = M->ctx;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    DevBuf dx, dy;
-    HIP_TRY(ctx, hipMalloc(&dx.p, (size_t)n * 6 * 4));
-    HIP_TRY(ctx, hipMalloc(&dy.p, (size_t)n * 4));
-    HIP_TRY(ctx, hipMemcpyAsync(dx.p, x, (size_t)n * 24, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, ptw::launch_mlp_infer(M->params, M->norm, (const float *)dx.p, n, (float *)dy.p, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(y, dy.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (n > M->f_cap) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (M->fx) (void)hipFree(M->fx);
+        if (M->fy) (void)hipFree(M->fy);
+        M->fx = M->fy = nullptr;
+        M->f_cap = 0;
+        const uint32_t cap = std::max<uint32_t>(n, 1024);
+        HIP_TRY(ctx, hipMalloc((void **)&M->fx, (size_t)cap * 6 * 4));
+        HIP_TRY(ctx, hipMalloc((void **)&M->fy, (size_t)cap * 4));
+        M->f_cap = cap;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(M->fx, x, (size_t)n * 24, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, ptw::launch_mlp_infer(M->params, M->norm, M->fx, n, M->fy, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(y, M->fy, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return PTW_OK;
 }

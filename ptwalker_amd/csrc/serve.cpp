@@ -28,6 +28,10 @@ using ptw::fail;
 constexpr uint32_t kPilotWalkers = 1000;
 constexpr uint32_t kPilotStream = 1u << 30;
 constexpr uint32_t kAuditStream = 1u << 31;
+// Walkers a batch takes at most (its first request excepted): about 60 ms of
+// walking on the whole device, so a large backlog is served as a run of
+// batches that alternate between the lanes rather than as one long batch.
+constexpr double kBatchWalkers = 4e8;
 
 enum Kind : uint8_t { PILOT = 0, MAIN = 1, AUDIT = 2 };
 
@@ -165,8 +169,8 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
 
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (int rc = ensure_lanes(ctx)) return rc;
-    // each lane's persistent kernels fill half the CUs, so a batch launched on
-    // an idle lane runs beside the other lane's batch instead of after it
+    // a lane's batch that takes all waiting work fills half the CUs with its
+    // persistent kernels, so a batch launched on the idle lane runs beside it
     const int lane_cus = std::max(1, ctx->num_cus / ptw::kServeLanes);
     size_t f0 = 0, tot0 = 0;
     HIP_TRY(ctx, hipMemGetInfo(&f0, &tot0));
@@ -184,18 +188,36 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
     std::vector<uint32_t> gq, gv;
     std::vector<uint64_t> gout;
 
+    auto item_walkers = [&](uint32_t r) {
+        return (double)results[r].budget * (audit_pick(C.seed, reqs[r].id, C.audit_rate) ? 11.0 : 1.0);
+    };
     // Composes one batch for an idle lane and queues it (no wait): the mains
     // (and sampled audits) of ready requests while their slices fit the lane's
     // HBM allowance, then the pilots of new arrivals.
-    auto launch = [&](int li, double t) -> int {
+    //  - A batch that leaves work waiting (a backlog: latency is set by the
+    //    queue) fills every CU and queues behind the other lane's batch.
+    //  - With `split` (both lanes idle, the waiting work fits one batch), the
+    //    batch takes about half of it and the other lane the rest, each on
+    //    half of the CUs, side by side.
+    //  - Otherwise it takes all waiting work on half of the CUs, so a request
+    //    that arrives meanwhile starts at once on the other lane.
+    auto launch = [&](int li, double t, bool split) -> int {
         ptw::ServeLane &SL = ctx->serve[li];
         InFlight &F = fl[li];
         F.items.clear();
+        double waiting = 0;
+        if (split)
+            for (uint32_t r : ready_q) waiting += item_walkers(r);
+        const size_t pilots = split ? (pilot_q.size() + 1) / 2 : pilot_q.size();
         uint64_t used = 0;
+        double walkers = 0;
         while (!ready_q.empty() && F.items.size() + 2 <= C.max_batch) {
             const uint32_t r = ready_q.front();
             const bool aud = audit_pick(C.seed, reqs[r].id, C.audit_rate);
             const uint64_t b = results[r].budget;
+            const double w = (double)b * (aud ? 11.0 : 1.0);
+            if (!F.items.empty() && (walkers + w > kBatchWalkers || (split && 2 * walkers >= waiting))) break;
+            walkers += w;
             uint64_t need = slice_bytes(V, b, C.tp) + (aud ? slice_bytes(V, 10 * b, C.tp) : 0);
             if (used + need > allowance && !F.items.empty()) {
                 ++st.admission_waits;
@@ -208,10 +230,12 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
             results[r].admit_s = t;
             results[r].audited = aud ? 1 : 0;
         }
-        while (!pilot_q.empty() && F.items.size() < C.max_batch) {
+        for (size_t k = 0; k < pilots && !pilot_q.empty() && F.items.size() < C.max_batch; ++k) {
             F.items.push_back(Item{pilot_q.front(), PILOT, kPilotWalkers});
             pilot_q.pop_front();
         }
+        const bool backlog = !split && (!ready_q.empty() || !pilot_q.empty());
+        const int cus = backlog ? ctx->num_cus : lane_cus;
         const size_t m = F.items.size();
         F.q.resize(m);
         for (size_t i = 0; i < m; ++i) {
@@ -228,7 +252,7 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
         F.d.seed = C.seed;
         F.d.semantics = PTW_SEM_REFERENCE;
         if (int rc = grow_staging(ctx, SL, m, K)) return rc;
-        const ptw::LaneRef L{SL.b, SL.stream, SL.ev0, SL.ev1, lane_cus};
+        const ptw::LaneRef L{SL.b, SL.stream, SL.ev0, SL.ev1, cus};
         if (int rc = ptw::batch_enqueue(ctx, L, F.d)) return rc;
         if (int rc = ptw::batch_enqueue_results(ctx, L, (uint32_t)m, K, SL.h_steps, SL.h_over, SL.h_ent, SL.h_tv,
                                                 SL.h_tc))
@@ -352,7 +376,15 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
         }
         for (int li = 0; li < ptw::kServeLanes && (!pilot_q.empty() || !ready_q.empty()); ++li) {
             if (fl[li].busy) continue;
-            if (int rc = launch(li, t)) return rc;
+            bool other_idle = false;
+            for (int lj = li + 1; lj < ptw::kServeLanes; ++lj) other_idle |= !fl[lj].busy;
+            bool split = false;
+            if (other_idle && ready_q.size() + pilot_q.size() >= 2 && ready_q.size() + 2 * pilot_q.size() <= C.max_batch) {
+                double waiting = 0;
+                for (uint32_t r : ready_q) waiting += item_walkers(r);
+                split = waiting <= kBatchWalkers;
+            }
+            if (int rc = launch(li, t, split)) return rc;
             progressed = true;
         }
         if (!progressed) {
