@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "walk_count_rmat22_traffic.json"),
                     help="HBM bytes per launch measured with rocprofv3 --pmc (JSON), if present")
     ap.add_argument("--workload", default="ppr-rwr",
-                    choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest", "node2vec", "gather"],
+                    choices=["ppr-rwr", "deepwalk", "batch", "serve", "cli", "ingest", "node2vec", "gather", "shapes"],
                     help="ppr-rwr: the headline workload; deepwalk: R-MAT scale 24 path corpus; "
                          "batch: 4096 PPR queries in one launch on R-MAT scale 24; "
                          "serve: learned- vs fixed-budget PPR serving on R-MAT scale 22; "
@@ -50,7 +50,8 @@ def parse():
                          "ingest: loading an R-MAT scale 22 edge list, device vs host parsing; "
                          "node2vec: second-order walks on R-MAT scale 22; "
                          "gather: random u32 gather rate against table size (the Infinity Cache knee), and "
-                         "the stationary-step rate on R-MAT scale 22")
+                         "the stationary-step rate on R-MAT scale 22; shapes: walk-kernel time over LDS "
+                         "hub-counter / visit-cache shapes on the headline graph (the sweep behind the defaults)")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--requests", type=int, default=3000)
     ap.add_argument("--rates", default="500,20000",
@@ -89,6 +90,8 @@ def main():
         return bench_node2vec(args)
     if args.workload == "gather":
         return bench_gather_sweep(args)
+    if args.workload == "shapes":
+        return bench_shape_sweep(args)
 
     import numpy as np
 
@@ -266,6 +269,64 @@ def bench_gather_sweep(args):
                                    f"{args.scale}", "vertices": gi.num_vertices, "edges": gi.num_edges},
             "gather": rows, "stationary_step": {"steps_per_s": sps, "kernel_ms": sms},
             "stream_copy_GBs": copy_gbs}
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+# (u16 LDS hub counters, visit-cache slots); the cache's slots and the hub
+# counters share the LDS, so the larger hub counts leave room for fewer slots
+SHAPES = [(8192, 4096), (16384, 2048), (36864, 1024), (55296, 1024), (73728, 1024), (77824, 512), (79872, 0),
+          (77824, -1)]
+
+
+def bench_shape_sweep(args):
+    """Walk-kernel time over launch shapes on the headline graph: PPR from the
+    max-degree and from the uniform source (1e8 walkers, tp 0.2) and PageRank
+    (1e8 walkers, tp 0.15); median kernel ms of 3 runs per shape, after one
+    warm-up.  Counts are compared across shapes (a shape changes where counts
+    accumulate, never what is counted)."""
+    import hashlib
+    import statistics
+
+    import numpy as np
+
+    import ptwalker_amd as ptw
+    from ptwalker_amd.workloads import walk_sources
+
+    W = int(args.walkers)
+    ctx = ptw.Context(0)
+    gi = ctx.rmat(scale=args.scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1)
+    hub, uni = walk_sources(gi, seed=1)[:2]
+    default = ctx.tuning()
+    apps = {"ppr_max_degree": ptw.PPR(args.tp, hub), "ppr_uniform": ptw.PPR(args.tp, uni),
+            "pagerank": ptw.PageRank(0.15)}
+    counts = np.zeros(gi.num_vertices, dtype=np.uint64)
+    rows, digests = [], {}
+    for hubs, slots in SHAPES:
+        ctx.set_tuning(lds_hubs=hubs, cache_slots=slots)
+        row = {"lds_hubs": hubs, "cache_slots": slots}
+        for name, app in apps.items():
+            ms = []
+            for i in range(4):
+                o = ctx.walk(ptw.make_desc(app, W, seed=1), counts=counts)
+                if i:
+                    ms.append(o.kernel_ms)
+            shape = o.launch_shape()
+            row[name] = statistics.median(ms)
+            row[name + "_shape"] = {k: shape[k] for k in ("lds_hubs", "cache_slots", "chunk")}
+            h = hashlib.sha1(counts.tobytes()).hexdigest()
+            row[name + "_identical"] = digests.setdefault(name, h) == h
+        rows.append(row)
+        print(f"# shapes {hubs}/{slots}: " + " ".join(f"{n} {row[n]:.2f} ms" for n in apps), file=sys.stderr,
+              flush=True)
+    ctx.set_tuning(**{k: getattr(default, k) for k in ("lds_hubs", "cache_slots")})
+    line = {"metric": "walk kernel ms", "value": min(r["ppr_max_degree"] for r in rows), "unit": "ms", "n_gpus": 1,
+            "steps": 3, "warmup": 1, "higher_is_better": False, "data": "synthetic",
+            "config": {"workload": f"PPR (max-degree, uniform source; tp {args.tp}) and PageRank (tp 0.15), "
+                                   f"{W:.0e} walkers, R-MAT scale {args.scale} ef16",
+                       "vertices": gi.num_vertices, "edges": gi.num_edges, "default_lds_hubs": default.lds_hubs,
+                       "default_cache_slots": default.cache_slots},
+            "shapes": rows}
     print(json.dumps(line), flush=True)
     ctx.close()
 
@@ -729,6 +790,16 @@ def bench_cli(args):
             ts.append(time.perf_counter() - t)
         cpu[th] = (r["steps"] / statistics.median(ts), statistics.median(ts), r["steps"])
     best_th = max(cpu, key=lambda k: cpu[k][0])
+    # the reference's own schedule (partitions from this host's L2, walker
+    # pools, migration, buffered hub sampling), walker init + walk scope
+    part = {}
+    for th in sorted({20, cores}):
+        ts = []
+        for _ in range(3):
+            p = oracle.ref_partitioned_walk(rp, col, oracle.APP_PAGERANK, 655360, tp=0.15, start_uniform=True, seed=1,
+                                            threads=th)
+            ts.append(p["seconds"])
+        part[str(th)] = p["steps"] / statistics.median(ts)
     line = {
         "metric": "walk-steps/s",
         "value": steps / walk_s,
@@ -756,7 +827,8 @@ def bench_cli(args):
                          "sample": f"the whole command: 655360 walkers, {cpu[best_th][2]} steps, "
                                    f"{cpu[best_th][1] * 1e3:.1f} ms (median of 3), reference process "
                                    f"(xorshift*, quantized geometric), {best_th} threads",
-                         "by_threads": {str(k): v[0] for k, v in cpu.items()}},
+                         "by_threads": {str(k): v[0] for k, v in cpu.items()},
+                         "partitioned_by_threads": part},
         "vs_cpu": (steps / walk_s) / cpu[best_th][0],
     }
     print(json.dumps(line), flush=True)
@@ -951,13 +1023,27 @@ def cpu_baseline(ctx, hub, args):
     if dt < 0.5 * args.cpu_seconds and n < 5e8:
         n = int(min(5e8, n * args.cpu_seconds / max(dt, 1e-3)))
         r, dt = run(n)
+    naive = {"steps_per_s": r["steps"] / dt, "walkers": n, "steps": r["steps"], "seconds": dt}
+    # the reference's own schedule on the same sample: partitions sized from
+    # this host's L2, walker pools, migration, buffered hub sampling; timed
+    # over walker init + walk, the scope the reference prints
+    t = time.perf_counter()
+    p = oracle.ref_partitioned_walk(rp, col, oracle.APP_PPR, n, tp=args.tp, source=hub, start_uniform=False, seed=1,
+                                    threads=cores)
+    part = {"steps_per_s": p["steps"] / p["seconds"], "walkers": n, "steps": p["steps"], "seconds": p["seconds"],
+            "with_partitioning_s": time.perf_counter() - t, "hub_partitions": p["hub_partitions"],
+            "low_partitions": p["low_partitions"], "l2_bytes": oracle.l2_cache_bytes()}
+    best = "partitioned" if part["steps_per_s"] > naive["steps_per_s"] else "walker_parallel"
+    b = part if best == "partitioned" else naive
     return {
-        "value": r["steps"] / dt,
+        "value": b["steps_per_s"],
         "unit": "steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"PPR from the max-degree vertex, {n} walkers ({r['steps']} steps, {dt:.1f} s), "
-                  f"xorshift* reference-semantics CPU path, {cores} threads",
+        "sample": f"PPR from the max-degree vertex, {n} walkers ({b['steps']} steps, {b['seconds']:.1f} s), "
+                  f"reference RNG (xorshift*, quantized geometric), {cores} threads, the faster of two CPU "
+                  f"schedules: {best}",
+        "schedules": {"walker_parallel": naive, "partitioned": part},
     }
 
 

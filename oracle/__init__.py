@@ -103,6 +103,11 @@ def lib():
         L.orc_ref_step_histogram.restype = None
         L.orc_xorshift_gen.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _P]
         L.orc_xorshift_gen.restype = None
+        L.orc_ref_partitioned_walk.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                               ctypes.c_uint64, ctypes.c_uint32, _P, _P, ctypes.POINTER(WalkStats),
+                                               ctypes.POINTER(ctypes.c_double), _P]
+        L.orc_ref_partitioned_walk.restype = ctypes.c_int
         L.orc_xorshift_gen_float.argtypes = [ctypes.c_uint64, ctypes.c_float, ctypes.c_uint64, _P]
         L.orc_xorshift_gen_float.restype = None
         L.orc_power_iteration.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_int,
@@ -246,6 +251,42 @@ def ref_walk(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, source: 
         raise ValueError("invalid reference-walk arguments")
     return dict(counts=counts, ends=None if ends is None else ends[:walkers], steps=st.steps, visits=st.visits,
                 dangling=st.dangling)
+
+
+def l2_cache_bytes() -> int:
+    """The host's per-core L2 size, which the reference reads to size its partitions."""
+    try:
+        n = os.sysconf("SC_LEVEL2_CACHE_SIZE")
+        if n and n > 0:
+            return int(n)
+    except (ValueError, OSError):
+        pass
+    return 1 << 20
+
+
+def ref_partitioned_walk(row_ptr, col, app: int, walkers: int, *, tp: float = 0.15, source: int = 0,
+                         start_uniform: bool = True, seed: int = 1, threads: int = 0, l2_bytes: int = 0,
+                         degree_bound: int = 32):
+    """The reference's own schedule (partitions, walker pools, migration, buffered hub
+    sampling) with its own RNG; returns counts (PAGERANK/PPR) or ends (RWR), the step
+    total, the walk time (walker init + walk) and the partition counts."""
+    rp, cl = _u32(row_ptr), _u32(col)
+    V = len(rp) - 1
+    counts = np.zeros(V, dtype=np.uint64) if app != APP_RWR else None
+    ends = np.zeros(max(walkers, 1), dtype=np.uint32) if app == APP_RWR else None
+    st = WalkStats()
+    secs = ctypes.c_double()
+    parts = np.zeros(2, dtype=np.uint32)
+    rc = lib().orc_ref_partitioned_walk(rp.ctypes.data, cl.ctypes.data, V, app, source, int(start_uniform), tp,
+                                        walkers, seed, threads, l2_bytes or l2_cache_bytes(), degree_bound,
+                                        counts.ctypes.data if counts is not None else None,
+                                        ends.ctypes.data if ends is not None else None, ctypes.byref(st),
+                                        ctypes.byref(secs), parts.ctypes.data)
+    if rc != 0:
+        raise ValueError("invalid partitioned-walk arguments")
+    return dict(counts=counts, ends=None if ends is None else ends[:walkers], steps=st.steps, visits=st.visits,
+                dangling=st.dangling, seconds=secs.value, hub_partitions=int(parts[0]),
+                low_partitions=int(parts[1]))
 
 
 def ref_step_histogram(tp: float, n: int, seed: int = 1, hist_len: int = 512) -> np.ndarray:

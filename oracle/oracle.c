@@ -607,3 +607,329 @@ int orc_power_iteration(const uint32_t *row_ptr, const uint32_t *col, uint32_t V
     free(z);
     return it;
 }
+
+/* ------------------------------------------------------------------ */
+/* The reference's own schedule: partitions, walker pools, migration,  */
+/* buffered hub sampling                                               */
+/* ------------------------------------------------------------------ */
+/* Graph side (src/core/graph.hpp:400-628): vertices of degree >= bound are
+ * hubs; they are shuffled and cut into hub partitions of about
+ * L2 * 0.9 / 80 vertices (an even number of partitions, half of them a
+ * multiple of the thread count); the other vertices are shuffled and cut in
+ * order into low partitions of at most L2 * 0.9 bytes (8 B per vertex + 4 B
+ * per edge, dp_partition).  The relabel puts the hubs first, then the low
+ * partitions, each padded to the largest one's vertex count, so a vertex's
+ * partition is arithmetic (vertex_partition_id_fuc, :159-169).  The
+ * reference first splits the low vertices into two max-cut groups; this
+ * restatement keeps one group and takes the first half of the low
+ * partitions as its "left" ones.
+ * Solver side (src/core/solver.hpp:758-1028, 194-498): every thread keeps a
+ * pool of 12 B walker records per partition; an iteration lets the threads
+ * claim partitions in the reference's map order, each under its lock, and
+ * advance every walker found in any thread's pool for it until the walker
+ * leaves the partition (it moves to the claiming thread's pool of its new
+ * partition) or ends.
+ * Sampling (src/core/sampler.hpp:43-167, 214-217): hub partitions draw
+ * from per-vertex buffers of pre-drawn neighbours, max(8, W L deg / E)
+ * entries (+1 if a power of two above 8; W = walkers still walking, L =
+ * floor(1 / tp) + 1), refilled when empty; low partitions pick directly.
+ * Counts (type 0) are plain increments: a partition's vertices are only
+ * touched by the thread that holds its lock. */
+typedef struct { uint32_t wid, vid, sid; } pw_walker;
+typedef struct { pw_walker *a; uint64_t n, cap; omp_lock_t lock; } pw_pool;
+typedef struct { uint32_t head, end; } pw_buf;
+
+typedef struct {
+    uint32_t H, vph, np_hub, hub_left, vpd, nlow, np;
+    uint64_t *rp;
+    uint32_t *cl, *newid, *oldid, *pend;
+} pw_graph;
+
+static inline uint32_t pw_part(const pw_graph *g, uint32_t v)
+{
+    return v < g->H ? v / g->vph : g->np_hub + (v - g->H) / g->vpd;
+}
+
+static void pw_shuffle(uint32_t *a, uint32_t n, xorshift_t *g)
+{
+    for (uint32_t i = n; i > 1; --i) {
+        uint32_t j = xs_gen(g, i);
+        uint32_t t = a[i - 1];
+        a[i - 1] = a[j];
+        a[j] = t;
+    }
+}
+
+static void pw_push(pw_pool *p, pw_walker w)
+{
+    if (p->n == p->cap) {
+        p->cap = p->cap ? 2 * p->cap : 4196; /* the reference's container size */
+        p->a = (pw_walker *)realloc(p->a, p->cap * sizeof(pw_walker));
+    }
+    p->a[p->n++] = w;
+}
+
+static uint32_t pw_buf_len(uint32_t deg, uint64_t walkers, uint32_t walk_len, uint64_t E)
+{
+    uint64_t len = E ? walkers * (uint64_t)walk_len * deg / E : 0;
+    if (len < 8) len = 8;
+    if (len > 8 && (len & (len - 1)) == 0) len++;
+    return (uint32_t)len;
+}
+
+int orc_ref_partitioned_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, int app, uint32_t source,
+                             int start_uniform, double tp, uint64_t walkers, uint64_t seed, int threads,
+                             uint64_t l2_bytes, uint32_t degree_bound, uint64_t *counts, uint32_t *ends,
+                             orc_walk_stats *st, double *walk_seconds, uint32_t *partitions)
+{
+    if (V == 0 || (app != ORC_APP_PAGERANK && app != ORC_APP_PPR && app != ORC_APP_RWR)) return -1;
+    if (!start_uniform && source >= V) return -1;
+    if (walkers > 0xFFFFFFFFull || !(tp > 0.0 && tp <= 1.0)) return -1;
+    const uint64_t E = row_ptr[V];
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+    if (l2_bytes == 0) l2_bytes = 1u << 20;
+    if (degree_bound == 0) degree_bound = 32;
+    xorshift_t gg;
+    xs_seed(&gg, seed, 0x9A47u);
+    /* ---- partitions and relabel ---- */
+    pw_graph G;
+    memset(&G, 0, sizeof(G));
+    uint32_t *hubs = (uint32_t *)malloc(sizeof(uint32_t) * V), *low = (uint32_t *)malloc(sizeof(uint32_t) * V);
+    uint32_t nh = 0, nl = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        if (row_ptr[v + 1] - row_ptr[v] >= degree_bound) hubs[nh++] = v;
+        else low[nl++] = v;
+    }
+    pw_shuffle(hubs, nh, &gg);
+    pw_shuffle(low, nl, &gg);
+    G.H = nh;
+    if (nh) {
+        uint64_t vph = l2_bytes * 9 / 10 / (8 + 8 + 64) - 1;
+        if (vph == 0) vph = 1;
+        uint32_t p = (uint32_t)(nh / vph + 1);
+        if (p & 1) ++p;
+        uint32_t left = p / 2;
+        if (left % (uint32_t)nt) left = (left / (uint32_t)nt + 1) * (uint32_t)nt;
+        G.np_hub = 2 * left;
+        G.hub_left = left;
+        G.vph = nh / G.np_hub + 1;
+    }
+    /* low partitions: consecutive runs of the shuffled order, at most subgraph bytes */
+    const uint64_t subgraph = l2_bytes * 9 / 10;
+    uint32_t *lcut = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)nl + 2));
+    uint32_t nlow = 0;
+    lcut[0] = 0;
+    {
+        uint64_t cur = 0;
+        for (uint32_t i = 0; i < nl; ++i) {
+            cur += 8 + 4ull * (row_ptr[low[i] + 1] - row_ptr[low[i]]);
+            if (cur >= subgraph) {
+                lcut[++nlow] = i + 1;
+                cur = 0;
+            }
+        }
+        if (lcut[nlow] != nl) lcut[++nlow] = nl;
+    }
+    G.nlow = nlow;
+    G.vpd = 1;
+    for (uint32_t c = 0; c < nlow; ++c)
+        if (lcut[c + 1] - lcut[c] > G.vpd) G.vpd = lcut[c + 1] - lcut[c];
+    G.np = G.np_hub + nlow;
+    const uint64_t Vp = (uint64_t)nh + (uint64_t)nlow * G.vpd;
+    G.newid = (uint32_t *)malloc(sizeof(uint32_t) * V);
+    G.oldid = (uint32_t *)malloc(sizeof(uint32_t) * Vp);
+    G.pend = (uint32_t *)malloc(sizeof(uint32_t) * (G.np ? G.np : 1));
+    for (uint64_t u = 0; u < Vp; ++u) G.oldid[u] = ORC_INVALID;
+    for (uint32_t i = 0; i < nh; ++i) {
+        G.newid[hubs[i]] = i;
+        G.oldid[i] = hubs[i];
+    }
+    for (uint32_t p = 0; p < G.np_hub; ++p) {
+        uint64_t e = (uint64_t)(p + 1) * G.vph;
+        G.pend[p] = (uint32_t)(e < nh ? e : nh);
+    }
+    for (uint32_t c = 0; c < nlow; ++c) {
+        const uint32_t base = nh + c * G.vpd;
+        for (uint32_t i = lcut[c]; i < lcut[c + 1]; ++i) {
+            G.newid[low[i]] = base + (i - lcut[c]);
+            G.oldid[base + (i - lcut[c])] = low[i];
+        }
+        G.pend[G.np_hub + c] = base + (lcut[c + 1] - lcut[c]);
+    }
+    free(hubs);
+    free(low);
+    free(lcut);
+    G.rp = (uint64_t *)malloc(sizeof(uint64_t) * (Vp + 1));
+    G.cl = (uint32_t *)malloc(sizeof(uint32_t) * (E ? E : 1));
+    G.rp[0] = 0;
+    for (uint64_t u = 0; u < Vp; ++u) {
+        const uint32_t o = G.oldid[u];
+        G.rp[u + 1] = G.rp[u] + (o == ORC_INVALID ? 0 : row_ptr[o + 1] - row_ptr[o]);
+    }
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 4096)
+    for (int64_t u = 0; u < (int64_t)Vp; ++u) {
+        const uint32_t o = G.oldid[u];
+        if (o == ORC_INVALID) continue;
+        for (uint32_t k = row_ptr[o]; k < row_ptr[o + 1]; ++k) G.cl[G.rp[u] + (k - row_ptr[o])] = G.newid[col[k]];
+    }
+    /* ---- hub edge buffers ---- */
+    const uint32_t walk_len = (uint32_t)(1.0 / tp) + 1;
+    pw_buf *hb = (pw_buf *)malloc(sizeof(pw_buf) * (nh ? nh : 1));
+    uint64_t units_n = 0;
+    for (uint32_t h = 0; h < nh; ++h) {
+        units_n += pw_buf_len((uint32_t)(G.rp[h + 1] - G.rp[h]), walkers, walk_len, E);
+        hb[h].end = (uint32_t)units_n;
+        hb[h].head = hb[h].end;
+    }
+    uint32_t *units = (uint32_t *)malloc(sizeof(uint32_t) * (units_n ? units_n : 1));
+    /* ---- pools, locks, map ---- */
+    pw_pool *pools = (pw_pool *)calloc((size_t)nt * G.np, sizeof(pw_pool));
+    for (size_t i = 0; i < (size_t)nt * G.np; ++i) omp_init_lock(&pools[i].lock);
+    omp_lock_t *plock = (omp_lock_t *)malloc(sizeof(omp_lock_t) * G.np);
+    for (uint32_t p = 0; p < G.np; ++p) omp_init_lock(&plock[p]);
+    const uint32_t low_left = nlow / 2;
+    uint32_t nmap = 0;
+    uint32_t *map = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)G.np + G.np_hub + 1));
+    for (uint32_t p = 0; p < G.hub_left; ++p) map[nmap++] = p;
+    for (uint32_t p = 0; p < low_left; ++p) map[nmap++] = G.np_hub + p;
+    for (uint32_t p = 0; p < G.hub_left; ++p) map[nmap++] = p;
+    for (uint32_t p = G.hub_left; p < G.np_hub; ++p) map[nmap++] = p;
+    for (uint32_t p = low_left; p < nlow; ++p) map[nmap++] = G.np_hub + p;
+    for (uint32_t p = G.hub_left; p < G.np_hub; ++p) map[nmap++] = p;
+    uint64_t *cnt = NULL;
+    if (app != ORC_APP_RWR) cnt = (uint64_t *)calloc(Vp ? Vp : 1, sizeof(uint64_t));
+    geo_table t;
+    geo_init(&t, tp);
+    xorshift_t *gens = (xorshift_t *)aligned_alloc(64, sizeof(xorshift_t) * (size_t)nt);
+    for (int k = 0; k < nt; ++k) xs_seed(&gens[k], seed, (uint64_t)k + 1);
+    const int type0 = app != ORC_APP_RWR;
+    uint64_t tot_steps = 0, tot_visits = 0, tot_dangling = 0;
+    const double t0 = omp_get_wtime();
+    /* ---- init_start_walks ---- */
+#pragma omp parallel num_threads(nt)
+    {
+        const int tid = omp_get_thread_num();
+        xorshift_t *g = &gens[tid];
+#pragma omp for schedule(static)
+        for (int64_t w = 0; w < (int64_t)walkers; ++w) {
+            const uint32_t v = G.newid[start_uniform ? xs_gen(g, V) : source];
+            pw_walker r = {(uint32_t)w, v, geo_draw(&t, g)};
+            pw_push(&pools[(size_t)tid * G.np + pw_part(&G, v)], r);
+        }
+    }
+    /* ---- iterations ---- */
+    uint64_t finished = 0;
+    while (finished < walkers) {
+        const uint64_t still = walkers - finished;
+        uint32_t next = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : tot_steps, tot_visits, tot_dangling, finished)
+        {
+            const int tid = omp_get_thread_num();
+            xorshift_t *g = &gens[tid];
+            pw_pool taken;
+            memset(&taken, 0, sizeof(taken));
+            for (;;) {
+                const uint32_t pos = __atomic_fetch_add(&next, 1, __ATOMIC_RELAXED);
+                if (pos >= nmap) break;
+                const uint32_t p = map[pos];
+                omp_set_lock(&plock[p]);
+                for (int s = 0; s < nt; ++s) {
+                    pw_pool *src = &pools[(size_t)s * G.np + p];
+                    omp_set_lock(&src->lock);
+                    pw_walker *a = src->a;
+                    const uint64_t n = src->n;
+                    const uint64_t cap = src->cap;
+                    src->a = taken.a;
+                    src->n = 0;
+                    src->cap = taken.cap;
+                    omp_unset_lock(&src->lock);
+                    taken.a = a;
+                    taken.cap = cap;
+                    for (uint64_t k = 0; k < n; ++k) {
+                        pw_walker w = a[k];
+                        for (;;) {
+                            if (type0) {
+                                cnt[w.vid]++;
+                                tot_visits++;
+                            }
+                            if (w.sid == 0) {
+                                finished++;
+                                if (ends) ends[w.wid] = G.oldid[w.vid];
+                                break;
+                            }
+                            const uint64_t beg = G.rp[w.vid];
+                            const uint32_t deg = (uint32_t)(G.rp[w.vid + 1] - beg);
+                            if (deg == 0) {
+                                tot_dangling++;
+                                finished++;
+                                if (ends) ends[w.wid] = G.oldid[w.vid];
+                                break;
+                            }
+                            uint32_t nx;
+                            if (w.vid < G.H) {
+                                pw_buf *b = &hb[w.vid];
+                                if (b->head == b->end) {
+                                    uint32_t len = pw_buf_len(deg, still, walk_len, E);
+                                    const uint32_t room = b->end - (w.vid ? hb[w.vid - 1].end : 0);
+                                    if (len > room) len = room;
+                                    for (uint32_t e = b->end - len; e < b->end; ++e) units[e] = G.cl[beg + xs_gen(g, deg)];
+                                    b->head = b->end - len;
+                                }
+                                nx = units[b->head++];
+                            } else {
+                                nx = G.cl[beg + xs_gen(g, deg)];
+                            }
+                            w.vid = nx;
+                            w.sid--;
+                            tot_steps++;
+                            const uint32_t q = pw_part(&G, nx);
+                            if (q != p) {
+                                pw_pool *dst = &pools[(size_t)tid * G.np + q];
+                                omp_set_lock(&dst->lock);
+                                pw_push(dst, w);
+                                omp_unset_lock(&dst->lock);
+                                break;
+                            }
+                        }
+                    }
+                }
+                omp_unset_lock(&plock[p]);
+            }
+            free(taken.a);
+        }
+    }
+    const double t1 = omp_get_wtime();
+    if (walk_seconds) *walk_seconds = t1 - t0;
+    if (partitions) {
+        partitions[0] = G.np_hub;
+        partitions[1] = nlow;
+    }
+    if (counts && cnt)
+        for (uint64_t u = 0; u < Vp; ++u)
+            if (G.oldid[u] != ORC_INVALID) counts[G.oldid[u]] += cnt[u];
+    if (st) {
+        st->steps = tot_steps;
+        st->visits = tot_visits;
+        st->dangling = tot_dangling;
+        st->proposals = 0;
+        st->exact_draws = 0;
+    }
+    for (size_t i = 0; i < (size_t)nt * G.np; ++i) {
+        omp_destroy_lock(&pools[i].lock);
+        free(pools[i].a);
+    }
+    for (uint32_t p = 0; p < G.np; ++p) omp_destroy_lock(&plock[p]);
+    free(pools);
+    free(plock);
+    free(map);
+    free(cnt);
+    free(gens);
+    free(units);
+    free(hb);
+    free(G.rp);
+    free(G.cl);
+    free(G.newid);
+    free(G.oldid);
+    free(G.pend);
+    return 0;
+}

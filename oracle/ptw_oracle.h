@@ -137,6 +137,20 @@ int orc_ref_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V,
                  uint64_t walkers, uint64_t seed, int threads,
                  uint64_t *counts, uint32_t *ends, orc_walk_stats *st);
 
+/* The reference's own schedule on the CPU, with its own RNG: hub/low
+ * partitions from a degree bound and the L2 size (l2_bytes 0: 1 MiB;
+ * degree_bound 0: 32), per-thread per-partition walker pools, partitions
+ * claimed in the reference's order under locks, migration, buffered hub
+ * sampling (src/core/graph.hpp:400-628, src/core/solver.hpp:194-498,
+ * 758-1028, src/core/sampler.hpp:43-167).  Apps PAGERANK/PPR (counts,
+ * accumulated into u64[V]) and RWR (ends u32[walkers]).  walk_seconds: walker
+ * init + walk (the reference's printed scope); partitions[2]: hub and low
+ * partition counts.  walkers <= 2^32 - 1. */
+int orc_ref_partitioned_walk(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, int app, uint32_t source,
+                             int start_uniform, double tp, uint64_t walkers, uint64_t seed, int threads,
+                             uint64_t l2_bytes, uint32_t degree_bound, uint64_t *counts, uint32_t *ends,
+                             orc_walk_stats *st, double *walk_seconds, uint32_t *partitions);
+
 /* The reference's quantized geometric draw: 1 + K with the 128-entry CDF and
  * xorshift* gen_float.  Runs `n` draws from one generator seeded with `seed`
  * and writes a histogram of min(L, hist_len - 1) into hist (u64[hist_len]). */

@@ -45,11 +45,20 @@ def test_learned_budget_beats_fixed_on_unseen_graph(ctx):
     cov_l = float(np.mean(learned >= need))
     cov_f = float(np.mean(fixed >= need))
     ratio = float(fixed.sum() / learned.sum())
+    # What any predictor can gain here: a fixed budget tuned on this very set
+    # to the learned budgets' coverage, against a perfect predictor that gives
+    # every query exactly its requirement (coverage 1)
+    eps_of = np.repeat([1e-3, 3e-4], len(src))
+    c2 = need * eps_of ** 2
+    bound = float((np.quantile(c2, cov_l) / eps_of ** 2).sum() / need.sum())
     print(f"coverage learned {cov_l:.3f} fixed {cov_f:.3f}; fixed/learned walk work {ratio:.2f}; "
-          f"fallbacks {fallback.mean():.3f}")
+          f"perfect-predictor bound {bound:.2f}; fallbacks {fallback.mean():.3f}")
     assert fallback.mean() < 0.05
     assert cov_l >= 0.90 and cov_l >= cov_f - 0.03
-    assert ratio > 1.25
+    # measured 1.44 (seeded: the walks, graph and sources repeat exactly); the
+    # margin covers the solver's and the network's float reductions
+    assert ratio > 1.40
+    assert ratio > 0.8 * bound
 
 
 def test_calibration_on_the_serving_graph(ctx):
@@ -89,4 +98,7 @@ def test_calibration_on_the_serving_graph(ctx):
     print(f"calibrated margin {cal.margin:.3f} fixed_c2 {cal.fixed_c2:.3f}; held-out realized coverage learned "
           f"{cl:.3f} fixed {cf:.3f}; fixed/learned work {work['fixed'] / work['learned']:.2f}; {stats}")
     assert cl >= 0.9 and cf >= 0.9 and abs(cl - cf) < 0.05
-    assert work["fixed"] / work["learned"] > 1.0
+    # measured 1.07 at equal held-out coverage: off its training scale (20) the
+    # network ranks difficulty less well (calibrated margin 1.90 here, 1.41 on
+    # scale 22), so the gain shrinks but stays
+    assert work["fixed"] / work["learned"] > 1.03

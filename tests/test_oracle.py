@@ -370,6 +370,69 @@ def test_reference_path_agrees_with_fp64():
     assert abs(mean.sum() - (1 + 1 / tp)) < 0.05
 
 
+# the reference's partitioned schedule: (L2 bytes, degree bound) pairs that give only a
+# low partition, hub partitions only, and both kinds (16 hub + 21 low on this graph)
+PARTITIONINGS = [(0, 1000), (4096, 12), (1024, 20)]
+
+
+@pytest.mark.parametrize("l2,bound", PARTITIONINGS)
+def test_partitioned_schedule_agrees_with_fp64(l2, bound):
+    rp, col = random_graph(400, 0.02, seed=6)
+    tp, W = 0.2, 300000
+    r = oracle.ref_partitioned_walk(rp, col, oracle.APP_PAGERANK, W, tp=tp, seed=5, threads=4, l2_bytes=l2,
+                                    degree_bound=bound)
+    assert r["hub_partitions"] + r["low_partitions"] > 0
+    if (l2, bound) == (1024, 20):
+        assert r["hub_partitions"] > 1 and r["low_partitions"] > 1
+    assert int(r["counts"].sum()) == W + r["steps"]
+    exp = oracle.power_iteration(rp, col, tp=tp, start=None, mode="visits")
+    sd = np.sqrt((2 / tp ** 2 + 3 / tp + 1) / W)
+    assert np.max(np.abs(r["counts"] / W - exp)) < 6 * sd
+    # single source
+    r = oracle.ref_partitioned_walk(rp, col, oracle.APP_PPR, W, tp=tp, source=11, start_uniform=False, seed=2,
+                                    threads=3, l2_bytes=l2, degree_bound=bound)
+    exp = oracle.power_iteration(rp, col, tp=tp, start=11, mode="visits")
+    assert np.max(np.abs(r["counts"] / W - exp)) < 6 * sd
+
+
+@pytest.mark.parametrize("l2,bound", PARTITIONINGS)
+def test_partitioned_schedule_rwr_endpoint_law(l2, bound):
+    rp, col = random_graph(400, 0.02, seed=6)
+    tp, W = 0.2, 300000
+    r = oracle.ref_partitioned_walk(rp, col, oracle.APP_RWR, W, tp=tp, source=4, start_uniform=False, seed=3,
+                                    threads=4, l2_bytes=l2, degree_bound=bound)
+    assert len(r["ends"]) == W and r["ends"].max() < 400
+    pe = oracle.power_iteration(rp, col, tp=tp, start=4, mode="ends")
+    f = np.bincount(r["ends"], minlength=400) / W
+    assert abs(pe.sum() - 1.0) < 1e-9
+    # per-vertex binomial bound, 6 sigma
+    assert np.all(np.abs(f - pe) <= 6 * np.sqrt(pe * (1 - pe) / W) + 1e-5)
+
+
+def test_partitioned_schedule_dangling_matches_naive_schedule():
+    # walkers that reach a vertex without out-edges end there, in both schedules
+    rp, col = random_graph(300, 0.03, seed=9, dangling=5)
+    tp, W = 0.2, 300000
+    a = oracle.ref_walk(rp, col, oracle.APP_RWR, W, tp=tp, source=4, start_uniform=False, seed=3, threads=3)
+    b = oracle.ref_partitioned_walk(rp, col, oracle.APP_RWR, W, tp=tp, source=4, start_uniform=False, seed=8,
+                                    threads=3, l2_bytes=1024, degree_bound=20)
+    assert b["dangling"] > 0
+    fa = np.bincount(a["ends"], minlength=300) / W
+    fb = np.bincount(b["ends"], minlength=300) / W
+    # two independent samples: difference variance <= 2 p (1 - p) / W
+    p = (fa + fb) / 2
+    assert np.all(np.abs(fa - fb) <= 6 * np.sqrt(2 * p * (1 - p) / W) + 1e-5)
+    assert fb[-5:].sum() > 0.04  # the dangling tail holds real mass
+
+
+def test_partitioned_schedule_rejects_bad_arguments():
+    rp, col = random_graph(20, 0.2, seed=1)
+    with pytest.raises(ValueError):
+        oracle.ref_partitioned_walk(rp, col, oracle.APP_PPR, 10, source=20, start_uniform=False)
+    with pytest.raises(ValueError):
+        oracle.ref_partitioned_walk(rp, col, oracle.APP_PPR, 10, tp=0.0)
+
+
 # ---- loader semantics ----------------------------------------------------------------
 def test_reference_loader_semantics(tmp_path):
     p = tmp_path / "g.txt"
