@@ -836,9 +836,10 @@ def bench_cli(args):
 
 def bench_ingest(args):
     """Graph load (ptw_graph_load) of an R-MAT edge list written as text and as
-    binary: the file parsed and the CSR built on the device against the same on
-    the host threads (mmap + parallel parse), both followed by the same degree
-    relabel and upload.  The reference reads the text with one thread
+    binary: the file parsed, the CSR built and degree-relabelled on the device
+    (the graph-id CSR and the relabel copied back as the host mirror) against
+    the host threads (mmap + parallel parse, relabel, upload).  The reference
+    reads the text with one thread
     (include/io.hpp:76-90).  Files are written to a temporary directory first
     (not timed); each load is timed from the call to the graph being resident."""
     import shutil
@@ -892,14 +893,15 @@ def bench_ingest(args):
             "seconds": {k: round(v, 3) for k, v in res.items()},
             "device_over_host": {"text": res["text_host_s"] / res["text_device_s"],
                                  "binary": res["binary_host_s"] / res["binary_device_s"]},
-            "roofline": {"bound": "hbm", "kernel": "whole load (file copy, parse, sorts, download, relabel, upload)",
+            "roofline": {"bound": "hbm", "kernel": "whole load (file read and copy, parse, sorts, relabel, host mirror)",
                          "achieved": size / res["text_device_s"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": size / res["text_device_s"] / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                         "note": "end-to-end input bytes per second; PCIe and the host relabel bound it"},
+                         "note": "end-to-end input bytes per second; the host file read, PCIe and the host "
+                                 "mirror's download bound it"},
             "cpu_baseline": {"value": gi.num_edges / res["text_host_s"], "unit": "edges/s",
                              "cores": min(cores, 16), "kind": "port",
                              "sample": "the same load with the file parsed and the CSR built by host threads "
-                                       "(mmap + parallel parse), same relabel and upload"},
+                                       "(mmap + parallel parse), relabelled on the host and uploaded"},
         }
         print(json.dumps(line), flush=True)
         ctx.close()

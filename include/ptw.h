@@ -501,6 +501,15 @@ int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint6
  * own accesses, without the dependence between a walker's steps.  No
  * counterpart in the reference. */
 int ptw_bench_step(ptw_ctx *ctx, uint64_t steps, uint64_t seed, double *steps_per_s, double *kernel_ms);
+/* Self-check of the device primitives behind graph ingest: exclusive scans
+ * (u8 -> u64, u32, u64), the stable radix sort of (u32 key, u32 value) pairs
+ * over the low `bits` (1..32) key bits, the flagged select and the u32 max,
+ * each run on n pseudo-random items drawn from `seed` (keys in [0, 2^bits),
+ * with runs of repeated keys) and compared element by element with a host
+ * computation (std::stable_sort for the sort).  Returns PTW_OK, or
+ * PTW_ERR_FORMAT with the first difference in ptw_last_error.  n < 2^32 - 1.
+ * No counterpart in the reference (which sorts on the host). */
+int ptw_check_primitives(ptw_ctx *ctx, uint64_t n, int bits, uint64_t seed);
 /* Streaming copy of `bytes` (multiple of 16) between two device buffers:
  * the stream roofline, and the known-size kernel that calibrates HBM byte
  * counters.  No counterpart in the reference. */

@@ -327,6 +327,7 @@ _SIGNATURES = {
     "ptw_buffer_zero": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "ptw_device_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "ptw_bench_step": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _P, _P]),
+    "ptw_check_primitives": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]),
     "ptw_bench_gather": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "ptw_bench_copy": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
@@ -651,6 +652,11 @@ class Context:
         ms = ctypes.c_double()
         self.check(self.lib.ptw_bench_step(self.h, steps, seed, ctypes.byref(sps), ctypes.byref(ms)))
         return sps.value, ms.value
+
+    def check_primitives(self, n: int, bits: int = 32, seed: int = 1):
+        """ptw_check_primitives: the ingest scans, sort, select and max on n random
+        items against a host computation; raises PtwError on the first difference."""
+        self.check(self.lib.ptw_check_primitives(self.h, n, bits, seed))
 
 
 class BudgetMLP:

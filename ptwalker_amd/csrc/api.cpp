@@ -14,6 +14,7 @@
 
 #include "../../include/ptw.h"
 #include "ctx.hpp"
+#include "prims.hpp"
 #include "engine.hpp"
 
 namespace {
@@ -180,7 +181,9 @@ int ensure_node2vec_tables(ptw_ctx *ctx)
     return PTW_OK;
 }
 
-int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
+// With `built`, the device graph was made on the device (ingest) and is taken
+// over as it is, with its relabel; otherwise it is relabelled here and uploaded.
+int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags, ptw::DeviceBuiltGraph *built = nullptr)
 {
     free_graph(ctx);
     ctx->host = std::move(g);
@@ -208,7 +211,18 @@ int install_graph(ptw_ctx *ctx, HostCsr &&g, uint32_t flags)
         return PTW_OK;
     }
     int rc;
-    if (flags & PTW_GRAPH_NO_RELABEL) {
+    if (built) {
+        d.row_ptr = built->row_ptr;
+        d.col = built->col;
+        d.perm = built->perm;
+        d.inv = built->inv;
+        d.relabelled = built->relabelled;
+        d.hubs = built->relabelled ? V : 0;
+        ctx->h_perm = std::move(built->h_perm);
+        ctx->h_inv = std::move(built->h_inv);
+        ctx->h_drow = std::move(built->h_drow);
+        *built = ptw::DeviceBuiltGraph{};
+    } else if (flags & PTW_GRAPH_NO_RELABEL) {
         if ((rc = upload(ctx, &d.row_ptr, h.row_ptr.data(), h.row_ptr.size()))) return rc;
         if ((rc = upload(ctx, &d.col, h.col.data(), h.col.size()))) return rc;
         d.relabelled = false;
@@ -487,8 +501,16 @@ int ptw_graph_load(ptw_ctx *ctx, const char *path, int format, int as_undirected
             on_device = stat(path, &st) == 0 && (uint64_t)st.st_size >= (16ull << 20);
         }
         HostCsr g;
-        int rc = on_device ? ptw::device_load_edge_list(ctx->stream, path, format, as_undirected != 0, g, ctx->err)
-                           : ptw::load_edge_list(path, format, as_undirected != 0, g, ctx->err);
+        if (on_device) {
+            ptw::DeviceBuiltGraph built;
+            int rc = ptw::device_load_edge_list(ctx->stream, path, format, as_undirected != 0,
+                                                !(flags & PTW_GRAPH_NO_RELABEL), g, built, ctx->err);
+            if (rc) return rc;
+            if (g.V == 0)
+                return fail(ctx, PTW_ERR_FORMAT, std::string("graph file '") + path + "' holds no edges");
+            return install_graph(ctx, std::move(g), flags, &built);
+        }
+        int rc = ptw::load_edge_list(path, format, as_undirected != 0, g, ctx->err);
         if (rc) return rc;
         if (g.V == 0) return fail(ctx, PTW_ERR_FORMAT, std::string("graph file '") + (path ? path : "") + "' holds no edges");
         return install_graph(ctx, std::move(g), flags);
@@ -938,6 +960,24 @@ int ptw_bench_step(ptw_ctx *ctx, uint64_t steps, uint64_t seed, double *steps_pe
     if (kernel_ms) *kernel_ms = ms;
     if (steps_per_s) *steps_per_s = ms > 0 ? (double)steps / (ms * 1e-3) : 0.0;
     return PTW_OK;
+}
+
+int ptw_check_primitives(ptw_ctx *ctx, uint64_t n, int bits, uint64_t seed)
+{
+    if (!ctx) return PTW_ERR_INVALID_ARG;
+    if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: no HIP device");
+    if (n >= 0xFFFFFFFFull || bits < 1 || bits > 32)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "n must be below 2^32 - 1 and bits in 1..32");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    try {
+        std::string msg;
+        const int r = ptw::prims::self_check(n, bits, seed, ctx->stream, msg);
+        if (r == -2) return fail(ctx, PTW_ERR_HIP, msg);
+        if (r != 0) return fail(ctx, PTW_ERR_FORMAT, msg);
+        return PTW_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PTW_ERR_OOM, "host allocation failed in the primitive self-check");
+    }
 }
 
 int ptw_bench_gather(ptw_ctx *ctx, uint64_t table_bytes, uint64_t gathers, uint64_t seed, double *gathers_per_s,

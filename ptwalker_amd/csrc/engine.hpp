@@ -168,9 +168,20 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s,
                        EffectiveShape *eff = nullptr);
 
-// Graph ingest on the device with the host loader's semantics (ingest.hip).
-int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, HostCsr &g,
-                          std::string &err);
+// A graph built on the device: its CSR in device order (degree descending
+// when relabelled, else graph ids), the relabel on both sides.  The device
+// buffers are owned by whoever receives it.
+struct DeviceBuiltGraph {
+    uint32_t *row_ptr = nullptr, *col = nullptr, *perm = nullptr, *inv = nullptr;
+    bool relabelled = false;
+    std::vector<uint32_t> h_perm, h_inv, h_drow;
+};
+
+// Graph ingest on the device with the host loader's semantics (ingest.hip):
+// `g` receives the graph-id CSR and names, `built` the device graph (with
+// the degree relabel of relabel_by_degree when `relabel`).
+int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, bool relabel,
+                          HostCsr &g, DeviceBuiltGraph &built, std::string &err);
 hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
                                       uint32_t V, uint64_t *out, bool accumulate,
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);

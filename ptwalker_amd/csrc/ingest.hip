@@ -23,9 +23,14 @@
 //      first element of each run holds the id's first position; sorting the
 //      runs by that position gives the new ids, scattered back to positions
 //   4. a stable radix sort of the edges by new source id gives the adjacency
-//      in edge order; a degree histogram and a scan give row_ptr
+//      in edge order, and row_ptr follows from the sorted sources
+//   5. (with the relabel) the degree-descending device order: a stable sort
+//      of the vertices by complemented degree (ties keep graph-id order), a
+//      scan of the degrees in that order, and one pass over the edges that
+//      writes each neighbour's device id to its place in the device CSR
+// The scans, sorts and select are prims.hip's.  The graph-id CSR and the
+// relabel come back to the host as its mirror; the device CSR stays.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <fcntl.h>
 #include <sys/stat.h>
@@ -40,6 +45,7 @@
 
 #include "../../include/ptw.h"
 #include "engine.hpp"
+#include "prims.hpp"
 
 namespace ptw {
 
@@ -263,6 +269,45 @@ __global__ void row_offsets_kernel(const uint32_t *ssrc, uint64_t E, uint32_t V,
     }
 }
 
+// Relabel, step 5: key = ~degree (ascending = degree descending), value =
+// graph id.
+__global__ void degree_keys_kernel(const uint32_t *row, uint32_t V, uint32_t *key, uint32_t *val)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        key[v] = ~(row[v + 1] - row[v]);
+        val[v] = (uint32_t)v;
+    }
+}
+
+// inv[d] is the graph id at device position d: perm is its inverse and
+// ddeg[d] its degree (scanned into the device row offsets).
+__global__ void permutation_kernel(const uint32_t *inv, const uint32_t *row, uint32_t V, uint32_t *perm,
+                                   uint32_t *ddeg)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; d < V; d += stride) {
+        const uint32_t u = inv[d];
+        perm[u] = (uint32_t)d;
+        ddeg[d] = row[u + 1] - row[u];
+    }
+}
+
+__global__ void set_word_kernel(uint32_t *p, uint32_t v) { *p = v; }
+
+// Edge i of source u (graph ids, sorted by source) is entry i - row[u] of
+// u's row; in the device CSR that row starts at drow[perm[u]] and holds
+// device ids.
+__global__ void device_col_kernel(const uint32_t *ssrc, const uint32_t *col, const uint32_t *row,
+                                  const uint32_t *perm, const uint32_t *drow, uint64_t E, uint32_t *dcol)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += stride) {
+        const uint32_t u = ssrc[i];
+        dcol[(uint64_t)drow[perm[u]] + (i - row[u])] = perm[col[i]];
+    }
+}
+
 uint32_t grid_for(uint64_t n)
 {
     uint64_t b = (n + 255) / 256;
@@ -285,6 +330,12 @@ struct Buffers {
         if (e == hipSuccess) p.push_back(q);
         *out = (T *)q;
         return e;
+    }
+    // hands a buffer to the caller: it is no longer freed here
+    void release(void *q)
+    {
+        for (void *&x : p)
+            if (x == q) x = nullptr;
     }
 };
 
@@ -313,28 +364,43 @@ struct Fd {
         }                                                                          \
     } while (0)
 
-// Stable radix sort of (key, value) pairs over `bits` key bits, with its
-// temporary storage.
-template <typename K, typename V>
-int sort_pairs(Buffers &B, const K *kin, K *kout, const V *vin, V *vout, uint64_t n, int bits, hipStream_t s,
-               std::string &err)
+// Temporary storage of the primitives, grown and kept for the whole load.
+struct Temp {
+    Buffers &B;
+    hipStream_t s;
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        // an earlier primitive may still be using the buffer
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        if (p) {
+            (void)hipFree(p);
+            B.release(p);
+            p = nullptr;
+            cap = 0;
+        }
+        if ((e = B.get(&p, bytes)) == hipSuccess) cap = bytes;
+        return e;
+    }
+};
+
+// Stable radix sort of (key, value) pairs over `bits` key bits.
+int sort_pairs(Temp &T, const uint32_t *kin, uint32_t *kout, const uint32_t *vin, uint32_t *vout, uint64_t n, int bits,
+               hipStream_t s, std::string &err)
 {
-    size_t tmp = 0;
-    ING_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, bits, s));
-    uint8_t *t = nullptr;
-    ING_TRY(B.get(&t, tmp));
-    ING_TRY(hipcub::DeviceRadixSort::SortPairs(t, tmp, kin, kout, vin, vout, n, 0, bits, s));
+    ING_TRY(T.need(prims::sort_temp_bytes(n)));
+    ING_TRY(prims::sort_pairs(kin, kout, vin, vout, n, bits, T.p, s));
     return PTW_OK;
 }
 
-template <typename T, typename U>
-int exclusive_sum(Buffers &B, const T *in, U *out, uint64_t n, hipStream_t s, std::string &err)
+template <typename X, typename U>
+int exclusive_sum(Temp &T, const X *in, U *out, uint64_t n, hipStream_t s, std::string &err)
 {
-    size_t tmp = 0;
-    ING_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s));
-    uint8_t *t = nullptr;
-    ING_TRY(B.get(&t, tmp));
-    ING_TRY(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, n, s));
+    ING_TRY(T.need(prims::scan_temp_bytes(n, sizeof(U))));
+    ING_TRY(prims::exclusive_scan(in, out, n, T.p, s));
     return PTW_OK;
 }
 
@@ -347,8 +413,8 @@ int bits_for(uint64_t max_value)
 
 }  // namespace
 
-int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, HostCsr &g,
-                          std::string &err)
+int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, bool relabel,
+                          HostCsr &g, DeviceBuiltGraph &built, std::string &err)
 {
     if (!path) {
         err = "graph path is NULL";
@@ -371,6 +437,7 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     }
     const uint64_t size = (uint64_t)st.st_size;
     Buffers B;
+    Temp T{B, s};
     int rc;
     // ---- 1. the file, padded with zeros to whole tiles ----
     const uint64_t tiles = (size + kTile - 1) / kTile;
@@ -448,7 +515,7 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
             hipLaunchKernelGGL(count_newlines_kernel, dim3(grid_for(tiles * 256)), dim3(kTileThreads), 0, s,
                                reinterpret_cast<const uint4 *>(file), tiles, tile_count);
             ING_TRY(hipGetLastError());
-            if ((rc = exclusive_sum(B, tile_count, tile_base, tiles, s, err))) return rc;
+            if ((rc = exclusive_sum(T, tile_count, tile_base, tiles, s, err))) return rc;
         }
         uint64_t last_base = 0, last_count = 0;
         if (tiles) {
@@ -478,11 +545,8 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
             hipLaunchKernelGGL(parse_lines_kernel, dim3(grid_for(lines)), dim3(256), 0, s, file, size, nl, n_nl,
                                lines, all, flags);
             ING_TRY(hipGetLastError());
-            size_t tmp = 0;
-            ING_TRY(hipcub::DeviceSelect::Flagged(nullptr, tmp, all, flags, pairs, n_sel, (int64_t)lines, s));
-            uint8_t *t = nullptr;
-            ING_TRY(B.get(&t, tmp));
-            ING_TRY(hipcub::DeviceSelect::Flagged(t, tmp, all, flags, pairs, n_sel, (int64_t)lines, s));
+            ING_TRY(T.need(prims::select_temp_bytes(lines)));
+            ING_TRY(prims::select_flagged(all, flags, pairs, lines, n_sel, T.p, s));
             ING_TRY(hipMemcpyAsync(&m, n_sel, 8, hipMemcpyDeviceToHost, s));
             ING_TRY(hipStreamSynchronize(s));
         }
@@ -503,13 +567,21 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     ING_TRY(B.get(&vals, n));
     ING_TRY(B.get(&skeys, n));
     ING_TRY(B.get(&svals, n));
+    // the id sort needs only the bits of the largest raw id
+    uint32_t *dmax = nullptr, max_id = 0;
+    ING_TRY(B.get(&dmax, 1));
+    ING_TRY(prims::max_u32(pairs, m, dmax, s));
+    ING_TRY(hipMemcpyAsync(&max_id, dmax, 4, hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(endpoints_kernel, dim3(grid_for(m)), dim3(256), 0, s, pairs, m, keys, vals);
     ING_TRY(hipGetLastError());
-    if ((rc = sort_pairs(B, keys, skeys, vals, svals, n, 32, s, err))) return rc;
+    ING_TRY(hipStreamSynchronize(s));
+    // one allocation covers every later primitive: the sorts are the largest
+    ING_TRY(T.need(std::max(prims::sort_temp_bytes(n), prims::scan_temp_bytes(n, 4))));
+    if ((rc = sort_pairs(T, keys, skeys, vals, svals, n, bits_for(max_id), s, err))) return rc;
     uint32_t *head = keys, *run_excl = vals;  // reuse
     hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n)), dim3(256), 0, s, skeys, n, head);
     ING_TRY(hipGetLastError());
-    if ((rc = exclusive_sum(B, head, run_excl, n, s, err))) return rc;
+    if ((rc = exclusive_sum(T, head, run_excl, n, s, err))) return rc;
     uint32_t last_excl = 0, last_head = 0;
     ING_TRY(hipMemcpyAsync(&last_excl, run_excl + n - 1, 4, hipMemcpyDeviceToHost, s));
     ING_TRY(hipMemcpyAsync(&last_head, head + n - 1, 4, hipMemcpyDeviceToHost, s));
@@ -527,7 +599,7 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     hipLaunchKernelGGL(first_positions_kernel, dim3(grid_for(n)), dim3(256), 0, s, head, run_excl, skeys, svals, n,
                        first_pos, run_idx, run_key);
     ING_TRY(hipGetLastError());
-    if ((rc = sort_pairs(B, first_pos, sfirst, run_idx, order, V, bits_for(n - 1), s, err))) return rc;
+    if ((rc = sort_pairs(T, first_pos, sfirst, run_idx, order, V, bits_for(n - 1), s, err))) return rc;
     hipLaunchKernelGGL(rank_runs_kernel, dim3(grid_for(V)), dim3(256), 0, s, order, run_key, V, new_id, names);
     ING_TRY(hipGetLastError());
     uint32_t *mapped = skeys;  // reuse: position -> new id
@@ -545,9 +617,32 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     hipLaunchKernelGGL(edges_kernel, dim3(grid_for(m)), dim3(256), 0, s, mapped, m, as_undirected ? 1 : 0, esrc,
                        edst);
     ING_TRY(hipGetLastError());
-    if ((rc = sort_pairs(B, esrc, ssrc, edst, col, E, bits_for(V - 1), s, err))) return rc;
+    if ((rc = sort_pairs(T, esrc, ssrc, edst, col, E, bits_for(V - 1), s, err))) return rc;
     hipLaunchKernelGGL(row_offsets_kernel, dim3(grid_for(E + 1)), dim3(256), 0, s, ssrc, E, V, row);
     ING_TRY(hipGetLastError());
+    // ---- 5. device order ----
+    uint32_t *perm = nullptr, *inv = nullptr, *drow = nullptr, *dcol = nullptr;
+    if (relabel) {
+        // the endpoint arrays (2m >= V entries) are free after the CSR sort
+        uint32_t *dkey = keys, *dval = vals, *skey = svals;
+        ING_TRY(B.get(&inv, V));
+        ING_TRY(B.get(&perm, V));
+        ING_TRY(B.get(&drow, (uint64_t)V + 1));
+        ING_TRY(B.get(&dcol, E));
+        hipLaunchKernelGGL(degree_keys_kernel, dim3(grid_for(V)), dim3(256), 0, s, row, V, dkey, dval);
+        ING_TRY(hipGetLastError());
+        if ((rc = sort_pairs(T, dkey, skey, dval, inv, V, 32, s, err))) return rc;
+        uint32_t *ddeg = dkey;
+        hipLaunchKernelGGL(permutation_kernel, dim3(grid_for(V)), dim3(256), 0, s, inv, row, V, perm, ddeg);
+        ING_TRY(hipGetLastError());
+        if ((rc = exclusive_sum(T, ddeg, drow, V, s, err))) return rc;
+        hipLaunchKernelGGL(set_word_kernel, dim3(1), dim3(1), 0, s, drow + V, (uint32_t)E);
+        ING_TRY(hipGetLastError());
+        hipLaunchKernelGGL(device_col_kernel, dim3(grid_for(E)), dim3(256), 0, s, ssrc, col, row, perm, drow, E,
+                           dcol);
+        ING_TRY(hipGetLastError());
+    }
+    // ---- the host mirror ----
     g.V = V;
     g.row_ptr.resize((size_t)V + 1);
     g.col.resize(E);
@@ -555,7 +650,24 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
     ING_TRY(hipMemcpyAsync(g.row_ptr.data(), row, ((uint64_t)V + 1) * 4, hipMemcpyDeviceToHost, s));
     ING_TRY(hipMemcpyAsync(g.col.data(), col, E * 4, hipMemcpyDeviceToHost, s));
     ING_TRY(hipMemcpyAsync(g.names.data(), names, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
+    built = DeviceBuiltGraph{};
+    if (relabel) {
+        built.h_perm.resize(V);
+        built.h_inv.resize(V);
+        built.h_drow.resize((size_t)V + 1);
+        ING_TRY(hipMemcpyAsync(built.h_perm.data(), perm, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
+        ING_TRY(hipMemcpyAsync(built.h_inv.data(), inv, (uint64_t)V * 4, hipMemcpyDeviceToHost, s));
+        ING_TRY(hipMemcpyAsync(built.h_drow.data(), drow, ((uint64_t)V + 1) * 4, hipMemcpyDeviceToHost, s));
+    }
     ING_TRY(hipStreamSynchronize(s));
+    // the device graph leaves with the caller
+    built.relabelled = relabel;
+    built.row_ptr = relabel ? drow : row;
+    built.col = relabel ? dcol : col;
+    built.perm = perm;
+    built.inv = inv;
+    for (void *q : {(void *)built.row_ptr, (void *)built.col, (void *)perm, (void *)inv})
+        if (q) B.release(q);
     return PTW_OK;
 }
 

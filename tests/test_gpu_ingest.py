@@ -118,3 +118,74 @@ def test_comment_only_file_is_rejected(ctx, tmp_path):
     p.write_text("# nothing\n\n")
     with pytest.raises(ptw.PtwError):
         ctx.load(str(p), ptw.FORMAT_TEXT, flags=ptw.GRAPH_INGEST_DEVICE)
+
+
+# ---- device primitives and the device-side degree relabel -------------------
+# sizes around one 4096-item tile, a few thousand tiles (two scan levels) and
+# 2^24 + 5 items (three: 4097 tiles -> 2 -> 1); digit widths that leave a
+# partial last pass
+@pytest.mark.parametrize("n,bits", [(0, 32), (1, 1), (4095, 8), (4096, 32), (4097, 13), (100_003, 7),
+                                    (3_000_001, 17), (16_777_221, 32), (16_777_221, 20)])
+def test_primitives_match_host(ctx, n, bits):
+    ctx.check_primitives(n, bits, seed=n + bits)
+
+
+def walk_outputs(c, V, seed=3):
+    info = c.info()
+    outs = []
+    for app in (ptw.PPR(0.2, int(info.max_degree_vertex)), ptw.PageRank(0.15)):
+        counts = np.zeros(V, dtype=np.uint64)
+        c.walk(ptw.make_desc(app, 200_000, seed=seed, dangling=ptw.DANGLING_STOP), counts=counts)
+        outs.append(counts)
+    W, L = 5000, 12
+    paths = np.zeros(W * L, dtype=np.uint32)
+    c.walk(ptw.make_desc(ptw.DeepWalk(L), W, seed=seed, dangling=ptw.DANGLING_STOP), paths=paths)
+    outs.append(paths)
+    return outs
+
+
+@pytest.mark.parametrize("fmt,undirected,no_relabel", [(ptw.FORMAT_TEXT, True, False), (ptw.FORMAT_BINARY, False, False),
+                                                       (ptw.FORMAT_TEXT, False, True)])
+def test_device_built_graph_walks_like_host_built(ctx, tmp_path, fmt, undirected, no_relabel):
+    """The device CSR (relabelled on the device, or in graph ids) walks exactly
+    like the one the host relabels and uploads: same counts, same paths."""
+    rng = np.random.default_rng(11)
+    n = 300_000
+    # skewed sparse ids out of order, so degrees spread and the relabel moves vertices
+    a = (rng.zipf(1.6, n) * 7919) % 4_000_000_000
+    b = (rng.zipf(1.6, n) * 104729 + 13) % 4_000_000_000
+    p = tmp_path / ("g.txt" if fmt == ptw.FORMAT_TEXT else "g.bin")
+    if fmt == ptw.FORMAT_TEXT:
+        p.write_text("\n".join(f"{x} {y}" for x, y in zip(a.tolist(), b.tolist())) + "\n")
+    else:
+        p.write_bytes(np.stack([a, b], 1).astype(np.uint32).tobytes())
+    flags = ptw.GRAPH_NO_RELABEL if no_relabel else 0
+    ctx.load(str(p), fmt, undirected, flags=flags | ptw.GRAPH_INGEST_DEVICE)
+    other = ptw.Context(0)
+    try:
+        other.load(str(p), fmt, undirected, flags=flags | ptw.GRAPH_INGEST_HOST)
+        i1, i2 = ctx.info(), other.info()
+        assert (i1.num_vertices, i1.num_edges, i1.max_degree_vertex, i1.relabelled) == \
+               (i2.num_vertices, i2.num_edges, i2.max_degree_vertex, i2.relabelled)
+        assert i1.relabelled == (0 if no_relabel else 1)
+        r1, c1 = ctx.export_csr()
+        r2, c2 = other.export_csr()
+        assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+        V = i1.num_vertices
+        for x, y in zip(walk_outputs(ctx, V), walk_outputs(other, V)):
+            assert np.array_equal(x, y)
+    finally:
+        other.close()
+
+
+def test_device_relabel_more_vertices_than_edges(ctx, host_ctx, tmp_path):
+    """A directed star read from its leaves: V = E + 1, every leaf but the
+    centre dangling, so the degree sort covers more vertices than edges."""
+    p = tmp_path / "star.txt"
+    p.write_text("".join(f"{i + 1} 0\n" for i in range(50_000)))
+    rd, cd = both(ctx, host_ctx, str(p), ptw.FORMAT_TEXT, False)
+    assert len(rd) - 1 == len(cd) + 1
+    counts = np.zeros(len(rd) - 1, dtype=np.uint64)
+    ctx.walk(ptw.make_desc(ptw.PageRank(0.3), 100_000, seed=2, dangling=ptw.DANGLING_STOP), counts=counts)
+    exp = oracle.walk_replay(rd, cd, oracle.APP_PAGERANK, 100_000, tp=0.3, seed=2)
+    assert np.array_equal(counts, exp["counts"])
