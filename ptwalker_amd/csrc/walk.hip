@@ -18,6 +18,14 @@
 // below `hubs` after the degree-descending relabel) in LDS u16 counters, two
 // per LDS word, flushed once per workgroup; other visits go to an LDS visit
 // cache or to device-scope u64 atomics.
+//
+// The endpoint app stages ends in LDS: each wavefront keeps the ends of its
+// last kEndSlots slices of walker ids (at most kEndSlice ids each) and
+// writes a slice out with whole-line stores once all of its walkers have
+// ended, where writing each end when it happens costs a partial-line write
+// per walker.  A slice whose slot is needed again while some of its walkers
+// still run (very long walks) writes the ends it has and lets the rest write
+// their own (a slot generation tells them).
 #include <algorithm>
 #include <mutex>
 
@@ -272,6 +280,16 @@ __device__ __forceinline__ void count_in_lds(const LdsCounters &L, uint32_t x, u
     atomicAdd(&counts[x], 1ull);
 }
 
+// Endpoint staging: slots per wavefront, walker ids per slot, and the LDS
+// words of one wavefront (the slots, then 5 words per slot: base lo, base
+// hi, length, ended, generation).
+constexpr uint32_t kEndSlots = 4;
+constexpr uint32_t kEndSlice = 256;
+constexpr uint32_t kEndMeta = 5;
+constexpr uint32_t kEndWaveWords = kEndSlots * kEndSlice + 32;
+static_assert(kEndSlots * kEndMeta <= 32, "slot words fit the wave's metadata area");
+static_assert((kEndSlots & (kEndSlots - 1)) == 0, "slot ring indexed by a mask");
+
 template <int KIND>
 __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk, uint32_t lds_hubs,
                                                    uint32_t cache_slots)
@@ -289,6 +307,16 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         __syncthreads();
     }
     const uint32_t lane = __lane_id();
+    // endpoint staging (chunk <= kEndSlice for this kind)
+    uint32_t *stage = nullptr, *meta = nullptr;
+    uint32_t slice_no = 0, cur_slot = 0;  // wave-uniform
+    uint64_t cur_base = 0;                // wave-uniform: first id of the current slice
+    uint32_t my_slot = 0, my_gen = 0, my_off = 0;
+    if (KIND == KIND_END) {
+        stage = lds_words + (threadIdx.x >> 6) * kEndWaveWords;
+        meta = stage + kEndSlots * kEndSlice;
+        if (lane < kEndSlots * kEndMeta) meta[lane] = 0;  // every slot free (ended == length == 0)
+    }
     uint64_t q_next = 0, q_end = 0;  // wave-uniform slice of local walker indices
     bool exhausted = false;
     bool active = false;
@@ -355,6 +383,34 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                 }
                 q_next = base;
                 q_end = base + chunk < p.walkers ? base + chunk : p.walkers;
+                if (KIND == KIND_END) {
+                    const uint32_t s = slice_no & (kEndSlots - 1);
+                    uint32_t *m = meta + s * kEndMeta;
+                    uint32_t *buf = stage + s * kEndSlice;
+                    const uint32_t old_len = m[2], old_done = m[3];
+                    if (old_done != old_len) {
+                        // the slot's older slice still has walkers running: its
+                        // ended walkers go out now, the running ones write their own
+                        const uint64_t ob = (uint64_t)m[0] | ((uint64_t)m[1] << 32);
+                        for (uint32_t i = lane; i < old_len; i += 64) {
+                            const uint32_t e = buf[i];
+                            if (e != PTW_INVALID_VERTEX) p.ends[ob + i] = e;
+                        }
+                    }
+                    const uint32_t len = (uint32_t)(q_end - q_next);
+                    for (uint32_t i = lane; i < len; i += 64) buf[i] = PTW_INVALID_VERTEX;
+                    const uint32_t gen = m[4] + 1;
+                    if (lane == 0) {
+                        m[0] = (uint32_t)q_next;
+                        m[1] = (uint32_t)(q_next >> 32);
+                        m[2] = len;
+                        m[3] = 0;
+                        m[4] = gen;
+                    }
+                    cur_slot = s;
+                    cur_base = q_next;
+                    ++slice_no;
+                }
             }
             const uint32_t n_idle = (uint32_t)__popcll(idle);
             const uint64_t avail = q_end - q_next;
@@ -362,6 +418,11 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             const uint32_t rank = lane_rank(idle);
             if (!active && rank < take) {
                 w = q_next + rank;
+                if (KIND == KIND_END) {
+                    my_slot = cur_slot;
+                    my_gen = meta[cur_slot * kEndMeta + 4];
+                    my_off = (uint32_t)(w - cur_base);
+                }
                 const uint64_t gw = p.wid_base + w;
                 wlo = (uint32_t)gw;
                 whi = (uint32_t)(gw >> 32);
@@ -394,6 +455,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         if (__ballot(active) == 0) break;
 
         // ---- one step of every active walker ----
+        bool staged_end = false;  // endpoint app: this lane's walker ended into its slot
         if (active) {
             const u32x4 r = philox4x32_10(wlo, whi, step, p.stream, p.seed_lo, p.seed_hi);
             bool stop;
@@ -433,7 +495,15 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             if (stop) {
                 my_steps += step;
                 my_dangling += dangled;
-                if (KIND == KIND_END) p.ends[w] = p.inv ? p.inv[v] : v;
+                if (KIND == KIND_END) {
+                    const uint32_t e = p.inv ? p.inv[v] : v;
+                    if (meta[my_slot * kEndMeta + 4] == my_gen) {
+                        stage[my_slot * kEndSlice + my_off] = e;
+                        staged_end = true;
+                    } else {
+                        p.ends[w] = e;
+                    }
+                }
                 if (KIND == KIND_PATH || KIND == KIND_N2V) {
                     path_flush_tail();
                     if (dangled) {
@@ -442,6 +512,22 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     }
                 }
                 active = false;
+            }
+        }
+        if (KIND == KIND_END && __ballot(staged_end) != 0) {
+            // count the step's ends per slot; a slot whose walkers have all
+            // ended goes out with whole-line stores and is free again
+            for (uint32_t s = 0; s < kEndSlots; ++s) {
+                const uint32_t c = (uint32_t)__popcll(__ballot(staged_end && my_slot == s));
+                if (c == 0) continue;
+                uint32_t *m = meta + s * kEndMeta;
+                const uint32_t done = m[3] + c, len = m[2];
+                if (lane == 0) m[3] = done;
+                if (done == len) {
+                    const uint64_t b = (uint64_t)m[0] | ((uint64_t)m[1] << 32);
+                    const uint32_t *buf = stage + s * kEndSlice;
+                    for (uint32_t i = lane; i < len; i += 64) p.ends[b + i] = buf[i];
+                }
             }
         }
     }
@@ -627,6 +713,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
         slots = shape.cache_slots > 0 ? (uint32_t)shape.cache_slots : 0u;
     }
     size_t lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
+    if (kind == KIND_END) lds = (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t);
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
@@ -645,7 +732,8 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
             per_cu = 1;
             hubs = hubs < 8192u ? hubs : 8192u;
             slots = p.start_mode == PTW_START_SOURCE ? (slots < 1024u ? slots : 1024u) : 0u;
-            lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
+            lds = kind == KIND_END ? (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t)
+                                   : (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
         }
     }
     const int blocks = per_cu * num_cus;
@@ -654,7 +742,8 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     const uint64_t waves = (uint64_t)blocks * (uint64_t)(block / 64);
     uint64_t fair = p.walkers / (waves * 4);
     fair = (fair + 63) / 64 * 64;
-    const uint32_t chunk = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(fair, (uint64_t)shape.chunk));
+    uint32_t chunk = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(fair, (uint64_t)shape.chunk));
+    if (kind == KIND_END && chunk > kEndSlice) chunk = kEndSlice;  // a slice fits its staging slot
     if (eff) {
         eff->block = block;
         eff->blocks = blocks;
@@ -665,7 +754,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     if (kind == KIND_COUNT)
         hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);
     else if (kind == KIND_END)
-        hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
+        hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), lds, s, p, chunk, 0u, 0u);
     else if (kind == KIND_N2V)
         hipLaunchKernelGGL(walk_kernel<KIND_N2V>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
     else

@@ -163,6 +163,31 @@ def test_counter_paths_explicit_shapes(ctx, hubs, slots):
         ctx.set_tuning(**default)
 
 
+# walker counts large enough that the slice length is the chunk asked for
+# (capped at 256) rather than the small-walk share
+@pytest.mark.parametrize("chunk,block,tp,W", [(64, 64, 0.02, 300_000), (64, 1024, 0.2, 300_000),
+                                              (128, 256, 0.05, 600_000), (256, 256, 0.2, 1_200_000),
+                                              (4096, 256, 0.01, 1_200_000)])
+def test_endpoint_staging_shapes(ctx, chunk, block, tp, W):
+    """RWR ends go through each wavefront's LDS slice slots (at most 256 ids
+    each, written out when a slice's walkers have all ended).  Short slices
+    with long walks (tp 0.02 at 64 ids) make many walkers outlive their slot,
+    so the slot's ended walkers are written early and the rest write their
+    own: ends and starts must still equal the replay."""
+    gi = ctx.rmat(scale=14, edge_factor=16, seed=2)
+    rp, col = ctx.export_csr()
+    default = ctx.tuning().as_dict()
+    try:
+        ctx.set_tuning(chunk=chunk, block=block, blocks_per_cu=1)
+        for app, n in ((ptw.RWR(tp, int(gi.max_degree_vertex)), W), (ptw.RWR(tp), W + 1)):
+            got = check(ctx, rp, col, app, n, seed=7)
+            assert 64 <= got.shape["chunk"] <= 256
+            if (chunk, block) in ((256, 256), (4096, 256)):
+                assert got.shape["chunk"] == 256
+    finally:
+        ctx.set_tuning(**default)
+
+
 def star(n):
     """Vertex 0 joined to leaves 1..n, both directions."""
     rp = np.zeros(n + 2, dtype=np.uint32)
