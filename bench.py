@@ -22,9 +22,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# the serving lanes need hardware queues of their own (HIP's default is 4);
-# set before HIP starts
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# the serving lanes need hardware queues of their own (HIP's default is 4,
+# and the variable may already be set to that); set before HIP starts
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 
@@ -275,16 +276,25 @@ def bench_gather_sweep(args):
 
 # (u16 LDS hub counters, visit-cache slots); the cache's slots and the hub
 # counters share the LDS, so the larger hub counts leave room for fewer slots
-SHAPES = [(8192, 4096), (16384, 2048), (36864, 1024), (55296, 1024), (73728, 1024), (77824, 512), (79872, 0),
-          (77824, -1)]
+# (lds_hubs, cache_slots, block, visit_log); blocks_per_cu stays 0, so the
+# occupancy query sets the workgroups per CU from each shape's LDS
+SHAPES = [(8192, 4096, 1024, 0), (36864, 1024, 1024, 0), (77824, 512, 1024, 0), (79872, 0, 1024, 0),
+          (77824, -1, 1024, 0),
+          (79872, 0, 1024, 1), (77824, -1, 1024, 1), (55296, 0, 1024, 1), (36864, 0, 1024, 1),
+          (36864, 0, 512, 1), (24576, 0, 512, 1), (16384, 0, 512, 1), (16384, 0, 256, 1), (8192, 0, 256, 1),
+          (0, 0, 256, 1), (0, 0, 1024, 1)]
+SHAPE_KEYS = ("lds_hubs", "cache_slots", "block", "visit_log")
+# (block, blocks_per_cu) of the endpoint kernel; 0: the occupancy query
+RWR_SHAPES = [(1024, 0), (1024, 1), (512, 0), (512, 2), (512, 3), (256, 0), (256, 4)]
 
 
 def bench_shape_sweep(args):
     """Walk-kernel time over launch shapes on the headline graph: PPR from the
     max-degree and from the uniform source (1e8 walkers, tp 0.2) and PageRank
     (1e8 walkers, tp 0.15); median kernel ms of 3 runs per shape, after one
-    warm-up.  Counts are compared across shapes (a shape changes where counts
-    accumulate, never what is counted)."""
+    warm-up.  A shape is LDS hub counters, visit-cache slots, workgroup size
+    and the visit log on or off.  Counts are compared across shapes (a shape
+    changes where counts accumulate, never what is counted)."""
     import hashlib
     import statistics
 
@@ -302,9 +312,10 @@ def bench_shape_sweep(args):
             "pagerank": ptw.PageRank(0.15)}
     counts = np.zeros(gi.num_vertices, dtype=np.uint64)
     rows, digests = [], {}
-    for hubs, slots in SHAPES:
-        ctx.set_tuning(lds_hubs=hubs, cache_slots=slots)
-        row = {"lds_hubs": hubs, "cache_slots": slots}
+    for shape_row in SHAPES:
+        setting = dict(zip(SHAPE_KEYS, shape_row))
+        ctx.set_tuning(**setting)
+        row = dict(setting)
         for name, app in apps.items():
             ms = []
             for i in range(4):
@@ -313,20 +324,42 @@ def bench_shape_sweep(args):
                     ms.append(o.kernel_ms)
             shape = o.launch_shape()
             row[name] = statistics.median(ms)
-            row[name + "_shape"] = {k: shape[k] for k in ("lds_hubs", "cache_slots", "chunk")}
+            row[name + "_shape"] = {k: shape[k] for k in ("lds_hubs", "cache_slots", "chunk", "block", "blocks",
+                                                          "visit_log")}
             h = hashlib.sha1(counts.tobytes()).hexdigest()
             row[name + "_identical"] = digests.setdefault(name, h) == h
         rows.append(row)
-        print(f"# shapes {hubs}/{slots}: " + " ".join(f"{n} {row[n]:.2f} ms" for n in apps), file=sys.stderr,
-              flush=True)
-    ctx.set_tuning(**{k: getattr(default, k) for k in ("lds_hubs", "cache_slots")})
+        print("# shapes " + "/".join(str(x) for x in shape_row) + ": " +
+              " ".join(f"{n} {row[n]:.2f} ms" for n in apps), file=sys.stderr, flush=True)
+    # the endpoint kernel has no counters: its shape is the workgroup size and
+    # how many workgroups a CU holds
+    ends = np.zeros(W, dtype=np.uint32)
+    rwr_apps = {"rwr_max_degree": ptw.RWR(args.tp, hub), "rwr_uniform": ptw.RWR(args.tp, uni)}
+    rwr_rows = []
+    for block, per_cu in RWR_SHAPES:
+        ctx.set_tuning(block=block, blocks_per_cu=per_cu)
+        row = {"block": block, "blocks_per_cu": per_cu}
+        for name, app in rwr_apps.items():
+            ms = []
+            for i in range(4):
+                o = ctx.walk(ptw.make_desc(app, W, seed=1), ends=ends)
+                if i:
+                    ms.append(o.kernel_ms)
+            row[name] = statistics.median(ms)
+            row[name + "_shape"] = {k: v for k, v in o.launch_shape().items() if k in ("block", "blocks", "chunk")}
+            h = hashlib.sha1(ends.tobytes()).hexdigest()
+            row[name + "_identical"] = digests.setdefault(name, h) == h
+        rwr_rows.append(row)
+        print(f"# rwr shapes {block}/{per_cu}: " + " ".join(f"{n} {row[n]:.2f} ms" for n in rwr_apps),
+              file=sys.stderr, flush=True)
+    ctx.set_tuning(**{k: getattr(default, k) for k in SHAPE_KEYS + ("blocks_per_cu",)})
     line = {"metric": "walk kernel ms", "value": min(r["ppr_max_degree"] for r in rows), "unit": "ms", "n_gpus": 1,
             "steps": 3, "warmup": 1, "higher_is_better": False, "data": "synthetic",
             "config": {"workload": f"PPR (max-degree, uniform source; tp {args.tp}) and PageRank (tp 0.15), "
                                    f"{W:.0e} walkers, R-MAT scale {args.scale} ef16",
-                       "vertices": gi.num_vertices, "edges": gi.num_edges, "default_lds_hubs": default.lds_hubs,
-                       "default_cache_slots": default.cache_slots},
-            "shapes": rows}
+                       "vertices": gi.num_vertices, "edges": gi.num_edges,
+                       "default": {k: getattr(default, k) for k in SHAPE_KEYS}},
+            "shapes": rows, "rwr_shapes": rwr_rows}
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -147,7 +147,8 @@ typedef struct ptw_walk_out {
     uint64_t steps;    /* sum of transitions over walkers */
     uint64_t visits;   /* count increments (count apps) */
     uint64_t dangling; /* walkers stopped at a degree-0 vertex */
-    double kernel_ms;  /* device time of the walk kernel (HIP events) */
+    double kernel_ms;  /* device time of the walk kernel and, for count apps, of the
+                          visit log's counting passes (HIP events) */
     double total_ms;   /* wall time of the call */
     /* the launch the walk ran with, after the automatic rules of
      * ptw_tuning (blocks_per_cu = 0, cache_slots = PTW_CACHE_AUTO) */
@@ -156,7 +157,8 @@ typedef struct ptw_walk_out {
     uint32_t launch_chunk;       /* walker ids per queue take */
     uint32_t launch_lds_hubs;    /* u16 LDS hub counters (count apps) */
     uint32_t launch_cache_slots; /* LDS visit-cache slots (count apps) */
-    uint32_t launch_reserved;
+    uint32_t launch_visit_log;   /* visit-log entries written (count apps; 0: the walk
+                                    counted with device atomics only) */
 } ptw_walk_out;
 
 /* ---- context ---------------------------------------------------------- */
@@ -467,6 +469,14 @@ typedef struct ptw_tuning {
     int32_t topk_tile;     /* LDS candidate slots of the top-k kernel (power of two, 256..4096) */
     int32_t dense_chunk;   /* walker ids per workgroup chunk of a dense batched query, a multiple
                               of 1024 up to 2^20; 0 = from the batch's dense walkers */
+    int32_t visit_log;     /* count apps: 1 (default) = visits that no LDS counter takes are
+                              appended to a device log and counted by id bucket after the
+                              walk (two u32 buffers of 1.1x the expected visits, kept by the
+                              context, at most 2^30 entries each; walks under 8 walkers per
+                              resident lane skip it); N > 1, a multiple of 1024 = the same
+                              with at most N entries; 0 = one device atomic per such visit.
+                              Visits beyond a full log take device atomics.  Counts are the
+                              same either way */
 } ptw_tuning;
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
 int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);

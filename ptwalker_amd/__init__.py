@@ -129,13 +129,14 @@ class WalkOut(ctypes.Structure):
         ("launch_chunk", ctypes.c_uint32),
         ("launch_lds_hubs", ctypes.c_uint32),
         ("launch_cache_slots", ctypes.c_uint32),
-        ("launch_reserved", ctypes.c_uint32),
+        ("launch_visit_log", ctypes.c_uint32),
     ]
 
     def launch_shape(self) -> dict:
         """The launch the walk ran with (after the automatic rules)."""
         return {"block": self.launch_block, "blocks": self.launch_blocks, "chunk": self.launch_chunk,
-                "lds_hubs": self.launch_lds_hubs, "cache_slots": self.launch_cache_slots}
+                "lds_hubs": self.launch_lds_hubs, "cache_slots": self.launch_cache_slots,
+                "visit_log": self.launch_visit_log}
 
 
 CACHE_AUTO = -1
@@ -208,6 +209,7 @@ class Tuning(ctypes.Structure):
         ("batch_chunk", ctypes.c_int32),
         ("topk_tile", ctypes.c_int32),
         ("dense_chunk", ctypes.c_int32),
+        ("visit_log", ctypes.c_int32),
     ]
 
     def as_dict(self):

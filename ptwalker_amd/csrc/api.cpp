@@ -23,9 +23,51 @@ using ptw::fail;
 using ptw::HostCsr;
 using ptw::upload;
 
+void free_vlog(ptw_ctx *ctx)
+{
+    ptw::VlogBuffers &b = ctx->vlog;
+    if (b.log) (void)hipFree(b.log);
+    if (b.parted) (void)hipFree(b.parted);
+    if (b.plan) (void)hipFree(b.plan);
+    b = ptw::VlogBuffers{};
+}
+
+// The visit log for a count walk of `walkers` walkers at termination
+// probability tp: room for 1.1x the expected visits plus one segment per
+// resident wavefront (each may leave one partly used), at most
+// kVlogMaxEntries and at most `limit` entries (a multiple of kVlogSeg; 0:
+// no limit).  The buffers grow and are kept.  Returns the entries the walk
+// may use, or 0 (device atomics) if the graph needs too many buckets or the
+// memory is not there.
+uint64_t ensure_vlog(ptw_ctx *ctx, uint64_t walkers, double tp, uint64_t limit = 0)
+{
+    const uint32_t V = ctx->dev.V;
+    const uint32_t shift = ptw::vlog_shift_for(V);
+    if (!shift) return 0;
+    const uint32_t nb = ptw::vlog_buckets_for(V, shift);
+    const double visits = 1.1 * (double)walkers * (1.0 + 1.0 / tp) + 64.0 * ctx->num_cus * ptw::kVlogSeg;
+    uint64_t cap = visits >= (double)ptw::kVlogMaxEntries ? ptw::kVlogMaxEntries : (uint64_t)visits;
+    if (limit && cap > limit) cap = limit;
+    cap = (cap + ptw::kVlogSeg - 1) / ptw::kVlogSeg * ptw::kVlogSeg;
+    ptw::VlogBuffers &b = ctx->vlog;
+    if (b.cap >= cap && b.plan_buckets >= nb) return cap;
+    const uint64_t alloc = std::max(cap, b.cap);
+    free_vlog(ctx);
+    if (hipMalloc((void **)&b.log, alloc * 4) != hipSuccess || hipMalloc((void **)&b.parted, alloc * 4) != hipSuccess ||
+        hipMalloc((void **)&b.plan, ptw::vlog_plan_words(nb) * 4) != hipSuccess) {
+        (void)hipGetLastError();  // the failed allocation is not a walk error
+        free_vlog(ctx);
+        return 0;
+    }
+    b.cap = alloc;
+    b.plan_buckets = nb;
+    return cap;
+}
+
 void free_graph(ptw_ctx *ctx)
 {
     ptw::free_batch(ctx);
+    free_vlog(ctx);
     auto &d = ctx->dev;
     if (d.row_ptr) (void)hipFree(d.row_ptr);
     if (d.col) (void)hipFree(d.col);
@@ -110,6 +152,14 @@ int warm_up(ptw_ctx *ctx)
         p.app = app;
         HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
         HIP_TRY(ctx, ptw::launch_walk(p, ctx->shape, ctx->num_cus, s));
+    }
+    // the visit log's passes over an empty log
+    if (const uint64_t cap = ensure_vlog(ctx, 0, 1.0)) {
+        const uint32_t shift = ptw::vlog_shift_for(ctx->dev.V);
+        const uint32_t nb = ptw::vlog_buckets_for(ctx->dev.V, shift);
+        HIP_TRY(ctx, ptw::launch_vlog_begin(ctx->vlog, nb, s));
+        HIP_TRY(ctx, ptw::launch_vlog_count(ctx->vlog, ctx->scratch + 3, cap, shift, nb, ctx->counts_dev,
+                                            ctx->num_cus, s));
     }
     HIP_TRY(ctx, ptw::launch_counts_to_graph_ids(ctx->counts_dev, ctx->dev.perm, std::min<uint32_t>(ctx->dev.V, 64),
                                                  ctx->counts_stage, 0, PTW_INVALID_VERTEX, 0, s));
@@ -366,8 +416,9 @@ int make_walk_launch(ptw_ctx *ctx, const ptw_walk_desc &d, ptw::WalkLaunch &p)
 // source.  Uniform starts spread their visits, and a source with more
 // neighbours than four times the cache spreads its first-step visits too thin
 // (about two per workgroup per neighbour at 1e8 walkers from a 1.6e5-degree
-// hub), so there the cache's LDS goes to hub counters (4 per slot).  An
-// explicit cache_slots is used as given.
+// hub), so there the cache's LDS goes to hub counters (4 per slot).  A walk
+// that counts through the visit log gives the cache up too (launch_walk).
+// An explicit cache_slots is used as given.
 ptw::LaunchShape walk_shape(const ptw_ctx *ctx, const ptw_walk_desc &d)
 {
     ptw::LaunchShape shape = ctx->shape;
@@ -380,6 +431,7 @@ ptw::LaunchShape walk_shape(const ptw_ctx *ctx, const ptw_walk_desc &d)
             shape.cache_slots = 0;
         } else {
             shape.cache_slots = ptw::kAutoCacheSlots;
+            shape.cache_auto = true;
         }
     }
     return shape;
@@ -390,13 +442,14 @@ void clear_walk_out(ptw_walk_out *out)
     out->steps = out->visits = out->dangling = 0;
     out->kernel_ms = out->total_ms = 0.0;
     out->launch_block = out->launch_blocks = out->launch_chunk = 0;
-    out->launch_lds_hubs = out->launch_cache_slots = out->launch_reserved = 0;
+    out->launch_lds_hubs = out->launch_cache_slots = 0;
+    out->launch_visit_log = 0;
 }
 
 // Totals from the scratch words (copied to host: [1] steps, [2] dangling),
 // the kernel time and the launch shape, then the dangling-walker verdict.
 int finish_walk_out(ptw_ctx *ctx, const ptw_walk_desc &d, const uint64_t *sc, const ptw::EffectiveShape &eff,
-                    double t0, ptw_walk_out *out)
+                    double t0, ptw_walk_out *out, uint64_t vlog_cap = 0)
 {
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -410,7 +463,7 @@ int finish_walk_out(ptw_ctx *ctx, const ptw_walk_desc &d, const uint64_t *sc, co
     out->launch_chunk = eff.chunk;
     out->launch_lds_hubs = eff.hubs;
     out->launch_cache_slots = eff.slots;
-    out->launch_reserved = 0;
+    out->launch_visit_log = eff.vlog ? (uint32_t)std::min<uint64_t>(sc[3], vlog_cap) : 0;
     if (sc[2] && d.dangling_policy == PTW_DANGLING_ERROR)
         return fail(ctx, PTW_ERR_DANGLING,
                     std::to_string(sc[2]) + " walkers reached a vertex without out-edges with steps left");
@@ -711,10 +764,26 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
 
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
     const ptw::LaunchShape shape = walk_shape(ctx, d);
+    const uint64_t log_cap =
+        count_app && W && shape.visit_log
+            ? ensure_vlog(ctx, W, d.tp, shape.visit_log > 1 ? (uint64_t)shape.visit_log : 0)
+            : 0;
+    if (log_cap) {
+        p.vlog = ctx->vlog.log;
+        p.vlog_cap = log_cap;
+        p.vlog_hist = ctx->vlog.hist();
+        p.vlog_shift = ptw::vlog_shift_for(V);
+        p.vlog_buckets = ptw::vlog_buckets_for(V, p.vlog_shift);
+        HIP_TRY(ctx, ptw::launch_vlog_begin(ctx->vlog, p.vlog_buckets, s));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
     ptw::EffectiveShape eff;
     if (W) HIP_TRY(ctx, ptw::launch_walk(p, shape, ctx->num_cus, s, &eff));
+    // the log's counting belongs to the walk's device time
+    if (eff.vlog)
+        HIP_TRY(ctx, ptw::launch_vlog_count(ctx->vlog, ctx->scratch + 3, log_cap, p.vlog_shift, p.vlog_buckets,
+                                            ctx->counts_dev, ctx->num_cus, s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     if (count_app) {
         const bool from_source = d.start_mode == PTW_START_SOURCE;
@@ -733,7 +802,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
     if (!on_dev && count_app) std::memcpy(out->counts, ctx->pinned + 8, (size_t)V * 8);
-    return finish_walk_out(ctx, d, ctx->pinned, eff, t0, out);
+    return finish_walk_out(ctx, d, ctx->pinned, eff, t0, out, log_cap);
 }
 
 int ptw_walk_static(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
@@ -802,6 +871,7 @@ int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)
     t->batch_chunk = ctx->shape.batch_chunk;
     t->topk_tile = ctx->shape.topk_tile;
     t->dense_chunk = ctx->shape.dense_chunk;
+    t->visit_log = ctx->shape.visit_log;
     return PTW_OK;
 }
 
@@ -827,7 +897,10 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
         return fail(ctx, PTW_ERR_INVALID_ARG, "topk_tile must be a power of two in [256, 4096]");
     if (t->dense_chunk < 0 || t->dense_chunk > (1 << 20) || t->dense_chunk % 1024)
         return fail(ctx, PTW_ERR_INVALID_ARG, "dense_chunk must be 0 or a multiple of 1024 up to 2^20");
+    if (t->visit_log < 0 || (t->visit_log > 1 && t->visit_log % (int32_t)ptw::kVlogSeg))
+        return fail(ctx, PTW_ERR_INVALID_ARG, "visit_log must be 0, 1 or a multiple of 1024 (entries)");
     ptw::drop_batch_graph(ctx);
+    ctx->shape.visit_log = t->visit_log;
     ctx->shape.dense_chunk = t->dense_chunk;
     ctx->shape.batch_chunk = t->batch_chunk;
     ctx->shape.topk_tile = t->topk_tile;

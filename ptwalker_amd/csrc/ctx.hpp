@@ -96,6 +96,7 @@ struct ptw_ctx {
     // (looked up at ptw_create with the device current)
     unsigned long long *static_scratch = nullptr;
     ptw::LaunchShape shape;
+    ptw::VlogBuffers vlog;  // visit log of count walks, grown on demand
     ptw::BatchState batch;
     ptw::ServeLane serve[ptw::kServeLanes];  // created by the first ptw_serve_replay
 };

@@ -121,9 +121,37 @@ struct WalkLaunch {
     uint32_t *ends;                  // graph ids
     uint32_t *starts;                // graph ids (end apps; nullptr: not written)
     uint32_t *paths;                 // graph ids
-    // scratch (device): [0] queue head, [1] steps, [2] dangling
+    // visit log (count apps; nullptr: device atomics): vlog_cap entries (a
+    // multiple of kVlogSeg), claimed in segments from scratch[3]; every
+    // appended entry is counted in vlog_hist[id >> vlog_shift]
+    uint32_t *vlog;
+    uint64_t vlog_cap;
+    uint32_t *vlog_hist;
+    uint32_t vlog_shift;
+    uint32_t vlog_buckets;
+    // scratch (device): [0] queue head, [1] steps, [2] dangling, [3] log head
     unsigned long long *scratch;
 };
+
+// Visit log: entries a wavefront claims at once, the filler of unused
+// entries, and the id buckets its partition uses (ids >> shift, buckets of
+// 2^13..2^15 ids, at most kVlogMaxBuckets).
+constexpr uint32_t kVlogSeg = 1024;
+constexpr uint32_t kNoVisit = 0xFFFFFFFFu;
+constexpr uint32_t kVlogMaxBuckets = 4096;
+constexpr uint32_t kVlogMinShift = 13, kVlogMaxShift = 15;
+// Bucket shift for V vertices: about 512 buckets, within the shift range;
+// 0 if the graph needs more than kVlogMaxBuckets buckets.
+inline uint32_t vlog_shift_for(uint32_t V)
+{
+    uint32_t lg = 0;
+    while (lg < 32 && (1ull << lg) < (uint64_t)V) ++lg;
+    uint32_t sh = lg > 9 ? lg - 9 : 0;
+    sh = sh < kVlogMinShift ? kVlogMinShift : sh > kVlogMaxShift ? kVlogMaxShift : sh;
+    const uint64_t nb = V ? (((uint64_t)V - 1) >> sh) + 1 : 1;
+    return nb <= kVlogMaxBuckets ? sh : 0;
+}
+inline uint32_t vlog_buckets_for(uint32_t V, uint32_t shift) { return V ? ((V - 1) >> shift) + 1 : 1; }
 
 struct LaunchShape {
     int block = 1024;        // threads per workgroup
@@ -134,12 +162,15 @@ struct LaunchShape {
     int batch_chunk = 16384;  // walker ids per chunk of a batched launch
     int topk_tile = 2048;    // top-k LDS candidate slots
     int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)
+    int visit_log = 1;       // count apps: log the visits LDS does not count (0: device atomics)
+    bool cache_auto = false; // cache_slots came from the automatic setting (a walk's shape only)
 };
 
 // The launch a walk actually ran with, after the automatic rules.
 struct EffectiveShape {
     int block = 0, blocks = 0;
     uint32_t chunk = 0, hubs = 0, slots = 0;
+    bool vlog = false;  // the visit log was used (the caller counts it)
 };
 
 // Visit-cache slots of the automatic setting (cache_slots = -1) where a cache
@@ -182,6 +213,32 @@ struct DeviceBuiltGraph {
 // the degree relabel of relabel_by_degree when `relabel`).
 int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_undirected, bool relabel,
                           HostCsr &g, DeviceBuiltGraph &built, std::string &err);
+// ---- visit log (vlog.hip) ---------------------------------------------------
+// Device buffers of the visit log of one walk: the log, its partition by id
+// bucket, and the partition's plan.
+struct VlogBuffers {
+    uint32_t *log = nullptr;     // cap entries
+    uint32_t *parted = nullptr;  // cap entries
+    uint64_t cap = 0;
+    // hist[NB] (zeroed before the walk), start[NB + 1], cursor[NB],
+    // task_first[NB + 1], then the segment length word
+    uint32_t *plan = nullptr;
+    uint32_t plan_buckets = 0;  // NB the plan area was sized for
+    uint32_t *hist() const { return plan; }
+};
+inline uint64_t vlog_plan_words(uint32_t nb) { return 4ull * nb + 4; }
+// Largest log a walk allocates (entries per buffer); a walk whose log fills
+// up counts its later visits with device atomics.
+constexpr uint64_t kVlogMaxEntries = 1ull << 30;
+// Zeroes the bucket counts the walk kernel adds to.
+hipError_t launch_vlog_begin(const VlogBuffers &b, uint32_t nb, hipStream_t s);
+// After the walk kernel: partitions the log's first min(*head, cap) entries
+// (head: the scratch word the walk claimed segments from; cap: the capacity
+// the walk was given, <= b.cap) by bucket, then adds each bucket's counts
+// into counts_dev (device ids).
+hipError_t launch_vlog_count(const VlogBuffers &b, const unsigned long long *head, uint64_t cap, uint32_t shift,
+                             uint32_t nb, unsigned long long *counts_dev, int num_cus, hipStream_t s);
+
 hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, const uint32_t *perm,
                                       uint32_t V, uint64_t *out, bool accumulate,
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);

@@ -17,7 +17,13 @@
 // Count apps privatize the `lds_hubs` highest-degree vertices (device ids
 // below `hubs` after the degree-descending relabel) in LDS u16 counters, two
 // per LDS word, flushed once per workgroup; other visits go to an LDS visit
-// cache or to device-scope u64 atomics.
+// cache, then to the visit log or to device-scope u64 atomics.  The log
+// takes the visits of a step from all lanes of a wavefront at once (ballot +
+// mbcnt) and appends them as consecutive u32 stores into the wavefront's
+// current segment of kVlogSeg entries, so a visit costs 4 B of a
+// line-sized write instead of one 32 B memory-side atomic; the visit log
+// kernels (vlog.hip) count the log after the walk.  Every logged visit is
+// also counted in its id bucket in LDS, which plans the log's partition.
 //
 // The endpoint app stages ends in LDS: each wavefront keeps the ends of its
 // last kEndSlots slices of walker ids (at most kEndSlice ids each) and
@@ -258,11 +264,13 @@ __device__ __forceinline__ void lds_add16(uint32_t *word, uint32_t shift, unsign
     }
 }
 
-__device__ __forceinline__ void count_in_lds(const LdsCounters &L, uint32_t x, unsigned long long *counts)
+// Counts a visit in LDS if it has a counter or a cache slot there; false
+// if it must be counted on the device.
+__device__ __forceinline__ bool count_in_lds(const LdsCounters &L, uint32_t x, unsigned long long *counts)
 {
     if (x < L.hubs) {
         lds_add16(&L.direct[x >> 1], (x & 1u) << 4, &counts[x]);
-        return;
+        return true;
     }
     if (L.mask != 0xFFFFFFFFu) {
         uint32_t h = (x * 0x9E3779B1u) & L.mask;
@@ -272,12 +280,12 @@ __device__ __forceinline__ void count_in_lds(const LdsCounters &L, uint32_t x, u
             if (k == kEmptyKey) k = atomicCAS(&L.keys[h], kEmptyKey, x);
             if (k == kEmptyKey || k == x) {
                 lds_add1(&L.vals[h], &counts[x]);
-                return;
+                return true;
             }
             h = (h + 1) & L.mask;
         }
     }
-    atomicAdd(&counts[x], 1ull);
+    return false;
 }
 
 // Endpoint staging: slots per wavefront, walker ids per slot, and the LDS
@@ -298,12 +306,17 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     const uint32_t hub_words = (lds_hubs + 1) >> 1;
     LdsCounters L{lds_words, lds_words + hub_words, lds_words + hub_words + cache_slots, lds_hubs,
                   cache_slots ? cache_slots - 1 : 0xFFFFFFFFu};
+    // visit log: per-bucket entries this workgroup appended
+    const bool logging = KIND == KIND_COUNT && p.vlog != nullptr;
+    uint32_t *bucket_lds = lds_words + hub_words + 2 * cache_slots;
     if (KIND == KIND_COUNT) {
         for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) L.direct[i] = 0;
         for (uint32_t i = threadIdx.x; i < cache_slots; i += blockDim.x) {
             L.keys[i] = kEmptyKey;
             L.vals[i] = 0;
         }
+        if (logging)
+            for (uint32_t i = threadIdx.x; i < p.vlog_buckets; i += blockDim.x) bucket_lds[i] = 0;
         __syncthreads();
     }
     const uint32_t lane = __lane_id();
@@ -331,11 +344,60 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     const uint32_t hot = p.start_mode == PTW_START_SOURCE ? p.source_dev : PTW_INVALID_VERTEX;
     uint64_t my_hot = 0;
 
+    // A visit that LDS does not count waits in `pend` for the wavefront's next
+    // log append (at most one per lane between appends).
+    uint32_t pend = kNoVisit;
     auto count_visit = [&](uint32_t x) {
-        if (x == hot)
+        if (x == hot) {
             ++my_hot;
-        else
-            count_in_lds(L, x, p.counts_dev);
+        } else if (!count_in_lds(L, x, p.counts_dev)) {
+            if (logging)
+                pend = x;
+            else
+                atomicAdd(&p.counts_dev[x], 1ull);
+        }
+    };
+    // The wavefront's log segment [seg_pos, seg_end) (wave-uniform).  A
+    // segment is claimed whole from the log head; once a claim finds the log
+    // full, the wavefront's later visits go to device atomics.
+    uint64_t seg_pos = 0, seg_end = 0;
+    bool log_full = false;
+    // Called by all lanes of the wavefront together.
+    auto log_append = [&]() {
+        if (!logging) return;
+        const uint64_t m = __ballot(pend != kNoVisit);
+        if (m == 0) return;
+        const uint32_t n = (uint32_t)__popcll(m);
+        const uint32_t r = lane_rank(m);
+        const uint64_t room = seg_end - seg_pos;
+        uint64_t slot = ~0ull;
+        if (r < room) slot = seg_pos + r;
+        if (n <= room) {
+            seg_pos += n;
+        } else {
+            seg_pos = seg_end;
+            if (!log_full) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&p.scratch[3], (unsigned long long)kVlogSeg);
+                base = wave_broadcast_first(base);
+                if (base + kVlogSeg <= p.vlog_cap) {
+                    if (r >= room) slot = base + (r - room);
+                    seg_pos = base + (n - room);
+                    seg_end = base + kVlogSeg;
+                } else {
+                    log_full = true;
+                }
+            }
+        }
+        if (pend != kNoVisit) {
+            if (slot != ~0ull) {
+                p.vlog[slot] = pend;
+                atomicAdd(&bucket_lds[pend >> p.vlog_shift], 1u);
+            } else {
+                atomicAdd(&p.counts_dev[pend], 1ull);
+            }
+        }
+        pend = kNoVisit;
     };
 
     // Path rows are written through a 4-entry register queue: every entry that
@@ -449,6 +511,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                 step = 0;
                 active = true;
             }
+            if (KIND == KIND_COUNT) log_append();
             q_next += take;
             idle = __ballot(!active);
         }
@@ -514,6 +577,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                 active = false;
             }
         }
+        if (KIND == KIND_COUNT) log_append();
         if (KIND == KIND_END && __ballot(staged_end) != 0) {
             // count the step's ends per slot; a slot whose walkers have all
             // ended goes out with whole-line stores and is free again
@@ -532,7 +596,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         }
     }
 
-    // ---- per-wave totals, then the LDS hub flush ----
+    // ---- the log segment's unused tail, per-wave totals, then the LDS flush ----
+    if (logging)
+        for (uint64_t i = seg_pos + lane; i < seg_end; i += 64) p.vlog[i] = kNoVisit;
     uint64_t hot64 = my_hot;
     for (int off = 32; off > 0; off >>= 1) {
         my_steps += __shfl_xor(my_steps, off);
@@ -555,6 +621,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             const uint32_t k = L.keys[i], c = L.vals[i];
             if (k != kEmptyKey && c) atomicAdd(&p.counts_dev[k], (unsigned long long)c);
         }
+        if (logging)
+            for (uint32_t i = threadIdx.x; i < p.vlog_buckets; i += blockDim.x)
+                if (bucket_lds[i]) atomicAdd(&p.vlog_hist[i], bucket_lds[i]);
     }
 }
 
@@ -700,8 +769,9 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s, EffectiveShape *eff)
+hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num_cus, hipStream_t s, EffectiveShape *eff)
 {
+    WalkLaunch p = p_in;
     const int kind = (p.app == PTW_APP_PAGERANK || p.app == PTW_APP_PPR) ? KIND_COUNT
                      : p.app == PTW_APP_RWR                            ? KIND_END
                      : p.app == PTW_APP_NODE2VEC                       ? KIND_N2V
@@ -711,9 +781,13 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
     if (kind == KIND_COUNT) {
         hubs = (uint32_t)shape.lds_hubs < p.hubs ? (uint32_t)shape.lds_hubs : p.hubs;
         slots = shape.cache_slots > 0 ? (uint32_t)shape.cache_slots : 0u;
+    } else {
+        p.vlog = nullptr;
     }
-    size_t lds = (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
-    if (kind == KIND_END) lds = (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t);
+    auto count_lds = [&]() {
+        return (((size_t)hubs + 1) / 2 + 2ull * slots + (p.vlog ? p.vlog_buckets : 0u)) * sizeof(uint32_t);
+    };
+    size_t lds = kind == KIND_END ? (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t) : count_lds();
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
@@ -732,10 +806,31 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
             per_cu = 1;
             hubs = hubs < 8192u ? hubs : 8192u;
             slots = p.start_mode == PTW_START_SOURCE ? (slots < 1024u ? slots : 1024u) : 0u;
-            lds = kind == KIND_END ? (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t)
-                                   : (((size_t)hubs + 1) / 2 + 2ull * slots) * sizeof(uint32_t);
+            p.vlog = nullptr;  // a few visits per vertex: the atomics are cheaper than the log's passes
         }
     }
+    if (kind == KIND_COUNT && p.vlog) {
+        // The log takes a source's hot neighbourhood as cheaply as the visit
+        // cache did (it only had to keep those visits off hot device
+        // counters), so an automatic cache gives its LDS to hub counters.
+        if (shape.cache_auto && slots) {
+            const uint32_t more = hubs + 4 * slots < (uint32_t)kMaxLdsHubs ? hubs + 4 * slots : (uint32_t)kMaxLdsHubs;
+            hubs = more < p.hubs ? more : p.hubs;
+            slots = 0;
+        }
+        // The log's bucket counters share the workgroup's LDS (160 KiB) with
+        // the hub counters and the cache; the hub range gives way.
+        constexpr uint32_t kLdsWords = 160u * 1024u / 4u;
+        if (hubs >= p.V || p.vlog_buckets == 0 || p.vlog_buckets > kVlogMaxBuckets ||
+            2ull * slots + p.vlog_buckets >= kLdsWords) {
+            p.vlog = nullptr;  // every visit has an LDS counter, or no room
+        } else {
+            const uint32_t max_hub_words = kLdsWords - 2 * slots - p.vlog_buckets;
+            if ((hubs + 1) / 2 > max_hub_words) hubs = 2 * max_hub_words;
+        }
+    }
+    if (kind == KIND_COUNT) lds = count_lds();
+    else if (kind == KIND_END) lds = (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t);
     const int blocks = per_cu * num_cus;
     // Walker ids per queue take: the configured chunk, cut down for small
     // walks so that every resident wavefront gets about four takes.
@@ -750,6 +845,7 @@ hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cu
         eff->chunk = chunk;
         eff->hubs = hubs;
         eff->slots = slots;
+        eff->vlog = p.vlog != nullptr;
     }
     if (kind == KIND_COUNT)
         hipLaunchKernelGGL(walk_kernel<KIND_COUNT>, dim3(blocks), dim3(block), lds, s, p, chunk, hubs, slots);

@@ -3,11 +3,13 @@ import sys
 
 import pytest
 
-# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default).  The
-# GPU session holds several contexts, each with up to three streams, and the
-# serving lanes of one context must not share a hardware queue or their
-# batches run one after the other; this takes effect before HIP starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default, and
+# the variable may already be set to that).  The GPU session holds several
+# contexts, each with up to three streams, and the serving lanes of one
+# context must not share a hardware queue or their batches run one after the
+# other; this takes effect before HIP starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:

@@ -211,30 +211,69 @@ def test_hub_counter_spill(ctx):
 def test_automatic_shape_rmat17(ctx):
     """The default (automatic) launch shape on a graph with more vertices than
     any LDS hub range (R-MAT scale 17: 90,202 vertices), so device ids above the
-    hub counters reach the visit cache or device atomics.  A source of degree
-    above 2048 and uniform starts give the cache's LDS to hub counters (79,872,
-    no cache); a degree-1 source keeps 77,824 hubs and 512 cache slots.  3e6
-    walkers keep the launch off the small-walk shape.  Counts equal the replay."""
+    hub counters reach the visit cache, the visit log or device atomics.  With
+    the log (the default) every walk gives the cache's LDS to hub counters
+    (79,872, no cache).  Without it, a source of degree above 2048 and uniform
+    starts do the same, and a degree-1 source keeps 77,824 hubs and 512 cache
+    slots.  3e6 walkers keep the launch off the small-walk shape.  Counts equal
+    the replay."""
     gi = ctx.rmat(scale=17, edge_factor=16, seed=1)
     assert gi.num_vertices > 81408
     rp, col = ctx.export_csr()
     deg = np.diff(rp.astype(np.int64))
     low = int(np.argmin(deg))
     assert ctx.tuning().cache_slots == ptw.CACHE_AUTO and ctx.tuning().blocks_per_cu == 0
-    got = check(ctx, rp, col, ptw.PPR(0.2, int(gi.max_degree_vertex)), 3_000_000, seed=7)
-    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (79872, 0), got.shape
-    got = check(ctx, rp, col, ptw.PPR(0.2, low), 3_000_000, seed=8)
-    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (77824, 512), got.shape
-    got = check(ctx, rp, col, ptw.PageRank(0.15), 3_000_000, seed=9)
-    assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (79872, 0), got.shape
-    # an explicit setting is used as given
+    assert ctx.tuning().visit_log == 1
     default = ctx.tuning().as_dict()
+    cases = [(ptw.PPR(0.2, int(gi.max_degree_vertex)), 7, (79872, 0), (79872, 0)),
+             (ptw.PPR(0.2, low), 8, (79872, 0), (77824, 512)),
+             (ptw.PageRank(0.15), 9, (79872, 0), (79872, 0))]
     try:
-        ctx.set_tuning(cache_slots=1024, lds_hubs=36864)
+        for log in (1, 0):
+            ctx.set_tuning(visit_log=log)
+            for app, seed, with_log, without in cases:
+                got = check(ctx, rp, col, app, 3_000_000, seed=seed)
+                want = with_log if log else without
+                assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == want, (log, got.shape)
+                assert (got.shape["visit_log"] > 0) == bool(log), got.shape
+        # an explicit setting is used as given
+        ctx.set_tuning(cache_slots=1024, lds_hubs=36864, visit_log=1)
         got = check(ctx, rp, col, ptw.PPR(0.2, int(gi.max_degree_vertex)), 3_000_000, seed=10)
         assert (got.shape["lds_hubs"], got.shape["cache_slots"]) == (36864, 1024), got.shape
     finally:
         ctx.set_tuning(**default)
+
+
+def test_visit_log_paths(ctx):
+    """Visits that no LDS counter takes go to the visit log and are counted by
+    id bucket after the walk; with visit_log = 0 they take device atomics, and
+    a log smaller than the walk's visits (64 Ki entries here) fills up and
+    sends the rest to device atomics.  R-MAT scale 18 (more vertices than the
+    LDS hub range), 4e6 walkers: counts equal the replay on every path."""
+    gi = ctx.rmat(scale=18, edge_factor=16, seed=3)
+    assert gi.num_vertices > 81408
+    rp, col = ctx.export_csr()
+    deg = np.diff(rp.astype(np.int64))
+    low = int(np.argmin(deg))
+    cases = [ptw.PPR(0.2, int(gi.max_degree_vertex)), ptw.PPR(0.15, low), ptw.PageRank(0.15)]
+    default = ctx.tuning().as_dict()
+    try:
+        for setting in (1, 0, 65536):
+            ctx.set_tuning(visit_log=setting)
+            for i, app in enumerate(cases):
+                print("visit_log", setting, app, flush=True)
+                got = check(ctx, rp, col, app, 4_000_000, seed=11 + i)
+                logged = got.shape["visit_log"]
+                if setting == 0:
+                    assert logged == 0
+                elif setting == 65536:
+                    assert logged == 65536, got.shape
+                else:
+                    assert 0 < logged < got.visits, got.shape
+    finally:
+        ctx.set_tuning(**default)
+    with pytest.raises(ptw.PtwError):
+        ctx.set_tuning(visit_log=1000)
 
 
 def test_device_outputs_and_accumulate(ctx):
@@ -466,3 +505,4 @@ def test_config_scale22_1e8_walkers_full_replay(ctx22, app_name):
         assert abs(got.steps / 1e8 - 5.0) < 0.01  # E[L] = 1/tp
         if app_name == "PPR":
             assert int(got.counts.sum()) == 100_000_000 + got.steps
+            assert got.shape["visit_log"] > 0, got.shape  # the non-hub visits went through the log
