@@ -68,7 +68,21 @@ def parse():
     ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--walks-per-vertex", type=int, default=10)
     ap.add_argument("--walk-len", type=int, default=80)
+    ap.add_argument("--tuning", default="",
+                    help="launch tuning for the ppr-rwr and deepwalk workloads, as key=value pairs separated by "
+                         "commas (ptw_tuning fields, e.g. block=256,blocks_per_cu=0)")
     return ap.parse_args()
+
+
+def apply_tuning(ctx, spec):
+    """Sets the ptw_tuning fields named in `spec` ("key=value,..."); returns them."""
+    kw = {}
+    for item in filter(None, (x.strip() for x in spec.split(","))):
+        k, v = item.split("=")
+        kw[k.strip()] = int(v)
+    if kw:
+        ctx.set_tuning(**kw)
+    return kw
 
 
 def main():
@@ -100,6 +114,7 @@ def main():
 
     W = int(args.walkers)
     ctx = ptw.Context(0)
+    tuning = apply_tuning(ctx, args.tuning)
     t0 = time.time()
     gi = ctx.rmat(scale=args.scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1)
     build_s = time.time() - t0
@@ -196,6 +211,7 @@ def main():
             "tp": args.tp,
             "walk_steps_per_bench_step": total_steps // args.steps,
             "graph_build_s": round(build_s, 2),
+            **({"tuning": tuning} if tuning else {}),
         },
         "roofline": {
             "bound": "hbm",
@@ -374,6 +390,7 @@ def bench_deepwalk(args):
 
     scale = args.scale if args.scale != 22 else 24
     ctx = ptw.Context(0)
+    tuning = apply_tuning(ctx, args.tuning)
     t0 = time.time()
     gi = ctx.rmat(scale=scale, edge_factor=16, seed=1)
     ctx.random_weights(seed=1)
@@ -421,7 +438,7 @@ def bench_deepwalk(args):
         "config": {"workload": f"DeepWalk corpus, R-MAT scale {scale} ef16, {args.walks_per_vertex} walks per vertex "
                                f"x {L} vertices, uniform + alias-weighted, paths resident in HBM",
                    "vertices": V, "edges": gi.num_edges, "walkers": W, "path_bytes": 4 * W * L,
-                   "graph_and_alias_build_s": round(build_s, 1)},
+                   "graph_and_alias_build_s": round(build_s, 1), **({"tuning": tuning} if tuning else {})},
         "roofline": {"bound": "hbm", "kernel": "walk_kernel<path> (unweighted)",
                      "achieved": 16.0 * steps / (uw_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": 16.0 * steps / (uw_ms * 1e-3) / 1e9 / 8000.0, "traffic": None, "bytes_per_step": 16,
@@ -489,6 +506,7 @@ def bench_batch(args):
 
     scale = args.scale if args.scale != 22 else 24
     ctx = ptw.Context(0)
+    tuning = apply_tuning(ctx, args.tuning)
     t0 = time.time()
     gi = ctx.rmat(scale=scale, edge_factor=16, seed=1)
     build_s = time.time() - t0

@@ -456,10 +456,11 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
 
 typedef struct ptw_tuning {
     int32_t block;         /* threads per workgroup: multiple of 64, 64..1024 */
-    int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query, and a walk
-                              under 8 walkers per resident lane then runs one workgroup of
-                              at most 512 threads per CU with at most 8192 hub counters and
-                              at most 1024 cache slots (none for uniform starts) */
+    int32_t blocks_per_cu; /* resident workgroups per CU; 0 = occupancy query, RWR walks then
+                              use workgroups of at most 512 threads, and a walk under 8
+                              walkers per resident lane runs one workgroup of at most 512
+                              threads per CU with at most 8192 hub counters and at most 1024
+                              cache slots (none for uniform starts) */
     int32_t chunk;         /* walker ids a wavefront takes from the queue at once (>= 64) */
     int32_t lds_hubs;      /* u16 LDS counters for the top-degree vertices, 0..81408 */
     int32_t cache_slots;   /* LDS visit-cache slots (power of two, <= 16384; 0 = off;
@@ -477,6 +478,13 @@ typedef struct ptw_tuning {
                               with at most N entries; 0 = one device atomic per such visit.
                               Visits beyond a full log take device atomics.  Counts are the
                               same either way */
+    int32_t edge_records;  /* neighbour records, built by the first walk that reads them and
+                              kept with the graph: per CSR slot the neighbour's id, row start,
+                              degree and graph id (16 B per edge; weighted path walks: both
+                              alias candidates, 32 B per edge), so a step is one dependent
+                              read.  1 (default) = every walk but node2vec (weighted count
+                              and endpoint walks keep the weighted-edge records); -1 =
+                              DEEPWALK walks only; 0 = never.  Results are the same */
 } ptw_tuning;
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
 int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);

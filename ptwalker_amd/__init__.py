@@ -210,6 +210,7 @@ class Tuning(ctypes.Structure):
         ("topk_tile", ctypes.c_int32),
         ("dense_chunk", ctypes.c_int32),
         ("visit_log", ctypes.c_int32),
+        ("edge_records", ctypes.c_int32),
     ]
 
     def as_dict(self):

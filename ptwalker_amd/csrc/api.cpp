@@ -23,6 +23,9 @@ using ptw::fail;
 using ptw::HostCsr;
 using ptw::upload;
 
+bool is_count_app(int app) { return app == PTW_APP_PAGERANK || app == PTW_APP_PPR; }
+bool is_path_app(int app) { return app == PTW_APP_DEEPWALK || app == PTW_APP_NODE2VEC; }
+
 void free_vlog(ptw_ctx *ctx)
 {
     ptw::VlogBuffers &b = ctx->vlog;
@@ -64,10 +67,72 @@ uint64_t ensure_vlog(ptw_ctx *ctx, uint64_t walkers, double tp, uint64_t limit =
     return cap;
 }
 
+// Neighbour records for walks (built on first use, kept with the graph):
+// unweighted {neighbour, row start, degree, graph id} per slot, or the
+// weighted pair per slot for path walks.  False (the walk reads the CSR) if
+// the memory is not there.
+bool ensure_edge_records(ptw_ctx *ctx, bool weighted)
+{
+    ptw::DeviceGraph &d = ctx->dev;
+    if (weighted ? d.wnrec != nullptr : d.nrec != nullptr) return true;
+    if (weighted && !d.wedge) return false;
+    const uint64_t n = (weighted ? 2 : 1) * std::max<uint64_t>(d.E, 1);
+    uint4 *rec = nullptr;
+    if (hipMalloc((void **)&rec, n * sizeof(uint4)) != hipSuccess) {
+        (void)hipGetLastError();  // a missing table is not a walk error
+        return false;
+    }
+    const hipError_t e = weighted ? ptw::launch_weighted_records(d.row_ptr, d.wedge, d.inv, d.E, rec, ctx->stream)
+                                  : ptw::launch_edge_records(d.row_ptr, d.col, d.inv, d.E, rec, ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(rec);
+        return false;
+    }
+    (weighted ? d.wnrec : d.nrec) = rec;
+    d.bytes += n * sizeof(uint4);
+    return true;
+}
+
+void free_edge_records(ptw_ctx *ctx, bool weighted_only)
+{
+    ptw::DeviceGraph &d = ctx->dev;
+    if (d.wnrec) {
+        (void)hipFree(d.wnrec);
+        d.bytes -= 32ull * std::max<uint64_t>(d.E, 1);
+        d.wnrec = nullptr;
+    }
+    if (!weighted_only && d.nrec) {
+        (void)hipFree(d.nrec);
+        d.bytes -= 16ull * std::max<uint64_t>(d.E, 1);
+        d.nrec = nullptr;
+    }
+}
+
+// Which records a walk reads (setting: -1 path walks, 0 none, 1 every walk
+// but node2vec).  Weighted records serve path walks only; other weighted
+// walks read the weighted-edge records.
+void pick_edge_records(ptw_ctx *ctx, const ptw_walk_desc &d, bool may_build, ptw::WalkLaunch &p)
+{
+    const int setting = ctx->shape.edge_records;
+    if (setting == 0 || d.app == PTW_APP_NODE2VEC) return;
+    if (setting < 0 && !is_path_app(d.app)) return;
+    const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
+    if (weighted && !is_path_app(d.app)) return;
+    if (!weighted ? ctx->dev.nrec == nullptr : ctx->dev.wnrec == nullptr) {
+        if (!may_build || !ensure_edge_records(ctx, weighted)) return;
+    }
+    if (weighted)
+        p.wnrec = ctx->dev.wnrec;
+    else
+        p.nrec = ctx->dev.nrec;
+}
+
 void free_graph(ptw_ctx *ctx)
 {
     ptw::free_batch(ctx);
     free_vlog(ctx);
+    free_edge_records(ctx, false);
     auto &d = ctx->dev;
     if (d.row_ptr) (void)hipFree(d.row_ptr);
     if (d.col) (void)hipFree(d.col);
@@ -118,6 +183,7 @@ int install_weights(ptw_ctx *ctx)
         }
     });
     ptw::drop_batch_graph(ctx);
+    free_edge_records(ctx, true);  // the weighted records follow the weights
     if (d.wedge) {
         (void)hipFree(d.wedge);
         d.bytes -= 16ull * h.E();
@@ -302,9 +368,6 @@ double now_ms()
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
-
-bool is_count_app(int app) { return app == PTW_APP_PAGERANK || app == PTW_APP_PPR; }
-bool is_path_app(int app) { return app == PTW_APP_DEEPWALK || app == PTW_APP_NODE2VEC; }
 
 // Descriptor and output checks shared by ptw_walk and ptw_walk_static.
 int check_walk(ptw_ctx *ctx, const ptw_walk_desc &d, const ptw_walk_out &out)
@@ -756,6 +819,7 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
 
     ptw::WalkLaunch p;
     if (int rc = make_walk_launch(ctx, d, p)) return rc;
+    pick_edge_records(ctx, d, true, p);
     p.counts_dev = ctx->counts_dev;
     p.ends = ends_dev;
     p.starts = starts_dev;
@@ -834,6 +898,7 @@ int ptw_walk_static(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
 
     ptw::WalkLaunch p;
     if (int rc = make_walk_launch(ctx, d, p)) return rc;
+    pick_edge_records(ctx, d, false, p);  // only records built by earlier walks: no allocation here
     p.counts_dev = counters;
     p.ends = d.app == PTW_APP_RWR ? out->ends : nullptr;
     p.starts = d.app == PTW_APP_RWR ? out->starts : nullptr;
@@ -872,6 +937,7 @@ int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t)
     t->topk_tile = ctx->shape.topk_tile;
     t->dense_chunk = ctx->shape.dense_chunk;
     t->visit_log = ctx->shape.visit_log;
+    t->edge_records = ctx->shape.edge_records;
     return PTW_OK;
 }
 
@@ -899,8 +965,11 @@ int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t)
         return fail(ctx, PTW_ERR_INVALID_ARG, "dense_chunk must be 0 or a multiple of 1024 up to 2^20");
     if (t->visit_log < 0 || (t->visit_log > 1 && t->visit_log % (int32_t)ptw::kVlogSeg))
         return fail(ctx, PTW_ERR_INVALID_ARG, "visit_log must be 0, 1 or a multiple of 1024 (entries)");
+    if (t->edge_records < -1 || t->edge_records > 1)
+        return fail(ctx, PTW_ERR_INVALID_ARG, "edge_records must be -1, 0 or 1");
     ptw::drop_batch_graph(ctx);
     ctx->shape.visit_log = t->visit_log;
+    ctx->shape.edge_records = t->edge_records;
     ctx->shape.dense_chunk = t->dense_chunk;
     ctx->shape.batch_chunk = t->batch_chunk;
     ctx->shape.topk_tile = t->topk_tile;

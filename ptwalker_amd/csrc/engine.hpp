@@ -37,6 +37,12 @@ struct DeviceGraph {
     // the alias slot's neighbour, 0}.  A weighted pick of slot j reads one
     // 16 B record instead of the alias entry and then the neighbour.
     uint4 *wedge = nullptr;
+    // neighbour records, built on first use: E x {neighbour, its row start,
+    // its degree, its graph id}, and 2E weighted ones for path walks (both
+    // alias candidates of a slot: row start, degree, graph id; the
+    // threshold in the first)
+    uint4 *nrec = nullptr;
+    uint4 *wnrec = nullptr;
     // transpose (in-edges, cut into pieces) and out-degrees for the FP64
     // solver, built on first use
     uint32_t *t_col = nullptr, *out_deg = nullptr, *piece_begin = nullptr, *vertex_piece = nullptr;
@@ -95,6 +101,10 @@ struct WalkLaunch {
     const uint32_t *perm;  // nullptr: identity
     const uint32_t *inv;   // nullptr: identity
     const uint4 *wedge;    // weighted-edge records; nullptr: uniform neighbour choice
+    // neighbour records (edge_records_kernel / weighted_records_kernel);
+    // nullptr: the walk reads row_ptr, col (or wedge) and inv per step
+    const uint4 *nrec;
+    const uint4 *wnrec;    // weighted (path walks only; used instead of wedge)
     uint32_t V;
     uint32_t hubs;         // LDS-privatized counters for device ids < hubs
     // process
@@ -164,6 +174,7 @@ struct LaunchShape {
     int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)
     int visit_log = 1;       // count apps: log the visits LDS does not count (0: device atomics)
     bool cache_auto = false; // cache_slots came from the automatic setting (a walk's shape only)
+    int edge_records = 1;    // neighbour records: 1 every walk but node2vec, -1 path walks only, 0 never
 };
 
 // The launch a walk actually ran with, after the automatic rules.
@@ -194,6 +205,10 @@ constexpr unsigned long long kEdgeSetEmpty = ~0ull;
 hipError_t launch_edge_set_build(const uint32_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
                                  unsigned long long *table, uint64_t mask, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_edge_records(const uint32_t *row_ptr, const uint32_t *col, const uint32_t *inv, uint64_t E,
+                               uint4 *rec, hipStream_t s);
+hipError_t launch_weighted_records(const uint32_t *row_ptr, const uint4 *wedge, const uint32_t *inv, uint64_t E,
+                                   uint4 *rec, hipStream_t s);
 // `shape.cache_slots` must be resolved (>= 0) by the caller; `eff`, if given,
 // receives the launch after the small-walk rule of blocks_per_cu = 0.
 hipError_t launch_walk(const WalkLaunch &p, const LaunchShape &shape, int num_cus, hipStream_t s,

@@ -206,6 +206,35 @@ __global__ void edge_set_build_kernel(const uint32_t *row_ptr, const uint32_t *c
     }
 }
 
+// Neighbour records, one per CSR slot e (device order): {neighbour u, u's
+// row start, u's degree, u's graph id}.
+__global__ void edge_records_kernel(const uint32_t *row_ptr, const uint32_t *col, const uint32_t *inv, uint64_t E,
+                                    uint4 *rec)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        const uint32_t u = col[e];
+        const uint32_t b = row_ptr[u];
+        rec[e] = uint4{u, b, row_ptr[u + 1] - b, inv ? inv[u] : u};
+    }
+}
+
+// Weighted neighbour records, two per CSR slot e from its weighted-edge
+// record {x, threshold, alias slot's neighbour z, 0}: rec[2e] = {x's row
+// start, x's degree, x's graph id, threshold}, rec[2e + 1] = the same for z
+// and 0.  Path walks only: no device ids.
+__global__ void weighted_records_kernel(const uint32_t *row_ptr, const uint4 *wedge, const uint32_t *inv,
+                                        uint64_t E, uint4 *rec)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        const uint4 w = wedge[e];
+        const uint32_t bx = row_ptr[w.x], bz = row_ptr[w.z];
+        rec[2 * e] = uint4{bx, row_ptr[w.x + 1] - bx, inv ? inv[w.x] : w.x, w.y};
+        rec[2 * e + 1] = uint4{bz, row_ptr[w.z + 1] - bz, inv ? inv[w.z] : w.z, 0u};
+    }
+}
+
 __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -337,6 +366,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     uint32_t wlo = 0, whi = 0;  // global walker id words
     uint32_t v = 0, step = 0;
     uint32_t prev = 0;  // node2vec: the vertex before v
+    // Neighbour records: a step reads its neighbour's record, which holds
+    // the neighbour's row (start, degree) and graph id for the next step, so
+    // a step is one dependent read instead of the row offsets, then the
+    // neighbour, then its graph id.
+    const bool use_rec = KIND != KIND_N2V && (p.nrec != nullptr || p.wnrec != nullptr);
+    uint32_t cur_beg = 0, cur_deg = 0, cur_g = 0;
     uint64_t my_steps = 0;
     uint32_t my_dangling = 0;
     // Single-source walks return to the source often: its visits are counted
@@ -503,10 +538,15 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     if (KIND == KIND_COUNT) count_visit(v);
                     if (KIND == KIND_END && p.starts) p.starts[w] = g;
                 }
+                if (use_rec) {
+                    cur_beg = p.row_ptr[v];
+                    cur_deg = p.row_ptr[v + 1] - cur_beg;
+                    cur_g = p.inv ? p.inv[v] : v;
+                }
                 if (KIND == KIND_PATH || KIND == KIND_N2V) {
                     pos = w * p.walk_len;
                     nbuf = 0;
-                    path_append(p.inv ? p.inv[v] : v);
+                    path_append(use_rec ? cur_g : p.inv ? p.inv[v] : v);
                 }
                 step = 0;
                 active = true;
@@ -530,8 +570,14 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             }
             bool dangled = false;
             if (!stop) {
-                const uint32_t beg = p.row_ptr[v];
-                const uint32_t deg = p.row_ptr[v + 1] - beg;
+                uint32_t beg, deg;
+                if (use_rec) {
+                    beg = cur_beg;
+                    deg = cur_deg;
+                } else {
+                    beg = p.row_ptr[v];
+                    deg = p.row_ptr[v + 1] - beg;
+                }
                 if (deg == 0) {
                     dangled = true;
                     stop = true;
@@ -541,7 +587,23 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                         nx = n2v_pick(p, wlo, whi, step, r, beg, deg, prev);
                     } else {
                         const uint32_t j = mulhi64(r.x, r.y, deg);
-                        if (p.wedge) {  // weighted: alias coin from the 4th word
+                        if (KIND != KIND_N2V && p.wnrec) {
+                            // weighted records of both alias candidates: the
+                            // coin picks one, which carries its row and name
+                            const uint4 a = p.wnrec[2 * ((uint64_t)beg + j)];
+                            const uint4 b = p.wnrec[2 * ((uint64_t)beg + j) + 1];
+                            const bool alias = r.w >= a.w;
+                            cur_beg = alias ? b.x : a.x;
+                            cur_deg = alias ? b.y : a.y;
+                            cur_g = alias ? b.z : a.z;
+                            nx = 0;  // path walks keep no device id on this path
+                        } else if (KIND != KIND_N2V && p.nrec) {
+                            const uint4 e = p.nrec[(uint64_t)beg + j];
+                            nx = e.x;
+                            cur_beg = e.y;
+                            cur_deg = e.z;
+                            cur_g = e.w;
+                        } else if (p.wedge) {  // weighted: alias coin from the 4th word
                             const uint4 e = p.wedge[beg + j];
                             nx = r.w >= e.y ? e.z : e.x;
                         } else {
@@ -552,14 +614,14 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     v = nx;
                     ++step;
                     if (KIND == KIND_COUNT) count_visit(v);
-                    if (KIND == KIND_PATH || KIND == KIND_N2V) path_append(p.inv ? p.inv[v] : v);
+                    if (KIND == KIND_PATH || KIND == KIND_N2V) path_append(use_rec ? cur_g : p.inv ? p.inv[v] : v);
                 }
             }
             if (stop) {
                 my_steps += step;
                 my_dangling += dangled;
                 if (KIND == KIND_END) {
-                    const uint32_t e = p.inv ? p.inv[v] : v;
+                    const uint32_t e = use_rec ? cur_g : p.inv ? p.inv[v] : v;
                     if (meta[my_slot * kEndMeta + 4] == my_gen) {
                         stage[my_slot * kEndSlice + my_off] = e;
                         staged_end = true;
@@ -760,6 +822,27 @@ hipError_t launch_edge_set_build(const uint32_t *row_ptr, const uint32_t *col, u
     return hipGetLastError();
 }
 
+hipError_t launch_edge_records(const uint32_t *row_ptr, const uint32_t *col, const uint32_t *inv, uint64_t E,
+                               uint4 *rec, hipStream_t s)
+{
+    if (E == 0) return hipSuccess;
+    uint64_t blocks = (E + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(edge_records_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, row_ptr, col, inv, E, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_weighted_records(const uint32_t *row_ptr, const uint4 *wedge, const uint32_t *inv, uint64_t E,
+                                   uint4 *rec, hipStream_t s)
+{
+    if (E == 0) return hipSuccess;
+    uint64_t blocks = (E + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(weighted_records_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, row_ptr, wedge, inv, E,
+                       rec);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
@@ -784,6 +867,9 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     } else {
         p.vlog = nullptr;
     }
+    if (kind == KIND_N2V) p.nrec = p.wnrec = nullptr;
+    if (kind != KIND_PATH) p.wnrec = nullptr;  // weighted records carry no device ids
+    if (p.wnrec) p.nrec = nullptr;
     auto count_lds = [&]() {
         return (((size_t)hubs + 1) / 2 + 2ull * slots + (p.vlog ? p.vlog_buckets : 0u)) * sizeof(uint32_t);
     };
@@ -792,6 +878,14 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
     if (per_cu <= 0) {
+        // The endpoint kernel's steps wait on one read each (neighbour
+        // records) and its LDS holds more workgroups than 1024-thread ones
+        // can use (SGPRs allow 28 waves per CU): 512-thread workgroups put
+        // 24 waves on a CU instead of 16.
+        if (kind == KIND_END && block > 512) {
+            block = 512;
+            lds = (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t);
+        }
         per_cu = kind == KIND_COUNT ? occupancy_blocks<KIND_COUNT>(block, lds)
                  : kind == KIND_END ? occupancy_blocks<KIND_END>(block, lds)
                  : kind == KIND_N2V ? occupancy_blocks<KIND_N2V>(block, lds)

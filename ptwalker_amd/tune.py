@@ -41,6 +41,7 @@ WALK_SPACE: Dict[str, List[int]] = {
     "lds_hubs": [16384, 36864, 55296, 77824],
     "cache_slots": [-1, 0, 512, 1024, 2048],  # -1: the library's per-walk choice
     "visit_log": [1, 0],
+    "edge_records": [1, 0],
 }
 
 BATCH_SPACE: Dict[str, List[int]] = {

@@ -276,6 +276,50 @@ def test_visit_log_paths(ctx):
         ctx.set_tuning(visit_log=1000)
 
 
+@pytest.mark.parametrize("relabel", [True, False])
+def test_edge_records_every_kind(ctx, relabel):
+    """Neighbour records (a slot's neighbour with its row and graph id) make a
+    step one dependent read; edge_records = 1 (default) puts every walk kind
+    but node2vec on them, 0 none, -1 DeepWalk only.  Weighted path walks read the pair of
+    alias candidates; new weights rebuild those.  Every result equals the
+    replay, with and without the degree relabel."""
+    gi = ctx.rmat(scale=14, edge_factor=16, seed=5, flags=0 if relabel else ptw.GRAPH_NO_RELABEL)
+    rp, col = ctx.export_csr()
+    hub = int(gi.max_degree_vertex)
+    low = int(np.argmin(np.diff(rp.astype(np.int64))))
+    apps = [(ptw.PPR(0.15, hub), 600_000), (ptw.PPR(0.2, low), 300_000), (ptw.PageRank(0.15), 300_000),
+            (ptw.RWR(0.2, hub), 300_000), (ptw.RWR(0.3), 300_000), (ptw.DeepWalk(20), 20_000),
+            (ptw.DeepWalk(9, start=ptw.START_ROUNDROBIN), 5_000)]
+    default = ctx.tuning().as_dict()
+    try:
+        for setting in (1, 0, -1):
+            ctx.set_tuning(edge_records=setting)
+            for i, (app, W) in enumerate(apps):
+                print("edge_records", setting, app, flush=True)
+                check(ctx, rp, col, app, W, seed=20 + i)
+        ctx.set_tuning(edge_records=1)
+        for wseed in (5, 6):
+            ctx.random_weights(seed=wseed)
+            w, _, _ = ctx.export_weights()
+            alias = oracle.build_alias(rp, w)
+            for app, W in [(ptw.DeepWalk(16), 5000), (ptw.PPR(0.2, hub), 200_000)]:
+                d = ptw.make_desc(app, W, seed=wseed, flags=ptw.FLAG_WEIGHTED)
+                paths = np.zeros(W * d.walk_len, dtype=np.uint32) if d.app == ptw.APP_DEEPWALK else None
+                counts = np.zeros(gi.num_vertices, dtype=np.uint64) if paths is None else None
+                out = ctx.walk(d, counts=counts, paths=paths)
+                ref = oracle.walk_replay(rp, col, d.app, W, tp=d.tp, seed=wseed, source=d.source,
+                                         start_mode=d.start_mode, walk_len=d.walk_len, alias=alias)
+                assert out.steps == ref["steps"]
+                if paths is not None:
+                    assert np.array_equal(paths.reshape(W, d.walk_len), ref["paths"])
+                else:
+                    assert np.array_equal(counts, ref["counts"])
+    finally:
+        ctx.set_tuning(**default)
+    with pytest.raises(ptw.PtwError):
+        ctx.set_tuning(edge_records=2)
+
+
 def test_device_outputs_and_accumulate(ctx):
     gi = ctx.rmat(scale=12, edge_factor=8, seed=3)
     rp, col = ctx.export_csr()
