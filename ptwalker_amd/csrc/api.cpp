@@ -67,11 +67,14 @@ uint64_t ensure_vlog(ptw_ctx *ctx, uint64_t walkers, double tp, uint64_t limit =
     return cap;
 }
 
+}  // namespace
+
 // Neighbour records for walks (built on first use, kept with the graph):
 // unweighted {neighbour, row start, degree, graph id} per slot, or the
 // weighted pair per slot for path walks.  False (the walk reads the CSR) if
-// the memory is not there.
-bool ensure_edge_records(ptw_ctx *ctx, bool weighted)
+// the memory is not there.  The build is waited for, so walks and batches on
+// other streams can read the records at once.
+bool ptw::ensure_edge_records(ptw_ctx *ctx, bool weighted)
 {
     ptw::DeviceGraph &d = ctx->dev;
     if (weighted ? d.wnrec != nullptr : d.nrec != nullptr) return true;
@@ -89,10 +92,17 @@ bool ensure_edge_records(ptw_ctx *ctx, bool weighted)
         (void)hipFree(rec);
         return false;
     }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(rec);
+        return false;
+    }
     (weighted ? d.wnrec : d.nrec) = rec;
     d.bytes += n * sizeof(uint4);
     return true;
 }
+
+namespace {
 
 void free_edge_records(ptw_ctx *ctx, bool weighted_only)
 {
@@ -120,7 +130,7 @@ void pick_edge_records(ptw_ctx *ctx, const ptw_walk_desc &d, bool may_build, ptw
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
     if (weighted && !is_path_app(d.app)) return;
     if (!weighted ? ctx->dev.nrec == nullptr : ctx->dev.wnrec == nullptr) {
-        if (!may_build || !ensure_edge_records(ctx, weighted)) return;
+        if (!may_build || !ptw::ensure_edge_records(ctx, weighted)) return;
     }
     if (weighted)
         p.wnrec = ctx->dev.wnrec;

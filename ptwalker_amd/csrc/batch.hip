@@ -187,6 +187,8 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
     const uint4 *wedge = p.hdr[5] ? p.wedge : nullptr;
+    const uint4 *nrec = wedge ? nullptr : p.nrec;  // neighbour records (unweighted batches)
+    uint32_t cur_beg = 0, cur_deg = 0;             // the row of v, from its record
     uint64_t c_next = 0, c_end = 0;  // wave-uniform current chunk
     uint32_t cq = kEmpty;
     bool exhausted = false;
@@ -267,6 +269,10 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                 }
                 w = c_next + rank;
                 v = Q.src;
+                if (nrec) {
+                    cur_beg = p.row_ptr[v];
+                    cur_deg = p.row_ptr[v + 1] - cur_beg;
+                }
                 step = 0;
                 if (!Q.ends) ++my_src;  // the start visit
                 active = true;
@@ -281,8 +287,8 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
             const bool coin = textbook || step >= 1;
             bool stop = coin && r.z >= Q.thr;
             if (!stop) {
-                const uint32_t beg = p.row_ptr[v];
-                const uint32_t deg = p.row_ptr[v + 1] - beg;
+                const uint32_t beg = nrec ? cur_beg : p.row_ptr[v];
+                const uint32_t deg = nrec ? cur_deg : p.row_ptr[v + 1] - beg;
                 if (deg == 0) {
                     stop = true;  // batches end walkers at vertices without out-edges
                 } else {
@@ -290,6 +296,11 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
                     if (wedge) {
                         const uint4 e = wedge[beg + j];
                         v = r.w >= e.y ? e.z : e.x;
+                    } else if (nrec) {
+                        const uint4 e = nrec[(uint64_t)beg + j];
+                        v = e.x;
+                        cur_beg = e.y;
+                        cur_deg = e.z;
                     } else {
                         v = p.col[beg + j];
                     }
@@ -343,8 +354,11 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
     const bool textbook = p.hdr[4] == PTW_SEM_TEXTBOOK;
     const uint4 *wedge = p.hdr[5] ? p.wedge : nullptr;
+    const uint4 *nrec = wedge ? nullptr : p.nrec;  // neighbour records (unweighted batches)
     uint32_t cq = kEmpty;  // workgroup-uniform: the query whose counts the LDS holds
     uint32_t src = 0, thr = 0, stream = 0;
+    uint32_t src_beg = 0, src_deg = 0;  // workgroup-uniform: the source's row
+    uint32_t cur_beg = 0, cur_deg = 0;  // the row of v, from its record
     bool ends = false;  // workgroup-uniform: count only each walker's last vertex
     uint64_t off = 0;
     uint32_t my_src = 0;
@@ -441,6 +455,8 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
             stream = bq.stream;
             ends = bq.ends_only != 0;
             off = bq.slice_off;
+            src_beg = p.row_ptr[src];
+            src_deg = p.row_ptr[src + 1] - src_beg;
         }
         const uint64_t start = s_start;
         const uint32_t len = s_len;
@@ -462,6 +478,8 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                     if (!active && rank < take) {
                         w = start + base + rank;
                         v = src;
+                        cur_beg = src_beg;
+                        cur_deg = src_deg;
                         step = 0;
                         if (!ends) ++my_src;  // the start visit
                         active = true;
@@ -474,8 +492,8 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                 const bool coin = textbook || step >= 1;
                 bool stop = coin && r.z >= thr;
                 if (!stop) {
-                    const uint32_t beg = p.row_ptr[v];
-                    const uint32_t deg = p.row_ptr[v + 1] - beg;
+                    const uint32_t beg = nrec ? cur_beg : p.row_ptr[v];
+                    const uint32_t deg = nrec ? cur_deg : p.row_ptr[v + 1] - beg;
                     if (deg == 0) {
                         stop = true;  // batches end walkers at vertices without out-edges
                     } else {
@@ -483,6 +501,11 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                         if (wedge) {
                             const uint4 e = wedge[beg + j];
                             v = r.w >= e.y ? e.z : e.x;
+                        } else if (nrec) {
+                            const uint4 e = nrec[(uint64_t)beg + j];
+                            v = e.x;
+                            cur_beg = e.y;
+                            cur_deg = e.z;
                         } else {
                             v = p.col[beg + j];
                         }

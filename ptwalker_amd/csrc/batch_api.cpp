@@ -148,6 +148,7 @@ BatchLaunch launch_args(ptw_ctx *ctx, const BatchState &b)
     p.row_ptr = ctx->dev.row_ptr;
     p.col = ctx->dev.col;
     p.wedge = ctx->dev.wedge;
+    p.nrec = ctx->dev.nrec;
     p.V = ctx->dev.V;
     p.queries = b.queries;
     p.chunks = b.chunks;
@@ -318,6 +319,9 @@ int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
     }
     if ((rc = clean_previous(ctx, L))) return rc;
     if ((rc = ensure_pools(ctx, L, P, d.num_queries, d.topk))) return rc;
+    // neighbour records (built once; a captured launch without them is redone)
+    if (!weighted && ctx->shape.edge_records == 1 && !ctx->dev.nrec && ensure_edge_records(ctx, false))
+        drop_exec(b);
 
     b.h_queries = std::move(P.q);
     b.h_chunks = std::move(P.chunks);

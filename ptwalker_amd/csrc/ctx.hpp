@@ -125,6 +125,10 @@ int upload(ptw_ctx *ctx, T **dst, const T *src, size_t n)
     return PTW_OK;
 }
 
+// Builds the graph's neighbour records (weighted: the pairs for path walks)
+// on the context's stream and waits for them; false if the memory is not
+// there.  Kept with the graph once built.
+bool ensure_edge_records(ptw_ctx *ctx, bool weighted);
 // Frees the batch pools, the serving lanes' included (graph change or
 // context teardown).
 void free_batch(ptw_ctx *ctx);

@@ -375,6 +375,7 @@ struct BatchLaunch {
     const uint32_t *row_ptr;
     const uint32_t *col;
     const uint4 *wedge;  // weighted-edge records, used when hdr[5] != 0
+    const uint4 *nrec;   // neighbour records for unweighted batches (nullptr: row_ptr and col)
     uint32_t V;
     const BatchQuery *queries;
     const BatchChunk *chunks;
