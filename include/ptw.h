@@ -472,8 +472,8 @@ typedef struct ptw_tuning {
                               of 1024 up to 2^20; 0 = from the batch's dense walkers */
     int32_t visit_log;     /* count apps: 1 (default) = visits that no LDS counter takes are
                               appended to a device log and counted by id bucket after the
-                              walk (two u32 buffers of 1.1x the expected visits, kept by the
-                              context, at most 2^30 entries each; walks under 8 walkers per
+                              walk (a u32 and a u16 buffer of 1.1x the expected visits, kept
+                              by the context, at most 2^30 entries each; walks under 8 walkers per
                               resident lane skip it); N > 1, a multiple of 1024 = the same
                               with at most N entries; 0 = one device atomic per such visit.
                               Visits beyond a full log take device atomics.  Counts are the

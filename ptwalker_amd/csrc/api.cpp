@@ -56,7 +56,7 @@ uint64_t ensure_vlog(ptw_ctx *ctx, uint64_t walkers, double tp, uint64_t limit =
     if (b.cap >= cap && b.plan_buckets >= nb) return cap;
     const uint64_t alloc = std::max(cap, b.cap);
     free_vlog(ctx);
-    if (hipMalloc((void **)&b.log, alloc * 4) != hipSuccess || hipMalloc((void **)&b.parted, alloc * 4) != hipSuccess ||
+    if (hipMalloc((void **)&b.log, alloc * 4) != hipSuccess || hipMalloc((void **)&b.parted, alloc * 2) != hipSuccess ||
         hipMalloc((void **)&b.plan, ptw::vlog_plan_words(nb) * 4) != hipSuccess) {
         (void)hipGetLastError();  // the failed allocation is not a walk error
         free_vlog(ctx);

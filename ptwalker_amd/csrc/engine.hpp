@@ -233,7 +233,7 @@ int device_load_edge_list(hipStream_t s, const char *path, int format, bool as_u
 // bucket, and the partition's plan.
 struct VlogBuffers {
     uint32_t *log = nullptr;     // cap entries
-    uint32_t *parted = nullptr;  // cap entries
+    uint16_t *parted = nullptr;  // cap entries: offsets inside their bucket
     uint64_t cap = 0;
     // hist[NB] (zeroed before the walk), start[NB + 1], cursor[NB],
     // task_first[NB + 1], then the segment length word

@@ -11,7 +11,8 @@
 //            cuts every bucket into count tasks of at most `seg` entries;
 //   scatter  each tile of 16384 entries is counting-sorted by bucket in
 //            LDS, reserves its run in every bucket it holds with one atomic
-//            per bucket, and writes the runs out with consecutive stores;
+//            per bucket, and writes the runs out with consecutive stores as
+//            u16 offsets inside the bucket (the bucket holds the rest);
 //   count    each task counts its bucket's ids in LDS u32 counters (one per
 //            id of the bucket) and adds the nonzero ones to the device
 //            counters: one atomic per vertex a task saw, not per visit.
@@ -96,10 +97,11 @@ __global__ __launch_bounds__(kThreads) void vlog_plan_kernel(uint32_t *plan, uin
 
 // Partition of the log's first min(head, cap) entries (a multiple of 4;
 // kNoVisit entries are skipped) into `out` by bucket, runs placed from the
-// plan's cursors.  LDS: the tile's keys, then cnt, off and gb per bucket.
+// plan's cursors, each entry as its offset inside its bucket (shift <= 16).
+// LDS: the tile's keys, then cnt, off and gb per bucket.
 __global__ __launch_bounds__(kThreads) void vlog_scatter_kernel(const uint32_t *log, const unsigned long long *head,
                                                                uint64_t cap, uint32_t shift, uint32_t nb,
-                                                               uint32_t *cursor, uint32_t *out)
+                                                               uint32_t *cursor, uint16_t *out)
 {
     extern __shared__ uint32_t sm[];
     __shared__ uint32_t wtot[kThreads / 64];
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(kThreads) void vlog_scatter_kernel(const uint32_t *
         for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
             const uint32_t y = keys[i];
             const uint32_t b = y >> shift;
-            out[gb[b] + (i - off[b])] = y;
+            out[gb[b] + (i - off[b])] = (uint16_t)(y & ((1u << shift) - 1u));
         }
         __syncthreads();  // the next tile reuses keys, cnt, off and gb
     }
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void vlog_scatter_kernel(const uint32_t *
 // Count tasks: task t of bucket b covers entries [start[b] + k seg,
 // min(start[b] + (k + 1) seg, start[b + 1])) of the partition, k = t -
 // task_first[b].  LDS: 2^shift u32 counters.
-__global__ __launch_bounds__(kThreads) void vlog_count_kernel(const uint32_t *parted, const uint32_t *plan, uint32_t shift,
+__global__ __launch_bounds__(kThreads) void vlog_count_kernel(const uint16_t *parted, const uint32_t *plan, uint32_t shift,
                                                              uint32_t nb, unsigned long long *counts)
 {
     extern __shared__ uint32_t c[];
@@ -185,12 +187,12 @@ __global__ __launch_bounds__(kThreads) void vlog_count_kernel(const uint32_t *pa
         for (; e + 3ull * blockDim.x < e1; e += 4ull * blockDim.x) {
             const uint32_t y0 = parted[e], y1 = parted[e + blockDim.x];
             const uint32_t y2 = parted[e + 2ull * blockDim.x], y3 = parted[e + 3ull * blockDim.x];
-            atomicAdd(&c[y0 & (width - 1)], 1u);
-            atomicAdd(&c[y1 & (width - 1)], 1u);
-            atomicAdd(&c[y2 & (width - 1)], 1u);
-            atomicAdd(&c[y3 & (width - 1)], 1u);
+            atomicAdd(&c[y0], 1u);
+            atomicAdd(&c[y1], 1u);
+            atomicAdd(&c[y2], 1u);
+            atomicAdd(&c[y3], 1u);
         }
-        for (; e < e1; e += blockDim.x) atomicAdd(&c[parted[e] & (width - 1)], 1u);
+        for (; e < e1; e += blockDim.x) atomicAdd(&c[parted[e]], 1u);
         __syncthreads();
         const uint64_t v0 = (uint64_t)lo << shift;
         for (uint32_t i = threadIdx.x; i < width; i += blockDim.x) {
