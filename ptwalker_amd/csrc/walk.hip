@@ -435,34 +435,38 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         pend = kNoVisit;
     };
 
-    // Path rows are written through a 4-entry register queue: every entry that
-    // completes an aligned 16 B chunk of the flat path array flushes the chunk
-    // with one dwordx4 store; a row's unaligned head and tail go out as dwords.
+    // Path rows are written through an 8-entry register queue: every entry
+    // that completes an aligned 32 B chunk of the flat path array flushes the
+    // chunk with two dwordx4 stores, so the L2 receives whole 32 B sectors
+    // (16 B stores leave half-written sectors that the L2 can evict between
+    // their two halves); a row's unaligned head and tail go out as dwords.
     uint64_t pos = 0;  // flat index of the next path entry
-    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nbuf = 0;
+    uint32_t qb[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // qb[7] is the newest entry
+    uint32_t nbuf = 0;
     auto path_append = [&](uint32_t x) {
-        b0 = b1;
-        b1 = b2;
-        b2 = b3;
-        b3 = x;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) qb[k] = qb[k + 1];
+        qb[7] = x;
         ++nbuf;
-        if ((pos & 3) == 3) {
-            if (nbuf >= 4) {
-                *reinterpret_cast<uint4 *>(p.paths + (pos - 3)) = uint4{b0, b1, b2, b3};
+        if ((pos & 7) == 7) {
+            if (nbuf >= 8) {
+                *reinterpret_cast<uint4 *>(p.paths + (pos - 7)) = uint4{qb[0], qb[1], qb[2], qb[3]};
+                *reinterpret_cast<uint4 *>(p.paths + (pos - 3)) = uint4{qb[4], qb[5], qb[6], qb[7]};
             } else {
-                if (nbuf >= 3) p.paths[pos - 2] = b1;
-                if (nbuf >= 2) p.paths[pos - 1] = b2;
-                p.paths[pos] = b3;
+                // qb[k] belongs at pos - 7 + k; the newest nbuf are this row's
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (8 - k <= (int)nbuf) p.paths[pos - 7 + k] = qb[k];
             }
             nbuf = 0;
         }
         ++pos;
     };
     auto path_flush_tail = [&]() {
-        // the last nbuf (< 4) entries sit at pos - nbuf .. pos - 1
-        if (nbuf >= 3) p.paths[pos - 3] = b1;
-        if (nbuf >= 2) p.paths[pos - 2] = b2;
-        if (nbuf >= 1) p.paths[pos - 1] = b3;
+        // the last nbuf (< 8) entries sit at pos - nbuf .. pos - 1 (qb[k] at pos - 8 + k)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (8 - k <= (int)nbuf) p.paths[pos - 8 + k] = qb[k];
         nbuf = 0;
     };
 
