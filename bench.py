@@ -171,7 +171,7 @@ def main():
         req_per_step = tj.get("memory_requests_per_step")
         lines_per_step = tj.get("l2_miss_lines_per_step")
         traffic_source = (f"not measured in this run: read from {os.path.relpath(args.traffic, ROOT)}, the "
-                          "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of this kernel on this workload scaled by the "
+                          "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of these kernels on this workload scaled by the "
                           "copy-kernel calibration (profiles/walk_count_rmat22_traffic.md)")
 
     # measured random-gather roofline at the CSR's col[] footprint, and the
@@ -215,7 +215,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "walk_kernel<count> (PPR)",
+            # ptw_walk's device time of a PPR launch: the walk and the visit
+            # log's counting passes that finish its counts
+            "kernel": "walk_kernel<count> + vlog_plan/scatter/count_kernel (PPR)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -232,9 +234,10 @@ def main():
             "frac": gather_frac,
             "note": "(PPR steps/s x 2 random gathers per step) / measured random u32 gathers/s",
         },
-        # the bound the counters show: memory requests (128 B L2-miss lines and
-        # 32 B memory-side atomics), against the random-gather request rate
-        # measured in this run (one 128 B line per gather)
+        # the bound the counters show: memory requests (128 B L2-miss lines,
+        # 32 B memory-side atomics and write requests), against the
+        # random-gather request rate measured in this run (one 128 B line per
+        # gather)
         "request_roofline": None if req_per_step is None else {
             "unit": "memory requests/s",
             "requests_per_step": req_per_step,
