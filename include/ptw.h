@@ -478,8 +478,9 @@ typedef struct ptw_tuning {
                               with at most N entries; 0 = one device atomic per such visit.
                               Visits beyond a full log take device atomics.  Counts are the
                               same either way */
-    int32_t edge_records;  /* neighbour records, built by the first walk that reads them and
-                              kept with the graph: per CSR slot the neighbour's id, row start,
+    int32_t edge_records;  /* neighbour records, built with the graph (weighted ones and any
+                              missing ones by the first walk that reads them) and kept with
+                              it: per CSR slot the neighbour's id, row start,
                               degree and graph id (16 B per edge; weighted path walks: both
                               alias candidates, 32 B per edge), so a step is one dependent
                               read.  1 (default) = every walk but node2vec (weighted count

@@ -229,8 +229,12 @@ int warm_up(ptw_ctx *ctx)
         HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
         HIP_TRY(ctx, ptw::launch_walk(p, ctx->shape, ctx->num_cus, s));
     }
-    // the visit log's passes over an empty log
-    if (const uint64_t cap = ensure_vlog(ctx, 0, 1.0)) {
+    // neighbour records belong to the graph's set-up, like the CSR: walks
+    // do not pay for building them (a failed allocation leaves walks on the CSR)
+    if (ctx->shape.edge_records != 0) (void)ptw::ensure_edge_records(ctx, false);
+    // the visit log's passes over an empty log (a minimal log; count walks
+    // grow it to their size)
+    if (const uint64_t cap = ensure_vlog(ctx, 0, 1.0, ptw::kVlogSeg)) {
         const uint32_t shift = ptw::vlog_shift_for(ctx->dev.V);
         const uint32_t nb = ptw::vlog_buckets_for(ctx->dev.V, shift);
         HIP_TRY(ctx, ptw::launch_vlog_begin(ctx->vlog, nb, s));
@@ -245,6 +249,27 @@ int warm_up(ptw_ctx *ctx)
     HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned + 8, ctx->counts_stage, (size_t)ctx->dev.V * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->counts_stage, ctx->pinned + 8, (size_t)ctx->dev.V * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
+    // One real walk of one walker through ptw_walk: the first walk of a
+    // process otherwise pays about 0.3 ms of one-time host-side set-up that
+    // the launches above do not take (measured: 0.54 ms for the first
+    // 655,360-walker PageRank on R-MAT scale 16, 0.26 ms for the next ones).
+    if (ctx->dev.V) {
+        ptw_walk_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d.app = PTW_APP_PAGERANK;
+        d.semantics = PTW_SEM_REFERENCE;
+        d.start_mode = PTW_START_UNIFORM;
+        d.dangling_policy = PTW_DANGLING_STOP;
+        d.tp = 1.0;
+        d.seed = 1;
+        d.walkers = 1;
+        d.walk_len = 1;
+        std::vector<uint64_t> counts(ctx->dev.V);
+        ptw_walk_out out;
+        std::memset(&out, 0, sizeof(out));
+        out.counts = counts.data();
+        if (int rc = ptw_walk(ctx, &d, &out)) return rc;
+    }
     return PTW_OK;
 }
 
@@ -839,8 +864,14 @@ int ptw_walk(ptw_ctx *ctx, const ptw_walk_desc *desc, ptw_walk_out *out)
     HIP_TRY(ctx, ptw::launch_fill_u64(ctx->scratch, 8, 0ull, s));
     if (count_app) HIP_TRY(ctx, ptw::launch_fill_u64(ctx->counts_dev, V, 0ull, s));
     const ptw::LaunchShape shape = walk_shape(ctx, d);
+    // With an automatic launch, a walk under 8 walkers per lane of one
+    // workgroup per CU counts with device atomics (launch_walk's small-walk
+    // rule; a count launch holds one workgroup per CU), so it takes no log
+    // and allocates none.
+    const bool small_walk =
+        shape.blocks_per_cu <= 0 && W < 8ull * (uint64_t)ctx->num_cus * (uint64_t)shape.block;
     const uint64_t log_cap =
-        count_app && W && shape.visit_log
+        count_app && W && shape.visit_log && !small_walk
             ? ensure_vlog(ctx, W, d.tp, shape.visit_log > 1 ? (uint64_t)shape.visit_log : 0)
             : 0;
     if (log_cap) {
