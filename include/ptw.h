@@ -291,7 +291,8 @@ typedef struct ptw_batch_desc {
     uint64_t seed;
     int32_t semantics;        /* PTW_SEM_* */
     uint32_t flags;           /* PTW_BATCH_* */
-    uint64_t dense_threshold; /* walkers above this use a dense slice; 0 = 100000 */
+    uint64_t dense_threshold; /* walkers above this are dense (workgroup chunks, a dense slice or
+                                 the visit log); 0 = 100000 */
 } ptw_batch_desc;
 
 typedef struct ptw_batch_out {
@@ -306,6 +307,11 @@ typedef struct ptw_batch_out {
     uint32_t hash_queries;
     double kernel_ms; /* device time: counter setup, walk, top-k */
     double total_ms;
+    /* dense queries counted through the visit log instead of a slice (visit_log
+       tuning != 0); a batch whose log fills is run again with slices, which
+       gives the same counts (log_rerun = 1) */
+    uint32_t logged_queries;
+    uint32_t log_rerun;
 } ptw_batch_out;
 
 int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out);
@@ -476,8 +482,13 @@ typedef struct ptw_tuning {
                               by the context, at most 2^30 entries each; walks under 8 walkers per
                               resident lane skip it); N > 1, a multiple of 1024 = the same
                               with at most N entries; 0 = one device atomic per such visit.
-                              Visits beyond a full log take device atomics.  Counts are the
-                              same either way */
+                              Visits beyond a full log take device atomics.  Batches: with
+                              visit_log != 0 (and at most 4096 id buckets of 2^13..2^15 ids),
+                              dense queries keep a hub slice and log the rest of their visits
+                              (u32 entry, u16 partition entry and one pair place per expected
+                              visit, 1.25x, held by the batch pools); N > 1 caps that log at
+                              N entries, and a batch whose log fills runs again with dense
+                              slices.  Counts are the same either way */
     int32_t edge_records;  /* neighbour records, built with the graph (weighted ones and any
                               missing ones by the first walk that reads them) and kept with
                               it: per CSR slot the neighbour's id, row start,

@@ -192,6 +192,8 @@ class BatchOut(ctypes.Structure):
         ("hash_queries", ctypes.c_uint32),
         ("kernel_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("logged_queries", ctypes.c_uint32),
+        ("log_rerun", ctypes.c_uint32),
     ]
 
 
@@ -529,7 +531,7 @@ class Context:
         del qs
         return BatchResult(tv[: n * topk].reshape(n, topk), tc[: n * topk].reshape(n, topk), steps[:n],
                            out.total_steps, out.slice_bytes, out.dense_queries, out.hash_queries, out.kernel_ms,
-                           out.total_ms, ent[:n])
+                           out.total_ms, ent[:n], out.logged_queries, bool(out.log_rerun))
 
     def batch_write_topk(self, path: str, sources, walkers, tps, result: "BatchResult", *, streams=None,
                          kinds=None, id_space: int = 1) -> None:
@@ -854,6 +856,8 @@ class BatchResult:
     kernel_ms: float
     total_ms: float
     entropy: np.ndarray      # (queries,) nats
+    logged_queries: int = 0  # dense queries counted through the visit log
+    log_rerun: bool = False  # the log filled and the batch ran again with slices
 
 
 @dataclass

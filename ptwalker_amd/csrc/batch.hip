@@ -155,11 +155,23 @@ __global__ void batch_begin_kernel(BatchLaunch p, uint32_t max_queries)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.queue[0] = 0;
         p.queue[1] = 0;
+        p.queue[2] = 0;
+        p.queue[3] = 0;
+        p.queue[4] = 0;
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < max_queries && i < nq; i += gridDim.x * blockDim.x) {
         p.touched_count[i] = 0;
         p.overflow[i] = 0;
         p.steps[i] = 0;
+        if (p.vlog) {
+            for (uint32_t w = 0; w < kBatchLogQWords; ++w) p.qlog[(uint64_t)i * kBatchLogQWords + w] = 0;
+        }
+    }
+    if (p.vlog) {
+        // the per-(query, bucket) entry counts the walk adds to
+        const uint64_t n = (uint64_t)(nq < max_queries ? nq : max_queries) * p.log_nb;
+        for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (uint64_t)gridDim.x * blockDim.x)
+            p.part_cnt[w] = 0;
     }
     // the top-k statistics, as one flat array of 16 B words
     const uint32_t q_used = nq < max_queries ? nq : max_queries;
@@ -333,7 +345,8 @@ __global__ __launch_bounds__(1024) void batch_walk_kernel(BatchLaunch p)
 // word x >> 1); the increment that takes one to 2^15 moves 2^15 to the slice.
 // The cache counts are u32 and cannot wrap: none exceeds its query's visit
 // total, which the host checks is below 2^32.
-__global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32_t hubs, uint32_t slots)
+__global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32_t hubs, uint32_t slots,
+                                                          uint32_t nb)
 {
     extern __shared__ uint32_t dense_lds[];
     const uint32_t hub_words = (hubs + 1) >> 1;
@@ -341,6 +354,9 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     uint32_t *const keys = dense_lds + hub_words;
     uint32_t *const vals = keys + slots;
     const uint32_t mask = slots ? slots - 1 : 0u;
+    // logged queries (nb > 0, no cache): entries per id bucket of the query
+    uint32_t *const bkt = dense_lds + hub_words;
+    const bool logging = nb != 0;
     __shared__ uint32_t s_q, s_len, s_off, s_done;
     __shared__ unsigned long long s_start;
 
@@ -349,6 +365,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
         keys[i] = kEmpty;
         vals[i] = 0;
     }
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) bkt[i] = 0;
     const uint32_t lane = __lane_id();
     const uint64_t n_dense = p.hdr[6];
     const uint32_t seed_lo = (uint32_t)p.hdr[3], seed_hi = (uint32_t)(p.hdr[3] >> 32);
@@ -360,12 +377,20 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     uint32_t src_beg = 0, src_deg = 0;  // workgroup-uniform: the source's row
     uint32_t cur_beg = 0, cur_deg = 0;  // the row of v, from its record
     bool ends = false;  // workgroup-uniform: count only each walker's last vertex
-    uint64_t off = 0;
+    uint64_t off = 0;  // workgroup-uniform: the query's dense slice, or its hub slice when logged
     uint32_t my_src = 0;
     uint64_t my_steps = 0;
+    // A visit the LDS does not take waits in `pend` for the wavefront's next
+    // log append (at most one per lane between appends).
+    uint32_t pend = kNoVisit;
+    // The wavefront's log segment: entries [seg_pos, seg_end) of segment
+    // seg_id (wave-uniform; seg_id = ~0 when it holds none).
+    uint64_t seg_pos = 0, seg_end = 0, seg_id = ~0ull;
+    // segments this batch may claim (hdr[8] <= the log's allocation)
+    const uint64_t seg_limit = logging ? (p.hdr[8] < p.log_segs ? p.hdr[8] : p.log_segs) : 0;
 
     // One count of the workgroup's query: its source in a register, the LDS
-    // hub counters, the LDS cache, then the query's dense slice.
+    // hub counters, then the LDS cache and the query's dense slice, or the log.
     auto count = [&](uint32_t x) {
         if (x == src) {
             ++my_src;
@@ -377,6 +402,8 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                 atomicSub(&direct[x >> 1], 0x8000u << sh);
                 atomicAdd(&p.dense[off + x], 0x8000u);
             }
+        } else if (logging) {
+            pend = x;
         } else {
             if (slots) {
                 uint32_t h = (x * 0x9E3779B1u) & mask;
@@ -395,6 +422,54 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
         }
     };
 
+    // Appends the wavefront's pending visits to its log segment as
+    // consecutive u32 stores; a full segment is closed and a new one claimed
+    // for the workgroup's query.  A claim beyond the log's capacity drops the
+    // visits and marks the query in p.overflow (the batch then fails).
+    // Called by all lanes of the wavefront together.
+    auto log_append = [&]() {
+        const uint64_t m = __ballot(pend != kNoVisit);
+        if (m == 0) return;
+        const uint32_t n = (uint32_t)__popcll(m);
+        const uint32_t r = lane_rank(m);
+        const uint64_t room = seg_end - seg_pos;
+        uint64_t slot = ~0ull;
+        if (r < room) slot = seg_pos + r;
+        if (n <= room) {
+            seg_pos += n;
+        } else {
+            unsigned long long s = 0;
+            if (lane == 0) {
+                if (seg_id != ~0ull) p.seg_n[seg_id] = kBatchLogSeg;  // filled to the end
+                s = atomicAdd(&p.queue[2], 1ull);
+                if (s < seg_limit) {
+                    p.seg_q[s] = cq;
+                    atomicAdd(&p.qlog[(uint64_t)cq * kBatchLogQWords], 1u);
+                } else {
+                    atomicAdd(&p.overflow[cq], n - (uint32_t)room);
+                }
+            }
+            s = wave_broadcast_first(s);
+            if (s < seg_limit) {
+                const uint64_t base = s * kBatchLogSeg;
+                if (r >= room) slot = base + (r - room);
+                seg_id = s;
+                seg_pos = base + (n - room);
+                seg_end = base + kBatchLogSeg;
+            } else {
+                seg_id = ~0ull;
+                seg_pos = seg_end = 0;
+            }
+        }
+        if (pend != kNoVisit) {
+            if (slot != ~0ull) {
+                p.vlog[slot] = pend;
+                atomicAdd(&bkt[pend >> p.log_shift], 1u);
+            }
+            pend = kNoVisit;
+        }
+    };
+
     // Workgroup-uniform: every thread calls it, between barriers.
     auto flush = [&]() {
         if (cq == kEmpty) return;
@@ -404,11 +479,27 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
             st += __shfl_xor(st, o);
         }
         if (lane == 0) {
-            if (hot) atomicAdd(&p.dense[off + src], (uint32_t)hot);
+            if (hot) {
+                if (logging)
+                    atomicAdd(&p.qlog[(uint64_t)cq * kBatchLogQWords + 4], (uint32_t)hot);
+                else
+                    atomicAdd(&p.dense[off + src], (uint32_t)hot);
+            }
             if (st) atomicAdd(&p.steps[cq], (unsigned long long)st);
+            // the wavefront's segment holds this query's entries only
+            if (logging && seg_id != ~0ull) p.seg_n[seg_id] = (uint32_t)(seg_pos - seg_id * kBatchLogSeg);
         }
+        seg_id = ~0ull;
+        seg_pos = seg_end = 0;
         my_src = 0;
         my_steps = 0;
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+            const uint32_t c = bkt[b];
+            if (c) {
+                atomicAdd(&p.part_cnt[(uint64_t)cq * nb + b], c);
+                bkt[b] = 0;
+            }
+        }
         for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) {
             const uint32_t c = direct[i];
             if (c) {
@@ -454,7 +545,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
             thr = bq.thr;
             stream = bq.stream;
             ends = bq.ends_only != 0;
-            off = bq.slice_off;
+            off = logging ? bq.hub_off : bq.slice_off;
             src_beg = p.row_ptr[src];
             src_deg = p.row_ptr[src + 1] - src_beg;
         }
@@ -519,6 +610,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                     active = false;
                 }
             }
+            if (logging) log_append();
         }
     }
     flush();
@@ -537,14 +629,27 @@ __device__ __forceinline__ void wave_lds_sync()
 // and 3 (most entries) in registers, larger counts in the workgroup's LDS
 // bins; c ln c in 2^-20 fixed point, so the sums do not depend on the order
 // of the additions.
+__device__ __forceinline__ unsigned long long clogc_fixed(uint32_t c)
+{
+    return c > 1 ? (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5) : 0ull;
+}
+// c ln c in 2^-20 fixed point for counts below kClogcTable, from the same
+// expression, for kernels whose lanes meet scattered counts (a table read
+// instead of a divergent double-precision log).
+constexpr uint32_t kClogcTable = 512;
+__device__ __forceinline__ void fill_clogc_table(unsigned long long *tab)
+{
+    for (uint32_t c = threadIdx.x; c < kClogcTable; c += blockDim.x) tab[c] = clogc_fixed(c);
+}
+
 struct LaneStats {
     uint32_t maxc = 0, n1 = 0, n2 = 0, n3 = 0;
     unsigned long long sum = 0, clogc = 0;
-    __device__ __forceinline__ void add(uint32_t c, uint32_t *s_hist)
+    __device__ __forceinline__ void add(uint32_t c, uint32_t *s_hist, const unsigned long long *tab = nullptr)
     {
         maxc = c > maxc ? c : maxc;
         sum += c;
-        if (c > 1) clogc += (unsigned long long)((double)c * log((double)c) * 1048576.0 + 0.5);
+        clogc += tab && c < kClogcTable ? tab[c] : clogc_fixed(c);
         if (c >= 4) {
             atomicAdd(&s_hist[count_bin(c)], 1u);
         } else {
@@ -673,6 +778,7 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
         const uint32_t q = (uint32_t)(t / segs);
         const uint32_t seg = (uint32_t)(t % segs);
         const BatchQuery bq = p.queries[q];
+        if (bq.kind == SLICE_LOG) continue;  // listed by batch_log_count_kernel
         uint2 *list = p.touched + bq.touched_off;
         uint2 e[16];
         bool valid[16];
@@ -749,6 +855,404 @@ __global__ __launch_bounds__(kCompactBlock) void batch_compact_kernel(BatchLaunc
     }
 }
 
+// ---- visit log of SLICE_LOG queries ------------------------------------------
+
+// Exclusive prefix of x over the workgroup (at most 1024 threads) and the
+// total.  Every thread calls it; s_wave holds 16 words.
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, unsigned long long *s_wave, uint64_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+    uint64_t incl = x;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint64_t before = 0;
+    total = 0;
+    for (uint32_t w = 0; w < waves; ++w) {
+        const uint64_t t = s_wave[w];
+        before += w < wave ? t : 0;
+        total += t;
+    }
+    __syncthreads();  // s_wave is reused by the next scan
+    return before + incl - x;
+}
+
+// Per logged query: the exclusive prefix of its bucket entries (the bucket's
+// first entry in the query's partition, and the scatter cursor) and its
+// entry total.
+__global__ __launch_bounds__(1024) void batch_log_rows_kernel(BatchLaunch p)
+{
+    __shared__ unsigned long long s_wave[16];
+    const uint32_t nq = (uint32_t)p.hdr[1], nb = p.log_nb;
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        if (p.queries[q].kind != SLICE_LOG) continue;  // workgroup-uniform
+        const uint64_t row = (uint64_t)q * nb;
+        const uint32_t b0 = threadIdx.x * per;
+        uint64_t mine = 0;
+        for (uint32_t b = b0; b < b0 + per && b < nb; ++b) mine += p.part_cnt[row + b];
+        uint64_t total = 0;
+        uint64_t at = block_excl_scan(mine, s_wave, total);
+        if (total >> 32) {
+            // u32 places cannot hold it (no walk comes near: the plan keeps
+            // every query's expected counts below 2^32 / 1.5): the query is
+            // marked as failed and its log is not partitioned
+            for (uint32_t b = b0; b < b0 + per && b < nb; ++b) p.part_cnt[row + b] = 0;
+            if (threadIdx.x == 0) {
+                atomicAdd(&p.overflow[q], 1u);
+                p.qlog[(uint64_t)q * kBatchLogQWords] = 0;
+                p.qlog[(uint64_t)q * kBatchLogQWords + 5] = 0;
+            }
+            continue;
+        }
+        for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
+            p.part_loc[row + b] = (uint32_t)at;
+            p.part_cur[row + b] = (uint32_t)at;
+            at += p.part_cnt[row + b];
+        }
+        if (threadIdx.x == 0) p.qlog[(uint64_t)q * kBatchLogQWords + 5] = (uint32_t)total;
+    }
+}
+
+// One workgroup: places every logged query's partition, segment list and
+// pair list one after the other (exclusive prefixes over the queries), and
+// gives the pair list its place in the touched pool (from hdr[7]) and its
+// capacity: a bucket holds at most its entries plus its hub counters plus
+// the source as distinct vertices.
+__global__ __launch_bounds__(1024) void batch_log_plan_kernel(BatchLaunch p)
+{
+    __shared__ unsigned long long s_wave[16];
+    const uint32_t nq = (uint32_t)p.hdr[1];
+    BatchQuery *qs = const_cast<BatchQuery *>(p.queries);
+    uint64_t c_ent = 0, c_seg = 0, c_pair = 0;
+    for (uint32_t base = 0; base < nq; base += blockDim.x) {
+        const uint32_t q = base + threadIdx.x;
+        uint64_t ent = 0, segs = 0, pairs = 0;
+        uint32_t *ql = p.qlog + (uint64_t)q * kBatchLogQWords;
+        bool logged = false;
+        if (q < nq && qs[q].kind == SLICE_LOG) {
+            logged = true;
+            ent = ql[5];
+            segs = ql[0];
+            pairs = ent + qs[q].hub_len + 1;
+            if (pairs > p.V) pairs = p.V;
+        }
+        uint64_t t_ent, t_seg, t_pair;
+        const uint64_t x_ent = block_excl_scan(ent, s_wave, t_ent);
+        const uint64_t x_seg = block_excl_scan(segs, s_wave, t_seg);
+        const uint64_t x_pair = block_excl_scan(pairs, s_wave, t_pair);
+        if (logged) {
+            p.q_base[q] = c_ent + x_ent;
+            ql[2] = (uint32_t)(c_seg + x_seg);
+            ql[1] = (uint32_t)(c_seg + x_seg);
+            qs[q].touched_off = p.hdr[7] + c_pair + x_pair;
+            qs[q].touched_cap = (uint32_t)pairs;
+        }
+        c_ent += t_ent;
+        c_seg += t_seg;
+        c_pair += t_pair;
+    }
+}
+
+// Lists every claimed segment under its query.
+__global__ void batch_log_seglist_kernel(BatchLaunch p)
+{
+    uint64_t used = p.queue[2];
+    if (used > p.hdr[8]) used = p.hdr[8];
+    if (used > p.log_segs) used = p.log_segs;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < used; s += stride) {
+        const uint32_t q = p.seg_q[s];
+        const uint32_t at = atomicAdd(&p.qlog[(uint64_t)q * kBatchLogQWords + 1], 1u);
+        p.seg_list[at] = (uint32_t)s;
+    }
+}
+
+// The logged queries after the walk, one query per workgroup at a time,
+// taken from queue[qword] in the order of log_order (largest first).  The
+// workgroup owns the query, so cursors and pair places are LDS values, not
+// device atomics.
+//
+// batch_log_partition_kernel: tiles of kBatchLogTile of the query's
+// segments (thread t holds entry t of each) are counted per id bucket; each
+// bucket's run continues at the bucket's LDS cursor, the tile is ordered by
+// bucket in LDS and goes out as u16 offsets (id - bucket start).
+//
+// batch_log_count_kernel: bucket by bucket, 2^log_shift u32 LDS counters
+// start from the hub slice where it covers the bucket, take the source's
+// register-counted visits and the bucket's entries, and the nonzero ones are
+// written in id order as the bucket's run of (vertex, count) pairs, after the
+// runs of the buckets before it.  The run's place and length are kept for
+// lookups; the counts go into the query's statistics.
+
+// Workgroup barrier that orders LDS accesses only: global loads and stores
+// in flight stay in flight across it (the kernels using it exchange nothing
+// through global memory within a launch).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// block_excl_scan with LDS-only barriers.
+__device__ __forceinline__ uint64_t block_excl_scan_lds(uint64_t x, unsigned long long *s_wave, uint64_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+    uint64_t incl = x;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    lds_barrier();
+    uint64_t before = 0;
+    total = 0;
+    for (uint32_t w = 0; w < waves; ++w) {
+        const uint64_t t = s_wave[w];
+        before += w < wave ? t : 0;
+        total += t;
+    }
+    lds_barrier();  // s_wave is reused by the next scan
+    return before + incl - x;
+}
+
+// Next logged query for this workgroup (kEmpty when none is left), as a
+// wave-uniform value.  Every thread calls it.
+__device__ __forceinline__ uint32_t take_logged_query(const BatchLaunch &p, uint32_t qword, uint32_t *s_take)
+{
+    lds_barrier();  // the previous query is done with the LDS
+    if (threadIdx.x == 0) {
+        const unsigned long long i = atomicAdd(&p.queue[qword], 1ull);
+        *s_take = i < p.hdr[9] ? p.log_order[i] : kEmpty;
+    }
+    lds_barrier();
+    return __builtin_amdgcn_readfirstlane(*s_take);
+}
+
+// Segments whose ids and fills a partition workgroup stages in LDS at once.
+constexpr uint32_t kPartSegBatch = 1024;
+
+// Dynamic LDS: the tile's staging (kBatchLogTile * kBatchLogSeg words), five
+// arrays of log_nb words, then the ids and fills of up to kPartSegBatch of
+// the query's segments.  The next tile's entries are loaded while a tile is
+// placed.
+__global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p)
+{
+    extern __shared__ uint32_t lp_lds[];
+    __shared__ unsigned long long s_wave[16];
+    __shared__ uint32_t s_take;
+    const uint32_t nb = p.log_nb, shift = p.log_shift;
+    const uint32_t off_mask = (1u << shift) - 1;
+    uint32_t *const stage = lp_lds;
+    uint32_t *const hist = stage + kBatchLogTile * kBatchLogSeg;
+    uint32_t *const start = hist + nb;
+    uint32_t *const fill = start + nb;
+    uint32_t *const base = fill + nb;
+    uint32_t *const cur = base + nb;
+    uint32_t *const seg_id = cur + nb;
+    uint32_t *const seg_fill = seg_id + kPartSegBatch;
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    for (;;) {
+        const uint32_t q = take_logged_query(p, 3, &s_take);
+        if (q == kEmpty) break;
+        const uint32_t *ql = p.qlog + (uint64_t)q * kBatchLogQWords;
+        const uint32_t nseg = ql[0], seg0 = ql[2];
+        const uint64_t qb = p.q_base[q];
+        const uint64_t row = (uint64_t)q * nb;
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cur[b] = p.part_loc[row + b];
+        for (uint32_t g0 = 0; g0 < nseg; g0 += kPartSegBatch) {
+            const uint32_t ng = nseg - g0 < kPartSegBatch ? nseg - g0 : kPartSegBatch;
+            lds_barrier();  // the previous group's ids are no longer read
+            for (uint32_t i = threadIdx.x; i < ng; i += blockDim.x) {
+                const uint32_t sg = p.seg_list[seg0 + g0 + i];
+                seg_id[i] = sg;
+                seg_fill[i] = p.seg_n[sg];
+            }
+            lds_barrier();
+            // entry threadIdx.x of each segment of tile t0 (kNoVisit past its fill)
+            auto load_tile = [&](uint32_t t0, uint32_t (&e)[kBatchLogTile]) {
+#pragma unroll
+                for (uint32_t k = 0; k < kBatchLogTile; ++k) {
+                    e[k] = kNoVisit;
+                    if (t0 + k < ng && threadIdx.x < seg_fill[t0 + k])
+                        e[k] = p.vlog[(uint64_t)seg_id[t0 + k] * kBatchLogSeg + threadIdx.x];
+                }
+            };
+            uint32_t nxt[kBatchLogTile];
+            load_tile(0, nxt);
+            for (uint32_t t0 = 0; t0 < ng; t0 += kBatchLogTile) {
+                uint32_t ent[kBatchLogTile];
+#pragma unroll
+                for (uint32_t k = 0; k < kBatchLogTile; ++k) ent[k] = nxt[k];
+                for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[b] = 0;
+                lds_barrier();
+#pragma unroll
+                for (uint32_t k = 0; k < kBatchLogTile; ++k)
+                    if (ent[k] != kNoVisit) atomicAdd(&hist[ent[k] >> shift], 1u);
+                if (t0 + kBatchLogTile < ng) load_tile(t0 + kBatchLogTile, nxt);  // in flight meanwhile
+                lds_barrier();
+                const uint32_t b0 = threadIdx.x * per;
+                uint64_t mine = 0;
+                for (uint32_t b = b0; b < b0 + per && b < nb; ++b) mine += hist[b];
+                uint64_t n_tile = 0;
+                uint32_t at = (uint32_t)block_excl_scan_lds(mine, s_wave, n_tile);
+                for (uint32_t b = b0; b < b0 + per && b < nb; ++b) {
+                    const uint32_t h = hist[b];
+                    start[b] = at;
+                    fill[b] = 0;
+                    base[b] = cur[b];
+                    cur[b] += h;
+                    at += h;
+                }
+                lds_barrier();
+#pragma unroll
+                for (uint32_t k = 0; k < kBatchLogTile; ++k) {
+                    if (ent[k] != kNoVisit) {
+                        const uint32_t b = ent[k] >> shift;
+                        const uint32_t pos = start[b] + atomicAdd(&fill[b], 1u);
+                        stage[pos] = (b << 16) | (ent[k] & off_mask);
+                    }
+                }
+                lds_barrier();
+                for (uint32_t i = threadIdx.x; i < (uint32_t)n_tile; i += blockDim.x) {
+                    const uint32_t e = stage[i];
+                    const uint32_t b = e >> 16;
+                    p.part16[qb + base[b] + (i - start[b])] = (uint16_t)(e & 0xFFFFu);
+                }
+                lds_barrier();  // stage, hist and the cursors are reused by the next tile
+            }
+        }
+    }
+}
+
+// Dynamic LDS: 2^log_shift u32 counters, a bitmap of the nonzero ones, then
+// the query's entry count and first entry per bucket (2 * log_nb words).
+// The counters are zero between buckets: a bucket sets the counters and bits
+// of its hub values, source visits and entries, and the output walks the
+// bitmap in id order (one word per thread), reads and clears each set
+// counter and writes its pair, so a bucket costs its entries, not its span.
+// 512-thread workgroups: two fit a CU, and one's barriers overlap the
+// other's work.
+constexpr uint32_t kLogCountBlock = 512;
+static_assert((1u << kBatchLogShift) / 32 == kLogCountBlock, "one bitmap word per thread");
+__global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLaunch p)
+{
+    extern __shared__ uint32_t lc_lds[];
+    __shared__ uint32_t s_hist[kCountBins];
+    __shared__ uint32_t s_max;
+    __shared__ unsigned long long s_sums[2];
+    __shared__ unsigned long long s_wave[16];
+    __shared__ uint32_t s_take;
+    __shared__ unsigned long long s_clogc[kClogcTable];
+    fill_clogc_table(s_clogc);
+    const uint32_t nb = p.log_nb, shift = p.log_shift;
+    const uint32_t span = 1u << shift;
+    const uint32_t off_mask = span - 1;
+    uint32_t *const cnt = lc_lds;
+    uint32_t *const bits = lc_lds + span;
+    uint32_t *const b_cnt = bits + span / 32;
+    uint32_t *const b_loc = b_cnt + nb;
+    for (uint32_t i = threadIdx.x; i < span + span / 32; i += blockDim.x) lc_lds[i] = 0;
+    for (uint32_t b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
+    if (threadIdx.x == 0) {
+        s_max = 0;
+        s_sums[0] = 0;
+        s_sums[1] = 0;
+    }
+    LaneStats ls;
+    const uint4 *part4 = reinterpret_cast<const uint4 *>(p.part16);
+    auto mark = [&](uint32_t i, uint32_t c) {
+        atomicAdd(&cnt[i], c);
+        atomicOr(&bits[i >> 5], 1u << (i & 31));
+    };
+    for (;;) {
+        const uint32_t q = take_logged_query(p, 4, &s_take);
+        if (q == kEmpty) break;
+        const uint32_t src = p.queries[q].source_dev;
+        const uint32_t hub_len = p.queries[q].hub_len;
+        const uint64_t hub_off = p.queries[q].hub_off;
+        const uint64_t touched_off = p.queries[q].touched_off;
+        const uint32_t src_visits = p.qlog[(uint64_t)q * kBatchLogQWords + 4];
+        const uint64_t qb = p.q_base[q];
+        const uint64_t row = (uint64_t)q * nb;
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+            b_cnt[b] = p.part_cnt[row + b];
+            b_loc[b] = p.part_loc[row + b];
+        }
+        lds_barrier();
+        uint32_t pairs = 0;  // workgroup-uniform: pairs written so far
+        for (uint32_t b = 0; b < nb; ++b) {
+            const uint64_t pi = row + b;
+            const uint32_t lo = b << shift;
+            const uint32_t hi = p.V - lo < span ? p.V : lo + span;
+            const uint32_t n = b_cnt[b];
+            const bool src_in = src >= lo && src < hi;
+            if (n == 0 && lo >= hub_len && !src_in) {
+                if (threadIdx.x == 0) {
+                    p.run_loc[pi] = pairs;
+                    p.run_len[pi] = 0;
+                }
+                continue;  // workgroup-uniform
+            }
+            // the bucket's entries [e0, e0 + n), as aligned 16 B chunks (the
+            // partition buffer has 8 entries of padding); the first chunk of
+            // each thread is loaded ahead
+            const uint64_t e0 = qb + b_loc[b];
+            const uint64_t a0 = e0 & ~7ull;
+            const uint32_t chunks = (uint32_t)((e0 + n - a0 + 7) >> 3);
+            uint4 first = uint4{0u, 0u, 0u, 0u};
+            if (threadIdx.x < chunks) first = part4[(a0 >> 3) + threadIdx.x];
+            // hub counters [lo, hub_end) of the bucket (the log holds none of
+            // those ids, the source's visits are in src_visits)
+            const uint32_t hub_end = hub_len < hi ? hub_len : hi;
+            for (uint32_t i = threadIdx.x; lo + i < hub_end; i += blockDim.x) {
+                const uint32_t c = p.dense[hub_off + lo + i];
+                if (c) mark(i, c);
+            }
+            if (src_in && threadIdx.x == 0 && src_visits) mark(src - lo, src_visits);
+            for (uint32_t j = threadIdx.x; j < chunks; j += blockDim.x) {
+                const uint4 v = j == threadIdx.x ? first : part4[(a0 >> 3) + j];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+                const uint64_t at = a0 + 8ull * j;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint64_t idx = at + e;
+                    if (idx >= e0 && idx < e0 + n) mark(((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu) & off_mask, 1u);
+                }
+            }
+            lds_barrier();
+            // output: thread t owns bitmap word t (ids lo + 32t ...), in id order
+            uint32_t word = bits[threadIdx.x];
+            uint64_t total = 0;
+            const uint32_t at = (uint32_t)block_excl_scan_lds((uint32_t)__popc(word), s_wave, total);
+            uint2 *out = p.touched + touched_off + pairs + at;
+            while (word) {
+                const uint32_t i = 32 * threadIdx.x + (uint32_t)(__ffs(word) - 1);
+                word &= word - 1;
+                const uint32_t c = cnt[i];
+                cnt[i] = 0;
+                *out++ = uint2{lo + i, c};
+                ls.add(c, s_hist, s_clogc);
+            }
+            bits[threadIdx.x] = 0;
+            if (threadIdx.x == 0) {
+                p.run_loc[pi] = pairs;
+                p.run_len[pi] = (uint32_t)total;
+            }
+            pairs += (uint32_t)total;
+            lds_barrier();  // the counters and bits are clear for the next bucket
+        }
+        if (threadIdx.x == 0) p.touched_count[q] = pairs;
+        flush_stats(ls, s_hist, &s_max, s_sums, p.tk_stats + q);
+    }
+}
+
 // ---- top-k -------------------------------------------------------------------
 
 struct Entries {
@@ -786,7 +1290,7 @@ __device__ __forceinline__ Entries entries_of(const BatchLaunch &p, uint32_t q)
     E.Q = p.queries[q];
     E.V = p.V;
     const uint32_t t = p.touched_count[q];
-    if (t <= E.Q.touched_cap) {
+    if (t <= E.Q.touched_cap || E.Q.kind == SLICE_LOG) {  // a logged list fits by construction
         E.mode = 0;
         E.n = t;
     } else if (E.Q.kind == SLICE_DENSE) {
@@ -1192,7 +1696,7 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(BatchLaunch p)
                 for (uint32_t i = (lo >> 2) + threadIdx.x; i < (hi >> 2); i += blockDim.x)
                     __builtin_nontemporal_store(z, &d4[i]);
             }
-        } else {
+        } else {  // hash or logged: the hub slice, then a hash query's table
             if ((uint64_t)seg * kSeg < Q.hub_len) {  // the hub slice, streamed
                 const uint32_t lo = seg * kSeg;
                 const uint32_t hi = lo + kSeg < Q.hub_len ? lo + kSeg : Q.hub_len;  // multiples of 64
@@ -1201,6 +1705,7 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(BatchLaunch p)
                 for (uint32_t i = (lo >> 2) + threadIdx.x; i < (hi >> 2); i += blockDim.x)
                     __builtin_nontemporal_store(z, &d4[i]);
             }
+            if (Q.kind == SLICE_LOG) continue;
             // tables hold a power of two >= 2 slots from an even slot: 16 B
             // stores of slot pairs are aligned
             const uint64_t cap = (uint64_t)Q.mask + 1;
@@ -1226,6 +1731,19 @@ __global__ void batch_gather_kernel(BatchLaunch p, uint64_t n, const uint32_t *q
         uint64_t c = 0;
         if (Q.kind == SLICE_DENSE) {
             c = p.dense[Q.slice_off + v];
+        } else if (Q.kind == SLICE_LOG) {
+            // binary search in the run of v's bucket (ids ascending)
+            const uint64_t pi = (uint64_t)qidx[i] * p.log_nb + (v >> p.log_shift);
+            const uint2 *run = p.touched + Q.touched_off + p.run_loc[pi];
+            uint32_t lo = 0, hi = p.run_len[pi];
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if (run[mid].x < v)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < p.run_len[pi] && run[lo].x == v) c = run[lo].y;
         } else if (v < Q.hub_len) {
             c = p.dense[Q.hub_off + v];
         } else {
@@ -1255,6 +1773,12 @@ __global__ void batch_export_kernel(BatchLaunch p, uint32_t q, const uint32_t *p
         for (uint64_t g = t0; g < p.V; g += stride) out[g] = p.dense[Q.slice_off + (perm ? perm[g] : g)];
     } else if (phase == 0) {
         for (uint64_t g = t0; g < p.V; g += stride) out[g] = 0;
+    } else if (Q.kind == SLICE_LOG) {
+        const uint32_t n = p.touched_count[q];
+        for (uint64_t i = t0; i < n; i += stride) {
+            const uint2 e = p.touched[Q.touched_off + i];
+            out[inv ? inv[e.x] : e.x] = e.y;
+        }
     } else {
         for (uint64_t i = t0; i <= Q.mask; i += stride) {
             const uint2 s = p.hash[Q.slice_off + i];
@@ -1281,13 +1805,15 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int walk_cus, int lds_hubs,
                              int cache_slots, hipStream_t s)
 {
-    const size_t dlds = (((size_t)lds_hubs + 1) / 2 + 2 * (size_t)cache_slots) * 4;
+    const uint32_t nb = p.vlog ? p.log_nb : 0u;
+    const uint32_t hubs = p.vlog ? batch_log_hubs(lds_hubs, nb) : (uint32_t)lds_hubs;
+    const uint32_t slots = p.vlog ? 0u : (uint32_t)cache_slots;
+    const size_t dlds = (((size_t)hubs + 1) / 2 + 2 * (size_t)slots + nb) * 4;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_dense_kernel, block, dlds) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
-    hipLaunchKernelGGL(batch_dense_kernel, dim3(per_cu * walk_cus), dim3(block), dlds, s, p, (uint32_t)lds_hubs,
-                       (uint32_t)cache_slots);
+    hipLaunchKernelGGL(batch_dense_kernel, dim3(per_cu * walk_cus), dim3(block), dlds, s, p, hubs, slots, nb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t lds = (size_t)(block / 64) * (kWaveSlots * 12 + kWaveHubs * 4);
@@ -1297,6 +1823,25 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int w
     hipLaunchKernelGGL(batch_walk_kernel, dim3(per_cu * walk_cus), dim3(block), lds, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (p.vlog) {
+        hipLaunchKernelGGL(batch_log_rows_kernel, dim3(num_cus * 2), dim3(1024), 0, s, p);
+        hipLaunchKernelGGL(batch_log_plan_kernel, dim3(1), dim3(1024), 0, s, p);
+        hipLaunchKernelGGL(batch_log_seglist_kernel, dim3(num_cus * 4), dim3(256), 0, s, p);
+        // one query per workgroup at a time: grids of what fits resident
+        const size_t lp_lds = ((size_t)kBatchLogTile * kBatchLogSeg + 5 * (size_t)nb + 2 * kPartSegBatch) * 4;
+        const size_t lc_lds = ((size_t)1 << p.log_shift) * 4 + ((size_t)1 << p.log_shift) / 8 + 8 * (size_t)nb;
+        int lp = 0, lc = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lp, batch_log_partition_kernel, kBatchLogSeg, lp_lds) !=
+                hipSuccess || lp < 1)
+            lp = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lc, batch_log_count_kernel, kLogCountBlock, lc_lds) !=
+                hipSuccess || lc < 1)
+            lc = 1;
+        hipLaunchKernelGGL(batch_log_partition_kernel, dim3(num_cus * lp), dim3(kBatchLogSeg), lp_lds, s, p);
+        hipLaunchKernelGGL(batch_log_count_kernel, dim3(num_cus * lc), dim3(kLogCountBlock), lc_lds, s, p);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(batch_compact_kernel, dim3(num_cus * 8), dim3(kCompactBlock), 0, s, p);
     return hipGetLastError();
 }

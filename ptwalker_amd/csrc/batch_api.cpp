@@ -36,12 +36,39 @@ void drop_exec(BatchState &b)
     b.exec = nullptr;
 }
 
+void free_log(BatchState &b)
+{
+    hfree(b.vlog);
+    hfree(b.seg_q);
+    hfree(b.seg_n);
+    hfree(b.seg_list);
+    hfree(b.part16);
+    hfree(b.qlog);
+    hfree(b.log_order);
+    hfree(b.q_base);
+    hfree(b.part_cnt);
+    hfree(b.part_loc);
+    hfree(b.part_cur);
+    hfree(b.run_loc);
+    hfree(b.run_len);
+    b.log_segs = b.log_q_cap = b.log_rows = 0;
+    b.log_shift = b.log_nb = 0;
+}
+
 struct Plan {
     std::vector<BatchQuery> q;
     std::vector<BatchChunk> chunks;  // workgroup chunks of dense queries, then wavefront chunks of hash queries
     uint64_t dense = 0, hash = 0, touched = 0, cand = 0;  // elements needed
     uint32_t n_dense = 0, n_hash = 0;
     uint64_t dense_chunks = 0;
+    // dense queries are logged (SLICE_LOG): buckets of 2^log_shift ids, log_nb of them
+    bool log = false;
+    uint32_t log_shift = 0, log_nb = 0;
+    uint64_t log_entries = 0;  // expected log entries, 1.25x (the pair lists' rule)
+    uint64_t log_hubs = 0;     // hub counters + 1 per logged query
+    uint64_t log_segs = 0;     // segments the log may claim
+    std::vector<uint32_t> log_order;  // logged queries, largest expected log first
+    uint64_t touched_total() const { return touched + (log ? log_segs * kBatchLogSeg + log_hubs : 0); }
 };
 
 // Walker ids per workgroup chunk of a dense query: about 1/48 of the
@@ -56,11 +83,18 @@ uint64_t dense_chunk_size(uint64_t walkers)
     return std::min<uint64_t>(std::max<uint64_t>(c, 16384), 49152);
 }
 
-int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
+int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P, bool allow_log = true)
 {
     const uint32_t V = ctx->dev.V;
     const uint64_t thr_dense = d.dense_threshold ? d.dense_threshold : 100000ull;
     const uint64_t chunk = (uint64_t)ctx->shape.batch_chunk;
+    // dense queries are logged unless visit_log is 0 or the graph needs more
+    // id buckets than the log's kernels hold
+    P.log = allow_log && ctx->shape.visit_log && V && ((uint64_t)(V - 1) >> kBatchLogShift) < kBatchLogMaxBuckets;
+    P.log_shift = P.log ? kBatchLogShift : 0u;
+    P.log_nb = P.log ? ((V - 1) >> kBatchLogShift) + 1 : 0u;
+    const uint32_t log_hub_len =
+        P.log ? std::min<uint32_t>(batch_log_hubs(ctx->shape.lds_hubs, P.log_nb), dense_pitch(V)) : 0u;
     P.q.resize(d.num_queries);
     for (uint32_t i = 0; i < d.num_queries; ++i) {
         const ptw_query &qq = d.queries[i];
@@ -85,7 +119,17 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         B.ends_only = ends ? 1u : 0u;
         B.pad = 0;
         const SliceSizes z = batch_slice_sizes(V, qq.walkers, qq.tp, thr_dense, ends);
-        if (z.dense) {
+        if (z.dense && P.log) {
+            // no slice: a hub slice (below) and the log, whose pair list
+            // the device places after the walk
+            B.kind = SLICE_LOG;
+            B.slice_off = 0;
+            B.mask = 0;
+            ++P.n_dense;
+            P.log_entries += (uint64_t)(1.25 * (double)qq.walkers * per_walker) + 64;
+            P.log_hubs += (uint64_t)log_hub_len + 1;
+            P.log_order.push_back(i);
+        } else if (z.dense) {
             B.kind = SLICE_DENSE;
             B.slice_off = P.dense;
             P.dense += z.slots;
@@ -99,12 +143,12 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
             ++P.n_hash;
         }
         // a multiple of 64 counters, like every dense slice: 256 B aligned
-        B.hub_len = z.hub_len;
+        B.hub_len = B.kind == SLICE_LOG ? log_hub_len : z.hub_len;
         B.hub_off = P.dense;
-        P.dense += z.hub_len;
+        P.dense += B.hub_len;
         B.touched_cap = z.touched_cap;
         B.touched_off = P.touched;
-        P.touched += z.touched_cap;
+        if (B.kind != SLICE_LOG) P.touched += z.touched_cap;
         // top-k candidates: no more than the query's distinct vertices while
         // its pair list holds them all, and never more than the LDS tile
         B.cand_cap = d.topk ? std::min<uint32_t>((uint32_t)ctx->shape.topk_tile, std::max<uint32_t>(z.touched_cap, 1))
@@ -113,15 +157,27 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P)
         P.cand += B.cand_cap;
     }
     for (uint32_t i = 0; i < d.num_queries; ++i) {
-        if (P.q[i].kind != SLICE_DENSE) continue;
+        if (P.q[i].kind == SLICE_HASH) continue;
         const uint64_t dchunk =
             ctx->shape.dense_chunk ? (uint64_t)ctx->shape.dense_chunk : dense_chunk_size(P.q[i].walkers);
         for (uint64_t s = 0; s < P.q[i].walkers; s += dchunk)
             P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(dchunk, P.q[i].walkers - s), s});
     }
     P.dense_chunks = P.chunks.size();
+    if (P.log) {
+        // largest expected log first, so the last queries counted are small ones
+        std::stable_sort(P.log_order.begin(), P.log_order.end(), [&](uint32_t x, uint32_t y) {
+            const double cx = (double)d.queries[x].walkers * batch_counts_per_walker(d.queries[x].tp, P.q[x].ends_only != 0);
+            const double cy = (double)d.queries[y].walkers * batch_counts_per_walker(d.queries[y].tp, P.q[y].ends_only != 0);
+            return cx > cy;
+        });
+        P.log_segs = batch_log_segments(P.log_entries, P.dense_chunks, ctx->shape.block, ctx->num_cus);
+        // visit_log = N > 1 caps the log at N entries (a full log reruns the batch with slices)
+        if (ctx->shape.visit_log > 1)
+            P.log_segs = std::min<uint64_t>(P.log_segs, (uint64_t)ctx->shape.visit_log / kBatchLogSeg);
+    }
     for (uint32_t i = 0; i < d.num_queries; ++i) {
-        if (P.q[i].kind == SLICE_DENSE) continue;
+        if (P.q[i].kind != SLICE_HASH) continue;
         for (uint64_t s = 0; s < P.q[i].walkers; s += chunk)
             P.chunks.push_back(BatchChunk{i, (uint32_t)std::min<uint64_t>(chunk, P.q[i].walkers - s), s});
     }
@@ -133,7 +189,13 @@ uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint
     uint64_t need = 0;
     if (P.dense > b.dense_cap) need += 4 * P.dense;
     if (P.hash > b.hash_cap) need += 8 * P.hash;
-    if (P.touched > b.touched_cap) need += 8 * P.touched;
+    if (P.touched_total() > b.touched_cap) need += 8 * P.touched_total();
+    if (P.log) {
+        if (P.log_segs > b.log_segs) need += P.log_segs * (kBatchLogSeg * 6 + 12);
+        const uint64_t rows = (uint64_t)nq * P.log_nb;
+        if (nq > b.log_q_cap) need += (uint64_t)nq * (4 * kBatchLogQWords + 8);
+        if (rows > b.log_rows) need += rows * 20;
+    }
     if (P.cand > b.cand_cap) need += 8 * P.cand;
     if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + sizeof(TopkStats) + 24);
     if (P.chunks.size() > b.chunk_cap) need += P.chunks.size() * sizeof(BatchChunk);
@@ -163,6 +225,24 @@ BatchLaunch launch_args(ptw_ctx *ctx, const BatchState &b)
     p.queue = b.queue;
     p.tk_stats = b.tk_stats;
     p.tk_cand = b.tk_cand;
+    if (b.vlog) {
+        p.vlog = b.vlog;
+        p.log_segs = b.log_segs;
+        p.seg_q = b.seg_q;
+        p.seg_n = b.seg_n;
+        p.seg_list = b.seg_list;
+        p.qlog = b.qlog;
+        p.log_order = b.log_order;
+        p.q_base = b.q_base;
+        p.log_shift = b.log_shift;
+        p.log_nb = b.log_nb;
+        p.part_cnt = b.part_cnt;
+        p.part_loc = b.part_loc;
+        p.part_cur = b.part_cur;
+        p.run_loc = b.run_loc;
+        p.run_len = b.run_len;
+        p.part16 = b.part16;
+    }
     return p;
 }
 
@@ -172,9 +252,64 @@ int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k
     BatchState &b = L.b;
     bool moved = false;
     if (!b.hdr) {
-        HIP_TRY(ctx, hipMalloc((void **)&b.hdr, 8 * sizeof(unsigned long long)));
-        HIP_TRY(ctx, hipMalloc((void **)&b.queue, 2 * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMalloc((void **)&b.hdr, kBatchHdrWords * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMalloc((void **)&b.queue, 8 * sizeof(unsigned long long)));
         moved = true;
+    }
+    if (!P.log && b.vlog) {
+        // a plan without logged queries: the launch must not run the log's kernels
+        free_log(b);
+        moved = true;
+    }
+    if (P.log) {
+        if (P.log_segs > b.log_segs) {
+            hfree(b.vlog);
+            hfree(b.seg_q);
+            hfree(b.seg_n);
+            hfree(b.seg_list);
+            hfree(b.part16);
+            const uint64_t cap = std::max<uint64_t>(P.log_segs, b.log_segs + b.log_segs / 2);
+            HIP_TRY(ctx, hipMalloc((void **)&b.vlog, cap * kBatchLogSeg * 4));
+            // 8 entries of padding: the counting reads aligned 16 B chunks
+            HIP_TRY(ctx, hipMalloc((void **)&b.part16, cap * kBatchLogSeg * 2 + 16));
+            HIP_TRY(ctx, hipMalloc((void **)&b.seg_q, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.seg_n, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.seg_list, cap * 4));
+            b.log_segs = cap;
+            moved = true;
+        }
+        if (nq > b.log_q_cap || !b.qlog) {
+            hfree(b.qlog);
+            hfree(b.q_base);
+            hfree(b.log_order);
+            const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(nq, 1), b.log_q_cap + b.log_q_cap / 2);
+            HIP_TRY(ctx, hipMalloc((void **)&b.qlog, cap * kBatchLogQWords * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.log_order, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.q_base, cap * 8));
+            b.log_q_cap = cap;
+            moved = true;
+        }
+        const uint64_t rows = std::max<uint64_t>((uint64_t)nq * P.log_nb, 1);
+        if (rows > b.log_rows || !b.part_cnt) {
+            hfree(b.part_cnt);
+            hfree(b.part_loc);
+            hfree(b.part_cur);
+            hfree(b.run_loc);
+            hfree(b.run_len);
+            const uint64_t cap = std::max<uint64_t>(rows, b.log_rows + b.log_rows / 2);
+            HIP_TRY(ctx, hipMalloc((void **)&b.part_cnt, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.part_loc, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.part_cur, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.run_loc, cap * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.run_len, cap * 4));
+            b.log_rows = cap;
+            moved = true;
+        }
+        if (b.log_shift != P.log_shift || b.log_nb != P.log_nb) {
+            b.log_shift = P.log_shift;
+            b.log_nb = P.log_nb;
+            moved = true;
+        }
     }
     if (P.dense > b.dense_cap) {
         hfree(b.dense);
@@ -193,9 +328,10 @@ int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k
         b.hash_cap = cap;
         moved = true;
     }
-    if (P.touched > b.touched_cap || !b.touched) {
+    if (P.touched_total() > b.touched_cap || !b.touched) {
         hfree(b.touched);
-        const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(P.touched, 1), b.touched_cap + b.touched_cap / 2);
+        const uint64_t cap =
+            std::max<uint64_t>(std::max<uint64_t>(P.touched_total(), 1), b.touched_cap + b.touched_cap / 2);
         HIP_TRY(ctx, hipMalloc((void **)&b.touched, cap * 8));
         b.touched_cap = cap;
         moved = true;
@@ -269,6 +405,7 @@ int clean_previous(ptw_ctx *ctx, LaneRef L)
 void free_state(BatchState &b)
 {
     drop_exec(b);
+    free_log(b);
     hfree(b.dense);
     hfree(b.hash);
     hfree(b.touched);
@@ -292,7 +429,7 @@ void free_state(BatchState &b)
 
 }  // namespace
 
-int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
+int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d, bool allow_log)
 {
     if (!ctx->has_graph) return fail(ctx, PTW_ERR_NO_GRAPH, "no graph loaded");
     if (ctx->device < 0) return fail(ctx, PTW_ERR_UNSUPPORTED, "host-only context: batches need a HIP device");
@@ -306,7 +443,7 @@ int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     BatchState &b = L.b;
     Plan P;
-    int rc = make_plan(ctx, d, P);
+    int rc = make_plan(ctx, d, P, allow_log);
     if (rc) return rc;
     // admission: what the plan needs beyond the pools must fit in free HBM
     const uint64_t extra = plan_bytes_beyond(b, P, d.num_queries, d.topk);
@@ -332,6 +469,10 @@ int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
     b.h_hdr[4] = (unsigned long long)d.semantics;
     b.h_hdr[5] = weighted ? 1 : 0;
     b.h_hdr[6] = P.dense_chunks;
+    b.h_hdr[7] = P.touched;  // logged queries' pair lists follow the others
+    b.h_hdr[8] = P.log_segs;
+    b.h_hdr[9] = P.log_order.size();
+    b.h_log_order = std::move(P.log_order);
     b.last_nq = d.num_queries;
     b.weighted = weighted;
     hipStream_t s = L.s;
@@ -340,6 +481,9 @@ int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d)
                                     hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(b.chunks, b.h_chunks.data(), b.h_chunks.size() * sizeof(BatchChunk),
                                     hipMemcpyHostToDevice, s));
+        if (!b.h_log_order.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(b.log_order, b.h_log_order.data(), b.h_log_order.size() * 4,
+                                        hipMemcpyHostToDevice, s));
     }
     HIP_TRY(ctx, hipMemcpyAsync(b.hdr, b.h_hdr, sizeof(b.h_hdr), hipMemcpyHostToDevice, s));
     b.dirty = true;
@@ -387,7 +531,8 @@ int batch_enqueue_results(ptw_ctx *ctx, LaneRef L, uint32_t nq, uint32_t k, unsi
 int batch_check(ptw_ctx *ctx, const ptw_batch_desc &d, const unsigned long long *steps, const uint32_t *over)
 {
     for (uint32_t i = 0; i < d.num_queries; ++i) {
-        if (over[i]) return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice");
+        if (over[i])
+            return fail(ctx, PTW_ERR_OOM, "query " + std::to_string(i) + " overflowed its hash slice or the visit log");
         const uint64_t total_counts =
             d.queries[i].walkers + (d.queries[i].kind == PTW_QUERY_ENDPOINTS ? 0 : steps[i]);
         if (total_counts >= 4294967296ull)
@@ -493,16 +638,29 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
     out->total_steps = 0;
     out->slice_bytes = 0;
     out->dense_queries = out->hash_queries = 0;
+    out->logged_queries = out->log_rerun = 0;
     out->kernel_ms = 0.0;
     const ptw::LaneRef L = ptw::main_lane(ctx);
-    int rc = ptw::batch_enqueue(ctx, L, d);
-    if (rc) return rc;
     std::vector<unsigned long long> steps(d.num_queries), tc((size_t)d.num_queries * d.topk);
     std::vector<uint32_t> over(d.num_queries);
-    if ((rc = ptw::batch_enqueue_results(ctx, L, d.num_queries, d.topk, steps.data(), over.data(), out->entropy,
-                                         d.topk ? out->topk_vertex : nullptr, tc.data())))
-        return rc;
-    HIP_TRY(ctx, hipStreamSynchronize(L.s));
+    auto run = [&](bool allow_log) -> int {
+        int r = ptw::batch_enqueue(ctx, L, d, allow_log);
+        if (r) return r;
+        if ((r = ptw::batch_enqueue_results(ctx, L, d.num_queries, d.topk, steps.data(), over.data(), out->entropy,
+                                            d.topk ? out->topk_vertex : nullptr, tc.data())))
+            return r;
+        HIP_TRY(ctx, hipStreamSynchronize(L.s));
+        return PTW_OK;
+    };
+    int rc = run(true);
+    if (rc) return rc;
+    bool logged = false;
+    for (const auto &q : L.b.h_queries) logged |= q.kind == ptw::SLICE_LOG;
+    if (logged && std::any_of(over.begin(), over.end(), [](uint32_t o) { return o != 0; })) {
+        // the visit log filled: the same batch with slices gives the same counts
+        if ((rc = run(false))) return rc;
+        out->log_rerun = 1;
+    }
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, L.ev0, L.ev1));
     out->kernel_ms = ms;
@@ -518,6 +676,12 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
         if (q.kind == ptw::SLICE_DENSE) {
             ++out->dense_queries;
             bytes += 4ull * ptw::dense_pitch(ctx->dev.V);
+        } else if (q.kind == ptw::SLICE_LOG) {
+            // hub slice, then per expected log entry: the entry, its partition place, a pair
+            ++out->dense_queries;
+            ++out->logged_queries;
+            bytes += 4ull * q.hub_len + 14ull * q.touched_cap;
+            continue;
         } else {
             ++out->hash_queries;
             bytes += 8ull * ((uint64_t)q.mask + 1) + 4ull * q.hub_len;

@@ -35,14 +35,25 @@ struct BatchState {
     uint32_t *topk_v = nullptr;
     unsigned long long *topk_c = nullptr;
     uint64_t topk_cap = 0;
-    unsigned long long *hdr = nullptr;    // chunks, queries, k, seed, semantics, weighted
-    unsigned long long *queue = nullptr;  // chunk queue head
+    unsigned long long *hdr = nullptr;    // chunks, queries, k, seed, semantics, weighted, dense chunks, log pairs
+    unsigned long long *queue = nullptr;  // chunk queue heads, log segment head, log tasks
+    // visit log of logged dense queries (BatchLaunch::vlog and the arrays after it)
+    uint32_t *vlog = nullptr, *seg_q = nullptr, *seg_n = nullptr, *seg_list = nullptr;
+    uint16_t *part16 = nullptr;
+    uint64_t log_segs = 0;  // segments allocated
+    uint32_t *qlog = nullptr, *log_order = nullptr;
+    unsigned long long *q_base = nullptr;
+    uint32_t *part_cnt = nullptr, *part_loc = nullptr, *part_cur = nullptr, *run_loc = nullptr, *run_len = nullptr;
+    uint64_t log_q_cap = 0;   // queries the per-query log arrays hold
+    uint64_t log_rows = 0;    // (query, bucket) entries the per-bucket arrays hold
+    uint32_t log_shift = 0, log_nb = 0;  // of the last plan with logged queries
     bool dirty = false;                   // the last batch's slices are not yet cleaned
     bool weighted = false;
     uint32_t last_nq = 0;
     std::vector<BatchQuery> h_queries;
     std::vector<BatchChunk> h_chunks;
-    unsigned long long h_hdr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> h_log_order;  // logged queries, largest expected log first
+    unsigned long long h_hdr[kBatchHdrWords] = {};
     // (query, vertex) gathers of the last batch, grown and kept
     uint32_t *g_q = nullptr, *g_v = nullptr;
     uint64_t *g_out = nullptr;
@@ -149,7 +160,8 @@ inline LaneRef main_lane(ptw_ctx *ctx) { return LaneRef{ctx->batch, ctx->stream,
 
 // ptw_batch_ppr's checks, plan, admission and device work on a lane, queued
 // without a wait; the caller queues the result copies and waits.
-int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d);
+// allow_log = false plans dense queries with slices even when visit_log is on.
+int batch_enqueue(ptw_ctx *ctx, LaneRef L, const ptw_batch_desc &d, bool allow_log = true);
 // Queues the copies of the lane's last batch results into host memory
 // (page-locked for an asynchronous copy); any pointer may be null.
 int batch_enqueue_results(ptw_ctx *ctx, LaneRef L, uint32_t nq, uint32_t k, unsigned long long *steps,

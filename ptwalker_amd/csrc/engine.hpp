@@ -259,7 +259,40 @@ hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, cons
                                       uint32_t extra_vertex, uint64_t extra, hipStream_t s);
 // ---- batched queries (batch.hip) -------------------------------------------
 
-enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1 };
+// SLICE_LOG: a dense query counted without a V-sized slice.  Its visits to
+// the LDS hub range go to a hub slice of hub_len counters; every other visit
+// is appended to the batch's visit log in segments of kBatchLogSeg entries,
+// each segment holding visits of one query.  After the walk the log is
+// partitioned by (query, id bucket) and every bucket is counted in LDS
+// straight into the query's (vertex, count) pair list.
+enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1, SLICE_LOG = 2 };
+
+constexpr uint32_t kBatchLogSeg = 1024;     // entries per wavefront segment
+constexpr uint32_t kBatchLogTile = 8;       // segments per partition tile
+constexpr uint32_t kBatchLogShift = 14;     // id bucket = device id >> kBatchLogShift
+constexpr uint32_t kBatchLogMaxBuckets = 4096;
+constexpr uint32_t kBatchLogQWords = 8;     // per-query words of the log plan (BatchLaunch::qlog)
+constexpr uint32_t kBatchHdrWords = 16;     // BatchLaunch::hdr
+// Workgroup LDS of the dense kernel (160 KiB less its static variables).
+constexpr uint32_t kDenseLdsBytes = 163840 - 256;
+
+// Hub counters a logged dense launch keeps for lds_hubs requested and nb
+// bucket counters beside them: a multiple of 64 that fits the LDS.
+inline uint32_t batch_log_hubs(int lds_hubs, uint32_t nb)
+{
+    const uint64_t words = (kDenseLdsBytes - 4ull * nb) / 4;
+    uint64_t h = std::min<uint64_t>((uint64_t)std::max(lds_hubs, 0), 2 * words);
+    return (uint32_t)(h & ~63ull);
+}
+// Log segments a batch of `entries` expected log entries may claim: every
+// segment is full except those a wavefront abandons when its workgroup
+// changes query or ends (at most one per wavefront per dense chunk, plus
+// one per resident wavefront, of which a CU holds at most 32).
+inline uint64_t batch_log_segments(uint64_t entries, uint64_t dense_chunks, int block, int num_cus)
+{
+    return (entries + kBatchLogSeg - 1) / kBatchLogSeg + dense_chunks * (uint64_t)(block / 64) +
+           32ull * (uint64_t)num_cus + 1;
+}
 
 // Dense slices are laid out every dense_pitch(V) counters: a multiple of 64,
 // so every slice starts on a 256 B boundary and can be read with 16 B loads.
@@ -326,6 +359,8 @@ struct BatchQuery {
     uint64_t hub_off;      // hash: u32 offset of the hub slice in the dense pool
     uint32_t ends_only;    // 1: count each walker's last vertex only (PTW_QUERY_ENDPOINTS)
     uint32_t pad;
+    // SLICE_LOG: hub_len / hub_off give its hub slice; touched_off and
+    // touched_cap are set on the device once the log's size is known
 };
 
 
@@ -382,7 +417,8 @@ struct BatchLaunch {
     // per-batch values live on the device so one captured graph serves every
     // batch: hdr[0] chunks, [1] queries, [2] top-k length, [3] seed,
     // [4] semantics, [5] weighted, [6] workgroup chunks of dense queries (they
-    // come first in the chunk table; the wavefront chunks of hash queries follow)
+    // come first in the chunk table; the wavefront chunks of hash queries
+    // follow), [7] and [8] the visit log's (below); kBatchHdrWords words
     const unsigned long long *hdr;
     uint32_t *dense;         // dense slice pool (u32 counters, device ids)
     uint2 *hash;             // hash table pool {key, count}
@@ -391,9 +427,31 @@ struct BatchLaunch {
     uint32_t *overflow;      // per query: hash insertions that found no slot
     unsigned long long *steps;  // per query
     double *entropy;            // per query, written by the top-k kernel
-    unsigned long long *queue;  // [0]: next wavefront chunk, [1]: next workgroup chunk
+    // [0]: next wavefront chunk, [1]: next workgroup chunk, [2]: next log
+    // segment, [3]: next logged query to partition, [4]: to count
+    unsigned long long *queue;
     TopkStats *tk_stats;        // per query
     unsigned long long *tk_cand;  // candidate pool: (count << 32 | ~device id)
+    // visit log of SLICE_LOG queries (vlog == nullptr: the batch has none);
+    // hdr[7] is the first pair of their lists in the touched pool, hdr[8] the
+    // segments this batch may claim (at most log_segs), hdr[9] the number of
+    // logged queries, counted in the order of log_order
+    uint32_t *vlog;              // log_segs segments of kBatchLogSeg device ids
+    uint64_t log_segs;
+    uint32_t *seg_q;             // per segment: its query
+    uint32_t *seg_n;             // per segment: entries written
+    uint32_t *seg_list;          // segment ids grouped by query (after the walk)
+    // per query, kBatchLogQWords words: [0] segments, [1] segment cursor,
+    // [2] first of its segments in seg_list, [4] visits to the source,
+    // [5] log entries
+    uint32_t *qlog;
+    const uint32_t *log_order;   // logged query indices, largest expected log first
+    unsigned long long *q_base;  // per query: first entry of its partition
+    uint32_t log_shift, log_nb;  // id bucket = device id >> log_shift; log_nb buckets
+    // per (query, bucket): entries, first entry in the query's partition,
+    // scatter cursor, first pair of its run in the pair list, run length
+    uint32_t *part_cnt, *part_loc, *part_cur, *run_loc, *run_len;
+    uint16_t *part16;            // partitioned entries: device id - bucket start
 };
 
 // Zeroes the per-query counters and the chunk queue of the batch in hdr.
@@ -403,6 +461,9 @@ hipError_t launch_batch_begin(const BatchLaunch &p, uint32_t max_queries, hipStr
 // of every query's slice into (vertex, count) pairs and count statistics.
 // The persistent walk kernels fill walk_cus CUs; the compaction grid is
 // sized for num_cus.
+// With p.vlog, dense queries are SLICE_LOG: the dense kernel keeps lds_hubs
+// hub counters (batch_log_hubs) and no cache, and the log is partitioned and
+// counted before the compaction of the hash queries.
 hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int walk_cus, int lds_hubs,
                              int cache_slots, hipStream_t s);
 // Per-query entropy and top-k (count desc, device id asc) into out_v (graph

@@ -122,7 +122,85 @@ def test_batch_weighted(ctx):
         assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
 
 
-def test_dense_hub_counter_spill(ctx):
+@pytest.fixture
+def visit_log(ctx, request):
+    """Runs the test with dense queries logged (1) or in dense slices (0)."""
+    t = ctx.tuning()
+    ctx.set_tuning(visit_log=request.param)
+    yield request.param
+    ctx.set_tuning(visit_log=t.visit_log)
+
+
+@pytest.fixture(scope="module")
+def g16(ctx):
+    gi = ctx.rmat(scale=16, edge_factor=16, seed=5)
+    rp, col = ctx.export_csr()
+    return gi, rp, col
+
+
+@pytest.mark.parametrize("hubs", [512, 77824])
+def test_logged_and_slice_batches_agree(ctx, g16, hubs):
+    # the same mixed batch (visit and endpoint queries, dense and hash) with
+    # dense queries logged and with dense slices: every output equal, and the
+    # logged run equal to the replay.  512 hub counters leave most visits to
+    # the log (several id buckets on this graph); 77824 cover every vertex.
+    gi, rp, col = g16
+    ctx.from_csr(rp, col)
+    n = 24
+    src, W, tp = make_queries(gi, n, seed=21, wmin=1000, wmax=400000)
+    kinds = np.where(np.arange(n) % 3 == 2, ptw.QUERY_ENDPOINTS, ptw.QUERY_VISITS)
+    t = ctx.tuning()
+    res = {}
+    try:
+        for vl in (1, 0):
+            ctx.set_tuning(visit_log=vl, lds_hubs=hubs)
+            r = ctx.batch_ppr(src, W, tp, topk=30, seed=8, kinds=kinds, dense_threshold=20000)
+            rng = np.random.default_rng(1)
+            qi = rng.integers(0, n, 4000)
+            vs = rng.integers(0, gi.num_vertices, 4000)
+            res[vl] = (r, [ctx.batch_counts(i) for i in range(n)], ctx.batch_gather(qi, vs), qi, vs)
+    finally:
+        ctx.set_tuning(visit_log=t.visit_log, lds_hubs=t.lds_hubs)
+    (a, ca, ga, qi, vs), (b, cb, gb, _, _) = res[1], res[0]
+    n_dense = int(np.sum(W > 20000))
+    assert n_dense >= 8
+    assert a.logged_queries == n_dense and a.dense_queries == n_dense and not a.log_rerun
+    assert b.logged_queries == 0 and b.dense_queries == n_dense
+    for i in range(n):
+        assert np.array_equal(ca[i], cb[i]), i
+    assert np.array_equal(a.topk_vertex, b.topk_vertex) and np.array_equal(a.topk_count, b.topk_count)
+    assert np.array_equal(a.steps, b.steps) and np.array_equal(a.entropy, b.entropy)
+    assert np.array_equal(ga, gb)
+    assert np.array_equal(ga, np.array([ca[q][v] for q, v in zip(qi, vs)], dtype=np.uint64))
+    for i in np.flatnonzero(W > 20000):
+        if kinds[i] == ptw.QUERY_ENDPOINTS:
+            ref = replay_endpoints(rp, col, int(src[i]), int(W[i]), float(tp[i]), int(i), seed=8)[0]
+        else:
+            ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), int(i), seed=8)["counts"]
+        assert np.array_equal(ca[i], ref), i
+
+
+def test_full_visit_log_reruns_with_slices(ctx, g16):
+    # a log of two segments cannot hold the dense queries' visits: the batch
+    # runs again with dense slices and its counts are still the replay's
+    gi, rp, col = g16
+    ctx.from_csr(rp, col)
+    src, W, tp = make_queries(gi, 6, seed=22, wmin=30000, wmax=100000)
+    t = ctx.tuning()
+    try:
+        ctx.set_tuning(visit_log=2048, lds_hubs=512)
+        r = ctx.batch_ppr(src, W, tp, topk=10, seed=4, dense_threshold=20000)
+        counts = [ctx.batch_counts(i) for i in range(6)]
+    finally:
+        ctx.set_tuning(visit_log=t.visit_log, lds_hubs=t.lds_hubs)
+    assert r.log_rerun and r.logged_queries == 0 and r.dense_queries == 6
+    for i in range(6):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=4)
+        assert np.array_equal(counts[i], ref["counts"]), i
+
+
+@pytest.mark.parametrize("visit_log", [1, 0], indirect=True)
+def test_dense_hub_counter_spill(ctx, visit_log):
     # star: hub 0 joined to 64 leaves; from a leaf every other visit is the hub,
     # so a 49152-walker chunk of one query takes about 1.2e5 hub visits and its
     # u16 LDS counter spills to the query's slice several times per chunk
@@ -161,8 +239,9 @@ def test_topk_tie_overflow_path(ctx):
         ctx.set_tuning(topk_tile=t.topk_tile)
 
 
+@pytest.mark.parametrize("visit_log", [1, 0], indirect=True)
 @pytest.mark.parametrize("dense_threshold", [1, 0])
-def test_pair_list_overflow_reads_slices(ctx, g12, dense_threshold):
+def test_pair_list_overflow_reads_slices(ctx, g12, dense_threshold, visit_log):
     # two or three walkers at tp = 0.01 take ~100 steps each, and the long
     # walks visit more distinct vertices than the pair list holds (1.25x the
     # expected visits + 64): top-k and the entropy of those queries read the
