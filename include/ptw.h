@@ -291,8 +291,9 @@ typedef struct ptw_batch_desc {
     uint64_t seed;
     int32_t semantics;        /* PTW_SEM_* */
     uint32_t flags;           /* PTW_BATCH_* */
-    uint64_t dense_threshold; /* walkers above this are dense (workgroup chunks, a dense slice or
-                                 the visit log); 0 = 100000 */
+    uint64_t dense_threshold; /* walkers above this are dense (workgroup chunks, the visit log or
+                                 a dense slice); 0 = 4096 when dense queries are logged
+                                 (visit_log != 0), 100000 with dense slices */
 } ptw_batch_desc;
 
 typedef struct ptw_batch_out {

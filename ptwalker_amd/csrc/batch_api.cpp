@@ -86,11 +86,12 @@ uint64_t dense_chunk_size(uint64_t walkers)
 int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P, bool allow_log = true)
 {
     const uint32_t V = ctx->dev.V;
-    const uint64_t thr_dense = d.dense_threshold ? d.dense_threshold : 100000ull;
     const uint64_t chunk = (uint64_t)ctx->shape.batch_chunk;
     // dense queries are logged unless visit_log is 0 or the graph needs more
     // id buckets than the log's kernels hold
-    P.log = allow_log && ctx->shape.visit_log && V && ((uint64_t)(V - 1) >> kBatchLogShift) < kBatchLogMaxBuckets;
+    P.log = allow_log && ctx->shape.visit_log && batch_log_fits(V);
+    const uint64_t thr_dense =
+        d.dense_threshold ? d.dense_threshold : P.log ? kDenseThresholdLogged : kDenseThresholdSlices;
     P.log_shift = P.log ? kBatchLogShift : 0u;
     P.log_nb = P.log ? ((V - 1) >> kBatchLogShift) + 1 : 0u;
     const uint32_t log_hub_len =
@@ -677,10 +678,11 @@ int ptw_batch_ppr(ptw_ctx *ctx, const ptw_batch_desc *desc, ptw_batch_out *out)
             ++out->dense_queries;
             bytes += 4ull * ptw::dense_pitch(ctx->dev.V);
         } else if (q.kind == ptw::SLICE_LOG) {
-            // hub slice, then per expected log entry: the entry, its partition place, a pair
             ++out->dense_queries;
             ++out->logged_queries;
-            bytes += 4ull * q.hub_len + 14ull * q.touched_cap;
+            const ptw_query &qq = d.queries[&q - L.b.h_queries.data()];
+            bytes += ptw::batch_query_bytes(ctx->dev.V, qq.walkers, qq.tp, qq.kind == PTW_QUERY_ENDPOINTS, 0, true,
+                                            q.hub_len);
             continue;
         } else {
             ++out->hash_queries;

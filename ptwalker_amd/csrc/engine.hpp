@@ -269,12 +269,22 @@ enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1, SLICE_LOG = 2 };
 
 constexpr uint32_t kBatchLogSeg = 1024;     // entries per wavefront segment
 constexpr uint32_t kBatchLogTile = 8;       // segments per partition tile
-constexpr uint32_t kBatchLogShift = 14;     // id bucket = device id >> kBatchLogShift
+constexpr uint32_t kBatchLogShift = 14;     // id bucket = device id >> kBatchLogShift (batch_log_fits)
 constexpr uint32_t kBatchLogMaxBuckets = 4096;
 constexpr uint32_t kBatchLogQWords = 8;     // per-query words of the log plan (BatchLaunch::qlog)
 constexpr uint32_t kBatchHdrWords = 16;     // BatchLaunch::hdr
 // Workgroup LDS of the dense kernel (160 KiB less its static variables).
 constexpr uint32_t kDenseLdsBytes = 163840 - 256;
+// Dense threshold (walkers) of a batch that gives none: logged dense queries
+// cost no V-sized slice, so any query worth a workgroup chunk is dense; with
+// slices, only the large ones.
+constexpr uint64_t kDenseThresholdLogged = 4096;
+constexpr uint64_t kDenseThresholdSlices = 100000;
+// Whether dense queries of a graph of V vertices can be logged (bucket count).
+inline bool batch_log_fits(uint32_t V)
+{
+    return V && ((uint64_t)(V - 1) >> kBatchLogShift) < kBatchLogMaxBuckets;
+}
 
 // Hub counters a logged dense launch keeps for lds_hubs requested and nb
 // bucket counters beside them: a multiple of 64 that fits the LDS.
@@ -340,6 +350,18 @@ inline SliceSizes batch_slice_sizes(uint32_t V, uint64_t walkers, double tp, uin
     z.hub_len = z.dense ? 0 : std::min<uint32_t>(dense_pitch(V), kHashHubLen);
     z.touched_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(V, expect), z.dense ? V : z.slots + z.hub_len);
     return z;
+}
+
+// HBM one batched query takes under the planner's rule: a logged dense query
+// holds its hub slice and, per expected count (1.25x), a log entry, a
+// partition entry and a pair place.
+inline uint64_t batch_query_bytes(uint32_t V, uint64_t walkers, double tp, bool ends_only, uint64_t dense_threshold,
+                                  bool logged, uint32_t log_hub_len)
+{
+    const SliceSizes z = batch_slice_sizes(V, walkers, tp, dense_threshold, ends_only);
+    if (!(z.dense && logged)) return z.bytes();
+    const uint64_t expect = (uint64_t)(1.25 * (double)walkers * batch_counts_per_walker(tp, ends_only)) + 64;
+    return 4ull * log_hub_len + 14ull * expect;
 }
 
 // One query of a batch, as the device sees it.

@@ -41,11 +41,18 @@ struct Item {
     uint64_t walkers;
 };
 
-// Slice bytes of one query under the batch planner's rule at the default
-// dense threshold.
-uint64_t slice_bytes(uint32_t V, uint64_t walkers, double tp)
+// HBM of one query under the batch planner's rule at the default dense
+// threshold (logged dense queries when the context logs them).
+uint64_t slice_bytes(const ptw_ctx *ctx, uint64_t walkers, double tp)
 {
-    return ptw::batch_slice_sizes(V, walkers, tp, 100000).bytes();
+    const uint32_t V = ctx->dev.V;
+    const bool logged = ctx->shape.visit_log && ptw::batch_log_fits(V);
+    const uint64_t thr = logged ? ptw::kDenseThresholdLogged : ptw::kDenseThresholdSlices;
+    const uint32_t hubs =
+        logged ? std::min<uint32_t>(ptw::batch_log_hubs(ctx->shape.lds_hubs, ((V - 1) >> ptw::kBatchLogShift) + 1),
+                                    ptw::dense_pitch(V))
+               : 0u;
+    return ptw::batch_query_bytes(V, walkers, tp, false, thr, logged, hubs);
 }
 
 void graph_features(const ptw::HostCsr &g, uint32_t s, float *f)
@@ -218,7 +225,7 @@ int ptw_serve_replay(ptw_ctx *ctx, ptw_mlp *mlp, const ptw_serve_config *cfg, co
             const double w = (double)b * (aud ? 11.0 : 1.0);
             if (!F.items.empty() && (walkers + w > kBatchWalkers || (split && 2 * walkers >= waiting))) break;
             walkers += w;
-            uint64_t need = slice_bytes(V, b, C.tp) + (aud ? slice_bytes(V, 10 * b, C.tp) : 0);
+            uint64_t need = slice_bytes(ctx, b, C.tp) + (aud ? slice_bytes(ctx, 10 * b, C.tp) : 0);
             if (used + need > allowance && !F.items.empty()) {
                 ++st.admission_waits;
                 break;

@@ -17,8 +17,9 @@ Knobs:
   batches batch_chunk (walker ids per queue entry of a hash query),
           dense_chunk (walker ids per workgroup chunk of a dense query; 0 =
           the planner's per-query rule), block, lds_hubs, cache_slots, dense_threshold
-          (walkers above which a query gets a dense counter slice rather than
-          a hash slice), topk_tile (LDS candidate slots of the top-k kernel)
+          (walkers above which a query is counted densely -- through the visit
+          log, or in a dense slice when visit_log is 0 -- rather than in a hash
+          slice), topk_tile (LDS candidate slots of the top-k kernel)
 
 Results are cached in a JSON file keyed by the graph's shape and a workload
 signature; `apply` restores a cached choice without timing anything.
@@ -51,7 +52,8 @@ BATCH_SPACE: Dict[str, List[int]] = {
     # the cache before the hub counts: 79872 hubs fit the LDS only without a cache
     "cache_slots": [0, 512],
     "lds_hubs": [36864, 55296, 77824, 79872],
-    "dense_threshold": [12_500, 25_000, 50_000, 100_000, 200_000],
+    # walkers above which a query is dense (logged or in a slice, by visit_log)
+    "dense_threshold": [1, 1_500, 4_096, 12_500, 25_000, 50_000, 100_000, 200_000],
     "topk_tile": [256, 512, 1024, 2048, 4096],
 }
 
@@ -155,7 +157,8 @@ def tune_batch(ctx, sources, walkers, tps, *, topk: int = 100, seed: int = 1, re
     base = ctx.tuning().as_dict()
     start = {k: base[k] for k in space if k in base}
     if "dense_threshold" in space:
-        start["dense_threshold"] = 100_000
+        # the planner's default for the context's counting
+        start["dense_threshold"] = 4_096 if base.get("visit_log", 1) else 100_000
     ref = ctx.batch_ppr(sources, walkers, tps, topk=topk, seed=seed, dense_threshold=start.get("dense_threshold", 0))
 
     def measure(c):

@@ -333,7 +333,7 @@ def test_hash_queries_hub_slice_and_table_scale20(ctx):
     W = np.exp(rng.uniform(np.log(200), np.log(20000), n)).astype(np.int64)
     W[:2] = 150000  # dense
     tp = rng.uniform(0.15, 0.3, n)
-    r = ctx.batch_ppr(src, W, tp, topk=50, seed=7)
+    r = ctx.batch_ppr(src, W, tp, topk=50, seed=7, dense_threshold=100000)
     assert r.dense_queries == 2 and r.hash_queries == n - 2
     table_hits = 0
     for i in range(n):
@@ -370,7 +370,7 @@ def test_config_batch_scale24_4096_queries(ctx24):
     """The batched configuration at its size: 4096 PPR queries on R-MAT scale 24
     (sources by degree, walkers log-uniform in [1e3, 1e6], tp uniform in
     [0.15, 0.3]), top-100, eager and as a captured graph.  32 queries -- the
-    8 largest (dense slices), the 8 smallest and 16 spread over the batch --
+    8 largest (dense, logged), the 8 smallest and 16 spread over the batch --
     are replayed alone and must match bit for bit (full count vector, steps,
     top-100); the batch's step total lies within 6 sigma of sum(W / tp)."""
     c, gi, rp, col = ctx24
