@@ -1134,26 +1134,31 @@ __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p
 // Dynamic LDS: 2^log_shift u32 counters, a bitmap of the nonzero ones, then
 // the query's entry count and first entry per bucket (2 * log_nb words).
 // The counters are zero between buckets: a bucket sets the counters and bits
-// of its hub values, source visits and entries, and the output walks the
-// bitmap in id order (one word per thread), reads and clears each set
-// counter and writes its pair, so a bucket costs its entries, not its span.
-// 512-thread workgroups: two fit a CU, and one's barriers overlap the
-// other's work.
+// of its hub values, source visits and entries, so it costs its entries, not
+// its span.  Output: wavefront w owns the bucket's w-th run of
+// kBatchRunIds ids (one bitmap word per lane), reads and clears each set
+// counter and writes the run's pairs in id order at a place it reserves on
+// the query's LDS pair cursor; runs of one query land in any order, which
+// the pair list allows (top-k reads it whole, lookups search a run).  Two
+// barriers per bucket; the next nonempty bucket's entries are loaded while
+// this one is counted.  512-thread workgroups: two fit a CU, and one's
+// barriers overlap the other's work.
 constexpr uint32_t kLogCountBlock = 512;
 static_assert((1u << kBatchLogShift) / 32 == kLogCountBlock, "one bitmap word per thread");
+static_assert(kBatchRunIds == 64 * 32, "a wavefront's run is one bitmap word per lane");
 __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLaunch p)
 {
     extern __shared__ uint32_t lc_lds[];
     __shared__ uint32_t s_hist[kCountBins];
     __shared__ uint32_t s_max;
     __shared__ unsigned long long s_sums[2];
-    __shared__ unsigned long long s_wave[16];
-    __shared__ uint32_t s_take;
+    __shared__ uint32_t s_take, s_pairs;
     __shared__ unsigned long long s_clogc[kClogcTable];
     fill_clogc_table(s_clogc);
     const uint32_t nb = p.log_nb, shift = p.log_shift;
     const uint32_t span = 1u << shift;
     const uint32_t off_mask = span - 1;
+    const uint32_t runs_per_bucket = span / kBatchRunIds;
     uint32_t *const cnt = lc_lds;
     uint32_t *const bits = lc_lds + span;
     uint32_t *const b_cnt = bits + span / 32;
@@ -1166,6 +1171,7 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
         s_sums[1] = 0;
     }
     LaneStats ls;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint4 *part4 = reinterpret_cast<const uint4 *>(p.part16);
     auto mark = [&](uint32_t i, uint32_t c) {
         atomicAdd(&cnt[i], c);
@@ -1185,29 +1191,41 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             b_cnt[b] = p.part_cnt[row + b];
             b_loc[b] = p.part_loc[row + b];
         }
+        if (threadIdx.x == 0) s_pairs = 0;
         lds_barrier();
-        uint32_t pairs = 0;  // workgroup-uniform: pairs written so far
-        for (uint32_t b = 0; b < nb; ++b) {
-            const uint64_t pi = row + b;
+        // buckets with something to count: entries, hub counters or the source
+        auto busy = [&](uint32_t b) {
+            const uint32_t lo = b << shift;
+            return b_cnt[b] != 0 || lo < hub_len || (src >= lo && src - lo < span);
+        };
+        auto next_busy = [&](uint32_t b) {
+            while (b < nb && !busy(b)) ++b;
+            return b;
+        };
+        // the first aligned 16 B chunk of bucket b's entries for this thread
+        auto first_chunk = [&](uint32_t b) {
+            uint4 v = uint4{0u, 0u, 0u, 0u};
+            if (b < nb) {
+                const uint64_t e0 = qb + b_loc[b];
+                const uint64_t a0 = e0 & ~7ull;
+                if (threadIdx.x < (uint32_t)((e0 + b_cnt[b] - a0 + 7) >> 3)) v = part4[(a0 >> 3) + threadIdx.x];
+            }
+            return v;
+        };
+        uint32_t b = next_busy(0);
+        for (uint32_t e = 0; e < b; ++e)  // buckets before the first busy one: no runs
+            if (threadIdx.x < runs_per_bucket) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
+        uint4 first = first_chunk(b);
+        while (b < nb) {  // workgroup-uniform
             const uint32_t lo = b << shift;
             const uint32_t hi = p.V - lo < span ? p.V : lo + span;
             const uint32_t n = b_cnt[b];
-            const bool src_in = src >= lo && src < hi;
-            if (n == 0 && lo >= hub_len && !src_in) {
-                if (threadIdx.x == 0) {
-                    p.run_loc[pi] = pairs;
-                    p.run_len[pi] = 0;
-                }
-                continue;  // workgroup-uniform
-            }
-            // the bucket's entries [e0, e0 + n), as aligned 16 B chunks (the
-            // partition buffer has 8 entries of padding); the first chunk of
-            // each thread is loaded ahead
             const uint64_t e0 = qb + b_loc[b];
             const uint64_t a0 = e0 & ~7ull;
             const uint32_t chunks = (uint32_t)((e0 + n - a0 + 7) >> 3);
-            uint4 first = uint4{0u, 0u, 0u, 0u};
-            if (threadIdx.x < chunks) first = part4[(a0 >> 3) + threadIdx.x];
+            const uint4 mine0 = first;
+            const uint32_t nxt = next_busy(b + 1);
+            first = first_chunk(nxt);  // in flight while this bucket is counted
             // hub counters [lo, hub_end) of the bucket (the log holds none of
             // those ids, the source's visits are in src_visits)
             const uint32_t hub_end = hub_len < hi ? hub_len : hi;
@@ -1215,9 +1233,9 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                 const uint32_t c = p.dense[hub_off + lo + i];
                 if (c) mark(i, c);
             }
-            if (src_in && threadIdx.x == 0 && src_visits) mark(src - lo, src_visits);
+            if (src >= lo && src < hi && threadIdx.x == 0 && src_visits) mark(src - lo, src_visits);
             for (uint32_t j = threadIdx.x; j < chunks; j += blockDim.x) {
-                const uint4 v = j == threadIdx.x ? first : part4[(a0 >> 3) + j];
+                const uint4 v = j == threadIdx.x ? mine0 : part4[(a0 >> 3) + j];
                 const uint32_t w[4] = {v.x, v.y, v.z, v.w};
                 const uint64_t at = a0 + 8ull * j;
 #pragma unroll
@@ -1227,11 +1245,20 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                 }
             }
             lds_barrier();
-            // output: thread t owns bitmap word t (ids lo + 32t ...), in id order
+            // output: wavefront w's run, lane l's bitmap word
             uint32_t word = bits[threadIdx.x];
-            uint64_t total = 0;
-            const uint32_t at = (uint32_t)block_excl_scan_lds((uint32_t)__popc(word), s_wave, total);
-            uint2 *out = p.touched + touched_off + pairs + at;
+            const uint32_t pc = (uint32_t)__popc(word);
+            uint32_t incl = pc;
+#pragma unroll
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const uint32_t run_n = __shfl(incl, 63);
+            uint32_t run_at = 0;
+            if (lane == 0 && run_n) run_at = atomicAdd(&s_pairs, run_n);
+            run_at = __shfl(run_at, 0);
+            uint2 *out = p.touched + touched_off + run_at + (incl - pc);
             while (word) {
                 const uint32_t i = 32 * threadIdx.x + (uint32_t)(__ffs(word) - 1);
                 word &= word - 1;
@@ -1241,14 +1268,18 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                 ls.add(c, s_hist, s_clogc);
             }
             bits[threadIdx.x] = 0;
-            if (threadIdx.x == 0) {
-                p.run_loc[pi] = pairs;
-                p.run_len[pi] = (uint32_t)total;
+            if (lane == 0) {
+                const uint64_t ri = (row + b) * runs_per_bucket + wave;
+                p.run_loc[ri] = run_at;
+                p.run_len[ri] = run_n;
             }
-            pairs += (uint32_t)total;
             lds_barrier();  // the counters and bits are clear for the next bucket
+            // buckets skipped on the way: no runs
+            for (uint32_t e = b + 1; e < nxt; ++e)
+                if (threadIdx.x < runs_per_bucket) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
+            b = nxt;
         }
-        if (threadIdx.x == 0) p.touched_count[q] = pairs;
+        if (threadIdx.x == 0) p.touched_count[q] = s_pairs;
         flush_stats(ls, s_hist, &s_max, s_sums, p.tk_stats + q);
     }
 }
@@ -1732,8 +1763,9 @@ __global__ void batch_gather_kernel(BatchLaunch p, uint64_t n, const uint32_t *q
         if (Q.kind == SLICE_DENSE) {
             c = p.dense[Q.slice_off + v];
         } else if (Q.kind == SLICE_LOG) {
-            // binary search in the run of v's bucket (ids ascending)
-            const uint64_t pi = (uint64_t)qidx[i] * p.log_nb + (v >> p.log_shift);
+            // binary search in the run of v's kBatchRunIds ids (ascending)
+            const uint64_t pi = (uint64_t)qidx[i] * p.log_nb * ((1u << p.log_shift) / kBatchRunIds) +
+                                v / kBatchRunIds;
             const uint2 *run = p.touched + Q.touched_off + p.run_loc[pi];
             uint32_t lo = 0, hi = p.run_len[pi];
             while (lo < hi) {

@@ -195,7 +195,7 @@ uint64_t plan_bytes_beyond(const BatchState &b, const Plan &P, uint32_t nq, uint
         if (P.log_segs > b.log_segs) need += P.log_segs * (kBatchLogSeg * 6 + 12);
         const uint64_t rows = (uint64_t)nq * P.log_nb;
         if (nq > b.log_q_cap) need += (uint64_t)nq * (4 * kBatchLogQWords + 8);
-        if (rows > b.log_rows) need += rows * 20;
+        if (rows > b.log_rows) need += rows * (12 + 8 * ((1ull << kBatchLogShift) / kBatchRunIds));
     }
     if (P.cand > b.cand_cap) need += 8 * P.cand;
     if (nq > b.q_cap) need += (uint64_t)nq * (sizeof(BatchQuery) + sizeof(TopkStats) + 24);
@@ -301,8 +301,9 @@ int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k
             HIP_TRY(ctx, hipMalloc((void **)&b.part_cnt, cap * 4));
             HIP_TRY(ctx, hipMalloc((void **)&b.part_loc, cap * 4));
             HIP_TRY(ctx, hipMalloc((void **)&b.part_cur, cap * 4));
-            HIP_TRY(ctx, hipMalloc((void **)&b.run_loc, cap * 4));
-            HIP_TRY(ctx, hipMalloc((void **)&b.run_len, cap * 4));
+            const uint64_t runs = cap * ((1ull << kBatchLogShift) / kBatchRunIds);
+            HIP_TRY(ctx, hipMalloc((void **)&b.run_loc, runs * 4));
+            HIP_TRY(ctx, hipMalloc((void **)&b.run_len, runs * 4));
             b.log_rows = cap;
             moved = true;
         }

@@ -270,6 +270,7 @@ enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1, SLICE_LOG = 2 };
 constexpr uint32_t kBatchLogSeg = 1024;     // entries per wavefront segment
 constexpr uint32_t kBatchLogTile = 8;       // segments per partition tile
 constexpr uint32_t kBatchLogShift = 14;     // id bucket = device id >> kBatchLogShift (batch_log_fits)
+constexpr uint32_t kBatchRunIds = 2048;     // ids per run of a logged query's pair list
 constexpr uint32_t kBatchLogMaxBuckets = 4096;
 constexpr uint32_t kBatchLogQWords = 8;     // per-query words of the log plan (BatchLaunch::qlog)
 constexpr uint32_t kBatchHdrWords = 16;     // BatchLaunch::hdr
@@ -471,7 +472,8 @@ struct BatchLaunch {
     unsigned long long *q_base;  // per query: first entry of its partition
     uint32_t log_shift, log_nb;  // id bucket = device id >> log_shift; log_nb buckets
     // per (query, bucket): entries, first entry in the query's partition,
-    // scatter cursor, first pair of its run in the pair list, run length
+    // scatter cursor; per (query, run of kBatchRunIds ids): first pair of the
+    // run in the query's pair list, run length
     uint32_t *part_cnt, *part_loc, *part_cur, *run_loc, *run_len;
     uint16_t *part16;            // partitioned entries: device id - bucket start
 };
