@@ -354,8 +354,9 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     uint32_t *const keys = dense_lds + hub_words;
     uint32_t *const vals = keys + slots;
     const uint32_t mask = slots ? slots - 1 : 0u;
-    // logged queries (nb > 0, no cache): entries per id bucket of the query
-    uint32_t *const bkt = dense_lds + hub_words;
+    // launches with logged queries (nb > 0): entries per id bucket of the
+    // workgroup's query when it is logged (the cache serves sliced queries)
+    uint32_t *const bkt = dense_lds + hub_words + 2 * slots;
     const bool logging = nb != 0;
     __shared__ uint32_t s_q, s_len, s_off, s_done;
     __shared__ unsigned long long s_start;
@@ -377,6 +378,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     uint32_t src_beg = 0, src_deg = 0;  // workgroup-uniform: the source's row
     uint32_t cur_beg = 0, cur_deg = 0;  // the row of v, from its record
     bool ends = false;  // workgroup-uniform: count only each walker's last vertex
+    bool qlogged = false;  // workgroup-uniform: the query is SLICE_LOG
     uint64_t off = 0;  // workgroup-uniform: the query's dense slice, or its hub slice when logged
     uint32_t my_src = 0;
     uint64_t my_steps = 0;
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
                 atomicSub(&direct[x >> 1], 0x8000u << sh);
                 atomicAdd(&p.dense[off + x], 0x8000u);
             }
-        } else if (logging) {
+        } else if (qlogged) {
             pend = x;
         } else {
             if (slots) {
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
         }
         if (lane == 0) {
             if (hot) {
-                if (logging)
+                if (qlogged)
                     atomicAdd(&p.qlog[(uint64_t)cq * kBatchLogQWords + 4], (uint32_t)hot);
                 else
                     atomicAdd(&p.dense[off + src], (uint32_t)hot);
@@ -545,7 +547,8 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
             thr = bq.thr;
             stream = bq.stream;
             ends = bq.ends_only != 0;
-            off = logging ? bq.hub_off : bq.slice_off;
+            qlogged = bq.kind == SLICE_LOG;
+            off = qlogged ? bq.hub_off : bq.slice_off;
             src_beg = p.row_ptr[src];
             src_deg = p.row_ptr[src + 1] - src_beg;
         }
@@ -1838,8 +1841,8 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int w
                              int cache_slots, hipStream_t s)
 {
     const uint32_t nb = p.vlog ? p.log_nb : 0u;
-    const uint32_t hubs = p.vlog ? batch_log_hubs(lds_hubs, nb) : (uint32_t)lds_hubs;
-    const uint32_t slots = p.vlog ? 0u : (uint32_t)cache_slots;
+    const uint32_t slots = (uint32_t)cache_slots;
+    const uint32_t hubs = p.vlog ? batch_log_hubs(lds_hubs, nb, slots) : (uint32_t)lds_hubs;
     const size_t dlds = (((size_t)hubs + 1) / 2 + 2 * (size_t)slots + nb) * 4;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_dense_kernel, block, dlds) != hipSuccess ||

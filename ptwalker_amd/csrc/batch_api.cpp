@@ -53,6 +53,7 @@ void free_log(BatchState &b)
     hfree(b.run_len);
     b.log_segs = b.log_q_cap = b.log_rows = 0;
     b.log_shift = b.log_nb = 0;
+    b.log_active = false;
 }
 
 struct Plan {
@@ -89,13 +90,15 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P, bool allow_log = t
     const uint64_t chunk = (uint64_t)ctx->shape.batch_chunk;
     // dense queries are logged unless visit_log is 0 or the graph needs more
     // id buckets than the log's kernels hold
-    P.log = allow_log && ctx->shape.visit_log && batch_log_fits(V);
+    const bool log_ok = allow_log && ctx->shape.visit_log && batch_log_fits(V);
     const uint64_t thr_dense =
-        d.dense_threshold ? d.dense_threshold : P.log ? kDenseThresholdLogged : kDenseThresholdSlices;
-    P.log_shift = P.log ? kBatchLogShift : 0u;
-    P.log_nb = P.log ? ((V - 1) >> kBatchLogShift) + 1 : 0u;
+        d.dense_threshold ? d.dense_threshold : log_ok ? kDenseThresholdLogged : kDenseThresholdSlices;
+    P.log_shift = log_ok ? kBatchLogShift : 0u;
+    P.log_nb = log_ok ? ((V - 1) >> kBatchLogShift) + 1 : 0u;
     const uint32_t log_hub_len =
-        P.log ? std::min<uint32_t>(batch_log_hubs(ctx->shape.lds_hubs, P.log_nb), dense_pitch(V)) : 0u;
+        log_ok ? std::min<uint32_t>(batch_log_hubs(ctx->shape.lds_hubs, P.log_nb, batch_cache_slots(ctx->shape)),
+                                    dense_pitch(V))
+               : 0u;
     P.q.resize(d.num_queries);
     for (uint32_t i = 0; i < d.num_queries; ++i) {
         const ptw_query &qq = d.queries[i];
@@ -120,7 +123,7 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P, bool allow_log = t
         B.ends_only = ends ? 1u : 0u;
         B.pad = 0;
         const SliceSizes z = batch_slice_sizes(V, qq.walkers, qq.tp, thr_dense, ends);
-        if (z.dense && P.log) {
+        if (z.dense && log_ok && (double)qq.walkers * per_walker <= kLogCountsPerVertex * V) {
             // no slice: a hub slice (below) and the log, whose pair list
             // the device places after the walk
             B.kind = SLICE_LOG;
@@ -130,6 +133,7 @@ int make_plan(ptw_ctx *ctx, const ptw_batch_desc &d, Plan &P, bool allow_log = t
             P.log_entries += (uint64_t)(1.25 * (double)qq.walkers * per_walker) + 64;
             P.log_hubs += (uint64_t)log_hub_len + 1;
             P.log_order.push_back(i);
+            P.log = true;
         } else if (z.dense) {
             B.kind = SLICE_DENSE;
             B.slice_off = P.dense;
@@ -226,7 +230,7 @@ BatchLaunch launch_args(ptw_ctx *ctx, const BatchState &b)
     p.queue = b.queue;
     p.tk_stats = b.tk_stats;
     p.tk_cand = b.tk_cand;
-    if (b.vlog) {
+    if (b.vlog && b.log_active) {
         p.vlog = b.vlog;
         p.log_segs = b.log_segs;
         p.seg_q = b.seg_q;
@@ -257,9 +261,10 @@ int ensure_pools(ptw_ctx *ctx, LaneRef L, const Plan &P, uint32_t nq, uint32_t k
         HIP_TRY(ctx, hipMalloc((void **)&b.queue, 8 * sizeof(unsigned long long)));
         moved = true;
     }
-    if (!P.log && b.vlog) {
-        // a plan without logged queries: the launch must not run the log's kernels
-        free_log(b);
+    // the pools are kept when a batch has no logged queries (a free would wait
+    // for the device); its launch does not run the log's kernels
+    if (P.log != b.log_active) {
+        b.log_active = P.log;
         moved = true;
     }
     if (P.log) {
@@ -388,7 +393,7 @@ int enqueue_run(ptw_ctx *ctx, LaneRef L, const BatchLaunch &p)
 {
     HIP_TRY(ctx, launch_batch_begin(p, L.b.q_cap, L.s));
     HIP_TRY(ctx, launch_batch_walk(p, ctx->shape.block, ctx->num_cus, L.cus, ctx->shape.lds_hubs,
-                                   ctx->shape.cache_slots < 0 ? ptw::kAutoCacheSlots : ctx->shape.cache_slots, L.s));
+                                   (int)ptw::batch_cache_slots(ctx->shape), L.s));
     HIP_TRY(ctx, launch_batch_topk(p, ctx->dev.inv, L.b.topk_v, L.b.topk_c, ctx->shape.topk_tile, ctx->num_cus,
                                    L.s));
     return PTW_OK;

@@ -47,6 +47,7 @@ struct BatchState {
     uint64_t log_q_cap = 0;   // queries the per-query log arrays hold
     uint64_t log_rows = 0;    // (query, bucket) entries the per-bucket arrays hold
     uint32_t log_shift = 0, log_nb = 0;  // of the last plan with logged queries
+    bool log_active = false;  // the current batch has logged queries (its launch runs the log's kernels)
     bool dirty = false;                   // the last batch's slices are not yet cleaned
     bool weighted = false;
     uint32_t last_nq = 0;

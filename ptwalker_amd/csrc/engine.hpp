@@ -188,6 +188,11 @@ struct EffectiveShape {
 // is kept, and the hub counters given the cache's LDS where it is not.
 constexpr int kAutoCacheSlots = 512;
 constexpr int kMaxLdsHubs = 81408;
+// Cache slots of a batched dense launch under a shape.
+inline uint32_t batch_cache_slots(const LaunchShape &s)
+{
+    return (uint32_t)(s.cache_slots < 0 ? kAutoCacheSlots : s.cache_slots);
+}
 
 // The no-allocation walk's scratch (walk.hip): 8 control words, then device-id
 // u64 visit counters for graphs of at most kStaticMaxVertices vertices.
@@ -281,17 +286,26 @@ constexpr uint32_t kDenseLdsBytes = 163840 - 256;
 // slices, only the large ones.
 constexpr uint64_t kDenseThresholdLogged = 4096;
 constexpr uint64_t kDenseThresholdSlices = 100000;
+// A dense query is logged while its expected counts are at most this many
+// per vertex.  Its log holds about 17.5 B per expected count where a slice
+// holds 12 B per vertex, and one workgroup partitions and counts a logged
+// query, where a slice is compacted by the whole device: beyond about one
+// count per vertex the slice takes less memory, its V-sized scan is a small
+// share of the query's walk, and a lone large query finishes sooner (serving
+// at 3.75 counts per vertex: p50 3.5 ms with slices, 17.6 ms logged).
+constexpr double kLogCountsPerVertex = 1.0;
 // Whether dense queries of a graph of V vertices can be logged (bucket count).
 inline bool batch_log_fits(uint32_t V)
 {
     return V && ((uint64_t)(V - 1) >> kBatchLogShift) < kBatchLogMaxBuckets;
 }
 
-// Hub counters a logged dense launch keeps for lds_hubs requested and nb
-// bucket counters beside them: a multiple of 64 that fits the LDS.
-inline uint32_t batch_log_hubs(int lds_hubs, uint32_t nb)
+// Hub counters a dense launch with logged queries keeps for lds_hubs
+// requested, beside nb bucket counters and the visit cache of its sliced
+// queries (slots): a multiple of 64 that fits the LDS.
+inline uint32_t batch_log_hubs(int lds_hubs, uint32_t nb, uint32_t slots)
 {
-    const uint64_t words = (kDenseLdsBytes - 4ull * nb) / 4;
+    const uint64_t words = (kDenseLdsBytes - 4ull * nb - 8ull * slots) / 4;
     uint64_t h = std::min<uint64_t>((uint64_t)std::max(lds_hubs, 0), 2 * words);
     return (uint32_t)(h & ~63ull);
 }
@@ -360,8 +374,9 @@ inline uint64_t batch_query_bytes(uint32_t V, uint64_t walkers, double tp, bool 
                                   bool logged, uint32_t log_hub_len)
 {
     const SliceSizes z = batch_slice_sizes(V, walkers, tp, dense_threshold, ends_only);
-    if (!(z.dense && logged)) return z.bytes();
-    const uint64_t expect = (uint64_t)(1.25 * (double)walkers * batch_counts_per_walker(tp, ends_only)) + 64;
+    const double counts = (double)walkers * batch_counts_per_walker(tp, ends_only);
+    if (!(z.dense && logged && counts <= kLogCountsPerVertex * V)) return z.bytes();
+    const uint64_t expect = (uint64_t)(1.25 * counts) + 64;
     return 4ull * log_hub_len + 14ull * expect;
 }
 

@@ -49,7 +49,8 @@ uint64_t slice_bytes(const ptw_ctx *ctx, uint64_t walkers, double tp)
     const bool logged = ctx->shape.visit_log && ptw::batch_log_fits(V);
     const uint64_t thr = logged ? ptw::kDenseThresholdLogged : ptw::kDenseThresholdSlices;
     const uint32_t hubs =
-        logged ? std::min<uint32_t>(ptw::batch_log_hubs(ctx->shape.lds_hubs, ((V - 1) >> ptw::kBatchLogShift) + 1),
+        logged ? std::min<uint32_t>(ptw::batch_log_hubs(ctx->shape.lds_hubs, ((V - 1) >> ptw::kBatchLogShift) + 1,
+                                                         ptw::batch_cache_slots(ctx->shape)),
                                     ptw::dense_pitch(V))
                : 0u;
     return ptw::batch_query_bytes(V, walkers, tp, false, thr, logged, hubs);

@@ -147,14 +147,14 @@ def test_logged_and_slice_batches_agree(ctx, g16, hubs):
     gi, rp, col = g16
     ctx.from_csr(rp, col)
     n = 24
-    src, W, tp = make_queries(gi, n, seed=21, wmin=1000, wmax=400000)
+    src, W, tp = make_queries(gi, n, seed=21, wmin=1000, wmax=100000)
     kinds = np.where(np.arange(n) % 3 == 2, ptw.QUERY_ENDPOINTS, ptw.QUERY_VISITS)
     t = ctx.tuning()
     res = {}
     try:
         for vl in (1, 0):
             ctx.set_tuning(visit_log=vl, lds_hubs=hubs)
-            r = ctx.batch_ppr(src, W, tp, topk=30, seed=8, kinds=kinds, dense_threshold=20000)
+            r = ctx.batch_ppr(src, W, tp, topk=30, seed=8, kinds=kinds, dense_threshold=4096)
             rng = np.random.default_rng(1)
             qi = rng.integers(0, n, 4000)
             vs = rng.integers(0, gi.num_vertices, 4000)
@@ -162,9 +162,12 @@ def test_logged_and_slice_batches_agree(ctx, g16, hubs):
     finally:
         ctx.set_tuning(visit_log=t.visit_log, lds_hubs=t.lds_hubs)
     (a, ca, ga, qi, vs), (b, cb, gb, _, _) = res[1], res[0]
-    n_dense = int(np.sum(W > 20000))
-    assert n_dense >= 8
-    assert a.logged_queries == n_dense and a.dense_queries == n_dense and not a.log_rerun
+    n_dense = int(np.sum(W > 4096))
+    # a dense query is logged while its expected counts stay within 1 per vertex
+    counts = W * np.where(kinds == ptw.QUERY_ENDPOINTS, 1.0, 1.0 + 1.0 / tp)
+    n_logged = int(np.sum((W > 4096) & (counts <= 1.0 * gi.num_vertices)))
+    assert n_logged >= 2 and n_dense - n_logged >= 2  # logged and sliced dense queries in one launch
+    assert a.logged_queries == n_logged and a.dense_queries == n_dense and not a.log_rerun
     assert b.logged_queries == 0 and b.dense_queries == n_dense
     for i in range(n):
         assert np.array_equal(ca[i], cb[i]), i
@@ -172,7 +175,7 @@ def test_logged_and_slice_batches_agree(ctx, g16, hubs):
     assert np.array_equal(a.steps, b.steps) and np.array_equal(a.entropy, b.entropy)
     assert np.array_equal(ga, gb)
     assert np.array_equal(ga, np.array([ca[q][v] for q, v in zip(qi, vs)], dtype=np.uint64))
-    for i in np.flatnonzero(W > 20000):
+    for i in np.flatnonzero(W > 4096):
         if kinds[i] == ptw.QUERY_ENDPOINTS:
             ref = replay_endpoints(rp, col, int(src[i]), int(W[i]), float(tp[i]), int(i), seed=8)[0]
         else:
@@ -185,11 +188,11 @@ def test_full_visit_log_reruns_with_slices(ctx, g16):
     # runs again with dense slices and its counts are still the replay's
     gi, rp, col = g16
     ctx.from_csr(rp, col)
-    src, W, tp = make_queries(gi, 6, seed=22, wmin=30000, wmax=100000)
+    src, W, tp = make_queries(gi, 6, seed=22, wmin=4200, wmax=6000)  # logged: counts <= V
     t = ctx.tuning()
     try:
         ctx.set_tuning(visit_log=2048, lds_hubs=512)
-        r = ctx.batch_ppr(src, W, tp, topk=10, seed=4, dense_threshold=20000)
+        r = ctx.batch_ppr(src, W, tp, topk=10, seed=4, dense_threshold=4096)
         counts = [ctx.batch_counts(i) for i in range(6)]
     finally:
         ctx.set_tuning(visit_log=t.visit_log, lds_hubs=t.lds_hubs)
@@ -199,23 +202,47 @@ def test_full_visit_log_reruns_with_slices(ctx, g16):
         assert np.array_equal(counts[i], ref["counts"]), i
 
 
-@pytest.mark.parametrize("visit_log", [1, 0], indirect=True)
-def test_dense_hub_counter_spill(ctx, visit_log):
-    # star: hub 0 joined to 64 leaves; from a leaf every other visit is the hub,
-    # so a 49152-walker chunk of one query takes about 1.2e5 hub visits and its
-    # u16 LDS counter spills to the query's slice several times per chunk
-    n = 64
+def star(n):
+    """Hub 0 joined to n leaves, both directions."""
     rp = np.zeros(n + 2, dtype=np.uint32)
     rp[1] = n
     rp[2:] = n + np.arange(1, n + 1)
     col = np.concatenate([np.arange(1, n + 1), np.zeros(n)]).astype(np.uint32)
+    return rp, col
+
+
+def test_dense_hub_counter_spill(ctx):
+    # star: hub 0 joined to 64 leaves; from a leaf every other visit is the hub,
+    # so a 49152-walker chunk of one query takes about 1.2e5 hub visits and its
+    # u16 LDS counter spills to the query's slice several times per chunk
+    rp, col = star(64)
     ctx.from_csr(rp, col)
     qs = ((1, 6_000_000, 0.2), (2, 3_000_000, 0.25))
     r = ctx.batch_ppr([q[0] for q in qs], [q[1] for q in qs], [q[2] for q in qs], topk=10, seed=3)
-    assert r.dense_queries == 2
+    assert r.dense_queries == 2 and r.logged_queries == 0  # far more counts than vertices: slices
     for i, (s, W, tp) in enumerate(qs):
         ref = replay_query(rp, col, s, W, tp, i, seed=3)
         assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+
+
+def test_logged_hub_counter_spill(ctx):
+    # star of 200000 leaves: a 30000-walker query from a leaf makes about 1.8e5
+    # counts (logged: at most one per vertex), half of them at the hub, all in
+    # one 32768-walker workgroup chunk, so the hub's u16 LDS counter spills to
+    # the query's hub slice about twice
+    rp, col = star(200_000)
+    ctx.from_csr(rp, col)
+    t = ctx.tuning()
+    try:
+        ctx.set_tuning(dense_chunk=32768)
+        r = ctx.batch_ppr([5], [30_000], [0.2], topk=10, seed=3, dense_threshold=4096)
+        got = ctx.batch_counts(0)
+    finally:
+        ctx.set_tuning(dense_chunk=t.dense_chunk)
+    assert r.logged_queries == 1
+    ref = replay_query(rp, col, 5, 30_000, 0.2, 0, seed=3)
+    assert ref["counts"][0] > 2 * 32768
+    assert np.array_equal(got, ref["counts"])
 
 
 def test_topk_tie_overflow_path(ctx):
