@@ -183,6 +183,51 @@ def test_logged_and_slice_batches_agree(ctx, g16, hubs):
         assert np.array_equal(ca[i], ref), i
 
 
+def test_logged_batch_graph_replays_and_weighted(ctx, g16):
+    # logged queries (and sliced ones) in a captured hipGraph replayed across
+    # batches with eager work in between, every replay equal to the eager
+    # result; then a weighted batch with logged queries against the replay
+    gi, rp, col = g16
+    ctx.from_csr(rp, col)
+    t = ctx.tuning()
+    try:
+        ctx.set_tuning(lds_hubs=512)
+        a = make_queries(gi, 20, seed=31, wmin=3000, wmax=40000)
+        b = make_queries(gi, 9, seed=32, wmin=3000, wmax=40000)
+        eager_a = ctx.batch_ppr(*a, topk=40, seed=4, dense_threshold=2048)
+        ca = [ctx.batch_counts(i) for i in (0, 7, 19)]
+        eager_b = ctx.batch_ppr(*b, topk=40, seed=4, dense_threshold=2048)
+        assert eager_a.logged_queries >= 4 and eager_a.dense_queries > eager_a.logged_queries
+        for rep in range(3):
+            ga = ctx.batch_ppr(*a, topk=40, seed=4, dense_threshold=2048, graph=True)
+            assert ga.logged_queries == eager_a.logged_queries
+            assert np.array_equal(ga.topk_vertex, eager_a.topk_vertex), rep
+            assert np.array_equal(ga.topk_count, eager_a.topk_count), rep
+            assert np.array_equal(ga.steps, eager_a.steps) and np.array_equal(ga.entropy, eager_a.entropy), rep
+            for i, c in zip((0, 7, 19), ca):
+                assert np.array_equal(ctx.batch_counts(i), c), (rep, i)
+            e = ctx.batch_ppr(*b, topk=40, seed=4, dense_threshold=2048)
+            assert np.array_equal(e.topk_count, eager_b.topk_count)
+            gb = ctx.batch_ppr(*b, topk=40, seed=4, dense_threshold=2048, graph=True)
+            assert np.array_equal(gb.topk_vertex, eager_b.topk_vertex), rep
+        for i in (0, 7, 19):
+            ref = replay_query(rp, col, int(a[0][i]), int(a[1][i]), float(a[2][i]), i, seed=4)
+            assert np.array_equal(ca[[0, 7, 19].index(i)], ref["counts"]), i
+        # weighted, logged
+        ctx.random_weights(seed=9)
+        w, _, _ = ctx.export_weights()
+        alias = oracle.build_alias(rp, w)
+        src, W, tp = make_queries(gi, 8, seed=33, wmin=3000, wmax=9000)
+        r = ctx.batch_ppr(src, W, tp, topk=10, seed=6, weighted=True, dense_threshold=2048)
+        assert r.logged_queries >= 4
+        for i in range(8):
+            ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=6, alias=alias)
+            assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+    finally:
+        ctx.set_tuning(lds_hubs=t.lds_hubs)
+        ctx.from_csr(rp, col)
+
+
 def test_full_visit_log_reruns_with_slices(ctx, g16):
     # a log of two segments cannot hold the dense queries' visits: the batch
     # runs again with dense slices and its counts are still the replay's
