@@ -619,15 +619,6 @@ __global__ __launch_bounds__(1024) void batch_dense_kernel(BatchLaunch p, uint32
     flush();
 }
 
-// Lane order within a wavefront: LDS writes of the wavefront are complete
-// and visible to its other lanes after this point.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // A lane's share of a query's count statistics (TopkStats): counts of 1, 2
 // and 3 (most entries) in registers, larger counts in the workgroup's LDS
 // bins; c ln c in 2^-20 fixed point, so the sums do not depend on the order

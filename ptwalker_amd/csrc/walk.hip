@@ -317,6 +317,13 @@ __device__ __forceinline__ bool count_in_lds(const LdsCounters &L, uint32_t x, u
     return false;
 }
 
+// Path staging: entries per lane and line (128 B); a lane's LDS row (odd
+// pitch), and a wavefront's LDS words: 64 rows, then 64 u64 line addresses
+// and 64 lane ids of its ready lines.
+constexpr uint32_t kPathChunk = 32;
+constexpr uint32_t kPathRow = kPathChunk + 1;
+constexpr uint32_t kPathWaveWords = 64 * kPathRow + 128 + 64;
+
 // Endpoint staging: slots per wavefront, walker ids per slot, and the LDS
 // words of one wavefront (the slots, then 5 words per slot: base lo, base
 // hi, length, ended, generation).
@@ -435,39 +442,74 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
         pend = kNoVisit;
     };
 
-    // Path rows are written through an 8-entry register queue: every entry
-    // that completes an aligned 32 B chunk of the flat path array flushes the
-    // chunk with two dwordx4 stores, so the L2 receives whole 32 B sectors
-    // (16 B stores leave half-written sectors that the L2 can evict between
-    // their two halves); a row's unaligned head and tail go out as dwords.
+    // Path rows go out through a per-lane LDS staging line of kPathChunk
+    // entries (lane l of wavefront w stages entry k at [w * kPathWaveWords +
+    // l * kPathRow + k]; the odd row pitch spreads lanes over the banks).
+    // An entry that completes an aligned kPathChunk-entry chunk of the flat
+    // path array marks the lane's line ready; after the step the wavefront
+    // writes its ready lines together, eight lanes to a line, so one store
+    // instruction writes eight whole 128 B lines (one request each) where a
+    // lane writing its own line would issue eight 16 B requests per line.  A
+    // row's unaligned head and tail go out from the lane itself (uint4 where
+    // aligned).
+    uint32_t *const pwave = lds_words + (threadIdx.x >> 6) * kPathWaveWords;
+    uint32_t *const pstage = pwave + lane * kPathRow;  // this lane's entry k: pstage[k]
+    unsigned long long *const fl_base = reinterpret_cast<unsigned long long *>(pwave + 64 * kPathRow);
+    uint32_t *const fl_lane = pwave + 64 * kPathRow + 128;
     uint64_t pos = 0;  // flat index of the next path entry
-    uint32_t qb[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // qb[7] is the newest entry
-    uint32_t nbuf = 0;
+    uint32_t nbuf = 0;  // staged entries of the current row not yet written
+    bool line_ready = false;  // the staged line is a whole chunk of this row, for the wavefront's flush
+    uint64_t line_base = 0;
+    // entries [k0, k1) of the current chunk, at flat indices first + (k - k0)
+    auto path_store_run = [&](uint64_t first, uint32_t k0, uint32_t k1) {
+        uint32_t k = k0;
+        for (; k < k1 && ((first + (k - k0)) & 3); ++k) p.paths[first + (k - k0)] = pstage[k];
+        for (; k + 4 <= k1; k += 4)
+            *reinterpret_cast<uint4 *>(p.paths + first + (k - k0)) =
+                uint4{pstage[k], pstage[k + 1], pstage[k + 2], pstage[k + 3]};
+        for (; k < k1; ++k) p.paths[first + (k - k0)] = pstage[k];
+    };
     auto path_append = [&](uint32_t x) {
-#pragma unroll
-        for (int k = 0; k < 7; ++k) qb[k] = qb[k + 1];
-        qb[7] = x;
+        const uint32_t k = (uint32_t)pos & (kPathChunk - 1);
+        pstage[k] = x;
         ++nbuf;
-        if ((pos & 7) == 7) {
-            if (nbuf >= 8) {
-                *reinterpret_cast<uint4 *>(p.paths + (pos - 7)) = uint4{qb[0], qb[1], qb[2], qb[3]};
-                *reinterpret_cast<uint4 *>(p.paths + (pos - 3)) = uint4{qb[4], qb[5], qb[6], qb[7]};
+        if (k == kPathChunk - 1) {
+            const uint64_t base = pos - (kPathChunk - 1);
+            if (nbuf >= kPathChunk) {
+                line_ready = true;
+                line_base = base;
             } else {
-                // qb[k] belongs at pos - 7 + k; the newest nbuf are this row's
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (8 - k <= (int)nbuf) p.paths[pos - 7 + k] = qb[k];
+                // the row began inside this chunk: its entries are the last nbuf
+                path_store_run(base + (kPathChunk - nbuf), kPathChunk - nbuf, kPathChunk);
             }
             nbuf = 0;
         }
         ++pos;
     };
     auto path_flush_tail = [&]() {
-        // the last nbuf (< 8) entries sit at pos - nbuf .. pos - 1 (qb[k] at pos - 8 + k)
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (8 - k <= (int)nbuf) p.paths[pos - 8 + k] = qb[k];
+        // the last nbuf (< kPathChunk) entries: flat pos - nbuf .. pos - 1, staged at k1 - nbuf .. k1 - 1
+        const uint32_t k1 = (uint32_t)pos & (kPathChunk - 1);
+        path_store_run(pos - nbuf, k1 - nbuf, k1);
         nbuf = 0;
+    };
+    // The wavefront's ready lines, written together.  Every lane calls it.
+    auto path_flush_lines = [&]() {
+        const uint64_t m = __ballot(line_ready);
+        if (m == 0) return;
+        const uint32_t n = (uint32_t)__popcll(m);
+        if (line_ready) {
+            const uint32_t r = lane_rank(m);
+            fl_base[r] = line_base;
+            fl_lane[r] = lane;
+        }
+        wave_lds_sync();
+        const uint32_t q = lane & 7;  // this lane's 16 B of each line
+        for (uint32_t r = lane >> 3; r < n; r += 8) {
+            const uint32_t *row = pwave + fl_lane[r] * kPathRow + 4 * q;
+            *reinterpret_cast<uint4 *>(p.paths + fl_base[r] + 4 * q) = uint4{row[0], row[1], row[2], row[3]};
+        }
+        wave_lds_sync();  // the lines are read before their lanes stage again
+        line_ready = false;
     };
 
     for (;;) {
@@ -644,6 +686,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
             }
         }
         if (KIND == KIND_COUNT) log_append();
+        if (KIND == KIND_PATH || KIND == KIND_N2V) path_flush_lines();
         if (KIND == KIND_END && __ballot(staged_end) != 0) {
             // count the step's ends per slot; a slot whose walkers have all
             // ended goes out with whole-line stores and is free again
@@ -877,7 +920,10 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     auto count_lds = [&]() {
         return (((size_t)hubs + 1) / 2 + 2ull * slots + (p.vlog ? p.vlog_buckets : 0u)) * sizeof(uint32_t);
     };
-    size_t lds = kind == KIND_END ? (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t) : count_lds();
+    auto path_lds = [&]() { return (size_t)(block / 64) * kPathWaveWords * sizeof(uint32_t); };
+    size_t lds = kind == KIND_END    ? (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t)
+                 : kind == KIND_COUNT ? count_lds()
+                                      : path_lds();
     // Workgroups beyond the resident capacity are harmless (no workgroup waits
     // on another): they start late and find the queue drained.
     int per_cu = shape.blocks_per_cu;
@@ -929,6 +975,7 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     }
     if (kind == KIND_COUNT) lds = count_lds();
     else if (kind == KIND_END) lds = (size_t)(block / 64) * kEndWaveWords * sizeof(uint32_t);
+    else lds = path_lds();
     const int blocks = per_cu * num_cus;
     // Walker ids per queue take: the configured chunk, cut down for small
     // walks so that every resident wavefront gets about four takes.
@@ -950,9 +997,9 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     else if (kind == KIND_END)
         hipLaunchKernelGGL(walk_kernel<KIND_END>, dim3(blocks), dim3(block), lds, s, p, chunk, 0u, 0u);
     else if (kind == KIND_N2V)
-        hipLaunchKernelGGL(walk_kernel<KIND_N2V>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
+        hipLaunchKernelGGL(walk_kernel<KIND_N2V>, dim3(blocks), dim3(block), lds, s, p, chunk, 0u, 0u);
     else
-        hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), 0, s, p, chunk, 0u, 0u);
+        hipLaunchKernelGGL(walk_kernel<KIND_PATH>, dim3(blocks), dim3(block), lds, s, p, chunk, 0u, 0u);
     return hipGetLastError();
 }
 

@@ -20,4 +20,13 @@ __device__ __forceinline__ uint64_t wave_broadcast_first(uint64_t x)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane order within a wavefront: LDS writes of the wavefront are complete
+// and visible to its other lanes after this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace ptw
