@@ -1031,11 +1031,23 @@ __device__ __forceinline__ uint32_t take_logged_query(const BatchLaunch &p, uint
 
 // Segments whose ids and fills a partition workgroup stages in LDS at once.
 constexpr uint32_t kPartSegBatch = 1024;
+// Dynamic LDS of a partition workgroup with tiles of `tile` segments and nb
+// id buckets, and the most it may take (160 KiB less its static variables).
+constexpr size_t kPartitionLdsBytes = 163840 - 256;
+inline size_t partition_lds(uint32_t tile, uint32_t nb)
+{
+    return ((size_t)tile * kBatchLogSeg + 5 * (size_t)nb + 2 * kPartSegBatch) * 4;
+}
+static_assert(((size_t)kBatchLogTile / 2 * kBatchLogSeg + 5 * (size_t)kBatchLogMaxBuckets + 2 * kPartSegBatch) * 4 <=
+                  kPartitionLdsBytes,
+              "half tiles fit beside the most buckets");
 
-// Dynamic LDS: the tile's staging (kBatchLogTile * kBatchLogSeg words), five
-// arrays of log_nb words, then the ids and fills of up to kPartSegBatch of
-// the query's segments.  The next tile's entries are loaded while a tile is
-// placed.
+// Dynamic LDS: the tile's staging (TILE * kBatchLogSeg words), five arrays of
+// log_nb words, then the ids and fills of up to kPartSegBatch of the query's
+// segments.  The next tile's entries are loaded while a tile is placed.  A
+// tile costs about 7 us beside its entries (barriers, the bucket scan, the
+// wait for the next tile), so tiles are as large as the LDS allows.
+template <uint32_t TILE>
 __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p)
 {
     extern __shared__ uint32_t lp_lds[];
@@ -1044,7 +1056,7 @@ __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p
     const uint32_t nb = p.log_nb, shift = p.log_shift;
     const uint32_t off_mask = (1u << shift) - 1;
     uint32_t *const stage = lp_lds;
-    uint32_t *const hist = stage + kBatchLogTile * kBatchLogSeg;
+    uint32_t *const hist = stage + TILE * kBatchLogSeg;
     uint32_t *const start = hist + nb;
     uint32_t *const fill = start + nb;
     uint32_t *const base = fill + nb;
@@ -1070,26 +1082,26 @@ __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p
             }
             lds_barrier();
             // entry threadIdx.x of each segment of tile t0 (kNoVisit past its fill)
-            auto load_tile = [&](uint32_t t0, uint32_t (&e)[kBatchLogTile]) {
+            auto load_tile = [&](uint32_t t0, uint32_t (&e)[TILE]) {
 #pragma unroll
-                for (uint32_t k = 0; k < kBatchLogTile; ++k) {
+                for (uint32_t k = 0; k < TILE; ++k) {
                     e[k] = kNoVisit;
                     if (t0 + k < ng && threadIdx.x < seg_fill[t0 + k])
                         e[k] = p.vlog[(uint64_t)seg_id[t0 + k] * kBatchLogSeg + threadIdx.x];
                 }
             };
-            uint32_t nxt[kBatchLogTile];
+            uint32_t nxt[TILE];
             load_tile(0, nxt);
-            for (uint32_t t0 = 0; t0 < ng; t0 += kBatchLogTile) {
-                uint32_t ent[kBatchLogTile];
+            for (uint32_t t0 = 0; t0 < ng; t0 += TILE) {
+                uint32_t ent[TILE];
 #pragma unroll
-                for (uint32_t k = 0; k < kBatchLogTile; ++k) ent[k] = nxt[k];
+                for (uint32_t k = 0; k < TILE; ++k) ent[k] = nxt[k];
                 for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[b] = 0;
                 lds_barrier();
 #pragma unroll
-                for (uint32_t k = 0; k < kBatchLogTile; ++k)
+                for (uint32_t k = 0; k < TILE; ++k)
                     if (ent[k] != kNoVisit) atomicAdd(&hist[ent[k] >> shift], 1u);
-                if (t0 + kBatchLogTile < ng) load_tile(t0 + kBatchLogTile, nxt);  // in flight meanwhile
+                if (t0 + TILE < ng) load_tile(t0 + TILE, nxt);  // in flight meanwhile
                 lds_barrier();
                 const uint32_t b0 = threadIdx.x * per;
                 uint64_t mine = 0;
@@ -1106,7 +1118,7 @@ __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p
                 }
                 lds_barrier();
 #pragma unroll
-                for (uint32_t k = 0; k < kBatchLogTile; ++k) {
+                for (uint32_t k = 0; k < TILE; ++k) {
                     if (ent[k] != kNoVisit) {
                         const uint32_t b = ent[k] >> shift;
                         const uint32_t pos = start[b] + atomicAdd(&fill[b], 1u);
@@ -1174,12 +1186,14 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
     for (;;) {
         const uint32_t q = take_logged_query(p, 4, &s_take);
         if (q == kEmpty) break;
-        const uint32_t src = p.queries[q].source_dev;
-        const uint32_t hub_len = p.queries[q].hub_len;
-        const uint64_t hub_off = p.queries[q].hub_off;
-        const uint64_t touched_off = p.queries[q].touched_off;
-        const uint32_t src_visits = p.qlog[(uint64_t)q * kBatchLogQWords + 4];
-        const uint64_t qb = p.q_base[q];
+        // per-query values in scalar registers: the loop below then waits on
+        // no load of them (its prefetched entries stay in flight)
+        const uint32_t src = __builtin_amdgcn_readfirstlane(p.queries[q].source_dev);
+        const uint32_t hub_len = __builtin_amdgcn_readfirstlane(p.queries[q].hub_len);
+        const uint64_t hub_off = wave_broadcast_first(p.queries[q].hub_off);
+        const uint64_t touched_off = wave_broadcast_first(p.queries[q].touched_off);
+        const uint32_t src_visits = __builtin_amdgcn_readfirstlane(p.qlog[(uint64_t)q * kBatchLogQWords + 4]);
+        const uint64_t qb = wave_broadcast_first(p.q_base[q]);
         const uint64_t row = (uint64_t)q * nb;
         for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
             b_cnt[b] = p.part_cnt[row + b];
@@ -1217,19 +1231,9 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             const uint64_t e0 = qb + b_loc[b];
             const uint64_t a0 = e0 & ~7ull;
             const uint32_t chunks = (uint32_t)((e0 + n - a0 + 7) >> 3);
-            const uint4 mine0 = first;
-            const uint32_t nxt = next_busy(b + 1);
-            first = first_chunk(nxt);  // in flight while this bucket is counted
-            // hub counters [lo, hub_end) of the bucket (the log holds none of
-            // those ids, the source's visits are in src_visits)
-            const uint32_t hub_end = hub_len < hi ? hub_len : hi;
-            for (uint32_t i = threadIdx.x; lo + i < hub_end; i += blockDim.x) {
-                const uint32_t c = p.dense[hub_off + lo + i];
-                if (c) mark(i, c);
-            }
-            if (src >= lo && src < hi && threadIdx.x == 0 && src_visits) mark(src - lo, src_visits);
-            for (uint32_t j = threadIdx.x; j < chunks; j += blockDim.x) {
-                const uint4 v = j == threadIdx.x ? mine0 : part4[(a0 >> 3) + j];
+            // The entries first: the chunk loaded ahead is used before the next
+            // bucket's load is issued, so waiting for it does not wait for that.
+            auto mark_chunk = [&](uint4 v, uint32_t j) {
                 const uint32_t w[4] = {v.x, v.y, v.z, v.w};
                 const uint64_t at = a0 + 8ull * j;
 #pragma unroll
@@ -1237,7 +1241,29 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                     const uint64_t idx = at + e;
                     if (idx >= e0 && idx < e0 + n) mark(((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu) & off_mask, 1u);
                 }
+            };
+            if (threadIdx.x < chunks) mark_chunk(first, threadIdx.x);
+            for (uint32_t j = threadIdx.x + kLogCountBlock; j < chunks; j += kLogCountBlock)
+                mark_chunk(part4[(a0 >> 3) + j], j);
+            const uint32_t nxt = next_busy(b + 1);
+            first = first_chunk(nxt);  // in flight while this bucket is finished
+            // hub counters [lo, hub_end) of the bucket (the log holds none of
+            // those ids, the source's visits are in src_visits), eight loads
+            // in flight per thread
+            const uint32_t hub_end = hub_len < hi ? hub_len : hi;
+            const uint32_t hub_n = hub_end > lo ? hub_end - lo : 0u;
+            for (uint32_t i0 = 0; i0 < hub_n; i0 += 8 * kLogCountBlock) {
+                uint32_t c[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const uint32_t i = i0 + k * kLogCountBlock + threadIdx.x;
+                    c[k] = i < hub_n ? p.dense[hub_off + lo + i] : 0u;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    if (c[k]) mark(i0 + k * kLogCountBlock + threadIdx.x, c[k]);
             }
+            if (src >= lo && src < hi && threadIdx.x == 0 && src_visits) mark(src - lo, src_visits);
             lds_barrier();
             // output: wavefront w's run, lane l's bitmap word
             uint32_t word = bits[threadIdx.x];
@@ -1854,16 +1880,18 @@ hipError_t launch_batch_walk(const BatchLaunch &p, int block, int num_cus, int w
         hipLaunchKernelGGL(batch_log_plan_kernel, dim3(1), dim3(1024), 0, s, p);
         hipLaunchKernelGGL(batch_log_seglist_kernel, dim3(num_cus * 4), dim3(256), 0, s, p);
         // one query per workgroup at a time: grids of what fits resident
-        const size_t lp_lds = ((size_t)kBatchLogTile * kBatchLogSeg + 5 * (size_t)nb + 2 * kPartSegBatch) * 4;
+        const uint32_t tile = partition_lds(kBatchLogTile, nb) <= kPartitionLdsBytes ? kBatchLogTile : kBatchLogTile / 2;
+        const size_t lp_lds = partition_lds(tile, nb);
         const size_t lc_lds = ((size_t)1 << p.log_shift) * 4 + ((size_t)1 << p.log_shift) / 8 + 8 * (size_t)nb;
+        auto partition = tile == kBatchLogTile ? batch_log_partition_kernel<kBatchLogTile>
+                                               : batch_log_partition_kernel<kBatchLogTile / 2>;
         int lp = 0, lc = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lp, batch_log_partition_kernel, kBatchLogSeg, lp_lds) !=
-                hipSuccess || lp < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lp, partition, kBatchLogSeg, lp_lds) != hipSuccess || lp < 1)
             lp = 1;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lc, batch_log_count_kernel, kLogCountBlock, lc_lds) !=
                 hipSuccess || lc < 1)
             lc = 1;
-        hipLaunchKernelGGL(batch_log_partition_kernel, dim3(num_cus * lp), dim3(kBatchLogSeg), lp_lds, s, p);
+        hipLaunchKernelGGL(partition, dim3(num_cus * lp), dim3(kBatchLogSeg), lp_lds, s, p);
         hipLaunchKernelGGL(batch_log_count_kernel, dim3(num_cus * lc), dim3(kLogCountBlock), lc_lds, s, p);
         e = hipGetLastError();
         if (e != hipSuccess) return e;

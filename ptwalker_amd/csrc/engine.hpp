@@ -273,7 +273,7 @@ hipError_t launch_counts_to_graph_ids(const unsigned long long *counts_dev, cons
 enum SliceKind : uint32_t { SLICE_DENSE = 0, SLICE_HASH = 1, SLICE_LOG = 2 };
 
 constexpr uint32_t kBatchLogSeg = 1024;     // entries per wavefront segment
-constexpr uint32_t kBatchLogTile = 8;       // segments per partition tile
+constexpr uint32_t kBatchLogTile = 32;      // segments per partition tile (half where the LDS lacks room)
 constexpr uint32_t kBatchLogShift = 14;     // id bucket = device id >> kBatchLogShift (batch_log_fits)
 constexpr uint32_t kBatchRunIds = 2048;     // ids per run of a logged query's pair list
 constexpr uint32_t kBatchLogMaxBuckets = 4096;
