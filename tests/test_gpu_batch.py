@@ -290,6 +290,55 @@ def test_logged_hub_counter_spill(ctx):
     assert np.array_equal(got, ref["counts"])
 
 
+def broom(leaves, decoys=100):
+    """Source 1 joined to 2 and 3, both joined to hub 0, which has `leaves`
+    leaves; then `decoys` stars of five leaves, whose centres outrank 1, 2 and
+    3 by degree."""
+    edges = [(1, 2), (1, 3), (2, 0), (3, 0)]
+    nxt = 4
+    for _ in range(leaves):
+        edges.append((0, nxt))
+        nxt += 1
+    for _ in range(decoys):
+        c = nxt
+        nxt += 1
+        for _ in range(5):
+            edges.append((c, nxt))
+            nxt += 1
+    e = np.array(edges, dtype=np.int64)
+    u = np.concatenate([e[:, 0], e[:, 1]])
+    v = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((v, u))
+    u, v = u[order], v[order]
+    rp = np.zeros(nxt + 1, dtype=np.uint32)
+    np.add.at(rp, u + 1, 1)
+    return np.cumsum(rp).astype(np.uint32), v.astype(np.uint32)
+
+
+def test_logged_count_overflow_paths(ctx):
+    # A logged query with large counts of every origin: with 64 hub counters
+    # the source's two neighbours are not hubs, and each takes over 2^16
+    # logged visits (entries of one id bucket beyond any u16); hub 0 (hub
+    # slice) and the source (its own register) pass 0xFFFF.  Expected counts
+    # stay below one per vertex, so the query is logged.
+    rp, col = broom(1_300_000)
+    ctx.from_csr(rp, col)
+    t = ctx.tuning()
+    try:
+        ctx.set_tuning(lds_hubs=64)
+        r = ctx.batch_ppr([1], [200_000], [0.2], topk=10, seed=4, dense_threshold=4096)
+        got = ctx.batch_counts(0)
+    finally:
+        ctx.set_tuning(lds_hubs=t.lds_hubs)
+    assert r.logged_queries == 1
+    ref = replay_query(rp, col, 1, 200_000, 0.2, 0, seed=4)
+    c = ref["counts"]
+    assert c[2] > 2 * 32768 and c[3] > 2 * 32768 and c[0] > 0xFFFF and c[1] > 0xFFFF
+    assert np.array_equal(got, c)
+    v, cc = oracle.topk(c, rp, 10)
+    assert np.array_equal(r.topk_vertex[0], v) and np.array_equal(r.topk_count[0], cc)
+
+
 def test_topk_tie_overflow_path(ctx):
     # star: hub 0 joined to 5000 leaves; tp = 1 makes each walker visit the hub then one
     # leaf, so leaf counts are small and tied far beyond the 256-slot tile
