@@ -495,9 +495,10 @@ typedef struct ptw_tuning {
                               it: per CSR slot the neighbour's id, row start,
                               degree and graph id (16 B per edge; weighted path walks: both
                               alias candidates, 32 B per edge), so a step is one dependent
-                              read.  1 (default) = every walk but node2vec (weighted count
-                              and endpoint walks keep the weighted-edge records); -1 =
-                              DEEPWALK walks only; 0 = never.  Results are the same */
+                              read.  1 (default) = every walk (weighted count and endpoint
+                              walks keep the weighted-edge records; a node2vec proposal
+                              reads its candidate's record); -1 = DEEPWALK and NODE2VEC
+                              walks only; 0 = never.  Results are the same */
 } ptw_tuning;
 int ptw_tuning_get(ptw_ctx *ctx, ptw_tuning *t);
 int ptw_tuning_set(ptw_ctx *ctx, const ptw_tuning *t);

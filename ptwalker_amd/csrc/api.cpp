@@ -119,13 +119,13 @@ void free_edge_records(ptw_ctx *ctx, bool weighted_only)
     }
 }
 
-// Which records a walk reads (setting: -1 path walks, 0 none, 1 every walk
-// but node2vec).  Weighted records serve path walks only; other weighted
-// walks read the weighted-edge records.
+// Which records a walk reads (setting: -1 path walks, 0 none, 1 every
+// walk).  Weighted records serve path walks only; other weighted walks read
+// the weighted-edge records.
 void pick_edge_records(ptw_ctx *ctx, const ptw_walk_desc &d, bool may_build, ptw::WalkLaunch &p)
 {
     const int setting = ctx->shape.edge_records;
-    if (setting == 0 || d.app == PTW_APP_NODE2VEC) return;
+    if (setting == 0) return;
     if (setting < 0 && !is_path_app(d.app)) return;
     const bool weighted = (d.flags & PTW_FLAG_WEIGHTED) != 0;
     if (weighted && !is_path_app(d.app)) return;

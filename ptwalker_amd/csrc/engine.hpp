@@ -174,7 +174,7 @@ struct LaunchShape {
     int dense_chunk = 0;     // walker ids per workgroup chunk of a dense batched query (0: automatic)
     int visit_log = 1;       // count apps: log the visits LDS does not count (0: device atomics)
     bool cache_auto = false; // cache_slots came from the automatic setting (a walk's shape only)
-    int edge_records = 1;    // neighbour records: 1 every walk but node2vec, -1 path walks only, 0 never
+    int edge_records = 1;    // neighbour records: 1 every walk, -1 path walks only, 0 never
 };
 
 // The launch a walk actually ran with, after the automatic rules.

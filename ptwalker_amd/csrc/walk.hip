@@ -112,6 +112,14 @@ __device__ __forceinline__ uint64_t n2v_weight(const WalkLaunch &p, uint32_t x, 
 __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
                               uint32_t deg, uint32_t prev);
 
+// CSR slot k as a neighbour record {id, row start, degree, graph id}: read
+// from the records, or only its id from `col` when the walk has none.
+__device__ __forceinline__ uint4 n2v_slot(const WalkLaunch &p, uint64_t k)
+{
+    if (p.nrec) return p.nrec[k];
+    return uint4{p.col[k], 0u, 0u, 0u};
+}
+
 // The next vertex from the row [beg, beg + deg) of the current vertex.
 // Attempt a draws Philox(wid, step + a * 2^20) (attempt 0's block is `r`, the
 // step's own), proposes a uniform slot and accepts it iff r.z < its weight.
@@ -124,9 +132,12 @@ __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, u
 // SOSP 2019).  A p < 1 walk then needs about half the proposals.  After
 // kN2vAttempts rejections one exact draw from the same integer weights
 // follows, so the law is exactly w(x) / sum w whichever attempt decides.
-__device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, u32x4 r,
-                             uint32_t beg, uint32_t deg, uint32_t prev)
+// Returns the chosen neighbour's record (n2v_slot); the return edge's extra
+// slots give prev_rec, the record of prev.
+__device__ uint4 n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, u32x4 r,
+                          uint32_t beg, uint32_t deg, uint4 prev_rec)
 {
+    const uint32_t prev = prev_rec.x;
     // The same decisions with fewer searches: a coin below every weight
     // accepts whatever x is, a coin at or above both non-return weights
     // rejects any x but prev; only coins in between need x's class.
@@ -138,34 +149,37 @@ __device__ uint32_t n2v_pick(const WalkLaunch &p, uint32_t wlo, uint32_t whi, ui
             const uint32_t s = mulhi64(r.x, r.y, deg + p.n2v_fold);
             const uint64_t z = r.z;
             if (s >= deg) {  // an extra slot of the return edge
-                if (z < p.w_fold) return prev;
+                if (z < p.w_fold) return prev_rec;
                 continue;
             }
-            const uint32_t x = p.col[beg + s];
-            if (x == prev || z < w_min) return x;  // prev's own slot weighs 2^32
+            const uint4 e = n2v_slot(p, (uint64_t)beg + s);
+            const uint32_t x = e.x;
+            if (x == prev || z < w_min) return e;  // prev's own slot weighs 2^32
             if (z >= w_io) continue;
-            if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return x;
+            if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return e;
         }
-        return n2v_exact(p, wlo, whi, step, beg, deg, prev);
+        return n2v_slot(p, (uint64_t)beg + n2v_exact(p, wlo, whi, step, beg, deg, prev));
     }
     const uint64_t w_min = p.w_ret < p.w_in ? (p.w_ret < p.w_out ? p.w_ret : p.w_out) : (p.w_in < p.w_out ? p.w_in : p.w_out);
     for (uint32_t a = 0; a < kN2vAttempts; ++a) {
         if (a) r = philox4x32_10(wlo, whi, step + (a << 20), p.stream, p.seed_lo, p.seed_hi);
-        const uint32_t x = p.col[beg + mulhi64(r.x, r.y, deg)];
+        const uint4 e = n2v_slot(p, (uint64_t)beg + mulhi64(r.x, r.y, deg));
+        const uint32_t x = e.x;
         const uint64_t z = r.z;
-        if (z < w_min) return x;
+        if (z < w_min) return e;
         if (x == prev) {
-            if (z < p.w_ret) return x;
+            if (z < p.w_ret) return e;
             continue;
         }
         if (z >= w_io) continue;
-        if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return x;
+        if (z < (edge_has(p.eset, p.eset_mask, prev, x) ? p.w_in : p.w_out)) return e;
     }
-    return n2v_exact(p, wlo, whi, step, beg, deg, prev);
+    return n2v_slot(p, (uint64_t)beg + n2v_exact(p, wlo, whi, step, beg, deg, prev));
 }
 
 // The exact draw after kN2vAttempts rejections: a pick from the whole row by
-// cumulative integer weight (the return edge at its full w_ret).
+// cumulative integer weight (the return edge at its full w_ret); returns the
+// slot in the row.
 __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, uint32_t step, uint32_t beg,
                               uint32_t deg, uint32_t prev)
 {
@@ -175,11 +189,10 @@ __device__ uint32_t n2v_exact(const WalkLaunch &p, uint32_t wlo, uint32_t whi, u
     const uint64_t pick = __umul64hi(((uint64_t)r.y << 32) | r.x, total);  // floor(u * total / 2^64)
     uint64_t cum = 0;
     for (uint32_t k = 0; k < deg; ++k) {
-        const uint32_t x = p.col[beg + k];
-        cum += n2v_weight(p, x, prev);
-        if (pick < cum) return x;
+        cum += n2v_weight(p, p.col[beg + k], prev);
+        if (pick < cum) return k;
     }
-    return p.col[beg + deg - 1];  // not reached: pick < total
+    return deg - 1;  // not reached: pick < total
 }
 
 // One thread per edge: its source is the last row starting at or before it
@@ -377,8 +390,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
     // the neighbour's row (start, degree) and graph id for the next step, so
     // a step is one dependent read instead of the row offsets, then the
     // neighbour, then its graph id.
-    const bool use_rec = KIND != KIND_N2V && (p.nrec != nullptr || p.wnrec != nullptr);
+    const bool use_rec = p.nrec != nullptr || p.wnrec != nullptr;
     uint32_t cur_beg = 0, cur_deg = 0, cur_g = 0;
+    uint32_t prev_beg = 0, prev_deg = 0, prev_g = 0;  // node2vec with records: prev's row and graph id
     uint64_t my_steps = 0;
     uint32_t my_dangling = 0;
     // Single-source walks return to the source often: its visits are counted
@@ -629,8 +643,20 @@ __global__ __launch_bounds__(1024) void walk_kernel(WalkLaunch p, uint32_t chunk
                     stop = true;
                 } else {
                     uint32_t nx;
-                    if (KIND == KIND_N2V && step > 0) {
-                        nx = n2v_pick(p, wlo, whi, step, r, beg, deg, prev);
+                    if (KIND == KIND_N2V && use_rec) {
+                        // the pick returns the neighbour's record; v's becomes prev's
+                        const uint4 e =
+                            step > 0 ? n2v_pick(p, wlo, whi, step, r, beg, deg, uint4{prev, prev_beg, prev_deg, prev_g})
+                                     : p.nrec[(uint64_t)beg + mulhi64(r.x, r.y, deg)];
+                        prev_beg = beg;
+                        prev_deg = deg;
+                        prev_g = cur_g;
+                        nx = e.x;
+                        cur_beg = e.y;
+                        cur_deg = e.z;
+                        cur_g = e.w;
+                    } else if (KIND == KIND_N2V && step > 0) {
+                        nx = n2v_pick(p, wlo, whi, step, r, beg, deg, uint4{prev, 0u, 0u, 0u}).x;
                     } else {
                         const uint32_t j = mulhi64(r.x, r.y, deg);
                         if (KIND != KIND_N2V && p.wnrec) {
@@ -914,7 +940,7 @@ hipError_t launch_walk(const WalkLaunch &p_in, const LaunchShape &shape, int num
     } else {
         p.vlog = nullptr;
     }
-    if (kind == KIND_N2V) p.nrec = p.wnrec = nullptr;
+    if (kind == KIND_N2V) p.wnrec = nullptr;  // node2vec walks are unweighted
     if (kind != KIND_PATH) p.wnrec = nullptr;  // weighted records carry no device ids
     if (p.wnrec) p.nrec = nullptr;
     auto count_lds = [&]() {

@@ -1,6 +1,6 @@
 """Node2vec on the device: paths bit-exact against the oracle's replay for
 several (p, q), start rules, seeds, streams and walker-id offsets, with and
-without the degree relabel, with the return edge folded (p < 1 on simple
+without the degree relabel and the neighbour records, with the return edge folded (p < 1 on simple
 symmetric graphs) and unfolded (graphs with a one-way edge or a repeated
 neighbour), on a graph where the exact fallback decides a fifth of the steps
 at its centre, and equal to DeepWalk when p = q = 1; the CLI binary end to
@@ -47,6 +47,19 @@ def test_parity_rmat12(ctx, rmat12, p, q):
     check(ctx, rp, col, 20000, 20, p, q, seed=3)
 
 
+@pytest.mark.parametrize("p,q", [(0.5, 2.0), (2.0, 0.5), (1.0 / 3000, 0.5)])
+def test_parity_without_records(ctx, rmat12, p, q):
+    # proposals read `col` and a step the row offsets when the graph keeps
+    # no neighbour records: the same paths
+    gi, rp, col = rmat12
+    t = ctx.tuning()
+    try:
+        ctx.set_tuning(edge_records=0)
+        check(ctx, rp, col, 20000, 20, p, q, seed=3)
+    finally:
+        ctx.set_tuning(edge_records=t.edge_records)
+
+
 def test_parity_starts_seeds_streams(ctx, rmat12):
     gi, rp, col = rmat12
     check(ctx, rp, col, 5000, 12, 0.5, 2.0, seed=11, stream=7, wid_base=(1 << 33) + 5)
@@ -65,10 +78,16 @@ def test_fallback_heavy_and_no_relabel(ctx):
     # the symmetric star ring folds the return edge; one directed chord keeps
     # it unfolded, and then the exact fallback decides a fifth of the centre's
     # steps
-    for rp, col in (star_ring(40), with_chord(*star_ring(40), 5, 25)):
-        for flags in (0, ptw.GRAPH_NO_RELABEL):
-            ctx.from_csr(rp, col, flags=flags)
-            check(ctx, rp, col, 20000, 30, 1.0 / 64, 64.0, seed=3)
+    t = ctx.tuning()
+    try:
+        for rp, col in (star_ring(40), with_chord(*star_ring(40), 5, 25)):
+            for flags in (0, ptw.GRAPH_NO_RELABEL):
+                ctx.from_csr(rp, col, flags=flags)
+                for rec in (1, 0):
+                    ctx.set_tuning(edge_records=rec)
+                    check(ctx, rp, col, 20000, 30, 1.0 / 64, 64.0, seed=3)
+    finally:
+        ctx.set_tuning(edge_records=t.edge_records)
 
 
 def test_self_loop_folded(ctx):
