@@ -339,6 +339,27 @@ def test_logged_count_overflow_paths(ctx):
     assert np.array_equal(r.topk_vertex[0], v) and np.array_equal(r.topk_count[0], cc)
 
 
+def test_logged_batch_many_buckets(ctx):
+    # a ring of 2e7 vertices has 1221 id buckets of 16384 ids: the log's
+    # partition then takes half tiles (16 segments) to leave the LDS to the
+    # bucket arrays; queries near the start, the middle and the wrap-around
+    n = 20_000_000
+    rp = (2 * np.arange(n + 1, dtype=np.uint64)).astype(np.uint32)
+    i = np.arange(n, dtype=np.int64)
+    nb = np.stack([(i - 1) % n, (i + 1) % n], axis=1)
+    nb.sort(axis=1)
+    col = nb.reshape(-1).astype(np.uint32)
+    ctx.from_csr(rp, col)
+    src, W, tp = [5, 10_000_000, n - 3], [50_000, 100_000, 200_000], [0.2, 0.15, 0.3]
+    r = ctx.batch_ppr(src, W, tp, topk=20, seed=7, dense_threshold=4096)
+    assert r.logged_queries == 3
+    for q in range(3):
+        ref = replay_query(rp, col, src[q], W[q], tp[q], q, seed=7)
+        assert np.array_equal(ctx.batch_counts(q), ref["counts"])
+        v, c = oracle.topk(ref["counts"], rp, 20)
+        assert np.array_equal(r.topk_vertex[q], v) and np.array_equal(r.topk_count[q], c)
+
+
 def test_topk_tie_overflow_path(ctx):
     # star: hub 0 joined to 5000 leaves; tp = 1 makes each walker visit the hub then one
     # leaf, so leaf counts are small and tied far beyond the 256-slot tile
