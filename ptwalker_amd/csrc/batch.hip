@@ -1148,9 +1148,53 @@ __global__ __launch_bounds__(1024) void batch_log_partition_kernel(BatchLaunch p
 // the pair list allows (top-k reads it whole, lookups search a run).  Two
 // barriers per bucket; the next nonempty bucket's entries are loaded while
 // this one is counted.  512-thread workgroups: two fit a CU, and one's
-// barriers overlap the other's work.
+// barriers overlap the other's work.  Small buckets come last, one per
+// wavefront (below).
 constexpr uint32_t kLogCountBlock = 512;
+// Buckets of at most kSmallBucket entries, with no hub counter and not the
+// source's, are counted by one wavefront each, without LDS counters or
+// workgroup barriers: a sort of their entries in registers (kSmallRegs per
+// lane), then the runs of equal ids.  On 4096 PPR queries on R-MAT scale 24
+// the counting took 9.4 ms without this path and 7.8, 7.5, 8.0 and 10.5 ms
+// with 1, 2, 4 and 8 keys per lane.
+constexpr uint32_t kSmallRegs = 2;
+constexpr uint32_t kSmallBucket = 64 * kSmallRegs;
 static_assert((1u << kBatchLogShift) / 32 == kLogCountBlock, "one bitmap word per thread");
+
+// Ascending bitonic sort of the wavefront's 64 * R keys, element k * 64 +
+// lane in key[k]: partners 64 or more apart are in the same lane's
+// registers, nearer ones in another lane of the same register.
+template <uint32_t R>
+__device__ __forceinline__ void wave_sort(uint32_t (&key)[R])
+{
+    static_assert((R & (R - 1)) == 0, "a power of two of registers");
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t size = 2; size <= 64 * R; size <<= 1) {
+#pragma unroll
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= 64) {
+                const uint32_t d = stride >> 6;
+#pragma unroll
+                for (uint32_t k = 0; k < R; ++k) {
+                    if (k & d) continue;  // k is the lower element of the pair (k, k | d)
+                    const bool asc = ((k * 64 + lane) & size) == 0;
+                    const uint32_t a = key[k], b = key[k | d];
+                    key[k] = asc ? min(a, b) : max(a, b);
+                    key[k | d] = asc ? max(a, b) : min(a, b);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < R; ++k) {
+                    const uint32_t other = (uint32_t)__shfl_xor(key[k], stride);
+                    const bool asc = ((k * 64 + lane) & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    key[k] = (lower == asc) ? min(key[k], other) : max(key[k], other);
+                }
+            }
+        }
+    }
+}
 static_assert(kBatchRunIds == 64 * 32, "a wavefront's run is one bitmap word per lane");
 __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLaunch p)
 {
@@ -1201,11 +1245,15 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
         }
         if (threadIdx.x == 0) s_pairs = 0;
         lds_barrier();
-        // buckets with something to count: entries, hub counters or the source
-        auto busy = [&](uint32_t b) {
+        // buckets with something to count: entries, hub counters or the
+        // source; the small ones (entries only) go to one wavefront each after
+        // the others
+        auto special = [&](uint32_t b) {
             const uint32_t lo = b << shift;
-            return b_cnt[b] != 0 || lo < hub_len || (src >= lo && src - lo < span);
+            return lo < hub_len || (src >= lo && src - lo < span);
         };
+        auto small = [&](uint32_t b) { return b_cnt[b] != 0 && b_cnt[b] <= kSmallBucket && !special(b); };
+        auto busy = [&](uint32_t b) { return (b_cnt[b] != 0 || special(b)) && !small(b); };
         auto next_busy = [&](uint32_t b) {
             while (b < nb && !busy(b)) ++b;
             return b;
@@ -1221,8 +1269,8 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             return v;
         };
         uint32_t b = next_busy(0);
-        for (uint32_t e = 0; e < b; ++e)  // buckets before the first busy one: no runs
-            if (threadIdx.x < runs_per_bucket) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
+        for (uint32_t e = 0; e < b; ++e)  // buckets before the first busy one: no runs (small ones later)
+            if (threadIdx.x < runs_per_bucket && !small(e)) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
         uint4 first = first_chunk(b);
         while (b < nb) {  // workgroup-uniform
             const uint32_t lo = b << shift;
@@ -1294,11 +1342,81 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                 p.run_len[ri] = run_n;
             }
             lds_barrier();  // the counters and bits are clear for the next bucket
-            // buckets skipped on the way: no runs
+            // buckets skipped on the way: no runs (small ones later)
             for (uint32_t e = b + 1; e < nxt; ++e)
-                if (threadIdx.x < runs_per_bucket) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
+                if (threadIdx.x < runs_per_bucket && !small(e)) p.run_len[(row + e) * runs_per_bucket + threadIdx.x] = 0;
             b = nxt;
         }
+        // small buckets, one per wavefront: sort the entries, count the runs
+        // of equal ids, write the distinct ids in order with their counts
+        for (uint32_t sb = wave; sb < nb; sb += kLogCountBlock / 64) {
+            if (!small(sb)) continue;  // wave-uniform
+            const uint32_t n = b_cnt[sb];
+            const uint64_t e0 = qb + b_loc[sb];
+            uint32_t key[kSmallRegs];
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+                const uint32_t e = k * 64 + lane;
+                key[k] = e < n ? (uint32_t)p.part16[e0 + e] : 0xFFFFFFFFu;
+            }
+            wave_sort(key);
+            // first element of each run of equal keys (padding sorts last)
+            uint64_t fm[kSmallRegs];
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+                // the element before: lane - 1 of this register, or lane 63 of the previous one
+                const uint32_t up = (uint32_t)__shfl_up(key[k], 1);
+                const uint32_t tail = k ? (uint32_t)__shfl(key[k ? k - 1 : 0], 63) : 0u;
+                const uint32_t before = lane ? up : tail;
+                const bool first = key[k] != 0xFFFFFFFFu && (k * 64 + lane == 0 || before != key[k]);
+                fm[k] = __ballot(first);
+            }
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallRegs; ++k) m += (uint32_t)__popcll(fm[k]);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&s_pairs, m);
+            base = __shfl(base, 0);
+            uint2 *out = p.touched + touched_off + base;
+            const uint32_t lo = sb << shift;
+            const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull);  // lanes below this one
+            uint32_t rank0 = 0;  // firsts in the registers before k
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+                if ((fm[k] >> lane) & 1ull) {
+                    // the next first element (or n) ends this run
+                    uint32_t next = n;
+                    const uint64_t above = lane < 63 ? fm[k] >> (lane + 1) : 0ull;
+                    if (above) {
+                        next = k * 64 + lane + 1 + (uint32_t)__ffsll((long long)above) - 1;
+                    } else {
+#pragma unroll
+                        for (uint32_t k2 = kSmallRegs - 1; k2 > k; --k2)
+                            if (fm[k2]) next = k2 * 64 + (uint32_t)__ffsll((long long)fm[k2]) - 1;
+                    }
+                    const uint32_t c = next - (k * 64 + lane);
+                    out[rank0 + (uint32_t)__popcll(fm[k] & below)] = uint2{lo + key[k], c};
+                    ls.add(c, s_hist, s_clogc);
+                }
+                rank0 += (uint32_t)__popcll(fm[k]);
+            }
+            // the bucket's runs of kBatchRunIds ids: place and length
+            uint32_t at = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < (1u << kBatchLogShift) / kBatchRunIds; ++r) {
+                uint32_t cnt_r = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < kSmallRegs; ++k)
+                    cnt_r += (uint32_t)__popcll(fm[k] & __ballot(key[k] != 0xFFFFFFFFu && key[k] / kBatchRunIds == r));
+                if (lane == r) {
+                    const uint64_t ri = (row + sb) * runs_per_bucket + r;
+                    p.run_loc[ri] = base + at;
+                    p.run_len[ri] = cnt_r;
+                }
+                at += cnt_r;
+            }
+        }
+        lds_barrier();  // every wavefront's pairs are reserved
         if (threadIdx.x == 0) p.touched_count[q] = s_pairs;
         flush_stats(ls, s_hist, &s_max, s_sums, p.tk_stats + q);
     }

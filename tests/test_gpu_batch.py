@@ -360,6 +360,31 @@ def test_logged_batch_many_buckets(ctx):
         assert np.array_equal(r.topk_vertex[q], v) and np.array_equal(r.topk_count[q], c)
 
 
+def test_logged_small_buckets(ctx):
+    # logged queries of 1100-3000 walkers on R-MAT scale 20 (64 id buckets)
+    # leave most buckets with a few dozen entries, counted one bucket per
+    # wavefront by a sort; the two larger queries' buckets take the LDS
+    # counters; every count vector and top-k equal to the replay
+    gi = ctx.rmat(scale=20, edge_factor=16, seed=3)
+    rp, col = ctx.export_csr()
+    rng = np.random.default_rng(4)
+    n = 10
+    src = rng.integers(0, gi.num_vertices, n)
+    W = np.concatenate([rng.integers(1100, 3000, n - 2), [25_000, 50_000]])
+    tp = rng.uniform(0.15, 0.3, n)
+    r = ctx.batch_ppr(src, W, tp, topk=25, seed=12, dense_threshold=1000)
+    assert r.logged_queries == n
+    for i in range(n):
+        ref = replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=12)
+        assert np.array_equal(ctx.batch_counts(i), ref["counts"]), i
+        v, c = oracle.topk(ref["counts"], rp, 25)
+        assert np.array_equal(r.topk_vertex[i], v) and np.array_equal(r.topk_count[i], c), i
+    qi = rng.integers(0, n, 3000)
+    vs = rng.integers(0, gi.num_vertices, 3000)
+    refs = [replay_query(rp, col, int(src[i]), int(W[i]), float(tp[i]), i, seed=12)["counts"] for i in range(n)]
+    assert np.array_equal(ctx.batch_gather(qi, vs), np.array([refs[q][v] for q, v in zip(qi, vs)], dtype=np.uint64))
+
+
 def test_topk_tie_overflow_path(ctx):
     # star: hub 0 joined to 5000 leaves; tp = 1 makes each walker visit the hub then one
     # leaf, so leaf counts are small and tied far beyond the 256-slot tile
