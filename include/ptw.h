@@ -292,8 +292,9 @@ typedef struct ptw_batch_desc {
     int32_t semantics;        /* PTW_SEM_* */
     uint32_t flags;           /* PTW_BATCH_* */
     uint64_t dense_threshold; /* walkers above this are dense (workgroup chunks, the visit log or
-                                 a dense slice); 0 = 4096 when dense queries are logged
-                                 (visit_log != 0), 100000 with dense slices */
+                                 a dense slice); 0 = 1 (every query dense) when dense
+                                 queries are logged (visit_log != 0), 100000 with dense
+                                 slices */
 } ptw_batch_desc;
 
 typedef struct ptw_batch_out {

@@ -282,9 +282,11 @@ constexpr uint32_t kBatchHdrWords = 16;     // BatchLaunch::hdr
 // Workgroup LDS of the dense kernel (160 KiB less its static variables).
 constexpr uint32_t kDenseLdsBytes = 163840 - 256;
 // Dense threshold (walkers) of a batch that gives none: logged dense queries
-// cost no V-sized slice, so any query worth a workgroup chunk is dense; with
+// cost no V-sized slice and a small one is counted a bucket per wavefront,
+// so every query is dense (4096 walkers: 71.9 ms for 4096 queries on R-MAT
+// scale 24, 1 walker: 69.5 ms; serving as fast, its latency lower); with
 // slices, only the large ones.
-constexpr uint64_t kDenseThresholdLogged = 4096;
+constexpr uint64_t kDenseThresholdLogged = 1;
 constexpr uint64_t kDenseThresholdSlices = 100000;
 // A dense query is logged while its expected counts are at most this many
 // per vertex.  Its log holds about 17.5 B per expected count where a slice

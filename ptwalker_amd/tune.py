@@ -158,7 +158,7 @@ def tune_batch(ctx, sources, walkers, tps, *, topk: int = 100, seed: int = 1, re
     start = {k: base[k] for k in space if k in base}
     if "dense_threshold" in space:
         # the planner's default for the context's counting
-        start["dense_threshold"] = 4_096 if base.get("visit_log", 1) else 100_000
+        start["dense_threshold"] = 1 if base.get("visit_log", 1) else 100_000
     ref = ctx.batch_ppr(sources, walkers, tps, topk=topk, seed=seed, dense_threshold=start.get("dense_threshold", 0))
 
     def measure(c):

@@ -23,8 +23,8 @@ def test_tune_batch_keeps_results_and_caches(ctx, tmp_path):
     space = {"batch_chunk": [4096, 16384], "block": [512, 1024], "dense_threshold": [50_000, 100_000],
              "topk_tile": [256, 2048]}
     r = tune.tune_batch(ctx, src, W, tp, topk=100, repeats=1, rounds=1, space=space, cache_path=path)
-    # the tuner starts from the planner's default threshold (4096 with logged dense queries)
-    start_thr = 4_096 if default["visit_log"] else 100_000
+    # the tuner starts from the planner's default threshold (1 with logged dense queries)
+    start_thr = 1 if default["visit_log"] else 100_000
     assert all(r.best[k] in space[k] or r.best[k] == default.get(k, start_thr) for k in r.best)
     assert r.best["topk_tile"] >= 200  # 256 cannot hold 2k candidates
     assert len(r.trials) >= 5 and r.best_ms <= r.start_ms
