@@ -32,6 +32,8 @@
 // workgroup narrows the count range with LDS histograms over the query's
 // entries until the candidates fit; ties that do not fit the tile are cut by
 // an exact radix select on the device id.
+#include <type_traits>
+
 #include "../../include/ptw.h"
 #include "engine.hpp"
 #include "philox.hpp"
@@ -1154,9 +1156,11 @@ constexpr uint32_t kLogCountBlock = 512;
 // Buckets of at most kSmallBucket entries, with no hub counter and not the
 // source's, are counted by one wavefront each, without LDS counters or
 // workgroup barriers: a sort of their entries in registers (kSmallRegs per
-// lane), then the runs of equal ids.  On 4096 PPR queries on R-MAT scale 24
-// the counting took 9.4 ms without this path and 7.8, 7.5, 8.0 and 10.5 ms
-// with 1, 2, 4 and 8 keys per lane.
+// lane; one where the bucket has at most 64 entries), then the runs of equal
+// ids.  On 4096 PPR queries on R-MAT scale 24 the counting took 9.4 ms
+// without this path and 7.8, 7.5, 8.0 and 10.5 ms with 1, 2, 4 and 8 keys per
+// lane (with every query logged: 8.0 ms at 2 keys, 7.8 ms with 1 key for the
+// buckets of at most 64 entries).
 constexpr uint32_t kSmallRegs = 2;
 constexpr uint32_t kSmallBucket = 64 * kSmallRegs;
 static_assert((1u << kBatchLogShift) / 32 == kLogCountBlock, "one bitmap word per thread");
@@ -1349,21 +1353,20 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
         }
         // small buckets, one per wavefront: sort the entries, count the runs
         // of equal ids, write the distinct ids in order with their counts
-        for (uint32_t sb = wave; sb < nb; sb += kLogCountBlock / 64) {
-            if (!small(sb)) continue;  // wave-uniform
-            const uint32_t n = b_cnt[sb];
-            const uint64_t e0 = qb + b_loc[sb];
-            uint32_t key[kSmallRegs];
+        // one small bucket sb of n entries from e0, R keys per lane
+        auto count_small = [&](auto regs, uint32_t sb, uint32_t n, uint64_t e0) {
+            constexpr uint32_t R = decltype(regs)::value;
+            uint32_t key[R];
 #pragma unroll
-            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+            for (uint32_t k = 0; k < R; ++k) {
                 const uint32_t e = k * 64 + lane;
                 key[k] = e < n ? (uint32_t)p.part16[e0 + e] : 0xFFFFFFFFu;
             }
             wave_sort(key);
             // first element of each run of equal keys (padding sorts last)
-            uint64_t fm[kSmallRegs];
+            uint64_t fm[R];
 #pragma unroll
-            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+            for (uint32_t k = 0; k < R; ++k) {
                 // the element before: lane - 1 of this register, or lane 63 of the previous one
                 const uint32_t up = (uint32_t)__shfl_up(key[k], 1);
                 const uint32_t tail = k ? (uint32_t)__shfl(key[k ? k - 1 : 0], 63) : 0u;
@@ -1373,7 +1376,7 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             }
             uint32_t m = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < kSmallRegs; ++k) m += (uint32_t)__popcll(fm[k]);
+            for (uint32_t k = 0; k < R; ++k) m += (uint32_t)__popcll(fm[k]);
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&s_pairs, m);
             base = __shfl(base, 0);
@@ -1382,7 +1385,7 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull);  // lanes below this one
             uint32_t rank0 = 0;  // firsts in the registers before k
 #pragma unroll
-            for (uint32_t k = 0; k < kSmallRegs; ++k) {
+            for (uint32_t k = 0; k < R; ++k) {
                 if ((fm[k] >> lane) & 1ull) {
                     // the next first element (or n) ends this run
                     uint32_t next = n;
@@ -1391,7 +1394,7 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                         next = k * 64 + lane + 1 + (uint32_t)__ffsll((long long)above) - 1;
                     } else {
 #pragma unroll
-                        for (uint32_t k2 = kSmallRegs - 1; k2 > k; --k2)
+                        for (uint32_t k2 = R - 1; k2 > k; --k2)
                             if (fm[k2]) next = k2 * 64 + (uint32_t)__ffsll((long long)fm[k2]) - 1;
                     }
                     const uint32_t c = next - (k * 64 + lane);
@@ -1406,7 +1409,7 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
             for (uint32_t r = 0; r < (1u << kBatchLogShift) / kBatchRunIds; ++r) {
                 uint32_t cnt_r = 0;
 #pragma unroll
-                for (uint32_t k = 0; k < kSmallRegs; ++k)
+                for (uint32_t k = 0; k < R; ++k)
                     cnt_r += (uint32_t)__popcll(fm[k] & __ballot(key[k] != 0xFFFFFFFFu && key[k] / kBatchRunIds == r));
                 if (lane == r) {
                     const uint64_t ri = (row + sb) * runs_per_bucket + r;
@@ -1415,6 +1418,15 @@ __global__ __launch_bounds__(kLogCountBlock) void batch_log_count_kernel(BatchLa
                 }
                 at += cnt_r;
             }
+        };
+        for (uint32_t sb = wave; sb < nb; sb += kLogCountBlock / 64) {
+            if (!small(sb)) continue;  // wave-uniform
+            const uint32_t n = b_cnt[sb];
+            const uint64_t e0 = qb + b_loc[sb];
+            if (n <= 64)  // wave-uniform: a one-register sort is cheaper
+                count_small(std::integral_constant<uint32_t, 1>{}, sb, n, e0);
+            else
+                count_small(std::integral_constant<uint32_t, kSmallRegs>{}, sb, n, e0);
         }
         lds_barrier();  // every wavefront's pairs are reserved
         if (threadIdx.x == 0) p.touched_count[q] = s_pairs;
